@@ -1,0 +1,206 @@
+/*
+ * gl_lists.c - display lists of libmtgl.
+ *
+ * Recording and replay follow the reference (src/lists.c, src/gl_api.c:2268-2499):
+ * recorded calls are stored with their arguments; GL_COMPILE records without
+ * executing, GL_COMPILE_AND_EXECUTE does both; replay calls the same GL entry
+ * points, so list geometry goes through the device pipeline with the matrices
+ * current at replay time.  List ids are 1-based, a contiguous run of freed ids
+ * is reused first, at most 1024 ids exist, nesting stops at 64 levels.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "gl_state.h"
+
+#define MTGL_EXPORT __attribute__((visibility("default")))
+#define CTX_OR_RETURN()        GLState *c = gls_current(); if (!c) return
+#define CTX_OR_RETURN_V(val)   GLState *c = gls_current(); if (!c) return (val)
+
+static gls_list *list_lookup(GLState *c, GLuint id)
+{
+    if (id == 0 || id > c->list_count) return NULL;
+    gls_list *l = &c->lists[id - 1];
+    return l->allocated ? l : NULL;
+}
+
+int gls_list_recording(GLState *c)
+{
+    return c && c->list_index != 0;
+}
+
+/* Append cmd to the list being compiled.  Returns 1 when the caller must not
+ * execute the call (GL_COMPILE), 0 otherwise (reference record_cmd,
+ * src/lists.c:185-197). */
+int gls_list_record(GLState *c, const gls_list_cmd *cmd)
+{
+    if (!c || c->list_index == 0) return 0;
+    gls_list *l = list_lookup(c, c->list_index);
+    if (l) {
+        if (l->count == l->capacity) {
+            size_t cap = l->capacity ? l->capacity * 2 : 64;
+            gls_list_cmd *p = (gls_list_cmd *)realloc(l->cmds, cap * sizeof(*p));
+            if (p) { l->cmds = p; l->capacity = cap; }
+        }
+        if (l->count < l->capacity) l->cmds[l->count++] = *cmd;
+    }
+    return c->list_mode == GL_COMPILE;
+}
+
+static void list_reset(gls_list *l)
+{
+    l->cmds = NULL;
+    l->count = 0;
+    l->capacity = 0;
+    l->valid = GL_FALSE;
+    l->allocated = GL_TRUE;
+}
+
+MTGL_EXPORT GLuint glGenLists(GLsizei range)
+{
+    CTX_OR_RETURN_V(0);
+    if (range < 0) { gls_set_error(c, GL_INVALID_VALUE); return 0; }
+    if (range == 0) return 0;
+    /* first run of `range` free slots among the existing ones */
+    size_t run = 0, start = 0;
+    for (size_t i = 0; i < c->list_count; i++) {
+        if (!c->lists[i].allocated) {
+            if (run == 0) start = i;
+            if (++run >= (size_t)range) {
+                for (GLsizei k = 0; k < range; k++) list_reset(&c->lists[start + (size_t)k]);
+                return (GLuint)(start + 1);
+            }
+        } else {
+            run = 0;
+        }
+    }
+    if (c->list_count + (size_t)range > GLS_MAX_LISTS) return 0;
+    GLuint first = (GLuint)(c->list_count + 1);
+    for (GLsizei k = 0; k < range; k++) list_reset(&c->lists[c->list_count + (size_t)k]);
+    c->list_count += (size_t)range;
+    return first;
+}
+
+MTGL_EXPORT void glDeleteLists(GLuint list, GLsizei range)
+{
+    CTX_OR_RETURN();
+    if (range < 0) { gls_set_error(c, GL_INVALID_VALUE); return; }
+    for (GLsizei i = 0; i < range; i++) {
+        gls_list *l = list_lookup(c, list + (GLuint)i);
+        if (!l) continue;
+        free(l->cmds);
+        l->cmds = NULL;
+        l->count = l->capacity = 0;
+        l->valid = GL_FALSE;
+        l->allocated = GL_FALSE;
+    }
+}
+
+MTGL_EXPORT void glNewList(GLuint list, GLenum mode)
+{
+    CTX_OR_RETURN();
+    if (c->list_index != 0) { gls_set_error(c, GL_INVALID_OPERATION); return; }
+    if (mode != GL_COMPILE && mode != GL_COMPILE_AND_EXECUTE) { gls_set_error(c, GL_INVALID_ENUM); return; }
+    gls_list *l = list_lookup(c, list);
+    if (!l) { gls_set_error(c, GL_INVALID_VALUE); return; }
+    l->count = 0;
+    l->valid = GL_FALSE;
+    c->list_index = list;
+    c->list_mode = mode;
+}
+
+MTGL_EXPORT void glEndList(void)
+{
+    CTX_OR_RETURN();
+    if (c->list_index == 0) { gls_set_error(c, GL_INVALID_OPERATION); return; }
+    gls_list *l = list_lookup(c, c->list_index);
+    if (l) l->valid = GL_TRUE;
+    c->list_index = 0;
+    c->list_mode = 0;
+}
+
+static void execute_list(GLState *c, GLuint id)
+{
+    gls_list *l = list_lookup(c, id);
+    if (!l || !l->valid) return;
+    for (size_t i = 0; i < l->count; i++) {
+        /* copy: a nested call may grow or free this list */
+        gls_list_cmd k = l->cmds[i];
+        const GLfloat *f = k.f;
+        switch (k.op) {
+        case LOP_BEGIN:         glBegin(k.e[0]); break;
+        case LOP_END:           glEnd(); break;
+        case LOP_VERTEX:        glVertex3f(f[0], f[1], f[2]); break;
+        case LOP_COLOR:         glColor4f(f[0], f[1], f[2], f[3]); break;
+        case LOP_TEXCOORD:      glTexCoord2f(f[0], f[1]); break;
+        case LOP_NORMAL:        glNormal3f(f[0], f[1], f[2]); break;
+        case LOP_TRANSLATEF:    glTranslatef(f[0], f[1], f[2]); break;
+        case LOP_ROTATEF:       glRotatef(f[0], f[1], f[2], f[3]); break;
+        case LOP_SCALEF:        glScalef(f[0], f[1], f[2]); break;
+        case LOP_PUSH_MATRIX:   glPushMatrix(); break;
+        case LOP_POP_MATRIX:    glPopMatrix(); break;
+        case LOP_LOAD_IDENTITY: glLoadIdentity(); break;
+        case LOP_MULT_MATRIXF:  glMultMatrixf(f); break;
+        case LOP_LOAD_MATRIXF:  glLoadMatrixf(f); break;
+        case LOP_MATRIX_MODE:   glMatrixMode(k.e[0]); break;
+        case LOP_ENABLE:        glEnable(k.e[0]); break;
+        case LOP_DISABLE:       glDisable(k.e[0]); break;
+        case LOP_BIND_TEXTURE:  glBindTexture(k.e[0], k.e[1]); break;
+        case LOP_BLEND_FUNC:    glBlendFunc(k.e[0], k.e[1]); break;
+        case LOP_DEPTH_FUNC:    glDepthFunc(k.e[0]); break;
+        case LOP_DEPTH_MASK:    glDepthMask((GLboolean)k.e[0]); break;
+        case LOP_CULL_FACE:     glCullFace(k.e[0]); break;
+        case LOP_FRONT_FACE:    glFrontFace(k.e[0]); break;
+        case LOP_SHADE_MODEL:   glShadeModel(k.e[0]); break;
+        case LOP_LIGHTF:        glLightf(k.e[0], k.e[1], f[0]); break;
+        case LOP_LIGHTFV:       glLightfv(k.e[0], k.e[1], f); break;
+        case LOP_MATERIALF:     glMaterialf(k.e[0], k.e[1], f[0]); break;
+        case LOP_MATERIALFV:    glMaterialfv(k.e[0], k.e[1], f); break;
+        case LOP_CALL_LIST:     glCallList(k.e[0]); break;
+        case LOP_ORTHO:         glOrtho(k.d[0], k.d[1], k.d[2], k.d[3], k.d[4], k.d[5]); break;
+        case LOP_FRUSTUM:       glFrustum(k.d[0], k.d[1], k.d[2], k.d[3], k.d[4], k.d[5]); break;
+        }
+        l = list_lookup(c, id);
+        if (!l) return;
+    }
+}
+
+MTGL_EXPORT void glCallList(GLuint list)
+{
+    CTX_OR_RETURN();
+    gls_list_cmd cmd = { LOP_CALL_LIST, { list } };
+    if (gls_list_record(c, &cmd)) return;
+    if (c->list_call_depth >= GLS_LIST_DEPTH) { gls_set_error(c, GL_STACK_OVERFLOW); return; }
+    c->list_call_depth++;
+    execute_list(c, list);
+    c->list_call_depth--;
+}
+
+MTGL_EXPORT void glCallLists(GLsizei n, GLenum type, const GLvoid *lists)
+{
+    CTX_OR_RETURN();
+    if (n < 0) { gls_set_error(c, GL_INVALID_VALUE); return; }
+    for (GLsizei i = 0; i < n; i++) {
+        GLuint offset;
+        switch (type) {
+        case GL_UNSIGNED_BYTE:  offset = ((const GLubyte *)lists)[i]; break;
+        case GL_UNSIGNED_SHORT: offset = ((const GLushort *)lists)[i]; break;
+        case GL_UNSIGNED_INT:   offset = ((const GLuint *)lists)[i]; break;
+        default: gls_set_error(c, GL_INVALID_ENUM); return;
+        }
+        glCallList(c->list_base + offset);
+    }
+}
+
+MTGL_EXPORT void glListBase(GLuint base)
+{
+    CTX_OR_RETURN();
+    c->list_base = base;
+}
+
+MTGL_EXPORT GLboolean glIsList(GLuint list)
+{
+    CTX_OR_RETURN_V(GL_FALSE);
+    gls_list *l = list_lookup(c, list);
+    return (l && l->valid) ? GL_TRUE : GL_FALSE;
+}

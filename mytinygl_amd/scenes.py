@@ -1,0 +1,777 @@
+"""Seeded scenes that drive a GLLibrary through the GL 1.5 API.
+
+Every scene is a function ``scene(g, W, H, seed)`` issuing GL calls on a bound
+library ``g`` whose context is current.  ``render()`` creates the context,
+clears every plane, runs the scene and returns (colour, depth, stencil).
+
+The workload configurations are the ones the benchmark and the parity tests
+use:
+
+* ``icosphere``  - 1,280-triangle icosphere, per-vertex colours through
+  GL_COLOR_MATERIAL, one directional light, Gouraud, depth test, 640x480,
+  glDrawArrays.
+* ``textured_sphere`` - 128x128 lat-long grid sphere (32,768 triangles) with
+  radial N(0, 0.05) displacement, 256x256 value-noise texture with
+  GL_LINEAR_MIPMAP_LINEAR / GL_LINEAR, perspective-correct, linear fog, one
+  light, 1920x1080, glDrawElements from buffer objects.
+* ``overdraw`` - 200k triangles with log-uniform areas of 1..1e4 px, blended
+  SRC_ALPHA / ONE_MINUS_SRC_ALPHA, depth LESS, stencil GREATER 8 / INCR,
+  back-face culling, 3840x2160.
+
+The remaining scenes are small feature scenes that together reach every branch
+of the per-vertex, clipping and per-fragment paths (reference src/raster.c,
+src/lighting.h, src/textures.c, src/clipping.h).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .gl import ENUMS as E
+
+SEED_BASE = 0x4D54474C00          # seeds 0x4D54474C01.. for the configurations
+
+
+def _f32(*v):
+    return np.asarray(v, np.float32)
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+# ---------------------------------------------------------------------------
+# geometry and texture generators
+# ---------------------------------------------------------------------------
+
+def icosphere(subdiv: int):
+    """Unit icosphere: (positions (V,3) float64, triangles (T,3) int)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+             (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+        new = []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            new += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = new
+    return np.array(verts), np.array(faces, np.int64)
+
+
+def value_noise_rgba(size: int, seed: int, octaves: int = 4, lattice: int = 16) -> np.ndarray:
+    """size x size RGBA8 value noise: `octaves` octaves of a seeded lattice^2
+    grid per channel, smoothstep interpolation, summed with halving weights and
+    quantised to bytes."""
+    rng = _rng(seed)
+    out = np.zeros((size, size, 4), np.float64)
+    total = 0.0
+    for o in range(octaves):
+        cells = lattice * (2 ** o)
+        grid = rng.random((cells + 1, cells + 1, 4))
+        grid[cells, :] = grid[0, :]
+        grid[:, cells] = grid[:, 0]
+        coord = (np.arange(size) + 0.5) * cells / size
+        i0 = np.floor(coord).astype(np.int64)
+        f = coord - i0
+        s = f * f * (3 - 2 * f)
+        gy0, gy1 = i0[:, None], i0[:, None] + 1
+        gx0, gx1 = i0[None, :], i0[None, :] + 1
+        sx, sy = s[None, :, None], s[:, None, None]
+        top = grid[gy0, gx0] * (1 - sx) + grid[gy0, gx1] * sx
+        bot = grid[gy1, gx0] * (1 - sx) + grid[gy1, gx1] * sx
+        w = 0.5 ** o
+        out += w * (top * (1 - sy) + bot * sy)
+        total += w
+    return np.clip(np.floor(out / total * 256.0), 0, 255).astype(np.uint8)
+
+
+def random_rgba(w: int, h: int, seed: int) -> np.ndarray:
+    return _rng(seed).integers(0, 256, size=(h, w, 4), dtype=np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# runner
+# ---------------------------------------------------------------------------
+
+def render(g, scene, W: int, H: int, seed: int = 0, clear=(0.0, 0.0, 0.0, 1.0), keep_context=False):
+    ctx = g.create_context(W, H)
+    try:
+        g.glClearColor(*clear)
+        g.glClearDepth(1.0)
+        g.glClearStencil(0)
+        g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"] | E["GL_STENCIL_BUFFER_BIT"])
+        scene(g, W, H, seed)
+        out = g.read_framebuffer(ctx, W, H)
+        err = g.glGetError()
+    finally:
+        if not keep_context:
+            g.destroy_context(ctx)
+            g.release()
+    if keep_context:
+        return out, err, ctx
+    return out, err
+
+
+def _perspective(g, W, H, near=0.1, far=100.0, half_h=0.1, dist=-3.0):
+    aspect = W / H
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glFrustum(-half_h * aspect, half_h * aspect, -half_h, half_h, near, far)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    if dist:
+        g.glTranslatef(0.0, 0.0, dist)
+
+
+# ---------------------------------------------------------------------------
+# workload configurations
+# ---------------------------------------------------------------------------
+
+def setup_icosphere(g, W=640, H=480, seed=SEED_BASE + 1):
+    pos, faces = icosphere(3)
+    rng = _rng(seed)
+    colors = rng.random((len(pos), 4)).astype(np.float32)
+    tri = faces.reshape(-1)
+    vp = g.keep(np.ascontiguousarray(pos[tri].astype(np.float32)))
+    vn = g.keep(vp.copy())
+    vc = g.keep(np.ascontiguousarray(colors[tri]))
+    g.glViewport(0, 0, W, H)
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glFrustum(-0.1 * 4.0 / 3.0, 0.1 * 4.0 / 3.0, -0.1, 0.1, 0.1, 100.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.fv(1, 1, 1, 0))
+    g.glEnable(E["GL_COLOR_MATERIAL"])
+    g.glColorMaterial(E["GL_FRONT_AND_BACK"], E["GL_AMBIENT_AND_DIFFUSE"])
+    g.glShadeModel(E["GL_SMOOTH"])
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glDepthFunc(E["GL_LESS"])
+    g.glTranslatef(0.0, 0.0, -3.0)
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_NORMAL_ARRAY"])
+    g.glEnableClientState(E["GL_COLOR_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 0, vp)
+    g.glNormalPointer(E["GL_FLOAT"], 0, vn)
+    g.glColorPointer(4, E["GL_FLOAT"], 0, vc)
+    return len(tri)
+
+
+def draw_icosphere(g, n):
+    g.glDrawArrays(E["GL_TRIANGLES"], 0, n)
+
+
+def scene_icosphere(g, W, H, seed=SEED_BASE + 1):
+    draw_icosphere(g, setup_icosphere(g, W, H, seed))
+
+
+def sphere_grid(n: int, seed: int, sigma: float = 0.05):
+    """(n+1)^2 lat-long vertices with per-unique-vertex radial N(0, sigma) noise
+    (seam column and poles shared) and 2 n^2 triangles."""
+    rng = _rng(seed)
+    noise = rng.standard_normal((n + 1, n)) * sigma
+    noise[0, :] = noise[0, 0]
+    noise[n, :] = noise[n, 0]
+    i = np.arange(n + 1)[:, None]
+    j = np.arange(n + 1)[None, :]
+    theta = np.pi * i / n
+    phi = 2 * np.pi * j / n
+    unit = np.stack([np.sin(theta) * np.cos(phi), np.cos(theta) * np.ones_like(phi),
+                     np.sin(theta) * np.sin(phi)], -1)
+    r = 1.0 + noise[:, np.arange(n + 1) % n]
+    posn = unit * r[..., None]
+    uv = np.stack([np.broadcast_to(j / n, (n + 1, n + 1)), np.broadcast_to(i / n, (n + 1, n + 1))], -1)
+    verts = np.concatenate([posn, unit, uv], -1).reshape(-1, 8).astype(np.float32)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v00 = ii * (n + 1) + jj
+    v01 = v00 + 1
+    v10 = v00 + (n + 1)
+    v11 = v10 + 1
+    idx = np.stack([v00, v10, v11, v00, v11, v01], -1).reshape(-1).astype(np.uint32)
+    return np.ascontiguousarray(verts), np.ascontiguousarray(idx)
+
+
+def setup_textured_sphere(g, W=1920, H=1080, seed=SEED_BASE + 2, grid=128, tex_size=256):
+    verts, idx = sphere_grid(grid, seed)
+    tex = value_noise_rgba(tex_size, SEED_BASE + 3)
+    g.glViewport(0, 0, W, H)
+    aspect = 16.0 / 9.0
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glFrustum(-0.1 * aspect, 0.1 * aspect, -0.1, 0.1, 0.1, 100.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.fv(1, 1, 1, 0))
+    g.glTranslatef(0.0, 0.0, -2.0)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glHint(E["GL_PERSPECTIVE_CORRECTION_HINT"], E["GL_NICEST"])
+    g.glEnable(E["GL_FOG"])
+    g.glFogi(E["GL_FOG_MODE"], E["GL_LINEAR"])
+    g.glFogf(E["GL_FOG_START"], 1.0)
+    g.glFogf(E["GL_FOG_END"], 3.0)
+    ids = np.zeros(1, np.uint32)
+    g.glGenTextures(1, ids)
+    g.glBindTexture(E["GL_TEXTURE_2D"], int(ids[0]))
+    g.glTexImage2D(E["GL_TEXTURE_2D"], 0, E["GL_RGBA"], tex_size, tex_size, 0, E["GL_RGBA"],
+                   E["GL_UNSIGNED_BYTE"], g.keep(np.ascontiguousarray(tex)))
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E["GL_LINEAR_MIPMAP_LINEAR"])
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"], E["GL_LINEAR"])
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_WRAP_S"], E["GL_REPEAT"])
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_WRAP_T"], E["GL_REPEAT"])
+    g.glEnable(E["GL_TEXTURE_2D"])
+    bufs = np.zeros(2, np.uint32)
+    g.glGenBuffers(2, bufs)
+    g.glBindBuffer(E["GL_ARRAY_BUFFER"], int(bufs[0]))
+    g.glBufferData(E["GL_ARRAY_BUFFER"], verts.nbytes, g.keep(verts), E["GL_STATIC_DRAW"])
+    g.glBindBuffer(E["GL_ELEMENT_ARRAY_BUFFER"], int(bufs[1]))
+    g.glBufferData(E["GL_ELEMENT_ARRAY_BUFFER"], idx.nbytes, g.keep(idx), E["GL_STATIC_DRAW"])
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_NORMAL_ARRAY"])
+    g.glEnableClientState(E["GL_TEXTURE_COORD_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 32, 0)
+    g.glNormalPointer(E["GL_FLOAT"], 32, 12)
+    g.glTexCoordPointer(2, E["GL_FLOAT"], 32, 24)
+    return len(idx)
+
+
+def draw_textured_sphere(g, n):
+    g.glDrawElements(E["GL_TRIANGLES"], n, E["GL_UNSIGNED_INT"], 0)
+
+
+def scene_textured_sphere(g, W, H, seed=SEED_BASE + 2):
+    draw_textured_sphere(g, setup_textured_sphere(g, W, H, seed))
+
+
+def overdraw_geometry(W: int, H: int, ntri: int, seed: int, max_log_area: float = 4.0):
+    """Triangles with area 10^U(0, max_log_area) px, centroid ~ U(screen), random
+    rotation and aspect in [1, 4], constant eye depth U(0.1, 100), RGB U[0,1],
+    alpha U(0.2, 1), winding flipped for half of them."""
+    rng = _rng(seed)
+    area = 10.0 ** rng.uniform(0.0, max_log_area, ntri)
+    cx = rng.uniform(0, W, ntri)
+    cy = rng.uniform(0, H, ntri)
+    rot = rng.uniform(0, 2 * np.pi, ntri)
+    aspect = rng.uniform(1.0, 4.0, ntri)
+    depth = rng.uniform(0.1, 100.0, ntri)
+    rgb = rng.random((ntri, 3))
+    alpha = rng.uniform(0.2, 1.0, ntri)
+    flip = rng.random(ntri) < 0.5
+    ang = np.array([0.0, 2 * np.pi / 3, 4 * np.pi / 3])
+    bx = np.cos(ang)[None, :] * aspect[:, None]
+    by = np.sin(ang)[None, :]
+    base_area = 0.5 * np.abs((bx[:, 1] - bx[:, 0]) * (by[:, 2] - by[:, 0]) - (bx[:, 2] - bx[:, 0]) * (by[:, 1] - by[:, 0]))
+    s = np.sqrt(area / base_area)
+    rx = (bx * np.cos(rot)[:, None] - by * np.sin(rot)[:, None]) * s[:, None] + cx[:, None]
+    ry = (bx * np.sin(rot)[:, None] + by * np.cos(rot)[:, None]) * s[:, None] + cy[:, None]
+    rx[flip, 1], rx[flip, 2] = rx[flip, 2].copy(), rx[flip, 1].copy()
+    ry[flip, 1], ry[flip, 2] = ry[flip, 2].copy(), ry[flip, 1].copy()
+    pos = np.stack([rx, ry, np.broadcast_to(-depth[:, None], rx.shape)], -1).reshape(-1, 3).astype(np.float32)
+    col = np.concatenate([rgb, alpha[:, None]], 1)
+    col = np.repeat(col, 3, axis=0).astype(np.float32)
+    return np.ascontiguousarray(pos), np.ascontiguousarray(col)
+
+
+def setup_overdraw(g, W=3840, H=2160, seed=SEED_BASE + 4, ntri=200_000, max_log_area=4.0):
+    pos, col = overdraw_geometry(W, H, ntri, seed, max_log_area)
+    g.glViewport(0, 0, W, H)
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glOrtho(0.0, float(W), 0.0, float(H), 0.1, 100.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    g.glEnable(E["GL_CULL_FACE"])
+    g.glCullFace(E["GL_BACK"])
+    g.glEnable(E["GL_BLEND"])
+    g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE_MINUS_SRC_ALPHA"])
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glDepthFunc(E["GL_LESS"])
+    g.glEnable(E["GL_STENCIL_TEST"])
+    g.glStencilFunc(E["GL_GREATER"], 8, 0xFF)
+    g.glStencilOp(E["GL_KEEP"], E["GL_KEEP"], E["GL_INCR"])
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_COLOR_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 0, g.keep(pos))
+    g.glColorPointer(4, E["GL_FLOAT"], 0, g.keep(col))
+    return len(pos)
+
+
+def draw_overdraw(g, n):
+    g.glDrawArrays(E["GL_TRIANGLES"], 0, n)
+
+
+def scene_overdraw(g, W, H, seed=SEED_BASE + 4, ntri=200_000):
+    draw_overdraw(g, setup_overdraw(g, W, H, seed, ntri))
+
+
+# ---------------------------------------------------------------------------
+# feature scenes
+# ---------------------------------------------------------------------------
+
+def _random_tris_immediate(g, rng, n, spread=3.0, alpha=(0.0, 1.0), tex=False, normals=False):
+    for _ in range(n):
+        for _k in range(3):
+            c = rng.random(4)
+            c[3] = rng.uniform(*alpha)
+            g.glColor4f(*c.astype(np.float32))
+            if tex:
+                g.glTexCoord2f(*rng.uniform(-1.0, 2.0, 2).astype(np.float32))
+            if normals:
+                g.glNormal3f(*rng.normal(size=3).astype(np.float32))
+            g.glVertex3f(*((rng.random(3) - 0.5) * spread).astype(np.float32))
+
+
+def scene_depth_funcs(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    funcs = ["GL_NEVER", "GL_LESS", "GL_EQUAL", "GL_LEQUAL", "GL_GREATER", "GL_NOTEQUAL", "GL_GEQUAL", "GL_ALWAYS"]
+    for k, fn in enumerate(funcs):
+        g.glDepthFunc(E[fn])
+        g.glDepthMask(bool(k % 3 != 2))
+        if k == 4:
+            g.glDepthRange(0.2, 0.7)
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 6)
+        g.glEnd()
+
+
+def scene_blend(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_BLEND"])
+    srcs = ["GL_ZERO", "GL_ONE", "GL_SRC_COLOR", "GL_ONE_MINUS_SRC_COLOR", "GL_DST_COLOR",
+            "GL_ONE_MINUS_DST_COLOR", "GL_SRC_ALPHA", "GL_ONE_MINUS_SRC_ALPHA", "GL_DST_ALPHA",
+            "GL_ONE_MINUS_DST_ALPHA", "GL_SRC_ALPHA_SATURATE", "GL_CONSTANT_COLOR", "GL_ONE_MINUS_CONSTANT_ALPHA"]
+    dsts = [s for s in srcs if s != "GL_SRC_ALPHA_SATURATE"]
+    for k in range(24):
+        g.glBlendFunc(E[srcs[rng.integers(len(srcs))]], E[dsts[rng.integers(len(dsts))]])
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 3)
+        g.glEnd()
+
+
+def scene_stencil(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_STENCIL_TEST"])
+    g.glEnable(E["GL_DEPTH_TEST"])
+    funcs = ["GL_NEVER", "GL_LESS", "GL_EQUAL", "GL_LEQUAL", "GL_GREATER", "GL_NOTEQUAL", "GL_GEQUAL", "GL_ALWAYS"]
+    ops = ["GL_KEEP", "GL_ZERO", "GL_REPLACE", "GL_INCR", "GL_INCR_WRAP", "GL_DECR", "GL_DECR_WRAP", "GL_INVERT"]
+    for k in range(30):
+        g.glStencilFunc(E[funcs[rng.integers(8)]], int(rng.integers(-3, 300)), int(rng.choice([0xFF, 0x0F, 0xF0, 0x7])))
+        g.glStencilOp(E[ops[rng.integers(8)]], E[ops[rng.integers(8)]], E[ops[rng.integers(8)]])
+        g.glStencilMask(int(rng.choice([0xFF, 0xFFFFFFFF, 0x3C, 0x01])))
+        g.glDepthFunc(E[funcs[1 + rng.integers(7)]])
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 2)
+        g.glEnd()
+
+
+def _upload_texture(g, data: np.ndarray, fmt="GL_RGBA"):
+    ids = np.zeros(1, np.uint32)
+    g.glGenTextures(1, ids)
+    g.glBindTexture(E["GL_TEXTURE_2D"], int(ids[0]))
+    h, w = data.shape[:2]
+    g.glTexImage2D(E["GL_TEXTURE_2D"], 0, E[fmt], w, h, 0, E[fmt], E["GL_UNSIGNED_BYTE"],
+                   g.keep(np.ascontiguousarray(data)))
+    return int(ids[0])
+
+
+def scene_textures(g, W, H, seed):
+    """Every min/mag filter, wrap mode and texture-environment mode, both
+    perspective hints, odd texture sizes (including < 2x2 without a level 1),
+    a texture matrix with w != 1, and texel alpha with the alpha test."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_TEXTURE_2D"])
+    g.glEnable(E["GL_DEPTH_TEST"])
+    texes = [_upload_texture(g, random_rgba(16, 16, seed)), _upload_texture(g, random_rgba(7, 5, seed + 1)),
+             _upload_texture(g, random_rgba(1, 9, seed + 2)),
+             _upload_texture(g, rng.integers(0, 256, (8, 8, 3), dtype=np.uint8), "GL_RGB"),
+             _upload_texture(g, rng.integers(0, 256, (4, 12), dtype=np.uint8), "GL_LUMINANCE"),
+             _upload_texture(g, rng.integers(0, 256, (6, 6, 2), dtype=np.uint8), "GL_LUMINANCE_ALPHA")]
+    mins = ["GL_NEAREST", "GL_LINEAR", "GL_NEAREST_MIPMAP_NEAREST", "GL_LINEAR_MIPMAP_NEAREST",
+            "GL_NEAREST_MIPMAP_LINEAR", "GL_LINEAR_MIPMAP_LINEAR"]
+    wraps = ["GL_REPEAT", "GL_CLAMP", "GL_CLAMP_TO_EDGE"]
+    envs = ["GL_MODULATE", "GL_DECAL", "GL_REPLACE", "GL_BLEND", "GL_ADD"]
+    g.glTexEnvfv(E["GL_TEXTURE_ENV"], E["GL_TEXTURE_ENV_COLOR"], g.fv(0.3, 0.6, 0.9, 0.5))
+    for k in range(36):
+        g.glBindTexture(E["GL_TEXTURE_2D"], texes[k % len(texes)])
+        g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E[mins[k % 6]])
+        g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"], E[["GL_NEAREST", "GL_LINEAR"][(k // 6) % 2]])
+        g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_WRAP_S"], E[wraps[k % 3]])
+        g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_WRAP_T"], E[wraps[(k // 3) % 3]])
+        g.glTexEnvi(E["GL_TEXTURE_ENV"], E["GL_TEXTURE_ENV_MODE"], E[envs[k % 5]])
+        g.glHint(E["GL_PERSPECTIVE_CORRECTION_HINT"], E[["GL_NICEST", "GL_FASTEST", "GL_DONT_CARE"][k % 3]])
+        if k % 7 == 3:
+            g.glEnable(E["GL_ALPHA_TEST"])
+            g.glAlphaFunc(E[["GL_GREATER", "GL_LESS", "GL_GEQUAL", "GL_NOTEQUAL"][k % 4]], float(rng.random()))
+        else:
+            g.glDisable(E["GL_ALPHA_TEST"])
+        g.glMatrixMode(E["GL_TEXTURE"])
+        g.glLoadIdentity()
+        if k % 5 == 1:
+            g.glTranslatef(0.25, -0.5, 0.0)
+            g.glScalef(3.0, 0.5, 1.0)
+        if k % 9 == 4:
+            g.glMultMatrixf(g.fv(1, 0, 0, 0.2, 0, 1, 0, 0.1, 0, 0, 1, 0, 0, 0, 0, 1.5))
+        g.glMatrixMode(E["GL_MODELVIEW"])
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 2, spread=(0.6 if k % 4 == 0 else 3.0), tex=True)
+        g.glEnd()
+
+
+def scene_fog(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H, dist=0.0)
+    g.glEnable(E["GL_FOG"])
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glFogfv(E["GL_FOG_COLOR"], g.fv(0.7, 0.2, 0.4, 0.6))
+    for k, mode in enumerate(["GL_LINEAR", "GL_EXP", "GL_EXP2", "GL_LINEAR"]):
+        g.glFogi(E["GL_FOG_MODE"], E[mode])
+        g.glFogf(E["GL_FOG_DENSITY"], float(rng.uniform(0.05, 0.6)))
+        g.glFogf(E["GL_FOG_START"], 1.0)
+        g.glFogf(E["GL_FOG_END"], 1.0 if k == 3 else 6.0)
+        g.glBegin(E["GL_TRIANGLES"])
+        for _ in range(5):
+            for _v in range(3):
+                g.glColor4f(*rng.random(4).astype(np.float32))
+                p = rng.uniform(-2, 2, 3)
+                p[2] = -rng.uniform(0.5, 9.0)
+                g.glVertex3f(*p.astype(np.float32))
+        g.glEnd()
+
+
+def scene_lighting(g, W, H, seed):
+    """Directional, positional (attenuated) and spot lights, specular with a
+    local viewer, NORMALIZE, every colour-material mode, two-sided lighting,
+    flat and Phong shading."""
+    rng = _rng(seed)
+    _perspective(g, W, H, dist=0.0)
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glLightModelfv(E["GL_LIGHT_MODEL_AMBIENT"], g.fv(0.1, 0.15, 0.2, 1.0))
+    for li in range(4):
+        L = E["GL_LIGHT0"] + li
+        g.glEnable(L)
+        g.glLightfv(L, E["GL_DIFFUSE"], g.fv(*rng.random(4)))
+        g.glLightfv(L, E["GL_SPECULAR"], g.fv(*rng.random(4)))
+        g.glLightfv(L, E["GL_AMBIENT"], g.fv(*(0.3 * rng.random(4))))
+        w = 0.0 if li == 0 else 1.0
+        g.glLightfv(L, E["GL_POSITION"], g.fv(*rng.uniform(-3, 3, 3), w))
+        if li >= 1:
+            g.glLightf(L, E["GL_CONSTANT_ATTENUATION"], 0.5)
+            g.glLightf(L, E["GL_LINEAR_ATTENUATION"], 0.2)
+            g.glLightf(L, E["GL_QUADRATIC_ATTENUATION"], 0.05)
+        if li == 2:
+            g.glLightfv(L, E["GL_SPOT_DIRECTION"], g.fv(0.0, 0.0, -1.0))
+            g.glLightf(L, E["GL_SPOT_CUTOFF"], 40.0)
+            g.glLightf(L, E["GL_SPOT_EXPONENT"], 3.0)
+    g.glMaterialfv(E["GL_FRONT_AND_BACK"], E["GL_SPECULAR"], g.fv(0.8, 0.8, 0.8, 1.0))
+    g.glMaterialf(E["GL_FRONT"], E["GL_SHININESS"], 20.0)
+    g.glMaterialfv(E["GL_BACK"], E["GL_DIFFUSE"], g.fv(0.2, 0.9, 0.3, 0.7))
+    g.glTranslatef(0.0, 0.0, -4.0)
+    modes = ["GL_AMBIENT", "GL_DIFFUSE", "GL_SPECULAR", "GL_EMISSION", "GL_AMBIENT_AND_DIFFUSE"]
+    faces = ["GL_FRONT", "GL_BACK", "GL_FRONT_AND_BACK"]
+    for k in range(12):
+        g.glShadeModel(E[["GL_SMOOTH", "GL_FLAT", "GL_PHONG"][k % 3]])
+        g.glLightModeli(E["GL_LIGHT_MODEL_TWO_SIDE"], int(k % 4 >= 2))
+        g.glLightModeli(E["GL_LIGHT_MODEL_LOCAL_VIEWER"], int(k % 5 == 1))
+        if k % 2:
+            g.glEnable(E["GL_NORMALIZE"])
+        else:
+            g.glDisable(E["GL_NORMALIZE"])
+        if k % 3 != 2:
+            g.glEnable(E["GL_COLOR_MATERIAL"])
+            g.glColorMaterial(E[faces[k % 3]], E[modes[k % 5]])
+        else:
+            g.glDisable(E["GL_COLOR_MATERIAL"])
+        g.glRotatef(float(rng.uniform(0, 360)), *rng.normal(size=3).astype(np.float32))
+        if k == 7:
+            g.glScalef(1.5, 0.5, 2.0)
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 3, spread=2.5, normals=True)
+        g.glEnd()
+
+
+def scene_clipping(g, W, H, seed):
+    """Triangles crossing every frustum plane, including the near plane and
+    vertices behind the eye."""
+    rng = _rng(seed)
+    _perspective(g, W, H, near=0.5, far=8.0, dist=0.0)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glBegin(E["GL_TRIANGLES"])
+    for _ in range(40):
+        for _v in range(3):
+            g.glColor4f(*rng.random(4).astype(np.float32))
+            g.glTexCoord2f(*rng.random(2).astype(np.float32))
+            p = rng.uniform(-6, 6, 3)
+            p[2] = rng.uniform(-12, 2)
+            g.glVertex3f(*p.astype(np.float32))
+    g.glEnd()
+
+
+def scene_cull(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_CULL_FACE"])
+    for k, (face, front) in enumerate([("GL_BACK", "GL_CCW"), ("GL_FRONT", "GL_CCW"), ("GL_BACK", "GL_CW"),
+                                       ("GL_FRONT_AND_BACK", "GL_CCW"), ("GL_FRONT", "GL_CW")]):
+        g.glCullFace(E[face])
+        g.glFrontFace(E[front])
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 8)
+        g.glEnd()
+
+
+def scene_primitives(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    for mode, nv in [("GL_TRIANGLE_STRIP", 11), ("GL_TRIANGLE_FAN", 9), ("GL_QUADS", 14), ("GL_QUAD_STRIP", 13),
+                     ("GL_POLYGON", 7), ("GL_TRIANGLES", 8)]:
+        g.glBegin(E[mode])
+        for _ in range(nv):
+            g.glColor3f(*rng.random(3).astype(np.float32))
+            g.glVertex3f(*((rng.random(3) - 0.5) * 3).astype(np.float32))
+        g.glEnd()
+
+
+def scene_arrays(g, W, H, seed):
+    """Client arrays of float and unsigned-byte components, sizes 2/3/4,
+    explicit strides, colour arrays of 3 components, 1-component texcoords,
+    negative first, glDrawElements with 8/16/32-bit client indices."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glEnable(E["GL_TEXTURE_2D"])
+    _upload_texture(g, random_rgba(8, 8, seed))
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E["GL_LINEAR"])
+    n = 30
+    pos = g.keep(((rng.random((n, 4)) - 0.5) * 3).astype(np.float32))
+    colb = g.keep(rng.integers(0, 256, (n, 3), dtype=np.uint8))
+    colf = g.keep((rng.random((n, 4)) * 1.4 - 0.2).astype(np.float32))
+    tc = g.keep(rng.random((n, 3)).astype(np.float32))
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_COLOR_ARRAY"])
+    g.glEnableClientState(E["GL_TEXTURE_COORD_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 16, pos)
+    g.glColorPointer(3, E["GL_UNSIGNED_BYTE"], 0, colb)
+    g.glTexCoordPointer(1, E["GL_FLOAT"], 12, tc)
+    g.glDrawArrays(E["GL_TRIANGLES"], 0, 12)
+    g.glVertexPointer(2, E["GL_FLOAT"], 16, pos)
+    g.glColorPointer(4, E["GL_FLOAT"], 0, colf)
+    g.glTexCoordPointer(2, E["GL_FLOAT"], 12, tc)
+    g.glDrawArrays(E["GL_TRIANGLE_STRIP"], -2, 10)
+    g.glVertexPointer(4, E["GL_FLOAT"], 0, pos)
+    for itype, dt in [("GL_UNSIGNED_BYTE", np.uint8), ("GL_UNSIGNED_SHORT", np.uint16), ("GL_UNSIGNED_INT", np.uint32)]:
+        idx = g.keep(rng.integers(0, n, 18).astype(dt))
+        g.glDrawElements(E["GL_TRIANGLES"], 18, E[itype], idx)
+    g.glDisableClientState(E["GL_COLOR_ARRAY"])
+    g.glColor4f(0.9, 0.1, 0.5, 1.0)
+    g.glDrawArrays(E["GL_TRIANGLE_FAN"], 5, 7)
+
+
+def scene_vbo(g, W, H, seed, oob_index=False):
+    """Interleaved buffer objects, element buffers with a byte offset and
+    glBufferSubData updates.  With oob_index, one index points past the end of
+    the array buffer: the reference reads outside the buffer there (undefined),
+    the device and the oracle both use the get_array_element defaults, so that
+    variant is compared device against oracle only."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glEnable(E["GL_COLOR_MATERIAL"])
+    n = 40
+    verts = np.concatenate([(rng.random((n, 3)) - 0.5) * 3, rng.normal(size=(n, 3)), rng.random((n, 4))], 1)
+    verts = g.keep(verts.astype(np.float32))
+    idx = g.keep(np.concatenate([rng.integers(0, n, 30), [n + 5 if oob_index else 7, 3, 4]]).astype(np.uint16))
+    bufs = np.zeros(2, np.uint32)
+    g.glGenBuffers(2, bufs)
+    g.glBindBuffer(E["GL_ARRAY_BUFFER"], int(bufs[0]))
+    g.glBufferData(E["GL_ARRAY_BUFFER"], verts.nbytes, verts, E["GL_STATIC_DRAW"])
+    patch = g.keep((rng.random(8) * 2 - 1).astype(np.float32))
+    g.glBufferSubData(E["GL_ARRAY_BUFFER"], 40 * 5, patch.nbytes, patch)
+    g.glBindBuffer(E["GL_ELEMENT_ARRAY_BUFFER"], int(bufs[1]))
+    g.glBufferData(E["GL_ELEMENT_ARRAY_BUFFER"], idx.nbytes, idx, E["GL_STATIC_DRAW"])
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_NORMAL_ARRAY"])
+    g.glEnableClientState(E["GL_COLOR_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 40, 0)
+    g.glNormalPointer(E["GL_FLOAT"], 40, 12)
+    g.glColorPointer(4, E["GL_FLOAT"], 40, 24)
+    g.glDrawElements(E["GL_TRIANGLES"], 33, E["GL_UNSIGNED_SHORT"], 0)
+    g.glDrawElements(E["GL_TRIANGLES"], 15, E["GL_UNSIGNED_SHORT"], 6)
+    g.glDrawArrays(E["GL_QUADS"], 2, 12)
+
+
+def scene_lists(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    base = g.glGenLists(3)
+    g.glNewList(base, E["GL_COMPILE"])
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 4)
+    g.glEnd()
+    g.glEndList()
+    g.glNewList(base + 1, E["GL_COMPILE_AND_EXECUTE"])
+    g.glPushMatrix()
+    g.glRotatef(30.0, 0.0, 0.0, 1.0)
+    g.glCallList(base)
+    g.glPopMatrix()
+    g.glEndList()
+    g.glNewList(base + 2, E["GL_COMPILE"])
+    g.glTranslatef(0.3, -0.2, 0.1)
+    g.glCallList(base + 1)
+    g.glEndList()
+    ids = g.keep(np.array([2, 1, 2], np.uint8))
+    g.glListBase(base)
+    g.glCallLists(3, E["GL_UNSIGNED_BYTE"], ids)
+    g.glCallList(base)
+
+
+def scene_scissor_viewport(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_SCISSOR_TEST"])
+    g.glScissor(W // 5, H // 4, W // 2, H // 2)
+    g.glViewport(-W // 6, H // 8, W, (3 * H) // 4)
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 10)
+    g.glEnd()
+    g.glClearColor(0.5, 0.25, 0.75, 1.0)
+    g.glClear(E["GL_COLOR_BUFFER_BIT"])
+    g.glViewport(W // 3, -H // 3, W, H * 2)
+    g.glScissor(0, 0, W, H)
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 10)
+    g.glEnd()
+
+
+def scene_colormask(g, W, H, seed):
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_BLEND"])
+    g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE"])
+    for m in range(16):
+        g.glColorMask(bool(m & 1), bool(m & 2), bool(m & 4), bool(m & 8))
+        g.glBegin(E["GL_TRIANGLES"])
+        _random_tris_immediate(g, rng, 2)
+        g.glEnd()
+
+
+def scene_edge_cases(g, W, H, seed):
+    """Robustness inputs of the reference's validation demo: NaN/Inf colours,
+    Inf and +-1e10 vertices, NaN texture coordinates, degenerate triangles."""
+    nan, inf = float("nan"), float("inf")
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glEnable(E["GL_TEXTURE_2D"])
+    _upload_texture(g, random_rgba(8, 8, seed))
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E["GL_LINEAR"])
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"], E["GL_LINEAR"])
+    tris = [
+        [(nan, 0.5, 0.2, 1.0), (-0.5, -0.5, 0.0)], [(0.2, inf, 0.3, 1.0), (0.5, -0.5, 0.0)], [(0.3, 0.4, -inf, nan), (0.0, 0.5, 0.0)],
+        [(1, 0, 0, 1), (1e10, 0.0, 0.0)], [(0, 1, 0, 1), (-1e10, 1e10, 0.0)], [(0, 0, 1, 1), (0.2, 0.2, -1e10)],
+        [(1, 1, 0, 1), (inf, 0.0, 0.0)], [(1, 0, 1, 1), (0.0, -inf, 0.0)], [(0, 1, 1, 1), (0.1, 0.1, 0.0)],
+        [(1, 1, 1, 1), (0.0, 0.0, 0.0)], [(1, 1, 1, 1), (0.0, 0.0, 0.0)], [(1, 1, 1, 1), (0.0, 0.0, 0.0)],
+        [(1, 0.5, 0, 1), (-0.8, -0.8, 0.0)], [(1, 0.5, 0, 1), (0.8, 0.8, 0.0)], [(1, 0.5, 0, 1), (0.0, 0.0, 0.0)],
+    ]
+    texcoords = [(nan, 0.0), (1e10, -1e10), (inf, 0.5), (0.5, nan), (-inf, 2.0)]
+    g.glBegin(E["GL_TRIANGLES"])
+    for k, (col, p) in enumerate(tris):
+        g.glColor4f(*col)
+        g.glTexCoord2f(*texcoords[k % len(texcoords)])
+        g.glVertex3f(*p)
+    g.glEnd()
+
+
+def scene_midprimitive_state(g, W, H, seed):
+    """State changed between the vertices of one primitive: matrices,
+    materials, lights and the colour-material side effect (reference
+    emit_vertex transforms each vertex with the state current at its call)."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glEnable(E["GL_LIGHT1"])
+    g.glLightfv(E["GL_LIGHT1"], E["GL_DIFFUSE"], g.fv(0.9, 0.2, 0.1, 1.0))
+    g.glLightfv(E["GL_LIGHT1"], E["GL_POSITION"], g.fv(-1.0, 2.0, 1.0, 1.0))
+    g.glShadeModel(E["GL_PHONG"])
+    g.glLightModeli(E["GL_LIGHT_MODEL_TWO_SIDE"], 1)
+    g.glBegin(E["GL_TRIANGLES"])
+    for k in range(24):
+        if k % 4 == 1:
+            g.glMaterialfv(E["GL_FRONT"], E["GL_DIFFUSE"], g.fv(*rng.random(4)))
+        if k % 5 == 2:
+            g.glRotatef(float(rng.uniform(-40, 40)), 0.0, 1.0, 0.0)
+        if k % 7 == 3:
+            g.glEnable(E["GL_COLOR_MATERIAL"])
+        if k % 7 == 6:
+            g.glDisable(E["GL_COLOR_MATERIAL"])
+        if k == 11:
+            g.glShadeModel(E["GL_SMOOTH"])
+        g.glColor4f(*rng.random(4).astype(np.float32))
+        g.glNormal3f(*rng.normal(size=3).astype(np.float32))
+        g.glVertex3f(*((rng.random(3) - 0.5) * 3).astype(np.float32))
+    g.glEnd()
+
+
+def scene_vbo_oob(g, W, H, seed):
+    scene_vbo(g, W, H, seed, oob_index=True)
+
+
+def scene_icosphere_small(g, W, H, seed):
+    draw_icosphere(g, setup_icosphere(g, W, H, seed))
+
+
+def scene_textured_sphere_small(g, W, H, seed):
+    draw_textured_sphere(g, setup_textured_sphere(g, W, H, seed, grid=24, tex_size=64))
+
+
+def scene_overdraw_small(g, W, H, seed):
+    draw_overdraw(g, setup_overdraw(g, W, H, seed, ntri=3000, max_log_area=3.0))
+
+
+FEATURE_SCENES = {
+    "depth_funcs": scene_depth_funcs,
+    "blend": scene_blend,
+    "stencil": scene_stencil,
+    "textures": scene_textures,
+    "fog": scene_fog,
+    "lighting": scene_lighting,
+    "clipping": scene_clipping,
+    "cull": scene_cull,
+    "primitives": scene_primitives,
+    "arrays": scene_arrays,
+    "vbo": scene_vbo,
+    "lists": scene_lists,
+    "scissor_viewport": scene_scissor_viewport,
+    "colormask": scene_colormask,
+    "edge_cases": scene_edge_cases,
+    "midprimitive_state": scene_midprimitive_state,
+    "icosphere_small": scene_icosphere_small,
+    "textured_sphere_small": scene_textured_sphere_small,
+    "overdraw_small": scene_overdraw_small,
+}

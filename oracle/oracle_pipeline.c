@@ -1,0 +1,1115 @@
+/*
+ * oracle_pipeline.c - CPU restatement of the MyTinyGL per-draw pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it implements the
+ * C-ABI seam of include/mtgl_dev.h on the host CPU by restating, operation for
+ * operation, the reference renderer's per-vertex, clipping, set-up, per-fragment
+ * and texture-sampling arithmetic.  It is linked (with the host GL layer) into
+ * oracle/_build/liboracle_gl.so, which only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load.  The shipped library libmtgl.so never links
+ * or loads it.
+ *
+ * Build strict: -std=c99 -O2 -ffp-contract=off, no fast-math (the reference's own
+ * Makefile flags -O3 -march=native -ffast-math change depth values and are not a
+ * valid oracle).
+ *
+ * Pinning (what checks each part):
+ *   - whole frames, bit for bit, against oracle/_ref/libmtgl_ref.so (the
+ *     reference sources compiled strict by oracle/Makefile) and against the
+ *     committed SHA-256 fixtures under tests/golden/ made from it;
+ *   - closed forms and invariants in tests/test_oracle_units.py (stencil and
+ *     depth truth tables, blend factors, RGBA8 pack/unpack, mip1 box filter,
+ *     clipping invariants, perspective vs affine interpolation, fill rule).
+ * The libm calls (log2f, expf, powf, cosf) are the host C library's, the same
+ * calls the reference makes.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "GL/gl.h"
+#include "mtgl_dev.h"
+
+#define OR_MAX_CLIP 12   /* reference MAX_CLIP_VERTS, src/clipping.h:16 */
+
+typedef struct { float x, y, z, w; } or_v4;
+typedef struct { float r, g, b, a; } or_col;
+
+/* post-transform vertex (reference vertex_t, src/graphics.h:438-446) */
+typedef struct {
+    or_v4 pos;
+    or_col color;
+    float tc[2];
+    float nrm[3];
+    float ez;
+    float ep[3];
+    float en[3];
+} or_vtx;
+
+typedef struct {
+    int32_t w, h;
+    uint32_t *px;
+    int32_t m1w, m1h;
+    uint32_t *m1;       /* level 1, NULL when the level-0 image is smaller than 2x2 */
+} or_tex;
+
+typedef struct {
+    uint8_t *data;
+    int64_t size;
+} or_buf;
+
+struct mtgl_dev {
+    int32_t w, h;
+    uint32_t *color;
+    float *depth;
+    uint8_t *stencil;
+    or_tex tex[MTGL_MAX_OBJECTS + 1];
+    or_buf buf[MTGL_MAX_OBJECTS + 1];
+    mtgl_dev_stats stats;
+};
+
+/* ------------------------------------------------------------------------- */
+/* scalar helpers                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* float -> int32 the way x86 cvttss2si does it: truncation, and INT_MIN for NaN
+ * or out-of-range inputs.  The reference's (int32_t) casts compile to this on the
+ * x86-64 hosts it runs on (src/raster.c:61-62, src/textures.c:385, :465, :529). */
+static int32_t or_cvtt(float f)
+{
+    if (f >= -2147483648.0f && f < 2147483648.0f) return (int32_t)f;
+    return INT32_MIN;
+}
+
+static int32_t or_isub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+
+static or_col or_unpack(uint32_t p)
+{
+    or_col c = { (p & 0xFF) / 255.0f, ((p >> 8) & 0xFF) / 255.0f,
+                 ((p >> 16) & 0xFF) / 255.0f, ((p >> 24) & 0xFF) / 255.0f };
+    return c;
+}
+
+/* reference color_to_rgba32, src/graphics.h:337-348 */
+static uint32_t or_pack(or_col c)
+{
+    float v[4] = { c.r, c.g, c.b, c.a };
+    uint32_t out = 0;
+    for (int k = 0; k < 4; k++) {
+        float x = v[k] < 0.0f ? 0.0f : (v[k] > 1.0f ? 1.0f : v[k]);
+        uint32_t byte = (uint32_t)or_cvtt(x * 255.0f) & 0xFFu;
+        out |= byte << (8 * k);
+    }
+    return out;
+}
+
+static or_col or_clamp(or_col c)
+{
+    or_col r = { c.r < 0 ? 0 : (c.r > 1 ? 1 : c.r), c.g < 0 ? 0 : (c.g > 1 ? 1 : c.g),
+                 c.b < 0 ? 0 : (c.b > 1 ? 1 : c.b), c.a < 0 ? 0 : (c.a > 1 ? 1 : c.a) };
+    return r;
+}
+
+static or_col or_add(or_col a, or_col b) { or_col r = { a.r + b.r, a.g + b.g, a.b + b.b, a.a + b.a }; return r; }
+static or_col or_mul(or_col a, or_col b) { or_col r = { a.r * b.r, a.g * b.g, a.b * b.b, a.a * b.a }; return r; }
+static or_col or_scale(or_col a, float s) { or_col r = { a.r * s, a.g * s, a.b * s, a.a * s }; return r; }
+
+/* reference color_lerp: a*(1-t) + b*t, src/graphics.h:293-295 */
+static or_col or_lerp(or_col a, or_col b, float t) { return or_add(or_scale(a, 1.0f - t), or_scale(b, t)); }
+
+static or_col or_from(mtgl_color c) { or_col r = { c.r, c.g, c.b, c.a }; return r; }
+
+static float or_dot3(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* reference vec3_normalize: multiply by 1/len, src/graphics.h:54-60 */
+static void or_normalize3(float *v)
+{
+    float len = sqrtf(or_dot3(v, v));
+    if (len > 0.0f) {
+        float s = 1.0f / len;
+        v[0] = v[0] * s; v[1] = v[1] * s; v[2] = v[2] * s;
+    }
+}
+
+/* reference mat4_mul_vec4, src/graphics.h:131-138 */
+static or_v4 or_xform(const float *m, or_v4 v)
+{
+    or_v4 r;
+    r.x = m[0] * v.x + m[4] * v.y + m[8] * v.z + m[12] * v.w;
+    r.y = m[1] * v.x + m[5] * v.y + m[9] * v.z + m[13] * v.w;
+    r.z = m[2] * v.x + m[6] * v.y + m[10] * v.z + m[14] * v.w;
+    r.w = m[3] * v.x + m[7] * v.y + m[11] * v.z + m[15] * v.w;
+    return r;
+}
+
+/* reference mat4_normal_matrix, src/graphics.h:229-260 */
+static void or_normal_matrix(const float *m, float *r)
+{
+    float a = m[0], b = m[1], c = m[2];
+    float d = m[4], e = m[5], f = m[6];
+    float g = m[8], h = m[9], i = m[10];
+    float det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+    memset(r, 0, 16 * sizeof(float));
+    if (fabsf(det) < 1e-10f) { r[0] = r[5] = r[10] = r[15] = 1.0f; return; }
+    float inv = 1.0f / det;
+    r[0] = (e * i - f * h) * inv;
+    r[1] = (d * i - f * g) * -inv;
+    r[2] = (d * h - e * g) * inv;
+    r[4] = (b * i - c * h) * -inv;
+    r[5] = (a * i - c * g) * inv;
+    r[6] = (a * h - b * g) * -inv;
+    r[8] = (b * f - c * e) * inv;
+    r[9] = (a * f - c * d) * -inv;
+    r[10] = (a * e - b * d) * inv;
+    r[15] = 1.0f;
+}
+
+/* ------------------------------------------------------------------------- */
+/* lighting (reference compute_lighting, src/lighting.h:53-142)               */
+/* ------------------------------------------------------------------------- */
+
+static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
+                       const mtgl_material *mat)
+{
+    or_col result = or_add(or_from(mat->emission), or_mul(or_from(mat->ambient), or_from(L->model_ambient)));
+    result.a = mat->diffuse.a;
+    float N[3] = { eye_normal[0], eye_normal[1], eye_normal[2] };
+    if (flags & MTGL_F_NORMALIZE) or_normalize3(N);
+    for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
+        const mtgl_light *lt = &L->lights[i];
+        if (!lt->enabled) continue;
+        float Ld[3];
+        float att = 1.0f;
+        if (lt->position[3] == 0.0f) {
+            Ld[0] = lt->position[0]; Ld[1] = lt->position[1]; Ld[2] = lt->position[2];
+            or_normalize3(Ld);
+        } else {
+            float to[3] = { lt->position[0] - eye_pos[0], lt->position[1] - eye_pos[1], lt->position[2] - eye_pos[2] };
+            float dist = sqrtf(or_dot3(to, to));
+            if (dist < 1e-6f) dist = 1e-6f;
+            float s = 1.0f / dist;
+            Ld[0] = to[0] * s; Ld[1] = to[1] * s; Ld[2] = to[2] * s;
+            float den = lt->constant_attenuation + lt->linear_attenuation * dist +
+                        lt->quadratic_attenuation * dist * dist;
+            if (den < 1e-6f) den = 1e-6f;
+            att = 1.0f / den;
+            if (lt->spot_cutoff < 180.0f) {
+                float sd[3] = { lt->spot_direction[0], lt->spot_direction[1], lt->spot_direction[2] };
+                or_normalize3(sd);
+                float cos_angle = -or_dot3(Ld, sd);
+                float cos_cutoff = cosf(lt->spot_cutoff * 3.14159265f / 180.0f);
+                if (cos_angle < cos_cutoff) att = 0.0f;
+                else att *= powf(cos_angle, lt->spot_exponent);
+            }
+        }
+        if (att <= 0.0f) continue;
+        result = or_add(result, or_scale(or_mul(or_from(mat->ambient), or_from(lt->ambient)), att));
+        float ndl = or_dot3(N, Ld);
+        if (ndl > 0.0f) {
+            result = or_add(result, or_scale(or_mul(or_from(mat->diffuse), or_from(lt->diffuse)), att * ndl));
+            if (mat->shininess > 0.0f) {
+                float V[3] = { 0.0f, 0.0f, 1.0f };
+                if (L->local_viewer) {
+                    V[0] = eye_pos[0] * -1.0f; V[1] = eye_pos[1] * -1.0f; V[2] = eye_pos[2] * -1.0f;
+                    or_normalize3(V);
+                }
+                float H[3] = { Ld[0] + V[0], Ld[1] + V[1], Ld[2] + V[2] };
+                or_normalize3(H);
+                float ndh = or_dot3(N, H);
+                if (ndh > 0.0f) {
+                    float spec = powf(ndh, mat->shininess) * att;
+                    result = or_add(result, or_scale(or_mul(or_from(mat->specular), or_from(lt->specular)), spec));
+                }
+            }
+        }
+    }
+    return or_clamp(result);
+}
+
+/* ------------------------------------------------------------------------- */
+/* vertex fetch and T&L                                                        */
+/* ------------------------------------------------------------------------- */
+
+/* source bytes of a stream: the draw's staged block or a buffer object */
+static const uint8_t *or_source(mtgl_dev *dev, const mtgl_draw *d, const mtgl_array *a, int64_t *size)
+{
+    if (a->buffer == 0) { *size = (int64_t)d->staged_bytes; return (const uint8_t *)d->staged; }
+    if (a->buffer > MTGL_MAX_OBJECTS || !dev->buf[a->buffer].data) { *size = 0; return NULL; }
+    *size = dev->buf[a->buffer].size;
+    return dev->buf[a->buffer].data;
+}
+
+/* reference get_array_element, src/gl_api.c:1758-1797; an element outside its
+ * source takes the defaults (the reference reads out of bounds there). */
+static void or_fetch(mtgl_dev *dev, const mtgl_draw *d, const mtgl_array *a, int32_t index, float *out, int n)
+{
+    int64_t size;
+    const uint8_t *src = or_source(dev, d, a, &size);
+    int64_t elem = (int64_t)a->size * (a->type == GL_FLOAT ? 4 : 1);
+    int64_t at = a->offset + (int64_t)index * a->stride;
+    if (!src || index < 0 || a->stride <= 0 || at < 0 || at + elem > size) {
+        for (int i = 0; i < n; i++) out[i] = i < 3 ? 0.0f : 1.0f;
+        return;
+    }
+    const uint8_t *p = src + at;
+    for (int i = 0; i < a->size && i < n; i++) {
+        if (a->type == GL_FLOAT) { float v; memcpy(&v, p + 4 * i, 4); out[i] = v; }
+        else out[i] = p[i] / 255.0f;
+    }
+    for (int i = a->size; i < n; i++) out[i] = i == 3 ? 1.0f : 0.0f;
+}
+
+static int32_t or_index(mtgl_dev *dev, const mtgl_draw *d, uint32_t i)
+{
+    if (d->index_type == 0) return (int32_t)((uint32_t)d->first + i);
+    int64_t size;
+    const uint8_t *src = or_source(dev, d, &d->indices, &size);
+    int64_t isz = d->index_type == GL_UNSIGNED_INT ? 4 : (d->index_type == GL_UNSIGNED_SHORT ? 2 : 1);
+    int64_t at = d->indices.offset + (int64_t)i * isz;
+    if (!src || at < 0 || at + isz > size) return -1;
+    if (isz == 4) { uint32_t v; memcpy(&v, src + at, 4); return (int32_t)v; }
+    if (isz == 2) { uint16_t v; memcpy(&v, src + at, 2); return (int32_t)v; }
+    return (int32_t)src[at];
+}
+
+/* glColor4f's sanitising (reference src/gl_api.c:708-722) */
+static float or_color_component(float v, float nan_value)
+{
+    if (isnan(v) || isinf(v)) v = nan_value;
+    return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+}
+
+/* reference emit_vertex, src/gl_api.c:263-348 */
+static void or_emit_vertex(const mtgl_vertex_state *vs, const float *obj, const float *color,
+                           const float *texcoord, const float *normal, or_vtx *out)
+{
+    or_v4 v = { obj[0], obj[1], obj[2], 1.0f };
+    or_v4 eye = or_xform(vs->modelview, v);
+    out->ez = -eye.z;
+    out->ep[0] = eye.x; out->ep[1] = eye.y; out->ep[2] = eye.z;
+
+    float nm[16];
+    or_normal_matrix(vs->modelview, nm);
+    or_v4 n4 = or_xform(nm, (or_v4){ normal[0], normal[1], normal[2], 0.0f });
+    out->en[0] = n4.x; out->en[1] = n4.y; out->en[2] = n4.z;
+    or_normalize3(out->en);
+
+    or_col vc = { color[0], color[1], color[2], color[3] };
+    int lit = (vs->flags & MTGL_F_LIGHTING) != 0;
+    if (lit && vs->shade_model != GL_PHONG) {
+        mtgl_material front = vs->lighting.front;
+        GLenum face = vs->color_material_face, mode = vs->color_material_mode;
+        if ((vs->flags & MTGL_F_COLOR_MATERIAL) && (face == GL_FRONT || face == GL_FRONT_AND_BACK)) {
+            or_col k = or_clamp(vc);
+            mtgl_color kk = { k.r, k.g, k.b, k.a };
+            if (mode == GL_AMBIENT || mode == GL_AMBIENT_AND_DIFFUSE) front.ambient = kk;
+            if (mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE) front.diffuse = kk;
+            if (mode == GL_SPECULAR) front.specular = kk;
+            if (mode == GL_EMISSION) front.emission = kk;
+        }
+        vc = or_light(&vs->lighting, vs->flags, out->ep, out->en, &front);
+    }
+    out->color = vc;
+
+    out->pos = or_xform(vs->projection, or_xform(vs->modelview, v));
+
+    or_v4 t4 = or_xform(vs->texture, (or_v4){ texcoord[0], texcoord[1], 0.0f, 1.0f });
+    if (t4.w != 0.0f && t4.w != 1.0f) { out->tc[0] = t4.x / t4.w; out->tc[1] = t4.y / t4.w; }
+    else { out->tc[0] = t4.x; out->tc[1] = t4.y; }
+    out->nrm[0] = normal[0]; out->nrm[1] = normal[1]; out->nrm[2] = normal[2];
+}
+
+/* ------------------------------------------------------------------------- */
+/* clipping (reference src/clipping.h:27-126, vertex_lerp src/graphics.h:465) */
+/* ------------------------------------------------------------------------- */
+
+static float or_plane(const or_v4 *p, int plane)
+{
+    switch (plane) {
+    case 0: return p->z + p->w;   /* near */
+    case 1: return p->w - p->z;   /* far */
+    case 2: return p->x + p->w;   /* left */
+    case 3: return p->w - p->x;   /* right */
+    case 4: return p->y + p->w;   /* bottom */
+    default: return p->w - p->y;  /* top */
+    }
+}
+
+static float or_lerpf(float a, float b, float t) { return a + t * (b - a); }
+
+static or_vtx or_vtx_lerp(const or_vtx *a, const or_vtx *b, float t)
+{
+    or_vtx v;
+    v.pos.x = or_lerpf(a->pos.x, b->pos.x, t);
+    v.pos.y = or_lerpf(a->pos.y, b->pos.y, t);
+    v.pos.z = or_lerpf(a->pos.z, b->pos.z, t);
+    v.pos.w = or_lerpf(a->pos.w, b->pos.w, t);
+    v.color = or_lerp(a->color, b->color, t);
+    for (int k = 0; k < 2; k++) v.tc[k] = or_lerpf(a->tc[k], b->tc[k], t);
+    for (int k = 0; k < 3; k++) v.nrm[k] = or_lerpf(a->nrm[k], b->nrm[k], t);
+    v.ez = or_lerpf(a->ez, b->ez, t);
+    for (int k = 0; k < 3; k++) v.ep[k] = or_lerpf(a->ep[k], b->ep[k], t);
+    for (int k = 0; k < 3; k++) v.en[k] = or_lerpf(a->en[k], b->en[k], t);
+    return v;
+}
+
+static void or_snap(or_vtx *v, int plane)
+{
+    switch (plane) {
+    case 0: v->pos.z = -v->pos.w; break;
+    case 1: v->pos.z = v->pos.w; break;
+    case 2: v->pos.x = -v->pos.w; break;
+    case 3: v->pos.x = v->pos.w; break;
+    case 4: v->pos.y = -v->pos.w; break;
+    default: v->pos.y = v->pos.w; break;
+    }
+}
+
+static int or_clip_plane(const or_vtx *in, int n, or_vtx *out, int plane)
+{
+    if (n == 0) return 0;
+    int m = 0;
+    const or_vtx *prev = &in[n - 1];
+    float pd = or_plane(&prev->pos, plane);
+    for (int i = 0; i < n; i++) {
+        const or_vtx *cur = &in[i];
+        float cd = or_plane(&cur->pos, plane);
+        if (pd >= 0) {
+            if (cd >= 0) {
+                if (m < OR_MAX_CLIP) out[m++] = *cur;
+            } else {
+                float den = pd - cd;
+                if (fabsf(den) > 1e-10f && m < OR_MAX_CLIP) {
+                    out[m] = or_vtx_lerp(prev, cur, pd / den);
+                    or_snap(&out[m], plane);
+                    m++;
+                }
+            }
+        } else if (cd >= 0) {
+            float den = pd - cd;
+            if (fabsf(den) > 1e-10f && m < OR_MAX_CLIP) {
+                out[m] = or_vtx_lerp(prev, cur, pd / den);
+                or_snap(&out[m], plane);
+                m++;
+            }
+            if (m < OR_MAX_CLIP) out[m++] = *cur;
+        }
+        prev = cur;
+        pd = cd;
+    }
+    return m;
+}
+
+static int or_clip_triangle(const or_vtx *tri, or_vtx *out)
+{
+    or_vtx a[OR_MAX_CLIP], b[OR_MAX_CLIP];
+    int n = or_clip_plane(tri, 3, a, 0);
+    if (!n) return 0;
+    n = or_clip_plane(a, n, b, 1);
+    if (!n) return 0;
+    n = or_clip_plane(b, n, a, 2);
+    if (!n) return 0;
+    n = or_clip_plane(a, n, b, 3);
+    if (!n) return 0;
+    n = or_clip_plane(b, n, a, 4);
+    if (!n) return 0;
+    return or_clip_plane(a, n, out, 5);
+}
+
+/* ------------------------------------------------------------------------- */
+/* textures (reference src/textures.c:272-557)                                */
+/* ------------------------------------------------------------------------- */
+
+static uint32_t or_texel(const uint32_t *px, int32_t w, int32_t h, uint32_t wrap_s, uint32_t wrap_t,
+                         int32_t x, int32_t y)
+{
+    if (wrap_s == GL_REPEAT) x = ((x % w) + w) % w;
+    else { if (x < 0) x = 0; else if (x >= w) x = w - 1; }
+    if (wrap_t == GL_REPEAT) y = ((y % h) + h) % h;
+    else { if (y < 0) y = 0; else if (y >= h) y = h - 1; }
+    return px[(size_t)y * (size_t)w + (size_t)x];
+}
+
+static uint32_t or_bilinear(uint32_t c00, uint32_t c10, uint32_t c01, uint32_t c11, float fx, float fy)
+{
+    or_col top = or_lerp(or_unpack(c00), or_unpack(c10), fx);
+    or_col bot = or_lerp(or_unpack(c01), or_unpack(c11), fx);
+    return or_pack(or_lerp(top, bot, fy));
+}
+
+/* one level, GL_LINEAR (bilinear, wrapped taps) or nearest (clamped index) */
+static uint32_t or_sample_level(const uint32_t *px, int32_t w, int32_t h, uint32_t wrap_s, uint32_t wrap_t,
+                                float u, float v, int linear, int *taps)
+{
+    float tx = u * w - 0.5f;
+    float ty = v * h - 0.5f;
+    if (linear) {
+        int32_t x0 = or_cvtt(floorf(tx));
+        int32_t y0 = or_cvtt(floorf(ty));
+        int32_t x1 = x0 + 1, y1 = y0 + 1;
+        float fx = tx - x0;
+        float fy = ty - y0;
+        *taps += 4;
+        return or_bilinear(or_texel(px, w, h, wrap_s, wrap_t, x0, y0), or_texel(px, w, h, wrap_s, wrap_t, x1, y0),
+                           or_texel(px, w, h, wrap_s, wrap_t, x0, y1), or_texel(px, w, h, wrap_s, wrap_t, x1, y1),
+                           fx, fy);
+    }
+    int32_t x = or_cvtt(floorf(tx + 0.5f));
+    int32_t y = or_cvtt(floorf(ty + 0.5f));
+    if (x < 0) x = 0;
+    if (x >= w) x = w - 1;
+    if (y < 0) y = 0;
+    if (y >= h) y = h - 1;
+    *taps += 1;
+    return px[(size_t)y * (size_t)w + (size_t)x];
+}
+
+/* reference texture_sample_mip1, src/textures.c:413-451: white without a level 1 */
+static uint32_t or_sample_mip1(const or_tex *t, const mtgl_raster_state *rs, float u, float v, uint32_t filter, int *taps)
+{
+    if (!t->m1) return 0xFFFFFFFFu;
+    int linear = filter == GL_LINEAR || filter == GL_LINEAR_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_LINEAR;
+    return or_sample_level(t->m1, t->m1w, t->m1h, rs->wrap_s, rs->wrap_t, u, v, linear, taps);
+}
+
+/* reference texture_sample_lod, src/textures.c:457-557 */
+static uint32_t or_sample(const or_tex *t, const mtgl_raster_state *rs, float u, float v, float lod, int *taps)
+{
+    if (rs->wrap_s == GL_REPEAT) { u = u - (float)or_cvtt(u); if (u < 0) u += 1.0f; }
+    else { if (u < 0.0f) u = 0.0f; if (u > 1.0f) u = 1.0f; }
+    if (rs->wrap_t == GL_REPEAT) { v = v - (float)or_cvtt(v); if (v < 0) v += 1.0f; }
+    else { if (v < 0.0f) v = 0.0f; if (v > 1.0f) v = 1.0f; }
+
+    uint32_t filter = lod > 0.0f ? rs->min_filter : rs->mag_filter;
+    if (filter == GL_NEAREST_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_NEAREST) {
+        if (lod >= 0.5f) return or_sample_mip1(t, rs, u, v, filter, taps);
+        filter = filter == GL_NEAREST_MIPMAP_NEAREST ? GL_NEAREST : GL_LINEAR;
+    } else if (filter == GL_NEAREST_MIPMAP_LINEAR || filter == GL_LINEAR_MIPMAP_LINEAR) {
+        if (lod > 0.0f) {
+            float cl = lod > 1.0f ? 1.0f : lod;
+            int base_linear = filter == GL_LINEAR_MIPMAP_LINEAR;
+            uint32_t c0 = or_sample_level(t->px, t->w, t->h, rs->wrap_s, rs->wrap_t, u, v, base_linear, taps);
+            uint32_t c1 = or_sample_mip1(t, rs, u, v, filter, taps);
+            return or_pack(or_lerp(or_unpack(c0), or_unpack(c1), cl));
+        }
+        filter = filter == GL_NEAREST_MIPMAP_LINEAR ? GL_NEAREST : GL_LINEAR;
+    }
+    return or_sample_level(t->px, t->w, t->h, rs->wrap_s, rs->wrap_t, u, v, filter == GL_LINEAR, taps);
+}
+
+/* ------------------------------------------------------------------------- */
+/* per-fragment tests and operations (reference src/raster.c:20-45, :343-448) */
+/* ------------------------------------------------------------------------- */
+
+static int or_compare(uint32_t func, float a, float b)
+{
+    switch (func) {
+    case GL_NEVER: return 0;
+    case GL_LESS: return a < b;
+    case GL_EQUAL: return a == b;
+    case GL_LEQUAL: return a <= b;
+    case GL_GREATER: return a > b;
+    case GL_NOTEQUAL: return a != b;
+    case GL_GEQUAL: return a >= b;
+    default: return 1;
+    }
+}
+
+static int or_stencil_test(uint32_t func, int32_t ref, uint32_t mask, uint8_t s)
+{
+    int32_t r = (int32_t)((uint32_t)ref & mask);
+    int32_t v = (int32_t)((uint32_t)s & mask);
+    switch (func) {
+    case GL_NEVER: return 0;
+    case GL_LESS: return r < v;
+    case GL_EQUAL: return r == v;
+    case GL_LEQUAL: return r <= v;
+    case GL_GREATER: return r > v;
+    case GL_NOTEQUAL: return r != v;
+    case GL_GEQUAL: return r >= v;
+    default: return 1;
+    }
+}
+
+static uint8_t or_stencil_op(uint32_t op, uint8_t s, int32_t ref)
+{
+    switch (op) {
+    case GL_KEEP: return s;
+    case GL_ZERO: return 0;
+    case GL_REPLACE: return (uint8_t)(ref & 0xFF);
+    case GL_INCR: return s < 255 ? (uint8_t)(s + 1) : 255;
+    case GL_INCR_WRAP: return (uint8_t)(s + 1);
+    case GL_DECR: return s > 0 ? (uint8_t)(s - 1) : 0;
+    case GL_DECR_WRAP: return (uint8_t)(s - 1);
+    case GL_INVERT: return (uint8_t)~s;
+    default: return s;
+    }
+}
+
+static or_col or_blend_factor(uint32_t f, or_col s, or_col d)
+{
+    or_col r;
+    switch (f) {
+    case GL_ZERO: r = (or_col){0, 0, 0, 0}; break;
+    case GL_ONE: r = (or_col){1, 1, 1, 1}; break;
+    case GL_SRC_COLOR: r = s; break;
+    case GL_ONE_MINUS_SRC_COLOR: r = (or_col){1 - s.r, 1 - s.g, 1 - s.b, 1 - s.a}; break;
+    case GL_DST_COLOR: r = d; break;
+    case GL_ONE_MINUS_DST_COLOR: r = (or_col){1 - d.r, 1 - d.g, 1 - d.b, 1 - d.a}; break;
+    case GL_SRC_ALPHA: r = (or_col){s.a, s.a, s.a, s.a}; break;
+    case GL_ONE_MINUS_SRC_ALPHA: r = (or_col){1 - s.a, 1 - s.a, 1 - s.a, 1 - s.a}; break;
+    case GL_DST_ALPHA: r = (or_col){d.a, d.a, d.a, d.a}; break;
+    case GL_ONE_MINUS_DST_ALPHA: r = (or_col){1 - d.a, 1 - d.a, 1 - d.a, 1 - d.a}; break;
+    case GL_SRC_ALPHA_SATURATE: {
+        float k = s.a < (1 - d.a) ? s.a : (1 - d.a);
+        r = (or_col){k, k, k, 1};
+        break;
+    }
+    default: r = (or_col){1, 1, 1, 1}; break;
+    }
+    return r;
+}
+
+static int or_in_fb(const mtgl_dev *dev, int32_t x, int32_t y)
+{
+    return x >= 0 && x < dev->w && y >= 0 && y < dev->h;
+}
+
+static uint8_t or_get_stencil(mtgl_dev *dev, int32_t x, int32_t y)
+{
+    return or_in_fb(dev, x, y) ? dev->stencil[(size_t)y * dev->w + x] : 0;
+}
+
+static void or_put_stencil_masked(mtgl_dev *dev, int32_t x, int32_t y, uint8_t nv, uint32_t wm)
+{
+    uint8_t old = or_get_stencil(dev, x, y);
+    uint8_t m8 = (uint8_t)(wm & 0xFF);
+    if (or_in_fb(dev, x, y)) dev->stencil[(size_t)y * dev->w + x] = (uint8_t)((old & ~m8) | (nv & m8));
+}
+
+/* reference write_pixel_masked, src/raster.c:20-45 */
+static void or_write_color(mtgl_dev *dev, uint32_t mask, int32_t x, int32_t y, or_col c)
+{
+    if (!or_in_fb(dev, x, y)) return;
+    uint32_t *p = &dev->color[(size_t)y * dev->w + x];
+    if (mask == 0xF) { *p = or_pack(c); return; }
+    if (mask == 0) return;
+    or_col d = or_unpack(*p);
+    if (mask & 1) d.r = c.r;
+    if (mask & 2) d.g = c.g;
+    if (mask & 4) d.b = c.b;
+    if (mask & 8) d.a = c.a;
+    *p = or_pack(d);
+}
+
+/* ------------------------------------------------------------------------- */
+/* triangle rasterisation (reference rasterize_triangle_smooth,               */
+/* src/raster.c:451-725)                                                       */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    int32_t x, y;
+    float z, winv;
+    or_col c;
+    float uv[2];
+    float ez;
+    float ep[3], en[3];
+} or_rv;
+
+static float or_edge(float ax, float ay, float bx, float by, float px, float py)
+{
+    return (px - ax) * (by - ay) - (py - ay) * (bx - ax);
+}
+
+static void or_raster(mtgl_dev *dev, const mtgl_raster_state *rs, const or_rv *v0, const or_rv *v1,
+                      const or_rv *v2, int back)
+{
+    int32_t minX = v0->x < v1->x ? (v0->x < v2->x ? v0->x : v2->x) : (v1->x < v2->x ? v1->x : v2->x);
+    int32_t minY = v0->y < v1->y ? (v0->y < v2->y ? v0->y : v2->y) : (v1->y < v2->y ? v1->y : v2->y);
+    int32_t maxX = v0->x > v1->x ? (v0->x > v2->x ? v0->x : v2->x) : (v1->x > v2->x ? v1->x : v2->x);
+    int32_t maxY = v0->y > v1->y ? (v0->y > v2->y ? v0->y : v2->y) : (v1->y > v2->y ? v1->y : v2->y);
+    const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
+    if (minX < vx) minX = vx;
+    if (minY < vy) minY = vy;
+    if (maxX >= vx + vw) maxX = vx + vw - 1;
+    if (maxY >= vy + vh) maxY = vy + vh - 1;
+    if (rs->flags & MTGL_F_SCISSOR_TEST) {
+        const int32_t sx = rs->scissor[0], sy = rs->scissor[1], sw = rs->scissor[2], sh = rs->scissor[3];
+        if (minX < sx) minX = sx;
+        if (minY < sy) minY = sy;
+        if (maxX >= sx + sw) maxX = sx + sw - 1;
+        if (maxY >= sy + sh) maxY = sy + sh - 1;
+    }
+    if (minX > maxX || minY > maxY) return;
+
+    const float fx0 = (float)v0->x, fy0 = (float)v0->y, fx1 = (float)v1->x, fy1 = (float)v1->y;
+    const float fx2 = (float)v2->x, fy2 = (float)v2->y;
+    float area = or_edge(fx0, fy0, fx1, fy1, fx2, fy2);
+    if (fabsf(area) < 0.5f) return;
+    float inv_area = 1.0f / area;
+    dev->stats.triangles_raster++;
+
+    const int depth_on = (rs->flags & MTGL_F_DEPTH_TEST) != 0;
+    const int stencil_on = (rs->flags & MTGL_F_STENCIL_TEST) != 0;
+    const int persp = rs->perspective_hint != GL_FASTEST;
+    const or_tex *tex = (rs->texture && rs->texture <= MTGL_MAX_OBJECTS && dev->tex[rs->texture].px)
+                            ? &dev->tex[rs->texture] : NULL;
+
+    float u0w = v0->uv[0] * v0->winv, v0w = v0->uv[1] * v0->winv;
+    float u1w = v1->uv[0] * v1->winv, v1w = v1->uv[1] * v1->winv;
+    float u2w = v2->uv[0] * v2->winv, v2w = v2->uv[1] * v2->winv;
+
+    float lod = 0.0f;
+    if (tex) {
+        float screen_area = fabsf(area) * 0.5f;
+        float du1 = (v1->uv[0] - v0->uv[0]) * tex->w;
+        float dv1 = (v1->uv[1] - v0->uv[1]) * tex->h;
+        float du2 = (v2->uv[0] - v0->uv[0]) * tex->w;
+        float dv2 = (v2->uv[1] - v0->uv[1]) * tex->h;
+        float texel_area = fabsf(du1 * dv2 - du2 * dv1) * 0.5f;
+        if (screen_area > 0.0f) {
+            float tpp = texel_area / screen_area;
+            if (tpp > 0.0f) {
+                lod = log2f(tpp) * 0.5f;
+                if (lod < 0.0f) lod = 0.0f;
+            }
+        }
+    }
+
+    const uint32_t cmask = rs->color_mask & 0xF;
+    const int blend_on = (rs->flags & MTGL_F_BLEND) != 0;
+    for (int32_t y = minY; y <= maxY; y++) {
+        for (int32_t x = minX; x <= maxX; x++) {
+            const float px = (float)x, py = (float)y;
+            float e0 = or_edge(fx1, fy1, fx2, fy2, px, py);
+            float e1 = or_edge(fx2, fy2, fx0, fy0, px, py);
+            float e2 = or_edge(fx0, fy0, fx1, fy1, px, py);
+            if (!((area > 0 && e0 >= 0 && e1 >= 0 && e2 >= 0) || (area < 0 && e0 <= 0 && e1 <= 0 && e2 <= 0)))
+                continue;
+            dev->stats.fragments++;
+            float b0 = e0 * inv_area, b1 = e1 * inv_area, b2 = e2 * inv_area;
+            float z = b0 * v0->z + b1 * v1->z + b2 * v2->z;
+            float depth = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
+
+            uint8_t sv = 0;
+            if (stencil_on) {
+                dev->stats.frag_bytes += 2;
+                sv = or_get_stencil(dev, x, y);
+                if (!or_stencil_test(rs->stencil_func, rs->stencil_ref, rs->stencil_mask, sv)) {
+                    or_put_stencil_masked(dev, x, y, or_stencil_op(rs->stencil_fail, sv, rs->stencil_ref),
+                                          rs->stencil_writemask);
+                    continue;
+                }
+            }
+            if (depth_on) {
+                dev->stats.frag_bytes += 4;
+                float stored = or_in_fb(dev, x, y) ? dev->depth[(size_t)y * dev->w + x] : 1.0f;
+                if (!or_compare(rs->depth_func, depth, stored)) {
+                    if (stencil_on)
+                        or_put_stencil_masked(dev, x, y, or_stencil_op(rs->stencil_zfail, sv, rs->stencil_ref),
+                                              rs->stencil_writemask);
+                    continue;
+                }
+            }
+            if (stencil_on)
+                or_put_stencil_masked(dev, x, y, or_stencil_op(rs->stencil_zpass, sv, rs->stencil_ref),
+                                      rs->stencil_writemask);
+
+            or_col c;
+            if (rs->shade_model == GL_FLAT) {
+                c = v2->c;
+            } else {
+                c.r = v0->c.r * b0 + v1->c.r * b1 + v2->c.r * b2;
+                c.g = v0->c.g * b0 + v1->c.g * b1 + v2->c.g * b2;
+                c.b = v0->c.b * b0 + v1->c.b * b1 + v2->c.b * b2;
+                c.a = v0->c.a * b0 + v1->c.a * b1 + v2->c.a * b2;
+            }
+            if (rs->flags & MTGL_F_LIGHTING) {
+                int phong = rs->shade_model == GL_PHONG;
+                int two = back && rs->lighting.two_side;
+                if (phong || two) {
+                    float ep[3], en[3];
+                    for (int k = 0; k < 3; k++) {
+                        ep[k] = v0->ep[k] * b0 + v1->ep[k] * b1 + v2->ep[k] * b2;
+                        en[k] = v0->en[k] * b0 + v1->en[k] * b1 + v2->en[k] * b2;
+                    }
+                    const mtgl_material *mat = &rs->lighting.front;
+                    if (two) {
+                        for (int k = 0; k < 3; k++) en[k] = en[k] * -1.0f;
+                        mat = &rs->lighting.back;
+                    }
+                    c = or_light(&rs->lighting, rs->flags, ep, en, mat);
+                }
+            }
+            if (tex) {
+                float u, v;
+                if (persp) {
+                    float uow = b0 * u0w + b1 * u1w + b2 * u2w;
+                    float vow = b0 * v0w + b1 * v1w + b2 * v2w;
+                    float oow = b0 * v0->winv + b1 * v1->winv + b2 * v2->winv;
+                    float w = 1.0f / oow;
+                    u = uow * w;
+                    v = vow * w;
+                } else {
+                    u = b0 * v0->uv[0] + b1 * v1->uv[0] + b2 * v2->uv[0];
+                    v = b0 * v0->uv[1] + b1 * v1->uv[1] + b2 * v2->uv[1];
+                }
+                int taps = 0;
+                or_col tc = or_unpack(or_sample(tex, rs, u, v, lod, &taps));
+                dev->stats.frag_bytes += 4u * (uint64_t)taps;
+                if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, tc.a, rs->alpha_ref)) continue;
+                switch (rs->tex_env_mode) {
+                case GL_REPLACE: c = tc; break;
+                case GL_DECAL:
+                    c = (or_col){ c.r + (tc.r - c.r) * tc.a, c.g + (tc.g - c.g) * tc.a, c.b + (tc.b - c.b) * tc.a, c.a };
+                    break;
+                case GL_BLEND: {
+                    or_col e = or_from(rs->tex_env_color);
+                    c = (or_col){ c.r * (1.0f - tc.r) + e.r * tc.r, c.g * (1.0f - tc.g) + e.g * tc.g,
+                                  c.b * (1.0f - tc.b) + e.b * tc.b, c.a * tc.a };
+                    break;
+                }
+                case GL_ADD: c = (or_col){ c.r + tc.r, c.g + tc.g, c.b + tc.b, c.a * tc.a }; break;
+                default: c = or_mul(c, tc); break;
+                }
+            }
+            if (rs->flags & MTGL_F_FOG) {
+                float fc = b0 * v0->ez + b1 * v1->ez + b2 * v2->ez;
+                float f;
+                switch (rs->fog_mode) {
+                case GL_LINEAR:
+                    f = rs->fog_end != rs->fog_start ? (rs->fog_end - fc) / (rs->fog_end - rs->fog_start) : 1.0f;
+                    break;
+                case GL_EXP: f = expf(-rs->fog_density * fc); break;
+                case GL_EXP2: { float d = rs->fog_density * fc; f = expf(-d * d); break; }
+                default: f = 1.0f; break;
+                }
+                if (f < 0.0f) f = 0.0f;
+                if (f > 1.0f) f = 1.0f;
+                /* reference color_lerp_rgb(fog_color, c, f): rgb towards the fragment,
+                 * alpha taken from the fog colour (src/graphics.h:298-305) */
+                or_col fg = or_from(rs->fog_color);
+                c = (or_col){ fg.r + (c.r - fg.r) * f, fg.g + (c.g - fg.g) * f, fg.b + (c.b - fg.b) * f, fg.a };
+            }
+            if (depth_on && rs->depth_mask) {
+                dev->stats.frag_bytes += 4;
+                if (or_in_fb(dev, x, y)) dev->depth[(size_t)y * dev->w + x] = depth;
+            }
+            if (blend_on) {
+                or_col dst = or_unpack(or_in_fb(dev, x, y) ? dev->color[(size_t)y * dev->w + x] : 0u);
+                or_col sf = or_blend_factor(rs->blend_src, c, dst);
+                or_col df = or_blend_factor(rs->blend_dst, c, dst);
+                c = or_clamp(or_add(or_mul(c, sf), or_mul(dst, df)));
+            }
+            c = or_clamp(c);
+            if (cmask) dev->stats.frag_bytes += (blend_on || cmask != 0xF) ? 8 : 4;
+            or_write_color(dev, cmask, x, y, c);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* per-triangle: clip, divide, fan, cull (reference src/raster.c:729-774,     */
+/* :901-958)                                                                   */
+/* ------------------------------------------------------------------------- */
+
+static void or_render_triangle(mtgl_dev *dev, const mtgl_raster_state *rs, const or_vtx *a, const or_vtx *b,
+                               const or_vtx *c, int *unsupported)
+{
+    or_vtx tri[3] = { *a, *b, *c };
+    or_vtx cl[OR_MAX_CLIP];
+    dev->stats.triangles_in++;
+    int n = or_clip_triangle(tri, cl);
+    if (n < 3) return;
+    for (int j = 0; j < n; j++) {
+        or_v4 *p = &cl[j].pos;
+        if (fabsf(p->w) < 1e-6f) { p->x = 0; p->y = 0; p->z = 0; p->w = 1; continue; }
+        float iw = 1.0f / p->w;
+        p->x *= iw; p->y *= iw; p->z *= iw; p->w = iw;
+    }
+    const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
+    for (int j = 1; j + 1 < n; j++) {
+        const or_vtx *src[3] = { &cl[0], &cl[j], &cl[j + 1] };
+        or_rv rv[3];
+        for (int k = 0; k < 3; k++) {
+            const or_vtx *s = src[k];
+            rv[k].x = or_cvtt((s->pos.x + 1.0f) * 0.5f * (float)vw + (float)vx);
+            rv[k].y = or_cvtt((1.0f - s->pos.y) * 0.5f * (float)vh + (float)vy);
+            rv[k].z = s->pos.z;
+            rv[k].winv = s->pos.w;
+            rv[k].c = s->color;
+            rv[k].uv[0] = s->tc[0]; rv[k].uv[1] = s->tc[1];
+            rv[k].ez = s->ez;
+            memcpy(rv[k].ep, s->ep, sizeof(rv[k].ep));
+            memcpy(rv[k].en, s->en, sizeof(rv[k].en));
+        }
+        float sa = (float)or_isub(rv[1].x, rv[0].x) * (float)or_isub(rv[2].y, rv[0].y) -
+                   (float)or_isub(rv[2].x, rv[0].x) * (float)or_isub(rv[1].y, rv[0].y);
+        if (rs->flags & MTGL_F_CULL_FACE) {
+            int front = rs->front_face == GL_CCW ? (sa < 0) : (sa > 0);
+            int cull = rs->cull_face_mode == GL_FRONT ? front : (rs->cull_face_mode == GL_BACK ? !front : 1);
+            if (cull) continue;
+        }
+        int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
+        uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
+        if (pm == GL_POINT || pm == GL_LINE) { *unsupported = 1; continue; }
+        or_raster(dev, rs, &rv[0], &rv[1], &rv[2], back);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* C-ABI                                                                       */
+/* ------------------------------------------------------------------------- */
+
+int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
+{
+    if (!out || width <= 0 || height <= 0 || width > MTGL_MAX_FB_DIM || height > MTGL_MAX_FB_DIM)
+        return MTGL_ERR_INVALID;
+    mtgl_dev *d = (mtgl_dev *)calloc(1, sizeof(mtgl_dev));
+    if (!d) return MTGL_ERR_OUT_OF_MEMORY;
+    size_t n = (size_t)width * (size_t)height;
+    d->w = width;
+    d->h = height;
+    d->color = (uint32_t *)calloc(n, sizeof(uint32_t));
+    d->depth = (float *)calloc(n, sizeof(float));
+    d->stencil = (uint8_t *)calloc(n, 1);
+    if (!d->color || !d->depth || !d->stencil) { mtgl_dev_destroy(d); return MTGL_ERR_OUT_OF_MEMORY; }
+    *out = d;
+    return MTGL_OK;
+}
+
+void mtgl_dev_destroy(mtgl_dev *d)
+{
+    if (!d) return;
+    for (int i = 0; i <= MTGL_MAX_OBJECTS; i++) {
+        free(d->tex[i].px);
+        free(d->tex[i].m1);
+        free(d->buf[i].data);
+    }
+    free(d->color);
+    free(d->depth);
+    free(d->stencil);
+    free(d);
+}
+
+int mtgl_dev_finish(mtgl_dev *d) { return d ? MTGL_OK : MTGL_ERR_INVALID; }
+
+int mtgl_dev_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                   uint32_t color, float depth, uint8_t stencil)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > d->w) x1 = d->w;
+    if (y1 > d->h) y1 = d->h;
+    for (int32_t y = y0; y < y1; y++) {
+        for (int32_t x = x0; x < x1; x++) {
+            size_t i = (size_t)y * d->w + x;
+            if (buffers & MTGL_BUF_COLOR) d->color[i] = color;
+            if (buffers & MTGL_BUF_DEPTH) d->depth[i] = depth;
+            if (buffers & MTGL_BUF_STENCIL) d->stencil[i] = stencil;
+        }
+    }
+    return MTGL_OK;
+}
+
+static int or_plane_ptr(mtgl_dev *d, uint32_t which, uint8_t **base, size_t *esz)
+{
+    if (which == MTGL_BUF_COLOR) { *base = (uint8_t *)d->color; *esz = 4; return 1; }
+    if (which == MTGL_BUF_DEPTH) { *base = (uint8_t *)d->depth; *esz = 4; return 1; }
+    if (which == MTGL_BUF_STENCIL) { *base = d->stencil; *esz = 1; return 1; }
+    return 0;
+}
+
+int mtgl_dev_read(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, int32_t w, int32_t h, void *dst)
+{
+    uint8_t *base; size_t esz;
+    if (!d || !dst || !or_plane_ptr(d, which, &base, &esz)) return MTGL_ERR_INVALID;
+    if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
+    for (int32_t r = 0; r < h; r++)
+        memcpy((uint8_t *)dst + (size_t)r * w * esz, base + ((size_t)(y + r) * d->w + x) * esz, (size_t)w * esz);
+    return MTGL_OK;
+}
+
+int mtgl_dev_write(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, int32_t w, int32_t h, const void *src)
+{
+    uint8_t *base; size_t esz;
+    if (!d || !src || !or_plane_ptr(d, which, &base, &esz)) return MTGL_ERR_INVALID;
+    if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
+    for (int32_t r = 0; r < h; r++)
+        memcpy(base + ((size_t)(y + r) * d->w + x) * esz, (const uint8_t *)src + (size_t)r * w * esz, (size_t)w * esz);
+    return MTGL_OK;
+}
+
+/* level 0 plus the 2x2 box-filtered level 1: (((c00 + c10) + c01) + c11) * 0.25,
+ * packed with truncation (reference texture_generate_mip1, src/textures.c:311-354) */
+int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, const uint32_t *texels)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS || w <= 0 || h <= 0 || w > MTGL_MAX_TEX_SIZE ||
+        h > MTGL_MAX_TEX_SIZE || !texels)
+        return MTGL_ERR_INVALID;
+    or_tex *t = &d->tex[id];
+    size_t n = (size_t)w * (size_t)h;
+    uint32_t *px = (uint32_t *)malloc(n * sizeof(uint32_t));
+    if (!px) return MTGL_ERR_OUT_OF_MEMORY;
+    memcpy(px, texels, n * sizeof(uint32_t));
+    uint32_t *m1 = NULL;
+    int32_t mw = 0, mh = 0;
+    if (w >= 2 && h >= 2) {
+        mw = w / 2; mh = h / 2;
+        m1 = (uint32_t *)malloc((size_t)mw * (size_t)mh * sizeof(uint32_t));
+        if (!m1) { free(px); return MTGL_ERR_OUT_OF_MEMORY; }
+        for (int32_t y = 0; y < mh; y++) {
+            for (int32_t x = 0; x < mw; x++) {
+                size_t s = (size_t)(2 * y) * w + (size_t)(2 * x);
+                or_col k = or_unpack(px[s]);
+                k = or_add(k, or_unpack(px[s + 1]));
+                k = or_add(k, or_unpack(px[s + w]));
+                k = or_add(k, or_unpack(px[s + w + 1]));
+                m1[(size_t)y * mw + x] = or_pack(or_scale(k, 0.25f));
+            }
+        }
+    }
+    free(t->px);
+    free(t->m1);
+    t->px = px; t->w = w; t->h = h;
+    t->m1 = m1; t->m1w = mw; t->m1h = mh;
+    return MTGL_OK;
+}
+
+int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    free(d->tex[id].px);
+    free(d->tex[id].m1);
+    memset(&d->tex[id], 0, sizeof(d->tex[id]));
+    return MTGL_OK;
+}
+
+int mtgl_dev_buffer_data(mtgl_dev *d, uint32_t id, int64_t size, const void *data)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    if (size <= 0) { free(d->buf[id].data); d->buf[id].data = NULL; d->buf[id].size = 0; return MTGL_OK; }
+    uint8_t *p = (uint8_t *)malloc((size_t)size);
+    if (!p) return MTGL_ERR_OUT_OF_MEMORY;
+    if (data) memcpy(p, data, (size_t)size);
+    else memset(p, 0, (size_t)size);
+    free(d->buf[id].data);
+    d->buf[id].data = p;
+    d->buf[id].size = size;
+    return MTGL_OK;
+}
+
+int mtgl_dev_buffer_sub_data(mtgl_dev *d, uint32_t id, int64_t offset, int64_t size, const void *data)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS || !d->buf[id].data || !data || offset < 0 || size < 0 ||
+        offset + size > d->buf[id].size)
+        return MTGL_ERR_INVALID;
+    memcpy(d->buf[id].data + offset, data, (size_t)size);
+    return MTGL_OK;
+}
+
+int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    free(d->buf[id].data);
+    d->buf[id].data = NULL;
+    d->buf[id].size = 0;
+    return MTGL_OK;
+}
+
+/* one draw: per-vertex T&L of every emitted vertex, then primitive assembly in
+ * the reference's flush order (src/raster.c:961-1017, :1199-1231) */
+int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
+{
+    if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
+    const uint32_t n = dr->count;
+    switch (dr->mode) {
+    case GL_TRIANGLES: case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_QUADS:
+    case GL_QUAD_STRIP: case GL_POLYGON:
+        break;
+    default:
+        return MTGL_ERR_UNSUPPORTED;
+    }
+    d->stats.draws++;
+    if (n == 0) return MTGL_OK;
+    or_vtx *vt = (or_vtx *)malloc((size_t)n * sizeof(or_vtx));
+    if (!vt) return MTGL_ERR_OUT_OF_MEMORY;
+    for (uint32_t i = 0; i < n; i++) {
+        int32_t idx = or_index(d, dr, i);
+        float pos[4], col[4], tc[2], nrm[3];
+        or_fetch(d, dr, &dr->position, idx, pos, 4);
+        if (dr->color.enabled) {
+            or_fetch(d, dr, &dr->color, idx, col, 4);
+            for (int k = 0; k < 4; k++) col[k] = or_color_component(col[k], k == 3 ? 1.0f : 0.0f);
+        } else {
+            memcpy(col, dr->cur_color, sizeof(col));
+        }
+        if (dr->texcoord.enabled) or_fetch(d, dr, &dr->texcoord, idx, tc, 2);
+        else memcpy(tc, dr->cur_texcoord, sizeof(tc));
+        if (dr->normal.enabled) or_fetch(d, dr, &dr->normal, idx, nrm, 3);
+        else memcpy(nrm, dr->cur_normal, sizeof(nrm));
+        if (dr->position.size == 2) pos[2] = 0.0f;
+        uint32_t si = 0;
+        if (dr->n_vstates > 1 && dr->state_index.enabled) {
+            int64_t size;
+            const uint8_t *src = or_source(d, dr, &dr->state_index, &size);
+            int64_t at = dr->state_index.offset + (int64_t)i * dr->state_index.stride;
+            if (src && at >= 0 && at + 4 <= size) memcpy(&si, src + at, 4);
+            if (si >= dr->n_vstates) si = dr->n_vstates - 1;
+        }
+        or_emit_vertex(&dr->vstates[si], pos, col, tc, nrm, &vt[i]);
+    }
+    const mtgl_raster_state *rs = dr->raster;
+    int unsupported = 0;
+    switch (dr->mode) {
+    case GL_TRIANGLES:
+        for (uint32_t i = 0; i + 2 < n; i += 3) or_render_triangle(d, rs, &vt[i], &vt[i + 1], &vt[i + 2], &unsupported);
+        break;
+    case GL_TRIANGLE_STRIP:
+        for (uint32_t i = 0; i + 2 < n; i++) {
+            if (i % 2 == 0) or_render_triangle(d, rs, &vt[i], &vt[i + 1], &vt[i + 2], &unsupported);
+            else or_render_triangle(d, rs, &vt[i + 1], &vt[i], &vt[i + 2], &unsupported);
+        }
+        break;
+    case GL_TRIANGLE_FAN:
+    case GL_POLYGON:
+        for (uint32_t i = 1; i + 1 < n; i++) or_render_triangle(d, rs, &vt[0], &vt[i], &vt[i + 1], &unsupported);
+        break;
+    case GL_QUADS:
+        for (uint32_t i = 0; i + 3 < n; i += 4) {
+            or_render_triangle(d, rs, &vt[i], &vt[i + 1], &vt[i + 2], &unsupported);
+            or_render_triangle(d, rs, &vt[i], &vt[i + 2], &vt[i + 3], &unsupported);
+        }
+        break;
+    case GL_QUAD_STRIP:
+        for (uint32_t i = 0; i + 3 < n; i += 2) {
+            or_render_triangle(d, rs, &vt[i], &vt[i + 1], &vt[i + 3], &unsupported);
+            or_render_triangle(d, rs, &vt[i], &vt[i + 3], &vt[i + 2], &unsupported);
+        }
+        break;
+    default:
+        break;
+    }
+    free(vt);
+    return unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+}
+
+int mtgl_dev_set_timing(mtgl_dev *d, int enable) { (void)enable; return d ? MTGL_OK : MTGL_ERR_INVALID; }
+
+int mtgl_dev_stats_get(mtgl_dev *d, mtgl_dev_stats *out)
+{
+    if (!d || !out) return MTGL_ERR_INVALID;
+    *out = d->stats;
+    return MTGL_OK;
+}
+
+int mtgl_dev_stats_reset(mtgl_dev *d)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    memset(&d->stats, 0, sizeof(d->stats));
+    return MTGL_OK;
+}
+
+const char *mtgl_dev_backend(void) { return "oracle:cpu (strict C restatement)"; }
