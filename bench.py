@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X per-draw pipeline on the 1080p textured-sphere frame.
+
+One step = one frame through the GL API of libmtgl.so: glClear of colour and
+depth, then glDrawElements of the 128x128 grid sphere (32,768 triangles,
+98,304 indices) from HBM-resident buffer objects with a 256x256
+GL_LINEAR_MIPMAP_LINEAR texture, perspective-correct interpolation, linear fog
+and one light, at 1920x1080.  Inputs (buffers, texture) are uploaded before the
+timed region.
+
+Reported: Mfragments/s (fragments = (triangle, pixel) pairs passing coverage,
+counted by the device), frames/s, the roofline of the fragment kernel
+(algorithmic bytes per launch / kernel time from HIP events on the context's
+stream) and a CPU baseline: the reference renderer compiled strict, one core,
+on a bounded sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mytinygl_amd.gl import GLLibrary, ENUMS as E  # noqa: E402
+from mytinygl_amd import scenes as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+W, H = 1920, 1080
+WORKLOAD = "textured_sphere_1920x1080"
+
+
+def frame(g, n):
+    g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"])
+    S.draw_textured_sphere(g, n)
+
+
+def traffic_from_profile():
+    """HBM bytes per fragment-kernel launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "k_raster_textured_sphere_1080p.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(budget_s: float):
+    """Frames of the same workload on the reference renderer (strict build, one
+    thread) until budget_s elapses; falls back to the oracle restatement when
+    the reference library is not present."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "libmtgl_ref.so")
+    kind = "reference"
+    if not os.path.exists(ref):
+        ref = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+        kind = "port"
+    if not os.path.exists(ref):
+        return None
+    g = GLLibrary(ref, "reference" if kind == "reference" else "oracle")
+    ctx = g.create_context(W, H)
+    g.glClearColor(0.0, 0.0, 0.0, 1.0)
+    n = S.setup_textured_sphere(g, W, H)
+    frame(g, n)                       # warm-up (lazy level-1 build in the reference)
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        frame(g, n)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or frames >= 200:
+            break
+    g.destroy_context(ctx)
+    g.release()
+    return {"frames": frames, "seconds": el, "kind": kind}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU split is a design note, not an implemented path)", file=sys.stderr)
+        sys.exit(2)
+
+    g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
+    ctx = g.create_context(W, H)
+    g.glClearColor(0.0, 0.0, 0.0, 1.0)
+    n = S.setup_textured_sphere(g, W, H)
+    for _ in range(args.warmup):
+        frame(g, n)
+    g.glFinish()
+
+    # fragments and algorithmic bytes of one frame, counted by the device
+    g.stats_reset(ctx)
+    frame(g, n)
+    st = g.stats(ctx)
+    frags_per_frame = st.fragments
+    bytes_per_frame = st.frag_bytes
+
+    g.stats_reset(ctx)
+    g.set_timing(ctx, True)
+    g.glFinish()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame(g, n)
+    g.glFinish()
+    t1 = time.perf_counter()
+    st = g.stats(ctx)
+    g.set_timing(ctx, False)
+    g.destroy_context(ctx)
+    g.release()
+
+    sec = t1 - t0
+    ms_per_step = sec * 1e3 / args.steps
+    fps = args.steps / sec
+    mfrag = frags_per_frame * fps / 1e6
+    k_ms = st.raster_ms / max(1, st.raster_launches)
+    bytes_per_launch = st.frag_bytes / max(1, st.raster_launches)
+    achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    cpu_obj = None
+    if cpu:
+        cpu_fps = cpu["frames"] / cpu["seconds"]
+        cpu_obj = {"value": round(frags_per_frame * cpu_fps / 1e6, 4), "unit": "Mfragments/s", "cores": 1,
+                   "kind": cpu["kind"],
+                   "sample": f"{cpu['frames']} frames of the same 1080p textured-sphere workload in "
+                             f"{cpu['seconds']:.1f} s, one thread ({os.cpu_count()} host CPUs visible)",
+                   "frames_per_s": round(cpu_fps, 4)}
+
+    out = {
+        "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
+        "value": round(mfrag, 3),
+        "unit": "Mfragments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "frames_per_s": round(fps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded grid sphere and value-noise texture)",
+        "config": {"workload": WORKLOAD, "width": W, "height": H, "triangles": n // 3,
+                   "fragments_per_frame": int(frags_per_frame), "algorithmic_bytes_per_frame": int(bytes_per_frame),
+                   "texture": "256x256 RGBA8, GL_LINEAR_MIPMAP_LINEAR/GL_LINEAR, GL_REPEAT",
+                   "fog": "GL_LINEAR 1..3"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profile(),
+                     "kernel": "k_raster", "kernel_ms": round(k_ms, 5), "launches": int(st.raster_launches),
+                     "bytes_per_launch": int(bytes_per_launch)},
+        "cpu_baseline": cpu_obj,
+    }
+    if cpu_obj:
+        out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

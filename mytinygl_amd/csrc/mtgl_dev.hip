@@ -1,0 +1,2066 @@
+// mtgl_dev.hip - MI355X (gfx950) implementation of the mtgl_dev C-ABI seam.
+//
+// Per draw (include/mtgl_dev.h, mtgl_dev_draw):
+//   K1 k_vertex   one thread per emitted vertex: attribute fetch with the
+//                 reference's get_array_element rules, glColor4f sanitising,
+//                 emit_vertex transform and lighting (src/gl_api.c:263-348,
+//                 src/lighting.h:53-142).
+//   K2 k_setup    one thread per assembled triangle: Sutherland-Hodgman clip
+//                 (src/clipping.h), perspective divide, fan split, integer
+//                 screen mapping, cull, bounding box, area, per-triangle LOD
+//                 (src/raster.c:451-529, :729-958).  Writes up to 7 fan
+//                 triangles per input triangle, ordered by (triangle, fan j).
+//   K3 binner     scan of per-triangle tile counts, load-balanced expansion to
+//                 (tile, triangle) pairs in submission order, stable LSD radix
+//                 sort by tile (LDS digit histograms, wavefront ballot ranks),
+//                 segment list of non-empty tiles.
+//   K4 k_raster   persistent waves; one 8x8 tile per wave, one pixel per lane;
+//                 the pixel's colour, depth and stencil stay in registers while
+//                 the wave walks the tile's bin in submission order, applying
+//                 the per-fragment pipeline of src/raster.c:532-724 exactly.
+//   K5 k_clear    scissor-aware fill (src/gl_api.c:409-457).
+//   K6 k_mip1     2x2 box filter of level 1 (src/textures.c:311-354).
+//
+// Bit-exactness: compiled with -ffp-contract=off and without fast-math, so
+// every float expression keeps the reference's operation order; fp32
+// division and sqrt are the correctly rounded sequences; float->int
+// conversions follow x86 cvttss2si (INT_MIN for NaN / out of range).  The
+// transcendental calls of the reference (log2f for the triangle LOD, expf for
+// GL_EXP/GL_EXP2 fog, powf for spot and specular terms) are evaluated in fp64
+// and rounded once; spot cutoff cosines arrive precomputed by the host C
+// library.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "GL/gl.h"
+#include "mtgl_dev.h"
+
+#define MTGL_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+constexpr int TILE = 8;                 // 8x8 pixels = one wave64
+constexpr int MAX_CLIP = 12;            // reference MAX_CLIP_VERTS: at most 10 fan triangles
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+constexpr int RADIX_BITS = 8;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+// ---------------------------------------------------------------------------
+// records in HBM
+// ---------------------------------------------------------------------------
+
+// post-transform vertex, hot part (48 B)
+struct __align__(16) VHot {
+    float x, y, z, w;
+    float r, g, b, a;
+    float s, t, ez, pad;
+};
+
+// post-transform vertex, eye-space part used by per-fragment lighting (32 B)
+struct __align__(16) VCold {
+    float ep[3];
+    float en[3];
+    float pad[2];
+};
+
+// fan triangle after setup (192 B); an unclipped triangle t owns slot t, a
+// clipped one owns a run of slots from the overflow pool after the first ntri
+struct __align__(16) RTri {
+    int32_t x0, y0, x1, y1, x2, y2;
+    int32_t minX, minY, maxX, maxY;      // viewport and scissor clamped
+    float area, inv_area;
+    float z0, z1, z2;
+    float w0, w1, w2;                    // 1/w
+    float c0[4], c1[4], c2[4];
+    float uv0[2], uv1[2], uv2[2];
+    float ez0, ez1, ez2;
+    float lod;
+    int32_t back;
+    int32_t pad0[3];                     // the first 176 bytes are what the fragment stage reads
+    int32_t tx0, ty0, tx1, ty1;          // tile box (framebuffer clamped)
+    int32_t ntiles;
+    int32_t pad[3];
+};
+static_assert(sizeof(RTri) == 208, "RTri layout");
+constexpr int RT_FRAG_VEC4 = 11;         // 176 bytes = 11 x 16 B staged in LDS per bin entry
+
+struct __align__(16) RCold {
+    float ep[9];
+    float en[9];
+    float pad[2];
+};
+
+struct Counters {
+    unsigned long long fragments;
+    unsigned long long frag_bytes;
+    unsigned long long tri_raster;   // fan triangles set up by the current draw
+    unsigned int nseg;
+    unsigned int work;
+    unsigned int unsupported;
+    unsigned int nclip;
+    unsigned int noverflow;       // overflow slots requested by clipped triangles
+    unsigned int overflow_full;   // a request did not fit: the host grows the pool and reruns
+};
+
+// attribute stream resolved to a device pointer
+struct DArr {
+    const uint8_t *src;
+    int64_t size;
+    int64_t offset;
+    int32_t stride;
+    int32_t comps;
+    uint32_t type;
+    uint32_t enabled;
+};
+
+struct VertexParams {
+    uint32_t count;
+    int32_t first;
+    uint32_t index_type;
+    uint32_t n_vstates;
+    DArr indices, position, color, texcoord, normal, state_index;
+    float cur_color[4], cur_texcoord[2], cur_normal[3];
+    const mtgl_vertex_state *vstates;
+    VHot *out_hot;
+    VCold *out_cold;              // null when no per-fragment lighting needs it
+};
+
+struct SetupParams {
+    uint32_t mode;
+    uint32_t count;
+    uint32_t ntri;
+    int32_t fb_w, fb_h;
+    int32_t tiles_x;
+    int32_t tex_w, tex_h;         // 0 when the draw is untextured
+    const mtgl_raster_state *rs;
+    const VHot *vhot;
+    const VCold *vcold;
+    RTri *rt;
+    RCold *rcold;
+    uint32_t *n_rt;               // fan triangles kept per input triangle
+    uint32_t *rec_base;           // first record slot per input triangle
+    unsigned long long *tri_tiles;
+    uint32_t *clip_list;          // triangles that cross a frustum plane
+    uint32_t overflow_cap;        // slots available after the first ntri
+    Counters *ctr;
+};
+
+struct RasterParams {
+    const mtgl_raster_state *rs;
+    const RTri *rt;
+    const RCold *rcold;
+    const uint32_t *vals;          // sorted pair values (record slot)
+    const uint32_t *seg_tiles;
+    const uint32_t *tile_start;
+    const uint32_t *tile_end;
+    uint32_t *color;
+    float *depth;
+    uint8_t *stencil;
+    int32_t fb_w, fb_h, tiles_x;
+    const uint32_t *tex;           // level 0 (null = untextured)
+    const uint32_t *tex1;          // level 1 (null = none: samples read as white)
+    int32_t tex_w, tex_h, tex1_w, tex1_h;
+    Counters *ctr;
+};
+
+// ---------------------------------------------------------------------------
+// scalar helpers (reference arithmetic, see file header)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int32_t cvtt(float f)
+{
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : INT32_MIN;
+}
+
+__device__ __forceinline__ int32_t isub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+
+__device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+struct Col { float r, g, b, a; };
+
+__device__ __forceinline__ Col col_add(Col a, Col b) { return {a.r + b.r, a.g + b.g, a.b + b.b, a.a + b.a}; }
+__device__ __forceinline__ Col col_mul(Col a, Col b) { return {a.r * b.r, a.g * b.g, a.b * b.b, a.a * b.a}; }
+__device__ __forceinline__ Col col_scale(Col a, float s) { return {a.r * s, a.g * s, a.b * s, a.a * s}; }
+__device__ __forceinline__ Col col_clamp(Col c) { return {clamp01(c.r), clamp01(c.g), clamp01(c.b), clamp01(c.a)}; }
+__device__ __forceinline__ Col col_lerp(Col a, Col b, float t) { return col_add(col_scale(a, 1.0f - t), col_scale(b, t)); }
+__device__ __forceinline__ Col col_of(const mtgl_color &c) { return {c.r, c.g, c.b, c.a}; }
+
+// clamp then truncate x*255 (reference color_to_rgba32): fmaxf maps NaN to 0,
+// which is also the byte x86 produces for a NaN channel, and the converted
+// value always lies in [0, 255]
+__device__ __forceinline__ uint32_t pack8(float v)
+{
+    float x = fminf(fmaxf(v, 0.0f), 1.0f);
+    return (uint32_t)(x * 255.0f);
+}
+
+__device__ __forceinline__ uint32_t col_pack(Col c)
+{
+    return pack8(c.r) | (pack8(c.g) << 8) | (pack8(c.b) << 16) | (pack8(c.a) << 24);
+}
+
+// byte/255.0f, correctly rounded: one product by 1/255 and one FMA residual
+// correction reproduce the IEEE quotient for all 256 byte values (checked
+// exhaustively by the test suite)
+__device__ __forceinline__ float u8_to_unit(uint32_t b)
+{
+    const float r = 1.0f / 255.0f;
+    float fb = (float)b;
+    float m = fb * r;
+    float e = __builtin_fmaf(-m, 255.0f, fb);
+    return __builtin_fmaf(e, r, m);
+}
+
+__device__ __forceinline__ Col col_unpack(uint32_t p)
+{
+    return {u8_to_unit(p & 0xFF), u8_to_unit((p >> 8) & 0xFF), u8_to_unit((p >> 16) & 0xFF), u8_to_unit(p >> 24)};
+}
+
+__device__ __forceinline__ float dot3(const float *a, const float *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__device__ __forceinline__ void normalize3(float *v)
+{
+    float len = sqrtf(dot3(v, v));
+    if (len > 0.0f) {
+        float s = 1.0f / len;
+        v[0] = v[0] * s; v[1] = v[1] * s; v[2] = v[2] * s;
+    }
+}
+
+__device__ __forceinline__ float pow_cr(float b, float e) { return (float)pow((double)b, (double)e); }
+__device__ __forceinline__ float exp_cr(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float log2_cr(float x) { return (float)log2((double)x); }
+
+// m * (x, y, z, w) with left-to-right row sums
+__device__ __forceinline__ void xform(const float *m, float x, float y, float z, float w, float *o)
+{
+    o[0] = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
+    o[1] = m[1] * x + m[5] * y + m[9] * z + m[13] * w;
+    o[2] = m[2] * x + m[6] * y + m[10] * z + m[14] * w;
+    o[3] = m[3] * x + m[7] * y + m[11] * z + m[15] * w;
+}
+
+// compute_lighting (reference src/lighting.h:53-142)
+__device__ Col compute_lighting(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
+                                Col m_amb, Col m_dif, Col m_spec, Col m_emi, float shininess)
+{
+    Col result = col_add(m_emi, col_mul(m_amb, col_of(L->model_ambient)));
+    result.a = m_dif.a;
+    float N[3] = {eye_normal[0], eye_normal[1], eye_normal[2]};
+    if (flags & MTGL_F_NORMALIZE) normalize3(N);
+    for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
+        const mtgl_light *lt = &L->lights[i];
+        if (!lt->enabled) continue;
+        float Ld[3];
+        float att = 1.0f;
+        if (lt->position[3] == 0.0f) {
+            Ld[0] = lt->position[0]; Ld[1] = lt->position[1]; Ld[2] = lt->position[2];
+            normalize3(Ld);
+        } else {
+            float to[3] = {lt->position[0] - eye_pos[0], lt->position[1] - eye_pos[1], lt->position[2] - eye_pos[2]};
+            float dist = sqrtf(dot3(to, to));
+            if (dist < 1e-6f) dist = 1e-6f;
+            float s = 1.0f / dist;
+            Ld[0] = to[0] * s; Ld[1] = to[1] * s; Ld[2] = to[2] * s;
+            float den = lt->constant_attenuation + lt->linear_attenuation * dist + lt->quadratic_attenuation * dist * dist;
+            if (den < 1e-6f) den = 1e-6f;
+            att = 1.0f / den;
+            if (lt->spot_cutoff < 180.0f) {
+                float sd[3] = {lt->spot_direction[0], lt->spot_direction[1], lt->spot_direction[2]};
+                normalize3(sd);
+                float cos_angle = -dot3(Ld, sd);
+                if (cos_angle < lt->spot_cos_cutoff) att = 0.0f;
+                else att *= pow_cr(cos_angle, lt->spot_exponent);
+            }
+        }
+        if (att <= 0.0f) continue;
+        result = col_add(result, col_scale(col_mul(m_amb, col_of(lt->ambient)), att));
+        float ndl = dot3(N, Ld);
+        if (ndl > 0.0f) {
+            result = col_add(result, col_scale(col_mul(m_dif, col_of(lt->diffuse)), att * ndl));
+            if (shininess > 0.0f) {
+                float V[3] = {0.0f, 0.0f, 1.0f};
+                if (L->local_viewer) {
+                    V[0] = eye_pos[0] * -1.0f; V[1] = eye_pos[1] * -1.0f; V[2] = eye_pos[2] * -1.0f;
+                    normalize3(V);
+                }
+                float H[3] = {Ld[0] + V[0], Ld[1] + V[1], Ld[2] + V[2]};
+                normalize3(H);
+                float ndh = dot3(N, H);
+                if (ndh > 0.0f) {
+                    float spec = pow_cr(ndh, shininess) * att;
+                    result = col_add(result, col_scale(col_mul(m_spec, col_of(lt->specular)), spec));
+                }
+            }
+        }
+    }
+    return col_clamp(result);
+}
+
+// ---------------------------------------------------------------------------
+// K1: vertex fetch, transform and lighting
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float ld_f32(const uint8_t *p)
+{
+    if (((uintptr_t)p & 3u) == 0) return *(const float *)p;
+    float v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+
+// reference get_array_element (src/gl_api.c:1758-1797); elements outside the
+// stream's source take the defaults
+__device__ void fetch(const DArr &a, int32_t index, float *out, int n)
+{
+    int64_t elem = (int64_t)a.comps * (a.type == GL_FLOAT ? 4 : 1);
+    int64_t at = a.offset + (int64_t)index * a.stride;
+    if (!a.src || index < 0 || a.stride <= 0 || at < 0 || at + elem > a.size) {
+        for (int i = 0; i < n; i++) out[i] = i < 3 ? 0.0f : 1.0f;
+        return;
+    }
+    const uint8_t *p = a.src + at;
+    for (int i = 0; i < n; i++) {
+        if (i < a.comps) out[i] = a.type == GL_FLOAT ? ld_f32(p + 4 * i) : (float)p[i] / 255.0f;
+        else out[i] = i == 3 ? 1.0f : 0.0f;
+    }
+}
+
+__device__ int32_t fetch_index(const VertexParams &P, uint32_t i)
+{
+    if (P.index_type == 0) return (int32_t)((uint32_t)P.first + i);
+    const DArr &a = P.indices;
+    int64_t isz = P.index_type == GL_UNSIGNED_INT ? 4 : (P.index_type == GL_UNSIGNED_SHORT ? 2 : 1);
+    int64_t at = a.offset + (int64_t)i * isz;
+    if (!a.src || at < 0 || at + isz > a.size) return -1;
+    const uint8_t *p = a.src + at;
+    if (isz == 4) {
+        uint32_t v;
+        if (((uintptr_t)p & 3u) == 0) v = *(const uint32_t *)p; else __builtin_memcpy(&v, p, 4);
+        return (int32_t)v;
+    }
+    if (isz == 2) {
+        uint16_t v;
+        if (((uintptr_t)p & 1u) == 0) v = *(const uint16_t *)p; else __builtin_memcpy(&v, p, 2);
+        return (int32_t)v;
+    }
+    return (int32_t)p[0];
+}
+
+__device__ __forceinline__ float sanitize(float v, float nan_value)
+{
+    if (__builtin_isnan(v) || __builtin_isinf(v)) v = nan_value;
+    return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+}
+
+__global__ void __launch_bounds__(256) k_vertex(VertexParams P)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.count) return;
+    int32_t idx = fetch_index(P, i);
+    float pos[4], col[4], tc[2], nrm[3];
+    fetch(P.position, idx, pos, 4);
+    if (P.position.comps == 2) pos[2] = 0.0f;
+    if (P.color.enabled) {
+        fetch(P.color, idx, col, 4);
+        for (int k = 0; k < 4; k++) col[k] = sanitize(col[k], k == 3 ? 1.0f : 0.0f);
+    } else {
+        for (int k = 0; k < 4; k++) col[k] = P.cur_color[k];
+    }
+    if (P.texcoord.enabled) fetch(P.texcoord, idx, tc, 2);
+    else { tc[0] = P.cur_texcoord[0]; tc[1] = P.cur_texcoord[1]; }
+    if (P.normal.enabled) fetch(P.normal, idx, nrm, 3);
+    else { nrm[0] = P.cur_normal[0]; nrm[1] = P.cur_normal[1]; nrm[2] = P.cur_normal[2]; }
+
+    uint32_t si = 0;
+    if (P.n_vstates > 1 && P.state_index.enabled) {
+        const DArr &a = P.state_index;
+        int64_t at = a.offset + (int64_t)i * a.stride;
+        if (a.src && at >= 0 && at + 4 <= a.size) __builtin_memcpy(&si, a.src + at, 4);
+        if (si >= P.n_vstates) si = P.n_vstates - 1;
+    }
+    const mtgl_vertex_state *vs = &P.vstates[si];
+
+    // emit_vertex (reference src/gl_api.c:263-348)
+    float eye[4];
+    xform(vs->modelview, pos[0], pos[1], pos[2], 1.0f, eye);
+    float en4[4];
+    xform(vs->normal_matrix, nrm[0], nrm[1], nrm[2], 0.0f, en4);
+    float en[3] = {en4[0], en4[1], en4[2]};
+    normalize3(en);
+    float ep[3] = {eye[0], eye[1], eye[2]};
+
+    Col vc = {col[0], col[1], col[2], col[3]};
+    if ((vs->flags & MTGL_F_LIGHTING) && vs->shade_model != GL_PHONG) {
+        const mtgl_material &f = vs->lighting.front;
+        Col amb = col_of(f.ambient), dif = col_of(f.diffuse), spe = col_of(f.specular), emi = col_of(f.emission);
+        uint32_t face = vs->color_material_face, mode = vs->color_material_mode;
+        if ((vs->flags & MTGL_F_COLOR_MATERIAL) && (face == GL_FRONT || face == GL_FRONT_AND_BACK)) {
+            Col k = col_clamp(vc);
+            if (mode == GL_AMBIENT || mode == GL_AMBIENT_AND_DIFFUSE) amb = k;
+            if (mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE) dif = k;
+            if (mode == GL_SPECULAR) spe = k;
+            if (mode == GL_EMISSION) emi = k;
+        }
+        vc = compute_lighting(&vs->lighting, vs->flags, ep, en, amb, dif, spe, emi, f.shininess);
+    }
+
+    float clip[4];
+    xform(vs->projection, eye[0], eye[1], eye[2], eye[3], clip);
+
+    float t4[4];
+    xform(vs->texture, tc[0], tc[1], 0.0f, 1.0f, t4);
+    float s_out, t_out;
+    if (t4[3] != 0.0f && t4[3] != 1.0f) { s_out = t4[0] / t4[3]; t_out = t4[1] / t4[3]; }
+    else { s_out = t4[0]; t_out = t4[1]; }
+
+    VHot h;
+    h.x = clip[0]; h.y = clip[1]; h.z = clip[2]; h.w = clip[3];
+    h.r = vc.r; h.g = vc.g; h.b = vc.b; h.a = vc.a;
+    h.s = s_out; h.t = t_out; h.ez = -eye[2]; h.pad = 0.0f;
+    P.out_hot[i] = h;
+    if (P.out_cold) {
+        VCold c;
+        c.ep[0] = ep[0]; c.ep[1] = ep[1]; c.ep[2] = ep[2];
+        c.en[0] = en[0]; c.en[1] = en[1]; c.en[2] = en[2];
+        c.pad[0] = c.pad[1] = 0.0f;
+        P.out_cold[i] = c;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2: primitive assembly, clipping, set-up
+// ---------------------------------------------------------------------------
+
+// triangles of a primitive (reference flush_*, src/raster.c:961-1017, :1199-1231)
+__host__ __device__ inline uint32_t tri_count(uint32_t mode, uint32_t n)
+{
+    switch (mode) {
+    case GL_TRIANGLES: return n / 3;
+    case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_POLYGON: return n >= 3 ? n - 2 : 0;
+    case GL_QUADS: return (n / 4) * 2;
+    case GL_QUAD_STRIP: return n >= 4 ? ((n - 4) / 2 + 1) * 2 : 0;
+    default: return 0;
+    }
+}
+
+__device__ __forceinline__ void tri_vertices(uint32_t mode, uint32_t t, uint32_t &a, uint32_t &b, uint32_t &c)
+{
+    switch (mode) {
+    case GL_TRIANGLES: a = 3 * t; b = a + 1; c = a + 2; break;
+    case GL_TRIANGLE_STRIP:
+        if (t % 2 == 0) { a = t; b = t + 1; } else { a = t + 1; b = t; }
+        c = t + 2;
+        break;
+    case GL_QUADS: {
+        uint32_t q = 4 * (t / 2);
+        if (t % 2 == 0) { a = q; b = q + 1; c = q + 2; } else { a = q; b = q + 2; c = q + 3; }
+        break;
+    }
+    case GL_QUAD_STRIP: {
+        uint32_t q = 2 * (t / 2);
+        if (t % 2 == 0) { a = q; b = q + 1; c = q + 3; } else { a = q; b = q + 3; c = q + 2; }
+        break;
+    }
+    default: a = 0; b = t + 1; c = t + 2; break;   // fan, polygon
+    }
+}
+
+// clip-space vertex carried through clipping (the object-space normal the
+// reference also interpolates is never read after clipping and is dropped)
+struct CV {
+    float x, y, z, w;
+    float r, g, b, a;
+    float s, t, ez;
+    float ep[3], en[3];
+};
+
+__device__ __forceinline__ float plane_dist(const CV &v, int plane)
+{
+    switch (plane) {
+    case 0: return v.z + v.w;
+    case 1: return v.w - v.z;
+    case 2: return v.x + v.w;
+    case 3: return v.w - v.x;
+    case 4: return v.y + v.w;
+    default: return v.w - v.y;
+    }
+}
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + t * (b - a); }
+
+// vertex_lerp (reference src/graphics.h:465-475) followed by snap_to_plane
+// (src/clipping.h:37-47)
+__device__ void cv_lerp_snap(const CV &a, const CV &b, float t, int plane, bool cold, CV &o)
+{
+    o.x = lerpf(a.x, b.x, t); o.y = lerpf(a.y, b.y, t); o.z = lerpf(a.z, b.z, t); o.w = lerpf(a.w, b.w, t);
+    o.r = a.r * (1.0f - t) + b.r * t;
+    o.g = a.g * (1.0f - t) + b.g * t;
+    o.b = a.b * (1.0f - t) + b.b * t;
+    o.a = a.a * (1.0f - t) + b.a * t;
+    o.s = lerpf(a.s, b.s, t); o.t = lerpf(a.t, b.t, t);
+    o.ez = lerpf(a.ez, b.ez, t);
+    if (cold) {
+        for (int k = 0; k < 3; k++) { o.ep[k] = lerpf(a.ep[k], b.ep[k], t); o.en[k] = lerpf(a.en[k], b.en[k], t); }
+    }
+    switch (plane) {
+    case 0: o.z = -o.w; break;
+    case 1: o.z = o.w; break;
+    case 2: o.x = -o.w; break;
+    case 3: o.x = o.w; break;
+    case 4: o.y = -o.w; break;
+    default: o.y = o.w; break;
+    }
+}
+
+// Sutherland-Hodgman against one plane (reference clip_polygon_plane_id,
+// src/clipping.h:50-99)
+__device__ int clip_plane(const CV *in, int n, CV *out, int plane, bool cold)
+{
+    if (n == 0) return 0;
+    int m = 0;
+    int pi = n - 1;
+    float pd = plane_dist(in[pi], plane);
+    for (int i = 0; i < n; i++) {
+        float cd = plane_dist(in[i], plane);
+        if (pd >= 0) {
+            if (cd >= 0) {
+                if (m < MAX_CLIP) out[m++] = in[i];
+            } else {
+                float den = pd - cd;
+                if (fabsf(den) > 1e-10f && m < MAX_CLIP) { cv_lerp_snap(in[pi], in[i], pd / den, plane, cold, out[m]); m++; }
+            }
+        } else if (cd >= 0) {
+            float den = pd - cd;
+            if (fabsf(den) > 1e-10f && m < MAX_CLIP) { cv_lerp_snap(in[pi], in[i], pd / den, plane, cold, out[m]); m++; }
+            if (m < MAX_CLIP) out[m++] = in[i];
+        }
+        pi = i;
+        pd = cd;
+    }
+    return m;
+}
+
+__device__ __forceinline__ void load_cv(const SetupParams &P, uint32_t v, bool cold, CV &o)
+{
+    const float4 *h = reinterpret_cast<const float4 *>(&P.vhot[v]);
+    float4 a = h[0], b = h[1], c = h[2];
+    o.x = a.x; o.y = a.y; o.z = a.z; o.w = a.w;
+    o.r = b.x; o.g = b.y; o.b = b.z; o.a = b.w;
+    o.s = c.x; o.t = c.y; o.ez = c.z;
+    if (cold) {
+        const VCold &q = P.vcold[v];
+        for (int k = 0; k < 3; k++) { o.ep[k] = q.ep[k]; o.en[k] = q.en[k]; }
+    } else {
+        for (int k = 0; k < 3; k++) { o.ep[k] = 0.0f; o.en[k] = 0.0f; }
+    }
+}
+
+__device__ __forceinline__ void perspective_divide(CV &v)
+{
+    if (fabsf(v.w) < 1e-6f) { v.x = 0.0f; v.y = 0.0f; v.z = 0.0f; v.w = 1.0f; return; }
+    float iw = 1.0f / v.w;
+    v.x *= iw; v.y *= iw; v.z *= iw; v.w = iw;
+}
+
+// one fan triangle (c[0], c[j], c[j+1]) after the divide; returns 1 when a
+// record was written
+__device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, const CV &A, const CV &B, const CV &C,
+                         bool cold, RTri *outp, RCold *coldp, int *tiles_out)
+{
+    const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
+    const CV *v[3] = {&A, &B, &C};
+    int32_t X[3], Y[3];
+    for (int k = 0; k < 3; k++) {
+        X[k] = cvtt((v[k]->x + 1.0f) * 0.5f * (float)vw + (float)vx);
+        Y[k] = cvtt((1.0f - v[k]->y) * 0.5f * (float)vh + (float)vy);
+    }
+    float sa = (float)isub(X[1], X[0]) * (float)isub(Y[2], Y[0]) - (float)isub(X[2], X[0]) * (float)isub(Y[1], Y[0]);
+    if (rs->flags & MTGL_F_CULL_FACE) {
+        int front = rs->front_face == GL_CCW ? (sa < 0) : (sa > 0);
+        int cull = rs->cull_face_mode == GL_FRONT ? front : (rs->cull_face_mode == GL_BACK ? !front : 1);
+        if (cull) return 0;
+    }
+    int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
+    uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
+    if (pm == GL_POINT || pm == GL_LINE) { atomicOr(&P.ctr->unsupported, 1u); return 0; }
+
+    int32_t minX = X[0] < X[1] ? (X[0] < X[2] ? X[0] : X[2]) : (X[1] < X[2] ? X[1] : X[2]);
+    int32_t minY = Y[0] < Y[1] ? (Y[0] < Y[2] ? Y[0] : Y[2]) : (Y[1] < Y[2] ? Y[1] : Y[2]);
+    int32_t maxX = X[0] > X[1] ? (X[0] > X[2] ? X[0] : X[2]) : (X[1] > X[2] ? X[1] : X[2]);
+    int32_t maxY = Y[0] > Y[1] ? (Y[0] > Y[2] ? Y[0] : Y[2]) : (Y[1] > Y[2] ? Y[1] : Y[2]);
+    if (minX < vx) minX = vx;
+    if (minY < vy) minY = vy;
+    if (maxX >= vx + vw) maxX = vx + vw - 1;
+    if (maxY >= vy + vh) maxY = vy + vh - 1;
+    if (rs->flags & MTGL_F_SCISSOR_TEST) {
+        const int32_t sx = rs->scissor[0], sy = rs->scissor[1], sw = rs->scissor[2], sh = rs->scissor[3];
+        if (minX < sx) minX = sx;
+        if (minY < sy) minY = sy;
+        if (maxX >= sx + sw) maxX = sx + sw - 1;
+        if (maxY >= sy + sh) maxY = sy + sh - 1;
+    }
+    if (minX > maxX || minY > maxY) return 0;
+    const float fx0 = (float)X[0], fy0 = (float)Y[0], fx1 = (float)X[1], fy1 = (float)Y[1];
+    const float fx2 = (float)X[2], fy2 = (float)Y[2];
+    float area = (fx2 - fx0) * (fy1 - fy0) - (fy2 - fy0) * (fx1 - fx0);
+    if (fabsf(area) < 0.5f) return 0;
+    // pixels outside the framebuffer have no observable effect: bin only the
+    // framebuffer part of the box
+    int32_t bx0 = minX < 0 ? 0 : minX, by0 = minY < 0 ? 0 : minY;
+    int32_t bx1 = maxX >= P.fb_w ? P.fb_w - 1 : maxX, by1 = maxY >= P.fb_h ? P.fb_h - 1 : maxY;
+    if (bx0 > bx1 || by0 > by1) return 0;
+
+    float lod = 0.0f;
+    if (P.tex_w > 0) {
+        float screen_area = fabsf(area) * 0.5f;
+        float du1 = (B.s - A.s) * (float)P.tex_w;
+        float dv1 = (B.t - A.t) * (float)P.tex_h;
+        float du2 = (C.s - A.s) * (float)P.tex_w;
+        float dv2 = (C.t - A.t) * (float)P.tex_h;
+        float texel_area = fabsf(du1 * dv2 - du2 * dv1) * 0.5f;
+        if (screen_area > 0.0f) {
+            float tpp = texel_area / screen_area;
+            if (tpp > 0.0f) {
+                lod = log2_cr(tpp) * 0.5f;
+                if (lod < 0.0f) lod = 0.0f;
+            }
+        }
+    }
+    RTri r;
+    r.x0 = X[0]; r.y0 = Y[0]; r.x1 = X[1]; r.y1 = Y[1]; r.x2 = X[2]; r.y2 = Y[2];
+    r.minX = minX; r.minY = minY; r.maxX = maxX; r.maxY = maxY;
+    r.area = area;
+    r.inv_area = 1.0f / area;
+    r.z0 = A.z; r.z1 = B.z; r.z2 = C.z;
+    r.w0 = A.w; r.w1 = B.w; r.w2 = C.w;
+    r.c0[0] = A.r; r.c0[1] = A.g; r.c0[2] = A.b; r.c0[3] = A.a;
+    r.c1[0] = B.r; r.c1[1] = B.g; r.c1[2] = B.b; r.c1[3] = B.a;
+    r.c2[0] = C.r; r.c2[1] = C.g; r.c2[2] = C.b; r.c2[3] = C.a;
+    r.uv0[0] = A.s; r.uv0[1] = A.t; r.uv1[0] = B.s; r.uv1[1] = B.t; r.uv2[0] = C.s; r.uv2[1] = C.t;
+    r.ez0 = A.ez; r.ez1 = B.ez; r.ez2 = C.ez;
+    r.lod = lod;
+    r.back = back;
+    r.pad0[0] = r.pad0[1] = r.pad0[2] = 0;
+    r.tx0 = bx0 / TILE; r.ty0 = by0 / TILE; r.tx1 = bx1 / TILE; r.ty1 = by1 / TILE;
+    r.ntiles = (r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    *outp = r;
+    if (cold) {
+        RCold q;
+        for (int k = 0; k < 3; k++) {
+            q.ep[k] = A.ep[k]; q.ep[3 + k] = B.ep[k]; q.ep[6 + k] = C.ep[k];
+            q.en[k] = A.en[k]; q.en[3 + k] = B.en[k]; q.en[6 + k] = C.en[k];
+        }
+        q.pad[0] = q.pad[1] = 0.0f;
+        *coldp = q;
+    }
+    *tiles_out = r.ntiles;
+    return 1;
+}
+
+// fast path: triangles inside every frustum plane pass through the clipper
+// unchanged (each plane re-emits the vertices in order), so they are set up
+// directly; the others are queued for k_setup_clip
+__global__ void __launch_bounds__(256) k_setup(SetupParams P)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.ntri) return;
+    const mtgl_raster_state *rs = P.rs;
+    const bool cold = P.vcold != nullptr;
+    uint32_t ia, ib, ic;
+    tri_vertices(P.mode, t, ia, ib, ic);
+    CV v[3];
+    load_cv(P, ia, cold, v[0]);
+    load_cv(P, ib, cold, v[1]);
+    load_cv(P, ic, cold, v[2]);
+    bool all_inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int pl = 0; pl < 6; pl++)
+            all_inside = all_inside && (plane_dist(v[k], pl) >= 0);
+    uint32_t nout = 0;
+    unsigned long long tiles = 0;
+    if (all_inside) {
+        for (int k = 0; k < 3; k++) perspective_divide(v[k]);
+        int nt = 0;
+        RTri *out = P.rt + t;
+        RCold *cout = cold ? P.rcold + t : nullptr;
+        if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, &nt)) { nout = 1; tiles = (unsigned long long)nt; }
+        if (nout) atomicAdd(&P.ctr->tri_raster, 1ull);
+    } else {
+        uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
+        P.clip_list[slot] = t;
+    }
+    P.n_rt[t] = nout;
+    P.rec_base[t] = t;
+    P.tri_tiles[t] = tiles;
+}
+
+// slow path: Sutherland-Hodgman in the reference's plane order (near, far,
+// left, right, bottom, top; src/clipping.h:106-126), divide, fan split
+__global__ void __launch_bounds__(64) k_setup_clip(SetupParams P)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *(volatile uint32_t *)&P.ctr->nclip) return;
+    const uint32_t t = P.clip_list[i];
+    const mtgl_raster_state *rs = P.rs;
+    const bool cold = P.vcold != nullptr;
+    uint32_t ia, ib, ic;
+    tri_vertices(P.mode, t, ia, ib, ic);
+    CV a[MAX_CLIP], b[MAX_CLIP];
+    load_cv(P, ia, cold, b[0]);
+    load_cv(P, ib, cold, b[1]);
+    load_cv(P, ic, cold, b[2]);
+    int n = clip_plane(b, 3, a, 0, cold);
+    n = n ? clip_plane(a, n, b, 1, cold) : 0;
+    n = n ? clip_plane(b, n, a, 2, cold) : 0;
+    n = n ? clip_plane(a, n, b, 3, cold) : 0;
+    n = n ? clip_plane(b, n, a, 4, cold) : 0;
+    n = n ? clip_plane(a, n, b, 5, cold) : 0;
+    uint32_t nout = 0;
+    unsigned long long tiles = 0;
+    uint32_t base = 0;
+    if (n >= 3) {
+        base = atomicAdd(&P.ctr->noverflow, (uint32_t)(n - 2));
+        if (base + (uint32_t)(n - 2) > P.overflow_cap) {
+            atomicOr(&P.ctr->overflow_full, 1u);
+            n = 0;
+        }
+    }
+    if (n >= 3) {
+        RTri *out = P.rt + P.ntri + base;
+        RCold *cout = cold ? P.rcold + P.ntri + base : nullptr;
+        for (int k = 0; k < n; k++) perspective_divide(b[k]);
+        for (int j = 1; j + 1 < n; j++) {
+            int nt = 0;
+            if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr, &nt)) {
+                nout++;
+                tiles += (unsigned long long)nt;
+            }
+        }
+    }
+    P.n_rt[t] = nout;
+    P.rec_base[t] = P.ntri + base;
+    P.tri_tiles[t] = tiles;
+    if (nout) atomicAdd(&P.ctr->tri_raster, (unsigned long long)nout);
+}
+
+// ---------------------------------------------------------------------------
+// scans (exclusive), one value per element; out may alias in
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+// block-wide exclusive scan of one value per thread (blockDim == SCAN_THREADS
+// or SORT_THREADS == 256); returns the exclusive prefix and the block total
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T *lds_waves, T &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    T inc = wave_inclusive_scan(v);
+    if (lane == 63) lds_waves[wave] = inc;
+    __syncthreads();
+    T base = 0;
+    total = 0;
+    for (int w = 0; w < nwaves; w++) {
+        T s = lds_waves[w];
+        if (w < wave) base += s;
+        total += s;
+    }
+    __syncthreads();
+    return base + inc - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, uint64_t n, T *block_sums)
+{
+    __shared__ T waves[SCAN_THREADS / 64];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    T s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint64_t i = base + (uint64_t)threadIdx.x * SCAN_ITEMS + k;
+        if (i < n) s += in[i];
+    }
+    T total;
+    block_exclusive_scan<T>(s, waves, total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// single workgroup: exclusive scan of the block sums in chunks, carrying the
+// running total; writes the grand total to *total_out
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_sums(T *sums, uint64_t nblocks, T *total_out)
+{
+    __shared__ T waves[SCAN_THREADS / 64];
+    T carry = 0;
+    for (uint64_t base = 0; base < nblocks; base += SCAN_THREADS) {
+        uint64_t i = base + threadIdx.x;
+        T v = i < nblocks ? sums[i] : (T)0;
+        T total;
+        T ex = block_exclusive_scan<T>(v, waves, total);
+        if (i < nblocks) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64_t n, const T *block_offsets, T *out)
+{
+    __shared__ T waves[SCAN_THREADS / 64];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T vals[SCAN_ITEMS];
+    T s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        vals[k] = base + k < n ? in[base + k] : (T)0;
+        s += vals[k];
+    }
+    T total;
+    T ex = block_exclusive_scan<T>(s, waves, total) + block_offsets[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += vals[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3a: expand triangles into (tile, fan triangle) pairs, in submission order
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
+                                                const uint32_t *rec_base, const RTri *rt, uint64_t npairs,
+                                                int32_t tiles_x, uint32_t *keys, uint32_t *vals)
+{
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npairs) return;
+    // last triangle whose first pair is <= p
+    uint32_t lo = 0, hi = ntri;   // invariant: pair_base[lo] <= p < pair_base[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (pair_base[mid] <= p) lo = mid; else hi = mid;
+    }
+    uint32_t t = lo;
+    uint64_t k = p - pair_base[t];
+    uint32_t nr = n_rt[t];
+    const uint32_t rb = rec_base[t];
+    const RTri *r = rt + rb;
+    uint32_t j = 0;
+    for (; j + 1 < nr; j++) {
+        uint64_t nt = (uint64_t)r[j].ntiles;
+        if (k < nt) break;
+        k -= nt;
+    }
+    const RTri &q = r[j];
+    int32_t tw = q.tx1 - q.tx0 + 1;
+    int32_t ty = q.ty0 + (int32_t)(k / (uint64_t)tw);
+    int32_t tx = q.tx0 + (int32_t)(k % (uint64_t)tw);
+    keys[p] = (uint32_t)(ty * tiles_x + tx);
+    vals[p] = rb + j;
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (key, value) pairs by key
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys, uint32_t n, int shift,
+                                                            uint32_t nblocks, uint32_t *counts)
+{
+    __shared__ uint32_t hist[RADIX];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        uint32_t i = base + r * SORT_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & (RADIX - 1)], 1u);
+    }
+    __syncthreads();
+    counts[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *keys_in, const uint32_t *vals_in,
+                                                               uint32_t n, int shift, uint32_t nblocks,
+                                                               const uint32_t *offsets, uint32_t *keys_out,
+                                                               uint32_t *vals_out)
+{
+    __shared__ uint32_t wave_hist[SORT_THREADS / 64][RADIX];
+    __shared__ uint32_t running[RADIX];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+    for (int w = 0; w < SORT_THREADS / 64; w++) wave_hist[w][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        uint32_t i = base + r * SORT_THREADS + threadIdx.x;
+        bool valid = i < n;
+        uint32_t key = valid ? keys_in[i] : 0u;
+        uint32_t val = valid ? vals_in[i] : 0u;
+        uint32_t d = (key >> shift) & (RADIX - 1);
+        // lanes of this wave holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RADIX_BITS; b++) {
+            uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        uint32_t below = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && below == 0) wave_hist[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = running[d] + below;
+            for (int w = 0; w < wave; w++) pos += wave_hist[w][d];
+            if (pos < n) {
+                keys_out[pos] = key;
+                vals_out[pos] = val;
+            }
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < SORT_THREADS / 64; w++) { add += wave_hist[w][threadIdx.x]; wave_hist[w][threadIdx.x] = 0; }
+        running[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3c: tile segments of the sorted pairs
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles, uint32_t *tile_start,
+                                                  uint32_t *tile_end, uint32_t *seg_tiles, Counters *ctr)
+{
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t k = keys[p];
+    bool ok = k < ntiles;
+    bool first = ok && (p == 0 || keys[p - 1] != k);
+    // one atomic per wave for the segment starts it holds
+    const int lane = threadIdx.x & 63;
+    uint64_t starts = __ballot(first);
+    uint32_t base = 0;
+    if (starts) {
+        if (lane == __builtin_ctzll(starts)) base = atomicAdd(&ctr->nseg, (uint32_t)__popcll(starts));
+        base = __shfl(base, (int)__builtin_ctzll(starts), 64);
+    }
+    if (first) {
+        tile_start[k] = p;
+        uint32_t s = base + (uint32_t)__popcll(starts & ((1ull << lane) - 1ull));
+        if (s < ntiles) seg_tiles[s] = k;
+    }
+    if (ok && (p == n - 1 || keys[p + 1] != k)) tile_end[k] = p + 1;
+}
+
+__global__ void k_reset(Counters *ctr)
+{
+    if (threadIdx.x == 0) {
+        ctr->tri_raster = 0;
+        ctr->nseg = 0;
+        ctr->work = 0;
+        ctr->unsupported = 0;
+        ctr->nclip = 0;
+        ctr->noverflow = 0;
+        ctr->overflow_full = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4: fine raster and per-fragment pipeline
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int compare_op(uint32_t func, float a, float b)
+{
+    switch (func) {
+    case GL_NEVER: return 0;
+    case GL_LESS: return a < b;
+    case GL_EQUAL: return a == b;
+    case GL_LEQUAL: return a <= b;
+    case GL_GREATER: return a > b;
+    case GL_NOTEQUAL: return a != b;
+    case GL_GEQUAL: return a >= b;
+    default: return 1;
+    }
+}
+
+__device__ __forceinline__ int stencil_test(uint32_t func, int32_t ref, uint32_t mask, uint32_t s)
+{
+    int32_t r = (int32_t)((uint32_t)ref & mask);
+    int32_t v = (int32_t)(s & mask);
+    switch (func) {
+    case GL_NEVER: return 0;
+    case GL_LESS: return r < v;
+    case GL_EQUAL: return r == v;
+    case GL_LEQUAL: return r <= v;
+    case GL_GREATER: return r > v;
+    case GL_NOTEQUAL: return r != v;
+    case GL_GEQUAL: return r >= v;
+    default: return 1;
+    }
+}
+
+__device__ __forceinline__ uint32_t stencil_op(uint32_t op, uint32_t s, int32_t ref)
+{
+    switch (op) {
+    case GL_KEEP: return s;
+    case GL_ZERO: return 0;
+    case GL_REPLACE: return (uint32_t)ref & 0xFFu;
+    case GL_INCR: return s < 255 ? s + 1 : 255;
+    case GL_INCR_WRAP: return (s + 1) & 0xFFu;
+    case GL_DECR: return s > 0 ? s - 1 : 0;
+    case GL_DECR_WRAP: return (s - 1) & 0xFFu;
+    case GL_INVERT: return (~s) & 0xFFu;
+    default: return s;
+    }
+}
+
+__device__ __forceinline__ Col blend_factor(uint32_t f, Col s, Col d)
+{
+    switch (f) {
+    case GL_ZERO: return {0.0f, 0.0f, 0.0f, 0.0f};
+    case GL_ONE: return {1.0f, 1.0f, 1.0f, 1.0f};
+    case GL_SRC_COLOR: return s;
+    case GL_ONE_MINUS_SRC_COLOR: return {1 - s.r, 1 - s.g, 1 - s.b, 1 - s.a};
+    case GL_DST_COLOR: return d;
+    case GL_ONE_MINUS_DST_COLOR: return {1 - d.r, 1 - d.g, 1 - d.b, 1 - d.a};
+    case GL_SRC_ALPHA: return {s.a, s.a, s.a, s.a};
+    case GL_ONE_MINUS_SRC_ALPHA: return {1 - s.a, 1 - s.a, 1 - s.a, 1 - s.a};
+    case GL_DST_ALPHA: return {d.a, d.a, d.a, d.a};
+    case GL_ONE_MINUS_DST_ALPHA: return {1 - d.a, 1 - d.a, 1 - d.a, 1 - d.a};
+    case GL_SRC_ALPHA_SATURATE: {
+        float k = s.a < (1 - d.a) ? s.a : (1 - d.a);
+        return {k, k, k, 1.0f};
+    }
+    default: return {1.0f, 1.0f, 1.0f, 1.0f};
+    }
+}
+
+struct TexLevel {
+    const uint32_t *px;
+    int32_t w, h;
+    bool pow2;
+};
+
+__device__ __forceinline__ int32_t wrap_coord(int32_t x, int32_t w, bool repeat, bool pow2)
+{
+    if (repeat) {
+        if (pow2) return x & (w - 1);
+        return ((x % w) + w) % w;
+    }
+    return x < 0 ? 0 : (x >= w ? w - 1 : x);
+}
+
+// one level: bilinear with wrapped taps (reference bilinear_filter,
+// get_texel_wrapped) or nearest with a clamped index
+__device__ uint32_t sample_level(const TexLevel &L, bool rep_s, bool rep_t, float u, float v, bool linear, int &taps)
+{
+    float tx = u * (float)L.w - 0.5f;
+    float ty = v * (float)L.h - 0.5f;
+    if (linear) {
+        int32_t x0 = cvtt(floorf(tx));
+        int32_t y0 = cvtt(floorf(ty));
+        int32_t x1 = x0 + 1, y1 = y0 + 1;
+        float fx = tx - (float)x0;
+        float fy = ty - (float)y0;
+        int32_t wx0 = wrap_coord(x0, L.w, rep_s, L.pow2), wx1 = wrap_coord(x1, L.w, rep_s, L.pow2);
+        int32_t wy0 = wrap_coord(y0, L.h, rep_t, L.pow2), wy1 = wrap_coord(y1, L.h, rep_t, L.pow2);
+        uint32_t c00 = L.px[(size_t)wy0 * L.w + wx0];
+        uint32_t c10 = L.px[(size_t)wy0 * L.w + wx1];
+        uint32_t c01 = L.px[(size_t)wy1 * L.w + wx0];
+        uint32_t c11 = L.px[(size_t)wy1 * L.w + wx1];
+        taps += 4;
+        Col top = col_lerp(col_unpack(c00), col_unpack(c10), fx);
+        Col bot = col_lerp(col_unpack(c01), col_unpack(c11), fx);
+        return col_pack(col_lerp(top, bot, fy));
+    }
+    int32_t x = cvtt(floorf(tx + 0.5f));
+    int32_t y = cvtt(floorf(ty + 0.5f));
+    if (x < 0) x = 0;
+    if (x >= L.w) x = L.w - 1;
+    if (y < 0) y = 0;
+    if (y >= L.h) y = L.h - 1;
+    taps += 1;
+    return L.px[(size_t)y * L.w + x];
+}
+
+// reference texture_sample_lod (src/textures.c:457-557)
+__device__ uint32_t sample_texture(const mtgl_raster_state *rs, const TexLevel &L0, const TexLevel &L1, float u, float v,
+                                   float lod, int &taps)
+{
+    const bool rep_s = rs->wrap_s == GL_REPEAT, rep_t = rs->wrap_t == GL_REPEAT;
+    if (rep_s) { u = u - (float)cvtt(u); if (u < 0) u += 1.0f; }
+    else { if (u < 0.0f) u = 0.0f; if (u > 1.0f) u = 1.0f; }
+    if (rep_t) { v = v - (float)cvtt(v); if (v < 0) v += 1.0f; }
+    else { if (v < 0.0f) v = 0.0f; if (v > 1.0f) v = 1.0f; }
+    uint32_t filter = lod > 0.0f ? rs->min_filter : rs->mag_filter;
+    if (filter == GL_NEAREST_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_NEAREST) {
+        if (lod >= 0.5f) {
+            if (!L1.px) return 0xFFFFFFFFu;
+            return sample_level(L1, rep_s, rep_t, u, v, filter == GL_LINEAR_MIPMAP_NEAREST, taps);
+        }
+        filter = filter == GL_NEAREST_MIPMAP_NEAREST ? GL_NEAREST : GL_LINEAR;
+    } else if (filter == GL_NEAREST_MIPMAP_LINEAR || filter == GL_LINEAR_MIPMAP_LINEAR) {
+        if (lod > 0.0f) {
+            float cl = lod > 1.0f ? 1.0f : lod;
+            bool lin = filter == GL_LINEAR_MIPMAP_LINEAR;
+            uint32_t c0 = sample_level(L0, rep_s, rep_t, u, v, lin, taps);
+            uint32_t c1 = L1.px ? sample_level(L1, rep_s, rep_t, u, v, lin, taps) : 0xFFFFFFFFu;
+            return col_pack(col_lerp(col_unpack(c0), col_unpack(c1), cl));
+        }
+        filter = filter == GL_NEAREST_MIPMAP_LINEAR ? GL_NEAREST : GL_LINEAR;
+    }
+    return sample_level(L0, rep_s, rep_t, u, v, filter == GL_LINEAR, taps);
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// per-pixel state a lane carries through its tile
+struct PixelState {
+    uint32_t c;          // packed colour
+    float d;             // depth
+    uint32_t s;          // stencil
+    bool dirty_c, dirty_d, dirty_s;
+};
+
+struct RasterConsts {
+    bool depth_on, stencil_on, blend_on, fog_on, alpha_on, lighting_on, textured, persp, flat, phong, unit_range;
+    uint32_t cmask;
+    TexLevel L0, L1;
+};
+
+__device__ __forceinline__ void stencil_write(const mtgl_raster_state *rs, PixelState &px, uint32_t op)
+{
+    uint32_t nv = stencil_op(op, px.s, rs->stencil_ref);
+    uint32_t m8 = rs->stencil_writemask & 0xFFu;
+    px.s = ((px.s & ~m8) | (nv & m8)) & 0xFFu;
+    px.dirty_s = true;
+}
+
+// one fragment of triangle r at pixel (x, y) whose barycentrics are (b0, b1, b2):
+// the per-fragment pipeline of the reference in its order (src/raster.c:546-721)
+__device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, const RTri *r, uint32_t rid, float b0,
+                               float b1, float b2, PixelState &px, unsigned long long &bytes)
+{
+    const mtgl_raster_state *rs = P.rs;
+    const float z = b0 * r->z0 + b1 * r->z1 + b2 * r->z2;
+    float depth;
+    if (K.unit_range) depth = (z + 1.0f) * 0.5f;   // (double)q * 1.0 + 0.0 rounds back to q
+    else depth = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
+
+    if (K.stencil_on) {
+        bytes += 2;
+        if (!stencil_test(rs->stencil_func, rs->stencil_ref, rs->stencil_mask, px.s)) {
+            stencil_write(rs, px, rs->stencil_fail);
+            return;
+        }
+    }
+    if (K.depth_on) {
+        bytes += 4;
+        if (!compare_op(rs->depth_func, depth, px.d)) {
+            if (K.stencil_on) stencil_write(rs, px, rs->stencil_zfail);
+            return;
+        }
+    }
+    if (K.stencil_on) stencil_write(rs, px, rs->stencil_zpass);
+
+    Col c;
+    if (K.flat) {
+        c = {r->c2[0], r->c2[1], r->c2[2], r->c2[3]};
+    } else {
+        c.r = r->c0[0] * b0 + r->c1[0] * b1 + r->c2[0] * b2;
+        c.g = r->c0[1] * b0 + r->c1[1] * b1 + r->c2[1] * b2;
+        c.b = r->c0[2] * b0 + r->c1[2] * b1 + r->c2[2] * b2;
+        c.a = r->c0[3] * b0 + r->c1[3] * b1 + r->c2[3] * b2;
+    }
+    if (K.lighting_on) {
+        const bool two = r->back && rs->lighting.two_side;
+        if (K.phong || two) {
+            const RCold *q = &P.rcold[rid];
+            float ep[3], en[3];
+            for (int k = 0; k < 3; k++) {
+                ep[k] = q->ep[k] * b0 + q->ep[3 + k] * b1 + q->ep[6 + k] * b2;
+                en[k] = q->en[k] * b0 + q->en[3 + k] * b1 + q->en[6 + k] * b2;
+            }
+            const mtgl_material *m = &rs->lighting.front;
+            if (two) {
+                for (int k = 0; k < 3; k++) en[k] = en[k] * -1.0f;
+                m = &rs->lighting.back;
+            }
+            c = compute_lighting(&rs->lighting, rs->flags, ep, en, col_of(m->ambient), col_of(m->diffuse),
+                                 col_of(m->specular), col_of(m->emission), m->shininess);
+        }
+    }
+    if (K.textured) {
+        float u, v;
+        if (K.persp) {
+            const float u0w = r->uv0[0] * r->w0, v0w = r->uv0[1] * r->w0;
+            const float u1w = r->uv1[0] * r->w1, v1w = r->uv1[1] * r->w1;
+            const float u2w = r->uv2[0] * r->w2, v2w = r->uv2[1] * r->w2;
+            float uow = b0 * u0w + b1 * u1w + b2 * u2w;
+            float vow = b0 * v0w + b1 * v1w + b2 * v2w;
+            float oow = b0 * r->w0 + b1 * r->w1 + b2 * r->w2;
+            float w = 1.0f / oow;
+            u = uow * w;
+            v = vow * w;
+        } else {
+            u = b0 * r->uv0[0] + b1 * r->uv1[0] + b2 * r->uv2[0];
+            v = b0 * r->uv0[1] + b1 * r->uv1[1] + b2 * r->uv2[1];
+        }
+        int taps = 0;
+        Col tc = col_unpack(sample_texture(rs, K.L0, K.L1, u, v, r->lod, taps));
+        bytes += 4u * (unsigned)taps;
+        if (K.alpha_on && !compare_op(rs->alpha_func, tc.a, rs->alpha_ref)) return;
+        switch (rs->tex_env_mode) {
+        case GL_REPLACE: c = tc; break;
+        case GL_DECAL:
+            c = {c.r + (tc.r - c.r) * tc.a, c.g + (tc.g - c.g) * tc.a, c.b + (tc.b - c.b) * tc.a, c.a};
+            break;
+        case GL_BLEND: {
+            Col e = col_of(rs->tex_env_color);
+            c = {c.r * (1.0f - tc.r) + e.r * tc.r, c.g * (1.0f - tc.g) + e.g * tc.g, c.b * (1.0f - tc.b) + e.b * tc.b,
+                 c.a * tc.a};
+            break;
+        }
+        case GL_ADD: c = {c.r + tc.r, c.g + tc.g, c.b + tc.b, c.a * tc.a}; break;
+        default: c = col_mul(c, tc); break;
+        }
+    }
+    if (K.fog_on) {
+        float fc = b0 * r->ez0 + b1 * r->ez1 + b2 * r->ez2;
+        float f;
+        switch (rs->fog_mode) {
+        case GL_LINEAR: f = rs->fog_end != rs->fog_start ? (rs->fog_end - fc) / (rs->fog_end - rs->fog_start) : 1.0f; break;
+        case GL_EXP: f = exp_cr(-rs->fog_density * fc); break;
+        case GL_EXP2: { float d = rs->fog_density * fc; f = exp_cr(-d * d); break; }
+        default: f = 1.0f; break;
+        }
+        if (f < 0.0f) f = 0.0f;
+        if (f > 1.0f) f = 1.0f;
+        // color_lerp_rgb(fog_color, c, f): the alpha is the fog colour's
+        Col fg = col_of(rs->fog_color);
+        c = {fg.r + (c.r - fg.r) * f, fg.g + (c.g - fg.g) * f, fg.b + (c.b - fg.b) * f, fg.a};
+    }
+    if (K.depth_on && rs->depth_mask) {
+        bytes += 4;
+        px.d = depth;
+        px.dirty_d = true;
+    }
+    if (K.blend_on) {
+        Col dst = col_unpack(px.c);
+        Col sf = blend_factor(rs->blend_src, c, dst);
+        Col df = blend_factor(rs->blend_dst, c, dst);
+        c = col_clamp(col_add(col_mul(c, sf), col_mul(dst, df)));
+    }
+    c = col_clamp(c);
+    if (K.cmask == 0xFu) {
+        bytes += K.blend_on ? 8 : 4;
+        px.c = col_pack(c);
+        px.dirty_c = true;
+    } else if (K.cmask != 0u) {
+        bytes += 8;
+        Col d = col_unpack(px.c);
+        if (K.cmask & 1u) d.r = c.r;
+        if (K.cmask & 2u) d.g = c.g;
+        if (K.cmask & 4u) d.b = c.b;
+        if (K.cmask & 8u) d.a = c.a;
+        px.c = col_pack(d);
+        px.dirty_c = true;
+    }
+}
+
+// inside test of the reference's edge functions at integer pixel (x, y)
+// (src/raster.c:299-302, :534-540); returns the three edge values
+__device__ __forceinline__ bool covers(const RTri *r, int32_t x, int32_t y, float &w0e, float &w1e, float &w2e)
+{
+    if (!(x >= r->minX && x <= r->maxX && y >= r->minY && y <= r->maxY)) return false;
+    const float px = (float)x, py = (float)y;
+    const float fx0 = (float)r->x0, fy0 = (float)r->y0, fx1 = (float)r->x1, fy1 = (float)r->y1;
+    const float fx2 = (float)r->x2, fy2 = (float)r->y2;
+    const float area = r->area;
+    w0e = (px - fx1) * (fy2 - fy1) - (py - fy1) * (fx2 - fx1);
+    w1e = (px - fx2) * (fy0 - fy2) - (py - fy2) * (fx0 - fx2);
+    w2e = (px - fx0) * (fy1 - fy0) - (py - fy0) * (fx1 - fx0);
+    return (area > 0 && w0e >= 0 && w1e >= 0 && w2e >= 0) || (area < 0 && w0e <= 0 && w1e <= 0 && w2e <= 0);
+}
+
+constexpr int CHUNK = 64;   // bin entries per coverage pass: one bit each in a lane's mask
+constexpr int RASTER_WAVES = 4;
+
+// the fragment-stage part of a record, as staged in LDS
+struct __align__(16) RFrag {
+    float4 v[RT_FRAG_VEC4];
+};
+
+__global__ void __launch_bounds__(256) k_raster(RasterParams P)
+{
+    // per wave: the records of the chunk being resolved and their slots
+    __shared__ RFrag s_rec[RASTER_WAVES][CHUNK];
+    __shared__ uint32_t s_rid[RASTER_WAVES][CHUNK];
+
+    const mtgl_raster_state *rs = P.rs;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    RasterConsts K;
+    const uint32_t flags = rs->flags;
+    K.depth_on = flags & MTGL_F_DEPTH_TEST;
+    K.stencil_on = flags & MTGL_F_STENCIL_TEST;
+    K.blend_on = flags & MTGL_F_BLEND;
+    K.fog_on = flags & MTGL_F_FOG;
+    K.alpha_on = flags & MTGL_F_ALPHA_TEST;
+    K.lighting_on = flags & MTGL_F_LIGHTING;
+    K.cmask = rs->color_mask & 0xFu;
+    K.textured = P.tex != nullptr;
+    K.persp = rs->perspective_hint != GL_FASTEST;
+    K.flat = rs->shade_model == GL_FLAT;
+    K.phong = rs->shade_model == GL_PHONG;
+    K.unit_range = rs->depth_near == 0.0 && rs->depth_far == 1.0;
+    K.L0 = {P.tex, P.tex_w, P.tex_h, P.tex_w > 0 && (P.tex_w & (P.tex_w - 1)) == 0 && (P.tex_h & (P.tex_h - 1)) == 0};
+    K.L1 = {P.tex1, P.tex1_w, P.tex1_h,
+            P.tex1_w > 0 && (P.tex1_w & (P.tex1_w - 1)) == 0 && (P.tex1_h & (P.tex1_h - 1)) == 0};
+    const bool need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
+    const uint32_t nseg = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nseg);
+    RFrag *recs = s_rec[wave];
+    uint32_t *rids = s_rid[wave];
+
+    unsigned long long frag_count = 0, frag_bytes = 0;
+    for (;;) {
+        uint32_t s = 0;
+        if (lane == 0) s = atomicAdd(&P.ctr->work, 1u);
+        s = __builtin_amdgcn_readfirstlane(s);
+        if (s >= nseg) break;
+        const uint32_t tile = P.seg_tiles[s];
+        const uint32_t e0 = P.tile_start[tile], e1 = P.tile_end[tile];
+        const int32_t tx = (int32_t)(tile % (uint32_t)P.tiles_x), ty = (int32_t)(tile / (uint32_t)P.tiles_x);
+        const int32_t x = tx * TILE + (lane & 7), y = ty * TILE + (lane >> 3);
+        const bool in_fb = x < P.fb_w && y < P.fb_h;
+        const size_t pix = in_fb ? (size_t)y * P.fb_w + x : 0;
+        PixelState px = {0u, 1.0f, 0u, false, false, false};
+        if (in_fb) {
+            if (need_color) px.c = P.color[pix];
+            if (K.depth_on) px.d = P.depth[pix];
+            if (K.stencil_on) px.s = P.stencil[pix];
+        }
+        for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
+            const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
+            // stage the chunk: lane k copies bin entry k's record into LDS
+            __builtin_amdgcn_wave_barrier();
+            if ((uint32_t)lane < cn) {
+                const uint32_t rid = P.vals[c0 + lane];
+                const float4 *src = reinterpret_cast<const float4 *>(&P.rt[rid]);
+#pragma unroll
+                for (int q = 0; q < RT_FRAG_VEC4; q++) recs[lane].v[q] = src[q];
+                rids[lane] = rid;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            // coverage pass: the wave walks the chunk in submission order and
+            // each lane records which triangles cover its pixel
+            uint64_t mask = 0;
+            for (uint32_t k = 0; k < cn; k++) {
+                const RTri *r = reinterpret_cast<const RTri *>(&recs[k]);
+                float w0e, w1e, w2e;
+                if (in_fb && covers(r, x, y, w0e, w1e, w2e)) mask |= 1ull << k;
+            }
+            frag_count += (unsigned long long)__popcll(mask);
+            // shading pass: every lane visits only its own fragments, lowest
+            // bit (earliest triangle) first
+            while (mask) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                const RTri *r = reinterpret_cast<const RTri *>(&recs[k]);
+                float w0e, w1e, w2e;
+                covers(r, x, y, w0e, w1e, w2e);
+                const float inv = r->inv_area;
+                shade_fragment(P, K, r, rids[k], w0e * inv, w1e * inv, w2e * inv, px, frag_bytes);
+            }
+        }
+        if (in_fb) {
+            if (px.dirty_c) P.color[pix] = px.c;
+            if (px.dirty_d) P.depth[pix] = px.d;
+            if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
+        }
+    }
+    frag_count = wave_sum_u64(frag_count);
+    frag_bytes = wave_sum_u64(frag_bytes);
+    if (lane == 0 && frag_count) {
+        atomicAdd(&P.ctr->fragments, frag_count);
+        atomicAdd(&P.ctr->frag_bytes, frag_bytes);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K5 clear, K6 level-1 box filter
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_clear(uint32_t *color, float *depth, uint8_t *stencil, int32_t fb_w,
+                                               int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t cv, float dv,
+                                               uint8_t sv)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)w * (uint64_t)h) return;
+    int32_t y = y0 + (int32_t)(i / (uint64_t)w), x = x0 + (int32_t)(i % (uint64_t)w);
+    size_t p = (size_t)y * fb_w + x;
+    if (color) color[p] = cv;
+    if (depth) depth[p] = dv;
+    if (stencil) stencil[p] = sv;
+}
+
+__global__ void __launch_bounds__(256) k_mip1(const uint32_t *src, int32_t w, uint32_t *dst, int32_t mw, int32_t mh)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mw * mh) return;
+    int32_t y = i / mw, x = i % mw;
+    size_t s = (size_t)(2 * y) * w + (size_t)(2 * x);
+    Col k = col_unpack(src[s]);
+    k = col_add(k, col_unpack(src[s + 1]));
+    k = col_add(k, col_unpack(src[s + w]));
+    k = col_add(k, col_unpack(src[s + w + 1]));
+    dst[i] = col_pack(col_scale(k, 0.25f));
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+namespace {
+
+struct DevTex {
+    int32_t w = 0, h = 0, m1w = 0, m1h = 0;
+    uint32_t *px = nullptr, *m1 = nullptr;
+};
+
+struct DevBuf {
+    uint8_t *data = nullptr;
+    int64_t size = 0;
+};
+
+// grow-only device allocation
+struct Arena {
+    void *ptr = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct mtgl_dev {
+    int device = 0;
+    int32_t w = 0, h = 0, tiles_x = 0, tiles_y = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *color = nullptr;
+    float *depth = nullptr;
+    uint8_t *stencil = nullptr;
+    DevTex tex[MTGL_MAX_OBJECTS + 1];
+    DevBuf buf[MTGL_MAX_OBJECTS + 1];
+    Counters *ctr = nullptr;           // device
+    Counters *ctr_host = nullptr;      // pinned mirror for readback
+    unsigned long long *pairs_host = nullptr;
+    uint8_t *pinned = nullptr;         // staging for per-draw uploads
+    size_t pinned_cap = 0;
+    hipEvent_t upload_done = nullptr;
+    Arena params, vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
+    uint32_t overflow_cap = 0;
+    Arena keys[2], vals[2], sort_counts, tile_start, tile_end, seg_tiles;
+    mtgl_dev_stats stats = {};
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
+    std::vector<hipEvent_t> free_events;
+};
+
+namespace {
+
+int map_err(hipError_t e)
+{
+    if (e == hipSuccess) return MTGL_OK;
+    if (e == hipErrorOutOfMemory) return MTGL_ERR_OUT_OF_MEMORY;
+    return MTGL_ERR_DEVICE;
+}
+
+#define HIPC(expr)                                                                                  \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            fprintf(stderr, "libmtgl: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                                      \
+            return map_err(e_);                                                                     \
+        }                                                                                           \
+    } while (0)
+
+int arena_reserve(mtgl_dev *d, Arena &a, size_t bytes)
+{
+    if (bytes <= a.cap) return MTGL_OK;
+    size_t cap = a.cap ? a.cap : 4096;
+    while (cap < bytes) cap *= 2;
+    HIPC(hipStreamSynchronize(d->stream));
+    if (a.ptr) HIPC(hipFree(a.ptr));
+    a.ptr = nullptr;
+    a.cap = 0;
+    HIPC(hipMalloc(&a.ptr, cap));
+    a.cap = cap;
+    return MTGL_OK;
+}
+
+template <typename T>
+T *as(Arena &a) { return reinterpret_cast<T *>(a.ptr); }
+
+int pinned_reserve(mtgl_dev *d, size_t bytes)
+{
+    if (d->upload_done) HIPC(hipEventSynchronize(d->upload_done));
+    if (bytes <= d->pinned_cap) return MTGL_OK;
+    size_t cap = d->pinned_cap ? d->pinned_cap : (1u << 20);
+    while (cap < bytes) cap *= 2;
+    if (d->pinned) HIPC(hipHostFree(d->pinned));
+    d->pinned = nullptr;
+    d->pinned_cap = 0;
+    HIPC(hipHostMalloc((void **)&d->pinned, cap, hipHostMallocDefault));
+    d->pinned_cap = cap;
+    return MTGL_OK;
+}
+
+unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// exclusive scan of n values of T (in may equal out); *total_dev receives the sum
+template <typename T>
+int device_scan(mtgl_dev *d, const T *in, T *out, uint64_t n, T *total_dev)
+{
+    if (n == 0) return MTGL_OK;
+    uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int rc = arena_reserve(d, d->scan_tmp, nb * sizeof(T));
+    if (rc) return rc;
+    T *sums = as<T>(d->scan_tmp);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, d->stream, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, d->stream, sums, nb, total_dev);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, d->stream, in, n,
+                       (const T *)sums, out);
+    HIPC(hipGetLastError());
+    return MTGL_OK;
+}
+
+// resolve an mtgl_array to a device stream
+DArr resolve(mtgl_dev *d, const mtgl_array &a, const uint8_t *staged_dev, uint64_t staged_bytes)
+{
+    DArr r = {};
+    r.enabled = a.enabled;
+    r.comps = a.size;
+    r.type = a.type;
+    r.stride = a.stride;
+    r.offset = a.offset;
+    if (!a.enabled) return r;
+    if (a.buffer == 0) {
+        r.src = staged_dev;
+        r.size = (int64_t)staged_bytes;
+    } else if (a.buffer <= MTGL_MAX_OBJECTS && d->buf[a.buffer].data) {
+        r.src = d->buf[a.buffer].data;
+        r.size = d->buf[a.buffer].size;
+    }
+    return r;
+}
+
+hipEvent_t take_event(mtgl_dev *d)
+{
+    if (!d->free_events.empty()) {
+        hipEvent_t e = d->free_events.back();
+        d->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int resolve_timing(mtgl_dev *d)
+{
+    for (auto &pr : d->pending_events) {
+        HIPC(hipEventSynchronize(pr.second));
+        float ms = 0.0f;
+        HIPC(hipEventElapsedTime(&ms, pr.first, pr.second));
+        d->stats.raster_ms += ms;
+        d->stats.raster_launches++;
+        d->free_events.push_back(pr.first);
+        d->free_events.push_back(pr.second);
+    }
+    d->pending_events.clear();
+    return MTGL_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+
+MTGL_API const char *mtgl_dev_backend(void)
+{
+    return "hip:gfx950 (K1 vertex, K2 setup, K3 binner, K4 tile raster)";
+}
+
+MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
+{
+    if (!out || width <= 0 || height <= 0 || width > MTGL_MAX_FB_DIM || height > MTGL_MAX_FB_DIM) return MTGL_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        fprintf(stderr, "libmtgl: no HIP device available\n");
+        return MTGL_ERR_DEVICE;
+    }
+    mtgl_dev *d = new (std::nothrow) mtgl_dev();
+    if (!d) return MTGL_ERR_OUT_OF_MEMORY;
+    *out = nullptr;
+    int rc = MTGL_OK;
+    do {
+        if (hipGetDevice(&d->device) != hipSuccess) { rc = MTGL_ERR_DEVICE; break; }
+        d->w = width;
+        d->h = height;
+        d->tiles_x = (width + TILE - 1) / TILE;
+        d->tiles_y = (height + TILE - 1) / TILE;
+        size_t n = (size_t)width * (size_t)height;
+        if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) { rc = MTGL_ERR_DEVICE; break; }
+        if ((rc = map_err(hipMalloc((void **)&d->color, n * 4))) != MTGL_OK) break;
+        if ((rc = map_err(hipMalloc((void **)&d->depth, n * 4))) != MTGL_OK) break;
+        if ((rc = map_err(hipMalloc((void **)&d->stencil, n))) != MTGL_OK) break;
+        if ((rc = map_err(hipMalloc((void **)&d->ctr, sizeof(Counters)))) != MTGL_OK) break;
+        if ((rc = map_err(hipHostMalloc((void **)&d->ctr_host, sizeof(Counters), hipHostMallocDefault))) != MTGL_OK) break;
+        if ((rc = map_err(hipHostMalloc((void **)&d->pairs_host, sizeof(unsigned long long), hipHostMallocDefault))) != MTGL_OK) break;
+        if ((rc = map_err(hipEventCreateWithFlags(&d->upload_done, hipEventDisableTiming))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->color, 0, n * 4, d->stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->depth, 0, n * 4, d->stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->stencil, 0, n, d->stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipStreamSynchronize(d->stream))) != MTGL_OK) break;
+    } while (0);
+    if (rc != MTGL_OK) {
+        mtgl_dev_destroy(d);
+        return rc;
+    }
+    *out = d;
+    return MTGL_OK;
+}
+
+MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
+{
+    if (!d) return;
+    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    for (int i = 0; i <= MTGL_MAX_OBJECTS; i++) {
+        if (d->tex[i].px) (void)hipFree(d->tex[i].px);
+        if (d->tex[i].m1) (void)hipFree(d->tex[i].m1);
+        if (d->buf[i].data) (void)hipFree(d->buf[i].data);
+    }
+    Arena *arenas[] = {&d->params, &d->vhot, &d->vcold, &d->rt, &d->rcold, &d->n_rt, &d->rec_base, &d->tri_tiles, &d->pair_base,
+                       &d->scan_tmp, &d->clip_list, &d->keys[0], &d->keys[1], &d->vals[0], &d->vals[1], &d->sort_counts,
+                       &d->tile_start, &d->tile_end, &d->seg_tiles};
+    for (Arena *a : arenas) if (a->ptr) (void)hipFree(a->ptr);
+    for (auto &pr : d->pending_events) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+    for (hipEvent_t e : d->free_events) (void)hipEventDestroy(e);
+    if (d->color) (void)hipFree(d->color);
+    if (d->depth) (void)hipFree(d->depth);
+    if (d->stencil) (void)hipFree(d->stencil);
+    if (d->ctr) (void)hipFree(d->ctr);
+    if (d->ctr_host) (void)hipHostFree(d->ctr_host);
+    if (d->pairs_host) (void)hipHostFree(d->pairs_host);
+    if (d->pinned) (void)hipHostFree(d->pinned);
+    if (d->upload_done) (void)hipEventDestroy(d->upload_done);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+MTGL_API int mtgl_dev_finish(mtgl_dev *d)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(d->stream));
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1, uint32_t color,
+                            float depth, uint8_t stencil)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > d->w) x1 = d->w;
+    if (y1 > d->h) y1 = d->h;
+    if (x0 >= x1 || y0 >= y1 || !(buffers & (MTGL_BUF_COLOR | MTGL_BUF_DEPTH | MTGL_BUF_STENCIL))) return MTGL_OK;
+    int32_t w = x1 - x0, h = y1 - y0;
+    uint64_t n = (uint64_t)w * (uint64_t)h;
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream,
+                       (buffers & MTGL_BUF_COLOR) ? d->color : nullptr, (buffers & MTGL_BUF_DEPTH) ? d->depth : nullptr,
+                       (buffers & MTGL_BUF_STENCIL) ? d->stencil : nullptr, d->w, x0, y0, w, h, color, depth, stencil);
+    HIPC(hipGetLastError());
+    return MTGL_OK;
+}
+
+static int plane_of(mtgl_dev *d, uint32_t which, uint8_t **base, size_t *esz)
+{
+    if (which == MTGL_BUF_COLOR) { *base = (uint8_t *)d->color; *esz = 4; return 1; }
+    if (which == MTGL_BUF_DEPTH) { *base = (uint8_t *)d->depth; *esz = 4; return 1; }
+    if (which == MTGL_BUF_STENCIL) { *base = d->stencil; *esz = 1; return 1; }
+    return 0;
+}
+
+MTGL_API int mtgl_dev_read(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, int32_t w, int32_t h, void *dst)
+{
+    uint8_t *base;
+    size_t esz;
+    if (!d || !dst || !plane_of(d, which, &base, &esz)) return MTGL_ERR_INVALID;
+    if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
+    if (w == 0 || h == 0) return MTGL_OK;
+    HIPC(hipStreamSynchronize(d->stream));
+    HIPC(hipMemcpy2D(dst, (size_t)w * esz, base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, (size_t)w * esz,
+                     (size_t)h, hipMemcpyDeviceToHost));
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_write(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, int32_t w, int32_t h, const void *src)
+{
+    uint8_t *base;
+    size_t esz;
+    if (!d || !src || !plane_of(d, which, &base, &esz)) return MTGL_ERR_INVALID;
+    if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
+    if (w == 0 || h == 0) return MTGL_OK;
+    HIPC(hipStreamSynchronize(d->stream));
+    HIPC(hipMemcpy2D(base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, src, (size_t)w * esz, (size_t)w * esz,
+                     (size_t)h, hipMemcpyHostToDevice));
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, const uint32_t *texels)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS || w <= 0 || h <= 0 || w > MTGL_MAX_TEX_SIZE || h > MTGL_MAX_TEX_SIZE ||
+        !texels)
+        return MTGL_ERR_INVALID;
+    size_t n = (size_t)w * (size_t)h;
+    uint32_t *px = nullptr, *m1 = nullptr;
+    int32_t mw = 0, mh = 0;
+    HIPC(hipMalloc((void **)&px, n * 4));
+    if (w >= 2 && h >= 2) {
+        mw = w / 2;
+        mh = h / 2;
+        hipError_t e = hipMalloc((void **)&m1, (size_t)mw * mh * 4);
+        if (e != hipSuccess) { (void)hipFree(px); return map_err(e); }
+    }
+    HIPC(hipMemcpy(px, texels, n * 4, hipMemcpyHostToDevice));
+    if (m1) {
+        hipLaunchKernelGGL(k_mip1, dim3(blocks_for((uint64_t)mw * mh, 256)), dim3(256), 0, d->stream, px, w, m1, mw, mh);
+        HIPC(hipGetLastError());
+    }
+    HIPC(hipStreamSynchronize(d->stream));
+    DevTex &t = d->tex[id];
+    if (t.px) HIPC(hipFree(t.px));
+    if (t.m1) HIPC(hipFree(t.m1));
+    t.px = px; t.w = w; t.h = h;
+    t.m1 = m1; t.m1w = mw; t.m1h = mh;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(d->stream));
+    DevTex &t = d->tex[id];
+    if (t.px) HIPC(hipFree(t.px));
+    if (t.m1) HIPC(hipFree(t.m1));
+    t = DevTex();
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_buffer_data(mtgl_dev *d, uint32_t id, int64_t size, const void *data)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(d->stream));
+    DevBuf &b = d->buf[id];
+    if (size <= 0) {
+        if (b.data) HIPC(hipFree(b.data));
+        b = DevBuf();
+        return MTGL_OK;
+    }
+    uint8_t *p = nullptr;
+    HIPC(hipMalloc((void **)&p, (size_t)size));
+    hipError_t e = data ? hipMemcpy(p, data, (size_t)size, hipMemcpyHostToDevice) : hipMemset(p, 0, (size_t)size);
+    if (e != hipSuccess) { (void)hipFree(p); return map_err(e); }
+    if (b.data) HIPC(hipFree(b.data));
+    b.data = p;
+    b.size = size;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_buffer_sub_data(mtgl_dev *d, uint32_t id, int64_t offset, int64_t size, const void *data)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS || !data || offset < 0 || size < 0) return MTGL_ERR_INVALID;
+    DevBuf &b = d->buf[id];
+    if (!b.data || offset + size > b.size) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(d->stream));
+    HIPC(hipMemcpy(b.data + offset, data, (size_t)size, hipMemcpyHostToDevice));
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
+{
+    if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(d->stream));
+    DevBuf &b = d->buf[id];
+    if (b.data) HIPC(hipFree(b.data));
+    b = DevBuf();
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
+{
+    if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
+    switch (dr->mode) {
+    case GL_TRIANGLES: case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_QUADS: case GL_QUAD_STRIP: case GL_POLYGON:
+        break;
+    default:
+        return MTGL_ERR_UNSUPPORTED;
+    }
+    d->stats.draws++;
+    const uint32_t count = dr->count;
+    const uint32_t ntri = tri_count(dr->mode, count);
+    if (ntri == 0) return MTGL_OK;
+    d->stats.triangles_in += ntri;
+    const mtgl_raster_state *rsh = dr->raster;
+    const bool need_cold = (rsh->flags & MTGL_F_LIGHTING) && (rsh->shade_model == GL_PHONG || rsh->lighting.two_side);
+
+    // ---- upload raster state, vertex states and staged bytes in one copy
+    const size_t rs_bytes = (sizeof(mtgl_raster_state) + 255) & ~(size_t)255;
+    const size_t vs_bytes = ((size_t)dr->n_vstates * sizeof(mtgl_vertex_state) + 255) & ~(size_t)255;
+    const size_t st_bytes = (size_t)dr->staged_bytes;
+    const size_t total = rs_bytes + vs_bytes + ((st_bytes + 255) & ~(size_t)255);
+    int rc;
+    if ((rc = pinned_reserve(d, total))) return rc;
+    if ((rc = arena_reserve(d, d->params, total))) return rc;
+    memcpy(d->pinned, rsh, sizeof(mtgl_raster_state));
+    memcpy(d->pinned + rs_bytes, dr->vstates, (size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+    if (st_bytes) memcpy(d->pinned + rs_bytes + vs_bytes, dr->staged, st_bytes);
+    HIPC(hipMemcpyAsync(d->params.ptr, d->pinned, total, hipMemcpyHostToDevice, d->stream));
+    HIPC(hipEventRecord(d->upload_done, d->stream));
+    const uint8_t *base = (const uint8_t *)d->params.ptr;
+    const mtgl_raster_state *rs_dev = (const mtgl_raster_state *)base;
+    const mtgl_vertex_state *vs_dev = (const mtgl_vertex_state *)(base + rs_bytes);
+    const uint8_t *staged_dev = base + rs_bytes + vs_bytes;
+
+    // ---- buffers
+    if ((rc = arena_reserve(d, d->vhot, (size_t)count * sizeof(VHot)))) return rc;
+    if (need_cold && (rc = arena_reserve(d, d->vcold, (size_t)count * sizeof(VCold)))) return rc;
+    // record pool: one slot per triangle plus an overflow pool for clipped ones
+    if (d->overflow_cap < 4096u) d->overflow_cap = 4096u;
+    uint32_t want_overflow = std::max<uint32_t>(d->overflow_cap, std::min<uint32_t>(ntri / 8u, 1u << 24));
+    d->overflow_cap = want_overflow;
+    if ((rc = arena_reserve(d, d->rt, ((size_t)ntri + d->overflow_cap) * sizeof(RTri)))) return rc;
+    if (need_cold && (rc = arena_reserve(d, d->rcold, ((size_t)ntri + d->overflow_cap) * sizeof(RCold)))) return rc;
+    if ((rc = arena_reserve(d, d->n_rt, (size_t)ntri * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(d, d->rec_base, (size_t)ntri * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(d, d->tri_tiles, (size_t)(ntri + 1) * sizeof(unsigned long long)))) return rc;
+    if ((rc = arena_reserve(d, d->pair_base, (size_t)(ntri + 1) * sizeof(unsigned long long)))) return rc;
+    if ((rc = arena_reserve(d, d->clip_list, (size_t)ntri * sizeof(uint32_t)))) return rc;
+
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, d->ctr);
+
+    // ---- K1
+    VertexParams vp = {};
+    vp.count = count;
+    vp.first = dr->first;
+    vp.index_type = dr->index_type;
+    vp.n_vstates = dr->n_vstates;
+    vp.indices = resolve(d, dr->indices, staged_dev, st_bytes);
+    vp.indices.enabled = dr->index_type != 0;
+    vp.position = resolve(d, dr->position, staged_dev, st_bytes);
+    vp.color = resolve(d, dr->color, staged_dev, st_bytes);
+    vp.texcoord = resolve(d, dr->texcoord, staged_dev, st_bytes);
+    vp.normal = resolve(d, dr->normal, staged_dev, st_bytes);
+    vp.state_index = resolve(d, dr->state_index, staged_dev, st_bytes);
+    memcpy(vp.cur_color, dr->cur_color, sizeof(vp.cur_color));
+    memcpy(vp.cur_texcoord, dr->cur_texcoord, sizeof(vp.cur_texcoord));
+    memcpy(vp.cur_normal, dr->cur_normal, sizeof(vp.cur_normal));
+    vp.vstates = vs_dev;
+    vp.out_hot = as<VHot>(d->vhot);
+    vp.out_cold = need_cold ? as<VCold>(d->vcold) : nullptr;
+    hipLaunchKernelGGL(k_vertex, dim3(blocks_for(count, 256)), dim3(256), 0, d->stream, vp);
+
+    // ---- K2
+    const DevTex *tx = (rsh->texture && rsh->texture <= MTGL_MAX_OBJECTS && d->tex[rsh->texture].px) ? &d->tex[rsh->texture]
+                                                                                                    : nullptr;
+    SetupParams sp = {};
+    sp.mode = dr->mode;
+    sp.count = count;
+    sp.ntri = ntri;
+    sp.fb_w = d->w;
+    sp.fb_h = d->h;
+    sp.tiles_x = d->tiles_x;
+    sp.tex_w = tx ? tx->w : 0;
+    sp.tex_h = tx ? tx->h : 0;
+    sp.rs = rs_dev;
+    sp.vhot = as<VHot>(d->vhot);
+    sp.vcold = need_cold ? as<VCold>(d->vcold) : nullptr;
+    sp.rt = as<RTri>(d->rt);
+    sp.rcold = need_cold ? as<RCold>(d->rcold) : nullptr;
+    sp.n_rt = as<uint32_t>(d->n_rt);
+    sp.rec_base = as<uint32_t>(d->rec_base);
+    sp.tri_tiles = as<unsigned long long>(d->tri_tiles);
+    sp.clip_list = as<uint32_t>(d->clip_list);
+    sp.overflow_cap = d->overflow_cap;
+    sp.ctr = d->ctr;
+    hipLaunchKernelGGL(k_setup, dim3(blocks_for(ntri, 256)), dim3(256), 0, d->stream, sp);
+    hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ntri, 64)), dim3(64), 0, d->stream, sp);
+    HIPC(hipGetLastError());
+
+    // ---- K3: pair offsets, total, then expand / sort / segment
+    unsigned long long *pair_base = as<unsigned long long>(d->pair_base);
+    if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, ntri, pair_base + ntri))) return rc;
+    HIPC(hipMemcpyAsync(d->pairs_host, pair_base + ntri, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
+    HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+    HIPC(hipStreamSynchronize(d->stream));
+    if (d->ctr_host->overflow_full) {
+        // the clipped triangles asked for more slots than the pool held: grow it
+        // to the total they requested and set them up again (nothing downstream ran)
+        uint32_t need = d->ctr_host->noverflow;
+        d->overflow_cap = need;
+        sp.overflow_cap = need;
+        if ((rc = arena_reserve(d, d->rt, ((size_t)ntri + need) * sizeof(RTri)))) return rc;
+        if (need_cold && (rc = arena_reserve(d, d->rcold, ((size_t)ntri + need) * sizeof(RCold)))) return rc;
+        sp.rt = as<RTri>(d->rt);
+        sp.rcold = need_cold ? as<RCold>(d->rcold) : nullptr;
+        // the fast path wrote its records into the old pool when it was reallocated
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, d->ctr);
+        hipLaunchKernelGGL(k_setup, dim3(blocks_for(ntri, 256)), dim3(256), 0, d->stream, sp);
+        hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ntri, 64)), dim3(64), 0, d->stream, sp);
+        HIPC(hipGetLastError());
+        if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, ntri, pair_base + ntri))) return rc;
+        HIPC(hipMemcpyAsync(d->pairs_host, pair_base + ntri, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
+        HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+        HIPC(hipStreamSynchronize(d->stream));
+        if (d->ctr_host->overflow_full) return MTGL_ERR_DEVICE;
+    }
+    const uint64_t npairs = *d->pairs_host;
+    const int unsupported = d->ctr_host->unsupported != 0;
+    d->stats.triangles_raster += d->ctr_host->tri_raster;
+    if (npairs == 0) return unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+    if (npairs > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    d->stats.bin_pairs += npairs;
+    const uint32_t n = (uint32_t)npairs;
+    for (int k = 0; k < 2; k++) {
+        if ((rc = arena_reserve(d, d->keys[k], (size_t)n * 4))) return rc;
+        if ((rc = arena_reserve(d, d->vals[k], (size_t)n * 4))) return rc;
+    }
+    const uint32_t ntiles = (uint32_t)(d->tiles_x * d->tiles_y);
+    if ((rc = arena_reserve(d, d->tile_start, (size_t)ntiles * 4))) return rc;
+    if ((rc = arena_reserve(d, d->tile_end, (size_t)ntiles * 4))) return rc;
+    if ((rc = arena_reserve(d, d->seg_tiles, (size_t)ntiles * 4))) return rc;
+    uint32_t *keys = as<uint32_t>(d->keys[0]), *vals = as<uint32_t>(d->vals[0]);
+    uint32_t *keys2 = as<uint32_t>(d->keys[1]), *vals2 = as<uint32_t>(d->vals[1]);
+    hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const unsigned long long *)pair_base,
+                       ntri, (const uint32_t *)sp.n_rt, (const uint32_t *)sp.rec_base, (const RTri *)sp.rt, (uint64_t)n,
+                       d->tiles_x, keys, vals);
+    int key_bits = 1;
+    while ((1u << key_bits) < ntiles) key_bits++;
+    const uint32_t nblocks = (n + SORT_TILE - 1) / SORT_TILE;
+    if ((rc = arena_reserve(d, d->sort_counts, (size_t)RADIX * nblocks * 4))) return rc;
+    uint32_t *counts = as<uint32_t>(d->sort_counts);
+    for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, d->stream, (const uint32_t *)keys, n, shift,
+                           nblocks, counts);
+        if ((rc = device_scan<uint32_t>(d, counts, counts, (uint64_t)RADIX * nblocks, nullptr))) return rc;
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, d->stream, (const uint32_t *)keys,
+                           (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
+        uint32_t *tk = keys; keys = keys2; keys2 = tk;
+        uint32_t *tv = vals; vals = vals2; vals2 = tv;
+    }
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const uint32_t *)keys, n, ntiles,
+                       as<uint32_t>(d->tile_start), as<uint32_t>(d->tile_end), as<uint32_t>(d->seg_tiles), d->ctr);
+    HIPC(hipGetLastError());
+
+    // ---- K4
+    RasterParams rp = {};
+    rp.rs = rs_dev;
+    rp.rt = sp.rt;
+    rp.rcold = sp.rcold;
+    rp.vals = vals;
+    rp.seg_tiles = as<uint32_t>(d->seg_tiles);
+    rp.tile_start = as<uint32_t>(d->tile_start);
+    rp.tile_end = as<uint32_t>(d->tile_end);
+    rp.color = d->color;
+    rp.depth = d->depth;
+    rp.stencil = d->stencil;
+    rp.fb_w = d->w;
+    rp.fb_h = d->h;
+    rp.tiles_x = d->tiles_x;
+    rp.tex = tx ? tx->px : nullptr;
+    rp.tex1 = tx ? tx->m1 : nullptr;
+    rp.tex_w = tx ? tx->w : 0;
+    rp.tex_h = tx ? tx->h : 0;
+    rp.tex1_w = tx ? tx->m1w : 0;
+    rp.tex1_h = tx ? tx->m1h : 0;
+    rp.ctr = d->ctr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (d->timing) {
+        ev0 = take_event(d);
+        ev1 = take_event(d);
+        if (ev0) HIPC(hipEventRecord(ev0, d->stream));
+    }
+    const unsigned nwaves = std::min<uint32_t>(n, 256u * 16u);
+    hipLaunchKernelGGL(k_raster, dim3((nwaves + 3) / 4), dim3(256), 0, d->stream, rp);
+    HIPC(hipGetLastError());
+    if (ev0 && ev1) {
+        HIPC(hipEventRecord(ev1, d->stream));
+        d->pending_events.emplace_back(ev0, ev1);
+    }
+    return unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int enable)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    d->timing = enable != 0;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_stats_get(mtgl_dev *d, mtgl_dev_stats *out)
+{
+    if (!d || !out) return MTGL_ERR_INVALID;
+    HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+    HIPC(hipStreamSynchronize(d->stream));
+    int rc = resolve_timing(d);
+    if (rc) return rc;
+    *out = d->stats;
+    out->fragments = d->ctr_host->fragments;
+    out->frag_bytes = d->ctr_host->frag_bytes;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_stats_reset(mtgl_dev *d)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(d->stream));
+    int rc = resolve_timing(d);
+    if (rc) return rc;
+    d->stats = mtgl_dev_stats();
+    HIPC(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
+    HIPC(hipStreamSynchronize(d->stream));
+    return MTGL_OK;
+}

@@ -754,6 +754,10 @@ def scene_overdraw_small(g, W, H, seed):
     draw_overdraw(g, setup_overdraw(g, W, H, seed, ntri=3000, max_log_area=3.0))
 
 
+# scenes whose reference behaviour is undefined (reads outside a buffer); they
+# are compared device against oracle only
+UNDEFINED_IN_REFERENCE = ("vbo_oob",)
+
 FEATURE_SCENES = {
     "depth_funcs": scene_depth_funcs,
     "blend": scene_blend,
@@ -766,6 +770,7 @@ FEATURE_SCENES = {
     "primitives": scene_primitives,
     "arrays": scene_arrays,
     "vbo": scene_vbo,
+    "vbo_oob": scene_vbo_oob,
     "lists": scene_lists,
     "scissor_viewport": scene_scissor_viewport,
     "colormask": scene_colormask,

@@ -687,6 +687,10 @@ static void or_raster(mtgl_dev *dev, const mtgl_raster_state *rs, const or_rv *v
             float e2 = or_edge(fx0, fy0, fx1, fy1, px, py);
             if (!((area > 0 && e0 >= 0 && e1 >= 0 && e2 >= 0) || (area < 0 && e0 <= 0 && e1 <= 0 && e2 <= 0)))
                 continue;
+            /* a pixel outside the framebuffer (viewport larger than it) reads the
+             * defaults and every write to it is dropped (src/framebuffer.h:92-134):
+             * it has no observable effect and is not counted */
+            if (!or_in_fb(dev, x, y)) continue;
             dev->stats.fragments++;
             float b0 = e0 * inv_area, b1 = e1 * inv_area, b2 = e2 * inv_area;
             float z = b0 * v0->z + b1 * v1->z + b2 * v2->z;
