@@ -56,6 +56,7 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr uint32_t HEAVY_BIN = 96;      // tiles with longer bins are rasterised first
 
 // ---------------------------------------------------------------------------
 // records in HBM
@@ -106,8 +107,7 @@ struct Counters {
     unsigned long long fragments;
     unsigned long long frag_bytes;
     unsigned long long tri_raster;   // fan triangles set up by the current draw
-    unsigned int nseg;
-    unsigned int work;
+    unsigned int nheavy;          // tiles in the heavy list of the current draw
     unsigned int unsupported;
     unsigned int nclip;
     unsigned int noverflow;       // overflow slots requested by clipped triangles
@@ -162,9 +162,10 @@ struct RasterParams {
     const RTri *rt;
     const RCold *rcold;
     const uint32_t *vals;          // sorted pair values (record slot)
-    const uint32_t *seg_tiles;
+    const uint32_t *heavy;         // tiles with more than HEAVY_BIN entries
+    uint32_t heavy_cap;            // blocks reserved for them
     const uint32_t *tile_start;
-    const uint32_t *tile_end;
+    const uint32_t *tile_end;      // 0 for an empty tile
     uint32_t *color;
     float *depth;
     uint8_t *stencil;
@@ -951,35 +952,42 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *k
 // ---------------------------------------------------------------------------
 
 __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles, uint32_t *tile_start,
-                                                  uint32_t *tile_end, uint32_t *seg_tiles, Counters *ctr)
+                                                  uint32_t *tile_end, uint32_t *heavy, uint32_t heavy_cap, Counters *ctr)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint32_t k = keys[p];
-    bool ok = k < ntiles;
-    bool first = ok && (p == 0 || keys[p - 1] != k);
-    // one atomic per wave for the segment starts it holds
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = p < n ? keys[p] : 0xFFFFFFFFu;
+    const bool ok = k < ntiles;
+    if (ok && (p == 0 || keys[p - 1] != k)) tile_start[k] = p;
+    const bool last = ok && (p == n - 1 || keys[p + 1] != k);
+    if (last) tile_end[k] = p + 1;
+    // a bin longer than HEAVY_BIN holds the same key HEAVY_BIN places before
+    // its last entry; one atomic per wave appends those tiles to the list
+    const bool hv = last && p >= HEAVY_BIN && keys[p - HEAVY_BIN] == k;
     const int lane = threadIdx.x & 63;
-    uint64_t starts = __ballot(first);
-    uint32_t base = 0;
-    if (starts) {
-        if (lane == __builtin_ctzll(starts)) base = atomicAdd(&ctr->nseg, (uint32_t)__popcll(starts));
-        base = __shfl(base, (int)__builtin_ctzll(starts), 64);
+    const uint64_t hb = __ballot(hv);
+    if (hb) {
+        const int leader = __builtin_ctzll(hb);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ctr->nheavy, (uint32_t)__popcll(hb));
+        base = __shfl(base, leader, 64);
+        if (hv) {
+            const uint32_t s = base + (uint32_t)__popcll(hb & ((1ull << lane) - 1ull));
+            if (s < heavy_cap) heavy[s] = k;
+        }
     }
-    if (first) {
-        tile_start[k] = p;
-        uint32_t s = base + (uint32_t)__popcll(starts & ((1ull << lane) - 1ull));
-        if (s < ntiles) seg_tiles[s] = k;
-    }
-    if (ok && (p == n - 1 || keys[p + 1] != k)) tile_end[k] = p + 1;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *dst, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
 }
 
 __global__ void k_reset(Counters *ctr)
 {
     if (threadIdx.x == 0) {
         ctr->tri_raster = 0;
-        ctr->nseg = 0;
-        ctr->work = 0;
+        ctr->nheavy = 0;
         ctr->unsupported = 0;
         ctr->nclip = 0;
         ctr->noverflow = 0;
@@ -1163,32 +1171,20 @@ __device__ __forceinline__ void stencil_write(const mtgl_raster_state *rs, Pixel
     px.dirty_s = true;
 }
 
-// one fragment of triangle r at pixel (x, y) whose barycentrics are (b0, b1, b2):
-// the per-fragment pipeline of the reference in its order (src/raster.c:546-721)
-__device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, const RTri *r, uint32_t rid, float b0,
-                               float b1, float b2, PixelState &px, unsigned long long &bytes)
+// The part of one fragment's pipeline (src/raster.c:546-721) that does not
+// read the framebuffer: window depth, colour through lighting, texturing and
+// fog, whether the alpha test keeps it, and how many texels it read.  It
+// depends only on the triangle and the pixel, so the fragments of a tile are
+// computed in parallel and in any order; resolve_fragment then applies the
+// stencil, depth, blend and mask steps in submission order.  meta = kept
+// (bit 0) | texel taps << 1.
+__device__ void fragment_source(const RasterParams &P, const RasterConsts &K, const RTri *r, uint32_t rid, float b0,
+                                float b1, float b2, Col &c_out, float &depth_out, uint32_t &meta_out)
 {
     const mtgl_raster_state *rs = P.rs;
     const float z = b0 * r->z0 + b1 * r->z1 + b2 * r->z2;
-    float depth;
-    if (K.unit_range) depth = (z + 1.0f) * 0.5f;   // (double)q * 1.0 + 0.0 rounds back to q
-    else depth = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
-
-    if (K.stencil_on) {
-        bytes += 2;
-        if (!stencil_test(rs->stencil_func, rs->stencil_ref, rs->stencil_mask, px.s)) {
-            stencil_write(rs, px, rs->stencil_fail);
-            return;
-        }
-    }
-    if (K.depth_on) {
-        bytes += 4;
-        if (!compare_op(rs->depth_func, depth, px.d)) {
-            if (K.stencil_on) stencil_write(rs, px, rs->stencil_zfail);
-            return;
-        }
-    }
-    if (K.stencil_on) stencil_write(rs, px, rs->stencil_zpass);
+    if (K.unit_range) depth_out = (z + 1.0f) * 0.5f;   // (double)q * 1.0 + 0.0 rounds back to q
+    else depth_out = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
 
     Col c;
     if (K.flat) {
@@ -1199,6 +1195,7 @@ __device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, con
         c.b = r->c0[2] * b0 + r->c1[2] * b1 + r->c2[2] * b2;
         c.a = r->c0[3] * b0 + r->c1[3] * b1 + r->c2[3] * b2;
     }
+    meta_out = 1u;
     if (K.lighting_on) {
         const bool two = r->back && rs->lighting.two_side;
         if (K.phong || two) {
@@ -1235,8 +1232,12 @@ __device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, con
         }
         int taps = 0;
         Col tc = col_unpack(sample_texture(rs, K.L0, K.L1, u, v, r->lod, taps));
-        bytes += 4u * (unsigned)taps;
-        if (K.alpha_on && !compare_op(rs->alpha_func, tc.a, rs->alpha_ref)) return;
+        if (K.alpha_on && !compare_op(rs->alpha_func, tc.a, rs->alpha_ref)) {
+            c_out = c;
+            meta_out = (uint32_t)taps << 1;
+            return;
+        }
+        meta_out = 1u | ((uint32_t)taps << 1);
         switch (rs->tex_env_mode) {
         case GL_REPLACE: c = tc; break;
         case GL_DECAL:
@@ -1267,11 +1268,49 @@ __device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, con
         Col fg = col_of(rs->fog_color);
         c = {fg.r + (c.r - fg.r) * f, fg.g + (c.g - fg.g) * f, fg.b + (c.b - fg.b) * f, fg.a};
     }
+    c_out = c;
+}
+
+// The framebuffer part of one fragment's pipeline, in the reference's order:
+// stencil test, depth test, stencil update, alpha-test discard, depth write,
+// blend, clamp and colour mask (src/raster.c:552-568, :651-721).  Byte
+// counting follows the reference's reads and writes of each step.
+// First half: stencil test, depth test, stencil update, the texel bytes the
+// fragment read, the alpha-test discard (kept == false), depth write and the
+// colour bytes.  Returns whether the colour step runs.
+__device__ __forceinline__ bool depth_stencil_step(const mtgl_raster_state *rs, const RasterConsts &K, float depth,
+                                                   uint32_t taps, bool kept, PixelState &px, unsigned long long &bytes)
+{
+    if (K.stencil_on) {
+        bytes += 2;
+        if (!stencil_test(rs->stencil_func, rs->stencil_ref, rs->stencil_mask, px.s)) {
+            stencil_write(rs, px, rs->stencil_fail);
+            return false;
+        }
+    }
+    if (K.depth_on) {
+        bytes += 4;
+        if (!compare_op(rs->depth_func, depth, px.d)) {
+            if (K.stencil_on) stencil_write(rs, px, rs->stencil_zfail);
+            return false;
+        }
+    }
+    if (K.stencil_on) stencil_write(rs, px, rs->stencil_zpass);
+    bytes += 4u * taps;
+    if (!kept) return false;
     if (K.depth_on && rs->depth_mask) {
         bytes += 4;
         px.d = depth;
         px.dirty_d = true;
     }
+    if (K.cmask == 0xFu) bytes += K.blend_on ? 8 : 4;
+    else if (K.cmask != 0u) bytes += 8;
+    return true;
+}
+
+// Second half: blend with the pixel's colour, clamp, colour mask, pack.
+__device__ __forceinline__ void color_step(const mtgl_raster_state *rs, const RasterConsts &K, Col c, PixelState &px)
+{
     if (K.blend_on) {
         Col dst = col_unpack(px.c);
         Col sf = blend_factor(rs->blend_src, c, dst);
@@ -1280,11 +1319,9 @@ __device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, con
     }
     c = col_clamp(c);
     if (K.cmask == 0xFu) {
-        bytes += K.blend_on ? 8 : 4;
         px.c = col_pack(c);
         px.dirty_c = true;
     } else if (K.cmask != 0u) {
-        bytes += 8;
         Col d = col_unpack(px.c);
         if (K.cmask & 1u) d.r = c.r;
         if (K.cmask & 2u) d.g = c.g;
@@ -1295,38 +1332,157 @@ __device__ void shade_fragment(const RasterParams &P, const RasterConsts &K, con
     }
 }
 
-// inside test of the reference's edge functions at integer pixel (x, y)
-// (src/raster.c:299-302, :534-540); returns the three edge values
-__device__ __forceinline__ bool covers(const RTri *r, int32_t x, int32_t y, float &w0e, float &w1e, float &w2e)
+// the reference's three edge functions at integer pixel (x, y)
+// (src/raster.c:299-302, :534-540), evaluated in its operation order
+__device__ __forceinline__ void edge_values(const RTri *r, int32_t x, int32_t y, float &w0e, float &w1e, float &w2e)
 {
-    if (!(x >= r->minX && x <= r->maxX && y >= r->minY && y <= r->maxY)) return false;
     const float px = (float)x, py = (float)y;
     const float fx0 = (float)r->x0, fy0 = (float)r->y0, fx1 = (float)r->x1, fy1 = (float)r->y1;
     const float fx2 = (float)r->x2, fy2 = (float)r->y2;
-    const float area = r->area;
     w0e = (px - fx1) * (fy2 - fy1) - (py - fy1) * (fx2 - fx1);
     w1e = (px - fx2) * (fy0 - fy2) - (py - fy2) * (fx0 - fx2);
     w2e = (px - fx0) * (fy1 - fy0) - (py - fy0) * (fx1 - fx0);
-    return (area > 0 && w0e >= 0 && w1e >= 0 && w2e >= 0) || (area < 0 && w0e <= 0 && w1e <= 0 && w2e <= 0);
 }
 
-constexpr int CHUNK = 64;   // bin entries per coverage pass: one bit each in a lane's mask
-constexpr int RASTER_WAVES = 4;
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(v, (unsigned)d, 64);
+        if (lane >= d) v += y;
+    }
+    return v;
+}
+
+constexpr int CHUNK = 64;          // bin entries per coverage pass: one bit each in a pixel's mask
+constexpr int FCAP = 128;          // fragment sources held in LDS per batch
 
 // the fragment-stage part of a record, as staged in LDS
 struct __align__(16) RFrag {
     float4 v[RT_FRAG_VEC4];
 };
 
-__global__ void __launch_bounds__(256) k_raster(RasterParams P)
+// what the in-order depth pass reads per bin entry: the edge-function terms
+// in the reference's operation order, 1/area and the three window z
+struct __align__(16) RCov {
+    float fx1, fy1, d0y, d0x;      // w0e = (x - fx1) * d0y - (y - fy1) * d0x
+    float fx2, fy2, d1y, d1x;      // w1e
+    float fx0, fy0, d2y, d2x;      // w2e
+    float inv_area, z0, z1, z2;
+};
+
+// field of the record held as 11 float4 registers
+template <int OFF>
+__device__ __forceinline__ float rec_f(const float4 *q)
 {
-    // per wave: the records of the chunk being resolved and their slots
-    __shared__ RFrag s_rec[RASTER_WAVES][CHUNK];
-    __shared__ uint32_t s_rid[RASTER_WAVES][CHUNK];
+    constexpr int v = OFF / 16, c = (OFF % 16) / 4;
+    return c == 0 ? q[v].x : (c == 1 ? q[v].y : (c == 2 ? q[v].z : q[v].w));
+}
+#define REC_F(q, field, extra) rec_f<(int)offsetof(RTri, field) + (extra)>(q)
+#define REC_I(q, field) __float_as_int(rec_f<(int)offsetof(RTri, field)>(q))
+
+// texels one fragment of a triangle with this LOD reads (the branch structure
+// of sample_texture: taps depend on the triangle, not on the pixel)
+__device__ __forceinline__ uint32_t texel_taps(const mtgl_raster_state *rs, const RasterConsts &K, float lod)
+{
+    if (!K.textured) return 0u;
+    const uint32_t filter = lod > 0.0f ? rs->min_filter : rs->mag_filter;
+    if (filter == GL_NEAREST_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_NEAREST) {
+        if (lod >= 0.5f) return K.L1.px ? (filter == GL_LINEAR_MIPMAP_NEAREST ? 4u : 1u) : 0u;
+        return filter == GL_LINEAR_MIPMAP_NEAREST ? 4u : 1u;
+    }
+    if (filter == GL_NEAREST_MIPMAP_LINEAR || filter == GL_LINEAR_MIPMAP_LINEAR) {
+        const uint32_t per = filter == GL_LINEAR_MIPMAP_LINEAR ? 4u : 1u;
+        return (lod > 0.0f && K.L1.px) ? 2u * per : per;
+    }
+    return filter == GL_LINEAR ? 4u : 1u;
+}
+
+template <int R, int I>
+__device__ __forceinline__ void cover_pixel(const float *a0, const float *a1, const float *a2, float b0, float b1,
+                                            float b2, int32_t &mlo, int32_t &mhi, uint32_t &tri_px)
+{
+    const bool p = (a0[I] >= b0) & (a1[I] >= b1) & (a2[I] >= b2);
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(p);
+    if ((int)threadIdx.x == R * TILE + I) {
+        mlo = (int32_t)(uint32_t)bal;
+        mhi = (int32_t)(uint32_t)(bal >> 32);
+    }
+    tri_px += p ? 1u : 0u;
+}
+
+template <int R>
+__device__ __forceinline__ void cover_row(const float *a0, const float *a1, const float *a2, const float *b0,
+                                          const float *b1, const float *b2, int32_t &mlo, int32_t &mhi,
+                                          uint32_t &tri_px)
+{
+    cover_pixel<R, 0>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 1>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 2>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 3>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 4>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 5>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 6>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 7>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// K4: one 8x8 tile per block of one wave.  Per chunk of 64 bin entries:
+//  1. coverage, one lane per triangle: the edge functions are separable,
+//     w = A(x) - B(y) with A and B the reference's rounded products, so each
+//     lane forms 8 column and 8 row terms per edge and a pixel's test is
+//     three compares (A and B are finite integers-valued floats, so
+//     fl(A - B) >= 0 exactly when A >= B); one ballot per pixel gives the
+//     mask of triangles covering it, written into that pixel's lane.
+//     Triangles with negative area are tested with A and B negated; the
+//     bounding box and the framebuffer edge fold in as -inf / +inf.
+//  2. depth and stencil in submission order, one pixel per lane, unless the
+//     alpha test (which reads the texture) decides depth writes.
+//  3. colour: when nothing blends, only the last fragment that passed a
+//     pixel can leave its colour, so one fragment per pixel is shaded;
+//     otherwise every passing fragment is shaded, one per lane in batches of
+//     at most FCAP, and blended per pixel in order.
+// Tiles whose bins exceed HEAVY_BIN entries come from a list and occupy the
+// first blocks, so the longest tiles start first.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster(RasterParams P)
+{
+    __shared__ RFrag s_rec[CHUNK];
+    __shared__ RCov s_cov[CHUNK];
+    __shared__ uint32_t s_rid[CHUNK];
+    __shared__ uint32_t s_tap[CHUNK];
+    __shared__ float4 s_col[FCAP];
+    __shared__ float s_dep[FCAP];
+    __shared__ uint16_t s_slot[FCAP];
+    __shared__ uint8_t s_meta[FCAP];
 
     const mtgl_raster_state *rs = P.rs;
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x;
+
+    uint32_t tile, e0, e1;
+    if (blockIdx.x < P.heavy_cap) {
+        const uint32_t nh = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nheavy);
+        if (blockIdx.x >= nh) return;
+        tile = P.heavy[blockIdx.x];
+        e0 = P.tile_start[tile];
+        e1 = P.tile_end[tile];
+    } else {
+        tile = blockIdx.x - P.heavy_cap;
+        e1 = P.tile_end[tile];
+        if (e1 == 0u) return;
+        e0 = P.tile_start[tile];
+        if (e1 - e0 > HEAVY_BIN) return;   // dispatched from the heavy list
+    }
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    e0 = __builtin_amdgcn_readfirstlane(e0);
+    e1 = __builtin_amdgcn_readfirstlane(e1);
+
     RasterConsts K;
     const uint32_t flags = rs->flags;
     K.depth_on = flags & MTGL_F_DEPTH_TEST;
@@ -1345,68 +1501,184 @@ __global__ void __launch_bounds__(256) k_raster(RasterParams P)
     K.L1 = {P.tex1, P.tex1_w, P.tex1_h,
             P.tex1_w > 0 && (P.tex1_w & (P.tex1_w - 1)) == 0 && (P.tex1_h & (P.tex1_h - 1)) == 0};
     const bool need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
-    const uint32_t nseg = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nseg);
-    RFrag *recs = s_rec[wave];
-    uint32_t *rids = s_rid[wave];
+    // the alpha test reads the texture, and its outcome gates the depth write
+    const bool alpha_dep = K.alpha_on && K.textured;
+    const bool last_writer = !alpha_dep && !K.blend_on && (K.cmask == 0xFu || K.cmask == 0u);
+
+    const int32_t tx = (int32_t)(tile % (uint32_t)P.tiles_x), ty = (int32_t)(tile / (uint32_t)P.tiles_x);
+    const int32_t X0 = tx * TILE, Y0 = ty * TILE;
+    const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
+    const float xf = (float)x, yf = (float)y;
+    const bool in_fb = x < P.fb_w && y < P.fb_h;
+    const size_t pix = in_fb ? (size_t)y * P.fb_w + x : 0;
+    PixelState px = {0u, 1.0f, 0u, false, false, false};
+    if (in_fb) {
+        if (need_color) px.c = P.color[pix];
+        if (K.depth_on) px.d = P.depth[pix];
+        if (K.stencil_on) px.s = P.stencil[pix];
+    }
 
     unsigned long long frag_count = 0, frag_bytes = 0;
-    for (;;) {
-        uint32_t s = 0;
-        if (lane == 0) s = atomicAdd(&P.ctr->work, 1u);
-        s = __builtin_amdgcn_readfirstlane(s);
-        if (s >= nseg) break;
-        const uint32_t tile = P.seg_tiles[s];
-        const uint32_t e0 = P.tile_start[tile], e1 = P.tile_end[tile];
-        const int32_t tx = (int32_t)(tile % (uint32_t)P.tiles_x), ty = (int32_t)(tile / (uint32_t)P.tiles_x);
-        const int32_t x = tx * TILE + (lane & 7), y = ty * TILE + (lane >> 3);
-        const bool in_fb = x < P.fb_w && y < P.fb_h;
-        const size_t pix = in_fb ? (size_t)y * P.fb_w + x : 0;
-        PixelState px = {0u, 1.0f, 0u, false, false, false};
-        if (in_fb) {
-            if (need_color) px.c = P.color[pix];
-            if (K.depth_on) px.d = P.depth[pix];
-            if (K.stencil_on) px.s = P.stencil[pix];
-        }
-        for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
-            const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
-            // stage the chunk: lane k copies bin entry k's record into LDS
-            __builtin_amdgcn_wave_barrier();
-            if ((uint32_t)lane < cn) {
-                const uint32_t rid = P.vals[c0 + lane];
-                const float4 *src = reinterpret_cast<const float4 *>(&P.rt[rid]);
+    for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
+        const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
+        __syncthreads();   // the previous chunk's records are no longer read
+        // ---- 1. stage entry `lane` and test it against the 64 pixels
+        const bool valid = (uint32_t)lane < cn;
+        float4 q[RT_FRAG_VEC4];
 #pragma unroll
-                for (int q = 0; q < RT_FRAG_VEC4; q++) recs[lane].v[q] = src[q];
-                rids[lane] = rid;
+        for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) {
+            const uint32_t rid = P.vals[c0 + lane];
+            const float4 *src = reinterpret_cast<const float4 *>(&P.rt[rid]);
+#pragma unroll
+            for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = src[i];
+#pragma unroll
+            for (int i = 0; i < RT_FRAG_VEC4; i++) s_rec[lane].v[i] = q[i];
+            s_rid[lane] = rid;
+        }
+        const float fx0 = (float)REC_I(q, x0), fy0 = (float)REC_I(q, y0);
+        const float fx1 = (float)REC_I(q, x1), fy1 = (float)REC_I(q, y1);
+        const float fx2 = (float)REC_I(q, x2), fy2 = (float)REC_I(q, y2);
+        const int32_t minX = REC_I(q, minX), minY = REC_I(q, minY);
+        const int32_t maxX = REC_I(q, maxX), maxY = REC_I(q, maxY);
+        const float area = REC_F(q, area, 0);
+        const float d0y = fy2 - fy1, d0x = fx2 - fx1;
+        const float d1y = fy0 - fy2, d1x = fx0 - fx2;
+        const float d2y = fy1 - fy0, d2x = fx1 - fx0;
+        if (valid) {
+            RCov cv;
+            cv.fx1 = fx1; cv.fy1 = fy1; cv.d0y = d0y; cv.d0x = d0x;
+            cv.fx2 = fx2; cv.fy2 = fy2; cv.d1y = d1y; cv.d1x = d1x;
+            cv.fx0 = fx0; cv.fy0 = fy0; cv.d2y = d2y; cv.d2x = d2x;
+            cv.inv_area = REC_F(q, inv_area, 0);
+            cv.z0 = REC_F(q, z0, 0); cv.z1 = REC_F(q, z1, 0); cv.z2 = REC_F(q, z2, 0);
+            s_cov[lane] = cv;
+            s_tap[lane] = texel_taps(rs, K, REC_F(q, lod, 0));
+        }
+        const bool live = valid && (area > 0.0f || area < 0.0f);
+        const float sg = area > 0.0f ? 1.0f : -1.0f;
+        float a0[TILE], a1[TILE], a2[TILE], b0[TILE], b1[TILE], b2[TILE];
+#pragma unroll
+        for (int i = 0; i < TILE; i++) {
+            const int32_t xi = X0 + i, yi = Y0 + i;
+            const float pxi = (float)xi, pyi = (float)yi;
+            a0[i] = sg * ((pxi - fx1) * d0y);
+            a1[i] = sg * ((pxi - fx2) * d1y);
+            a2[i] = sg * ((pxi - fx0) * d2y);
+            b0[i] = sg * ((pyi - fy1) * d0x);
+            b1[i] = sg * ((pyi - fy2) * d1x);
+            b2[i] = sg * ((pyi - fy0) * d2x);
+            if (!(live && xi >= minX && xi <= maxX && xi < P.fb_w)) a0[i] = -__builtin_inff();
+            if (!(yi >= minY && yi <= maxY && yi < P.fb_h)) b0[i] = __builtin_inff();
+        }
+        int32_t mlo = 0, mhi = 0;
+        uint32_t tri_px = 0;
+        cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<2>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<3>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<4>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<5>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<6>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        cover_row<7>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
+        (void)tri_px;
+        const uint64_t m = ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
+        frag_count += (unsigned long long)__popcll(m);
+        __syncthreads();   // staged records visible to every lane
+
+        // ---- 2. depth and stencil in order
+        uint64_t todo = m;   // fragments whose colour step runs
+        if (!alpha_dep) {
+            uint64_t t = m, pass = 0;
+            while (t) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(t);
+                t &= t - 1;
+                const RCov &cv = s_cov[k];
+                const float w0e = (xf - cv.fx1) * cv.d0y - (yf - cv.fy1) * cv.d0x;
+                const float w1e = (xf - cv.fx2) * cv.d1y - (yf - cv.fy2) * cv.d1x;
+                const float w2e = (xf - cv.fx0) * cv.d2y - (yf - cv.fy0) * cv.d2x;
+                const float bb0 = w0e * cv.inv_area, bb1 = w1e * cv.inv_area, bb2 = w2e * cv.inv_area;
+                const float z = bb0 * cv.z0 + bb1 * cv.z1 + bb2 * cv.z2;
+                float depth;
+                if (K.unit_range) depth = (z + 1.0f) * 0.5f;
+                else depth = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
+                if (depth_stencil_step(rs, K, depth, s_tap[k], true, px, frag_bytes)) pass |= 1ull << k;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            // coverage pass: the wave walks the chunk in submission order and
-            // each lane records which triangles cover its pixel
-            uint64_t mask = 0;
-            for (uint32_t k = 0; k < cn; k++) {
-                const RTri *r = reinterpret_cast<const RTri *>(&recs[k]);
+            todo = pass;
+        }
+
+        // ---- 3. colour
+        if (last_writer) {
+            if (K.cmask != 0u && todo) {
+                const uint32_t k = 63u - (uint32_t)__builtin_clzll(todo);
+                const RTri *r = reinterpret_cast<const RTri *>(&s_rec[k]);
                 float w0e, w1e, w2e;
-                if (in_fb && covers(r, x, y, w0e, w1e, w2e)) mask |= 1ull << k;
-            }
-            frag_count += (unsigned long long)__popcll(mask);
-            // shading pass: every lane visits only its own fragments, lowest
-            // bit (earliest triangle) first
-            while (mask) {
-                const uint32_t k = (uint32_t)__builtin_ctzll(mask);
-                mask &= mask - 1;
-                const RTri *r = reinterpret_cast<const RTri *>(&recs[k]);
-                float w0e, w1e, w2e;
-                covers(r, x, y, w0e, w1e, w2e);
+                edge_values(r, x, y, w0e, w1e, w2e);
                 const float inv = r->inv_area;
-                shade_fragment(P, K, r, rids[k], w0e * inv, w1e * inv, w2e * inv, px, frag_bytes);
+                Col c;
+                float dep;
+                uint32_t meta;
+                fragment_source(P, K, r, s_rid[k], w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
+                color_step(rs, K, c, px);
             }
+            continue;
         }
-        if (in_fb) {
-            if (px.dirty_c) P.color[pix] = px.c;
-            if (px.dirty_d) P.depth[pix] = px.d;
-            if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
+        // batches: each pixel takes its next L fragments, L the largest quota
+        // that keeps the batch within FCAP (L >= 1 fits since 64 <= FCAP)
+        uint64_t rem = todo;
+        while (__builtin_amdgcn_ballot_w64(rem != 0ull)) {
+            const uint32_t rc = (uint32_t)__popcll(rem);
+            uint32_t L = 1;
+#pragma unroll
+            for (uint32_t step = 32; step >= 1; step >>= 1)
+                if (wave_sum_u32(min(rc, L + step)) <= (uint32_t)FCAP) L += step;
+            const uint32_t take = min(rc, L);
+            const uint32_t ij = wave_incl_scan(take, lane);
+            const uint32_t off = ij - take;
+            const uint32_t fr = min((uint32_t)__builtin_amdgcn_readlane((int32_t)ij, 63), (uint32_t)FCAP);
+            {
+                uint32_t o = off;
+                for (uint32_t qn = 0; qn < take; qn++) {
+                    const uint32_t k = (uint32_t)__builtin_ctzll(rem);
+                    rem &= rem - 1;
+                    if (o < (uint32_t)FCAP) s_slot[o] = (uint16_t)((k << 6) | (uint32_t)lane);
+                    o++;
+                }
+            }
+            __syncthreads();
+            for (uint32_t f0 = 0; f0 < fr; f0 += 64) {
+                const uint32_t f = f0 + (uint32_t)lane;
+                if (f < fr) {
+                    const uint32_t sl = s_slot[f];
+                    const uint32_t k = sl >> 6, j = sl & 63u;
+                    const RTri *r = reinterpret_cast<const RTri *>(&s_rec[k]);
+                    float w0e, w1e, w2e;
+                    edge_values(r, X0 + (int32_t)(j & 7u), Y0 + (int32_t)(j >> 3), w0e, w1e, w2e);
+                    const float inv = r->inv_area;
+                    Col c;
+                    float dep;
+                    uint32_t meta;
+                    fragment_source(P, K, r, s_rid[k], w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
+                    s_col[f] = make_float4(c.r, c.g, c.b, c.a);
+                    s_dep[f] = dep;
+                    s_meta[f] = (uint8_t)meta;
+                }
+            }
+            __syncthreads();
+            for (uint32_t qn = 0; qn < take && off + qn < (uint32_t)FCAP; qn++) {
+                const uint32_t f = off + qn;
+                const float4 cc = s_col[f];
+                const uint32_t meta = s_meta[f];
+                if (!alpha_dep || depth_stencil_step(rs, K, s_dep[f], meta >> 1, (meta & 1u) != 0u, px, frag_bytes))
+                    color_step(rs, K, Col{cc.x, cc.y, cc.z, cc.w}, px);
+            }
+            __syncthreads();   // slots and sources are reused by the next batch
         }
+    }
+    if (in_fb) {
+        if (px.dirty_c) P.color[pix] = px.c;
+        if (px.dirty_d) P.depth[pix] = px.d;
+        if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
     }
     frag_count = wave_sum_u64(frag_count);
     frag_bytes = wave_sum_u64(frag_bytes);
@@ -1474,6 +1746,7 @@ struct Arena {
 
 struct mtgl_dev {
     int device = 0;
+    int cu_count = 256;
     int32_t w = 0, h = 0, tiles_x = 0, tiles_y = 0;
     hipStream_t stream = nullptr;
     uint32_t *color = nullptr;
@@ -1489,7 +1762,7 @@ struct mtgl_dev {
     hipEvent_t upload_done = nullptr;
     Arena params, vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
-    Arena keys[2], vals[2], sort_counts, tile_start, tile_end, seg_tiles;
+    Arena keys[2], vals[2], sort_counts, tile_start, tile_end, heavy_tiles;
     mtgl_dev_stats stats = {};
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
@@ -1637,6 +1910,9 @@ MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
     int rc = MTGL_OK;
     do {
         if (hipGetDevice(&d->device) != hipSuccess) { rc = MTGL_ERR_DEVICE; break; }
+        if (hipDeviceGetAttribute(&d->cu_count, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess ||
+            d->cu_count <= 0)
+            d->cu_count = 256;
         d->w = width;
         d->h = height;
         d->tiles_x = (width + TILE - 1) / TILE;
@@ -1675,7 +1951,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     }
     Arena *arenas[] = {&d->params, &d->vhot, &d->vcold, &d->rt, &d->rcold, &d->n_rt, &d->rec_base, &d->tri_tiles, &d->pair_base,
                        &d->scan_tmp, &d->clip_list, &d->keys[0], &d->keys[1], &d->vals[0], &d->vals[1], &d->sort_counts,
-                       &d->tile_start, &d->tile_end, &d->seg_tiles};
+                       &d->tile_start, &d->tile_end, &d->heavy_tiles};
     for (Arena *a : arenas) if (a->ptr) (void)hipFree(a->ptr);
     for (auto &pr : d->pending_events) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (hipEvent_t e : d->free_events) (void)hipEventDestroy(e);
@@ -1971,7 +2247,7 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     const uint32_t ntiles = (uint32_t)(d->tiles_x * d->tiles_y);
     if ((rc = arena_reserve(d, d->tile_start, (size_t)ntiles * 4))) return rc;
     if ((rc = arena_reserve(d, d->tile_end, (size_t)ntiles * 4))) return rc;
-    if ((rc = arena_reserve(d, d->seg_tiles, (size_t)ntiles * 4))) return rc;
+    if ((rc = arena_reserve(d, d->heavy_tiles, (size_t)ntiles * 4))) return rc;
     uint32_t *keys = as<uint32_t>(d->keys[0]), *vals = as<uint32_t>(d->vals[0]);
     uint32_t *keys2 = as<uint32_t>(d->keys[1]), *vals2 = as<uint32_t>(d->vals[1]);
     hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const unsigned long long *)pair_base,
@@ -1991,8 +2267,13 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         uint32_t *tk = keys; keys = keys2; keys2 = tk;
         uint32_t *tv = vals; vals = vals2; vals2 = tv;
     }
+    // every tile with more than HEAVY_BIN entries takes HEAVY_BIN + 1 of the n
+    const uint32_t heavy_cap = std::min<uint32_t>(ntiles, n / (HEAVY_BIN + 1u));
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, d->stream, as<uint32_t>(d->tile_end),
+                       ntiles, 0u);
     hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const uint32_t *)keys, n, ntiles,
-                       as<uint32_t>(d->tile_start), as<uint32_t>(d->tile_end), as<uint32_t>(d->seg_tiles), d->ctr);
+                       as<uint32_t>(d->tile_start), as<uint32_t>(d->tile_end), as<uint32_t>(d->heavy_tiles), heavy_cap,
+                       d->ctr);
     HIPC(hipGetLastError());
 
     // ---- K4
@@ -2001,7 +2282,8 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     rp.rt = sp.rt;
     rp.rcold = sp.rcold;
     rp.vals = vals;
-    rp.seg_tiles = as<uint32_t>(d->seg_tiles);
+    rp.heavy = as<uint32_t>(d->heavy_tiles);
+    rp.heavy_cap = heavy_cap;
     rp.tile_start = as<uint32_t>(d->tile_start);
     rp.tile_end = as<uint32_t>(d->tile_end);
     rp.color = d->color;
@@ -2023,8 +2305,8 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         ev1 = take_event(d);
         if (ev0) HIPC(hipEventRecord(ev0, d->stream));
     }
-    const unsigned nwaves = std::min<uint32_t>(n, 256u * 16u);
-    hipLaunchKernelGGL(k_raster, dim3((nwaves + 3) / 4), dim3(256), 0, d->stream, rp);
+    // heavy-list blocks first, then one block per tile (empty and heavy tiles exit)
+    hipLaunchKernelGGL(k_raster, dim3(heavy_cap + ntiles), dim3(64), 0, d->stream, rp);
     HIPC(hipGetLastError());
     if (ev0 && ev1) {
         HIPC(hipEventRecord(ev1, d->stream));
