@@ -57,6 +57,7 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 constexpr uint32_t HEAVY_BIN = 96;      // tiles with longer bins are rasterised first
+constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
 // records in HBM
@@ -76,26 +77,28 @@ struct __align__(16) VCold {
     float pad[2];
 };
 
-// fan triangle after setup (192 B); an unclipped triangle t owns slot t, a
+// fan triangle after setup (208 B); an unclipped triangle t owns slot t, a
 // clipped one owns a run of slots from the overflow pool after the first ntri
 struct __align__(16) RTri {
     int32_t x0, y0, x1, y1, x2, y2;
     int32_t minX, minY, maxX, maxY;      // viewport and scissor clamped
     float area, inv_area;
     float z0, z1, z2;
+    float lod;                           // the first 64 bytes are what coverage and the depth pass read
     float w0, w1, w2;                    // 1/w
     float c0[4], c1[4], c2[4];
     float uv0[2], uv1[2], uv2[2];
     float ez0, ez1, ez2;
-    float lod;
     int32_t back;
-    int32_t pad0[3];                     // the first 176 bytes are what the fragment stage reads
+    int32_t pad0[3];                     // the first 176 bytes are what colour computation reads
     int32_t tx0, ty0, tx1, ty1;          // tile box (framebuffer clamped)
     int32_t ntiles;
     int32_t pad[3];
 };
 static_assert(sizeof(RTri) == 208, "RTri layout");
 constexpr int RT_FRAG_VEC4 = 11;         // 176 bytes = 11 x 16 B staged in LDS per bin entry
+constexpr int RT_COV_VEC4 = 4;           // 64 bytes: vertices, box, area, 1/area, z, lod
+static_assert(offsetof(RTri, lod) == 60 && offsetof(RTri, pad0) == 164, "RTri layout");
 
 struct __align__(16) RCold {
     float ep[9];
@@ -107,7 +110,8 @@ struct Counters {
     unsigned long long fragments;
     unsigned long long frag_bytes;
     unsigned long long tri_raster;   // fan triangles set up by the current draw
-    unsigned int nheavy;          // tiles in the heavy list of the current draw
+    unsigned int nheavy;          // non-empty tiles of the current draw with bins over HEAVY_BIN entries
+    unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
     unsigned int nclip;
     unsigned int noverflow;       // overflow slots requested by clipped triangles
@@ -162,13 +166,17 @@ struct RasterParams {
     const RTri *rt;
     const RCold *rcold;
     const uint32_t *vals;          // sorted pair values (record slot)
-    const uint32_t *heavy;         // tiles with more than HEAVY_BIN entries
-    uint32_t heavy_cap;            // blocks reserved for them
+    const uint32_t *list;          // non-empty tiles: nheavy from the front, nlight from the back
+    uint32_t ntiles;
+    uint32_t part;                 // list items of this launch: 0 all, 1 the heavy ones, 2 the others
+    unsigned long long *stats;     // per block: fragments, algorithmic bytes (summed when read)
+    const float4 *cov;             // first 64 record bytes per pair, in pair order
     const uint32_t *tile_start;
     const uint32_t *tile_end;      // 0 for an empty tile
     uint32_t *color;
     float *depth;
     uint8_t *stencil;
+    uint32_t *writer;              // per pixel: 1 + record slot of the last passing fragment
     int32_t fb_w, fb_h, tiles_x;
     const uint32_t *tex;           // level 0 (null = untextured)
     const uint32_t *tex1;          // level 1 (null = none: samples read as white)
@@ -701,11 +709,14 @@ __global__ void __launch_bounds__(256) k_setup(SetupParams P)
         RTri *out = P.rt + t;
         RCold *cout = cold ? P.rcold + t : nullptr;
         if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, &nt)) { nout = 1; tiles = (unsigned long long)nt; }
-        if (nout) atomicAdd(&P.ctr->tri_raster, 1ull);
     } else {
         uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
         P.clip_list[slot] = t;
     }
+    // one counter update per wave
+    const uint64_t kept = __ballot(nout != 0);
+    if (kept && (threadIdx.x & 63) == __builtin_ctzll(__ballot(1)))
+        atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
     P.n_rt[t] = nout;
     P.rec_base[t] = t;
     P.tri_tiles[t] = tiles;
@@ -951,36 +962,101 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *k
 // K3c: tile segments of the sorted pairs
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles, uint32_t *tile_start,
-                                                  uint32_t *tile_end, uint32_t *heavy, uint32_t heavy_cap, Counters *ctr)
+constexpr int SEG_THREADS = 256;
+constexpr int SEG_ITEMS = 16;
+
+// Inclusive prefix sum over the 64 lanes of a wave with DPP row shifts and
+// row broadcasts (VALU only): Hillis-Steele within each row of 16 lanes,
+// then row 0's and row 1's totals carried into the rows above.
+__device__ __forceinline__ uint32_t dpp_incl_scan(uint32_t v)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t k = p < n ? keys[p] : 0xFFFFFFFFu;
-    const bool ok = k < ntiles;
-    if (ok && (p == 0 || keys[p - 1] != k)) tile_start[k] = p;
-    const bool last = ok && (p == n - 1 || keys[p + 1] != k);
-    if (last) tile_end[k] = p + 1;
-    // a bin longer than HEAVY_BIN holds the same key HEAVY_BIN places before
-    // its last entry; one atomic per wave appends those tiles to the list
-    const bool hv = last && p >= HEAVY_BIN && keys[p - HEAVY_BIN] == k;
-    const int lane = threadIdx.x & 63;
-    const uint64_t hb = __ballot(hv);
-    if (hb) {
-        const int leader = __builtin_ctzll(hb);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&ctr->nheavy, (uint32_t)__popcll(hb));
-        base = __shfl(base, leader, 64);
-        if (hv) {
-            const uint32_t s = base + (uint32_t)__popcll(hb & ((1ull << lane) - 1ull));
-            if (s < heavy_cap) heavy[s] = k;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xe, false);  // row_shr:4, lanes 4..15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xc, false);  // row_shr:8, lanes 8..15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+__device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(v), 63);
+}
+
+// Tile ranges of the sorted pair keys and the list of non-empty tiles:
+// tiles whose bins hold more than HEAVY_BIN entries from the front of `list`,
+// the others from its back, so that the raster kernels, which walk the list
+// from the front, start the longest tiles first.  Each thread owns SEG_ITEMS
+// consecutive keys; one atomic per block and list end.
+__global__ void __launch_bounds__(SEG_THREADS) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles,
+                                                          uint32_t *tile_start, uint32_t *tile_end, uint32_t *list,
+                                                          Counters *ctr)
+{
+    __shared__ uint32_t s_wave[2][SEG_THREADS / 64];
+    __shared__ uint32_t s_base[2];
+    const uint32_t p0 = (blockIdx.x * SEG_THREADS + threadIdx.x) * SEG_ITEMS;
+    uint32_t nh = 0, nl = 0;
+    for (int i = 0; i < SEG_ITEMS; i++) {
+        const uint32_t p = p0 + i;
+        if (p >= n) break;
+        const uint32_t k = keys[p];
+        if (k >= ntiles) continue;
+        if (p == 0 || keys[p - 1] != k) tile_start[k] = p;
+        if (p == n - 1 || keys[p + 1] != k) {
+            tile_end[k] = p + 1;
+            // a bin longer than HEAVY_BIN holds its key HEAVY_BIN places earlier
+            if (p >= HEAVY_BIN && keys[p - HEAVY_BIN] == k) nh++;
+            else nl++;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t ih = dpp_incl_scan(nh), il = dpp_incl_scan(nl);
+    if (lane == 63) {
+        s_wave[0][wave] = ih;
+        s_wave[1][wave] = il;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t th = 0, tl = 0;
+        for (int w = 0; w < SEG_THREADS / 64; w++) {
+            const uint32_t a = s_wave[0][w], b = s_wave[1][w];
+            s_wave[0][w] = th;
+            s_wave[1][w] = tl;
+            th += a;
+            tl += b;
+        }
+        s_base[0] = th ? atomicAdd(&ctr->nheavy, th) : 0u;
+        s_base[1] = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
+    }
+    __syncthreads();
+    if (nh == 0 && nl == 0) return;
+    uint32_t oh = s_base[0] + s_wave[0][wave] + ih - nh;
+    uint32_t ol = s_base[1] + s_wave[1][wave] + il - nl;
+    for (int i = 0; i < SEG_ITEMS; i++) {
+        const uint32_t p = p0 + i;
+        if (p >= n) break;
+        const uint32_t k = keys[p];
+        if (k >= ntiles || (p != n - 1 && keys[p + 1] == k)) continue;
+        if (p >= HEAVY_BIN && keys[p - HEAVY_BIN] == k) {
+            if (oh < ntiles) list[oh] = k;
+            oh++;
+        } else {
+            if (ol < ntiles) list[ntiles - 1u - ol] = k;
+            ol++;
         }
     }
 }
 
-__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *dst, uint32_t n, uint32_t v)
+// First 64 bytes of each pair's record in sorted (bin) order, so the depth
+// kernel reads every chunk of a bin as one contiguous 4 KiB block; one
+// thread per 16 bytes.
+__global__ void __launch_bounds__(256) k_gather_cov(const uint32_t *vals, const RTri *rt, uint32_t n, float4 *out)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = v;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n * RT_COV_VEC4) return;
+    const uint32_t e = (uint32_t)(i / RT_COV_VEC4), q = (uint32_t)(i % RT_COV_VEC4);
+    out[i] = reinterpret_cast<const float4 *>(rt + vals[e])[q];
 }
 
 __global__ void k_reset(Counters *ctr)
@@ -988,6 +1064,7 @@ __global__ void k_reset(Counters *ctr)
     if (threadIdx.x == 0) {
         ctr->tri_raster = 0;
         ctr->nheavy = 0;
+        ctr->nlight = 0;
         ctr->unsupported = 0;
         ctr->nclip = 0;
         ctr->noverflow = 0;
@@ -1027,6 +1104,31 @@ __device__ __forceinline__ int stencil_test(uint32_t func, int32_t ref, uint32_t
     case GL_GEQUAL: return r >= v;
     default: return 1;
     }
+}
+
+// GL comparison functions as accept masks: GL_NEVER..GL_ALWAYS are
+// 0x200 + (less ? 1 : 0) + (equal ? 2 : 0) + (greater ? 4 : 0); any other
+// value behaves as GL_ALWAYS, as in compare_op
+__device__ __forceinline__ uint32_t cmp_mask_of(uint32_t func)
+{
+    return (func >= GL_NEVER && func <= GL_ALWAYS) ? func - GL_NEVER : 7u;
+}
+
+// compare_op through an accept mask; unordered operands (NaN) pass only
+// GL_NOTEQUAL and GL_ALWAYS, the two masks holding both less and greater
+__device__ __forceinline__ bool cmp_f(uint32_t m, float a, float b)
+{
+    const bool lt = a < b, eq = a == b, gt = a > b;
+    const bool un = !(lt | eq | gt);
+    return (lt & ((m & 1u) != 0u)) | (eq & ((m & 2u) != 0u)) | (gt & ((m & 4u) != 0u)) | (un & ((m & 5u) == 5u));
+}
+
+// stencil_test through an accept mask
+__device__ __forceinline__ bool cmp_stencil(uint32_t m, int32_t ref, uint32_t mask, uint32_t s)
+{
+    const int32_t r = (int32_t)((uint32_t)ref & mask);
+    const int32_t v = (int32_t)(s & mask);
+    return ((r < v) & ((m & 1u) != 0u)) | ((r == v) & ((m & 2u) != 0u)) | ((r > v) & ((m & 4u) != 0u));
 }
 
 __device__ __forceinline__ uint32_t stencil_op(uint32_t op, uint32_t s, int32_t ref)
@@ -1069,6 +1171,22 @@ struct TexLevel {
     const uint32_t *px;
     int32_t w, h;
     bool pow2;
+};
+
+// the draw's fragment state, copied once per block into (scalar) registers
+struct RasterConsts {
+    bool depth_on, stencil_on, blend_on, fog_on, alpha_on, lighting_on, textured, persp, flat, phong, unit_range;
+    bool rep_s, rep_t, depth_mask, two_side, depth_write;
+    uint32_t pass_bytes;   // bytes a passing, kept fragment writes: depth and colour
+    uint32_t cmask;
+    uint32_t depth_cmp, stencil_cmp, alpha_cmp;   // accept masks (cmp_mask_of)
+    uint32_t stencil_mask, stencil_writemask, stencil_fail, stencil_zfail, stencil_zpass;
+    int32_t stencil_ref;
+    uint32_t blend_src, blend_dst, tex_env_mode, fog_mode, min_filter, mag_filter;
+    float alpha_ref, fog_density, fog_start, fog_end;
+    Col tex_env_color, fog_color;
+    double depth_near, depth_far;
+    TexLevel L0, L1;
 };
 
 __device__ __forceinline__ int32_t wrap_coord(int32_t x, int32_t w, bool repeat, bool pow2)
@@ -1114,15 +1232,15 @@ __device__ uint32_t sample_level(const TexLevel &L, bool rep_s, bool rep_t, floa
 }
 
 // reference texture_sample_lod (src/textures.c:457-557)
-__device__ uint32_t sample_texture(const mtgl_raster_state *rs, const TexLevel &L0, const TexLevel &L1, float u, float v,
+__device__ uint32_t sample_texture(const RasterConsts &K, const TexLevel &L0, const TexLevel &L1, float u, float v,
                                    float lod, int &taps)
 {
-    const bool rep_s = rs->wrap_s == GL_REPEAT, rep_t = rs->wrap_t == GL_REPEAT;
+    const bool rep_s = K.rep_s, rep_t = K.rep_t;
     if (rep_s) { u = u - (float)cvtt(u); if (u < 0) u += 1.0f; }
     else { if (u < 0.0f) u = 0.0f; if (u > 1.0f) u = 1.0f; }
     if (rep_t) { v = v - (float)cvtt(v); if (v < 0) v += 1.0f; }
     else { if (v < 0.0f) v = 0.0f; if (v > 1.0f) v = 1.0f; }
-    uint32_t filter = lod > 0.0f ? rs->min_filter : rs->mag_filter;
+    uint32_t filter = lod > 0.0f ? K.min_filter : K.mag_filter;
     if (filter == GL_NEAREST_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_NEAREST) {
         if (lod >= 0.5f) {
             if (!L1.px) return 0xFFFFFFFFu;
@@ -1142,12 +1260,6 @@ __device__ uint32_t sample_texture(const mtgl_raster_state *rs, const TexLevel &
     return sample_level(L0, rep_s, rep_t, u, v, filter == GL_LINEAR, taps);
 }
 
-__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
-{
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
 
 // per-pixel state a lane carries through its tile
 struct PixelState {
@@ -1157,16 +1269,11 @@ struct PixelState {
     bool dirty_c, dirty_d, dirty_s;
 };
 
-struct RasterConsts {
-    bool depth_on, stencil_on, blend_on, fog_on, alpha_on, lighting_on, textured, persp, flat, phong, unit_range;
-    uint32_t cmask;
-    TexLevel L0, L1;
-};
 
-__device__ __forceinline__ void stencil_write(const mtgl_raster_state *rs, PixelState &px, uint32_t op)
+__device__ __forceinline__ void stencil_write(const RasterConsts &K, PixelState &px, uint32_t op)
 {
-    uint32_t nv = stencil_op(op, px.s, rs->stencil_ref);
-    uint32_t m8 = rs->stencil_writemask & 0xFFu;
+    uint32_t nv = stencil_op(op, px.s, K.stencil_ref);
+    uint32_t m8 = K.stencil_writemask & 0xFFu;
     px.s = ((px.s & ~m8) | (nv & m8)) & 0xFFu;
     px.dirty_s = true;
 }
@@ -1184,7 +1291,7 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
     const mtgl_raster_state *rs = P.rs;
     const float z = b0 * r->z0 + b1 * r->z1 + b2 * r->z2;
     if (K.unit_range) depth_out = (z + 1.0f) * 0.5f;   // (double)q * 1.0 + 0.0 rounds back to q
-    else depth_out = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
+    else depth_out = (float)((double)((z + 1.0f) * 0.5f) * (K.depth_far - K.depth_near) + K.depth_near);
 
     Col c;
     if (K.flat) {
@@ -1197,7 +1304,7 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
     }
     meta_out = 1u;
     if (K.lighting_on) {
-        const bool two = r->back && rs->lighting.two_side;
+        const bool two = r->back && K.two_side;
         if (K.phong || two) {
             const RCold *q = &P.rcold[rid];
             float ep[3], en[3];
@@ -1231,20 +1338,20 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
             v = b0 * r->uv0[1] + b1 * r->uv1[1] + b2 * r->uv2[1];
         }
         int taps = 0;
-        Col tc = col_unpack(sample_texture(rs, K.L0, K.L1, u, v, r->lod, taps));
-        if (K.alpha_on && !compare_op(rs->alpha_func, tc.a, rs->alpha_ref)) {
+        Col tc = col_unpack(sample_texture(K, K.L0, K.L1, u, v, r->lod, taps));
+        if (K.alpha_on && !cmp_f(K.alpha_cmp, tc.a, K.alpha_ref)) {
             c_out = c;
             meta_out = (uint32_t)taps << 1;
             return;
         }
         meta_out = 1u | ((uint32_t)taps << 1);
-        switch (rs->tex_env_mode) {
+        switch (K.tex_env_mode) {
         case GL_REPLACE: c = tc; break;
         case GL_DECAL:
             c = {c.r + (tc.r - c.r) * tc.a, c.g + (tc.g - c.g) * tc.a, c.b + (tc.b - c.b) * tc.a, c.a};
             break;
         case GL_BLEND: {
-            Col e = col_of(rs->tex_env_color);
+            Col e = K.tex_env_color;
             c = {c.r * (1.0f - tc.r) + e.r * tc.r, c.g * (1.0f - tc.g) + e.g * tc.g, c.b * (1.0f - tc.b) + e.b * tc.b,
                  c.a * tc.a};
             break;
@@ -1256,16 +1363,16 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
     if (K.fog_on) {
         float fc = b0 * r->ez0 + b1 * r->ez1 + b2 * r->ez2;
         float f;
-        switch (rs->fog_mode) {
-        case GL_LINEAR: f = rs->fog_end != rs->fog_start ? (rs->fog_end - fc) / (rs->fog_end - rs->fog_start) : 1.0f; break;
-        case GL_EXP: f = exp_cr(-rs->fog_density * fc); break;
-        case GL_EXP2: { float d = rs->fog_density * fc; f = exp_cr(-d * d); break; }
+        switch (K.fog_mode) {
+        case GL_LINEAR: f = K.fog_end != K.fog_start ? (K.fog_end - fc) / (K.fog_end - K.fog_start) : 1.0f; break;
+        case GL_EXP: f = exp_cr(-K.fog_density * fc); break;
+        case GL_EXP2: { float d = K.fog_density * fc; f = exp_cr(-d * d); break; }
         default: f = 1.0f; break;
         }
         if (f < 0.0f) f = 0.0f;
         if (f > 1.0f) f = 1.0f;
         // color_lerp_rgb(fog_color, c, f): the alpha is the fog colour's
-        Col fg = col_of(rs->fog_color);
+        Col fg = K.fog_color;
         c = {fg.r + (c.r - fg.r) * f, fg.g + (c.g - fg.g) * f, fg.b + (c.b - fg.b) * f, fg.a};
     }
     c_out = c;
@@ -1278,27 +1385,42 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
 // First half: stencil test, depth test, stencil update, the texel bytes the
 // fragment read, the alpha-test discard (kept == false), depth write and the
 // colour bytes.  Returns whether the colour step runs.
-__device__ __forceinline__ bool depth_stencil_step(const mtgl_raster_state *rs, const RasterConsts &K, float depth,
+__device__ __forceinline__ bool depth_stencil_step(const RasterConsts &K, float depth,
                                                    uint32_t taps, bool kept, PixelState &px, unsigned long long &bytes)
 {
+    if (!K.stencil_on) {
+        // no stencil: the depth test alone, written to select rather than branch
+        bool pass = true;
+        if (K.depth_on) {
+            bytes += 4;
+            pass = cmp_f(K.depth_cmp, depth, px.d);
+        }
+        if (pass) bytes += 4u * taps + (kept ? K.pass_bytes : 0u);
+        const bool write = pass && kept;
+        if (write && K.depth_write) {
+            px.d = depth;
+            px.dirty_d = true;
+        }
+        return write;
+    }
     if (K.stencil_on) {
         bytes += 2;
-        if (!stencil_test(rs->stencil_func, rs->stencil_ref, rs->stencil_mask, px.s)) {
-            stencil_write(rs, px, rs->stencil_fail);
+        if (!cmp_stencil(K.stencil_cmp, K.stencil_ref, K.stencil_mask, px.s)) {
+            stencil_write(K, px, K.stencil_fail);
             return false;
         }
     }
     if (K.depth_on) {
         bytes += 4;
-        if (!compare_op(rs->depth_func, depth, px.d)) {
-            if (K.stencil_on) stencil_write(rs, px, rs->stencil_zfail);
+        if (!cmp_f(K.depth_cmp, depth, px.d)) {
+            if (K.stencil_on) stencil_write(K, px, K.stencil_zfail);
             return false;
         }
     }
-    if (K.stencil_on) stencil_write(rs, px, rs->stencil_zpass);
+    if (K.stencil_on) stencil_write(K, px, K.stencil_zpass);
     bytes += 4u * taps;
     if (!kept) return false;
-    if (K.depth_on && rs->depth_mask) {
+    if (K.depth_on && K.depth_mask) {
         bytes += 4;
         px.d = depth;
         px.dirty_d = true;
@@ -1309,12 +1431,12 @@ __device__ __forceinline__ bool depth_stencil_step(const mtgl_raster_state *rs, 
 }
 
 // Second half: blend with the pixel's colour, clamp, colour mask, pack.
-__device__ __forceinline__ void color_step(const mtgl_raster_state *rs, const RasterConsts &K, Col c, PixelState &px)
+__device__ __forceinline__ void color_step(const RasterConsts &K, Col c, PixelState &px)
 {
     if (K.blend_on) {
         Col dst = col_unpack(px.c);
-        Col sf = blend_factor(rs->blend_src, c, dst);
-        Col df = blend_factor(rs->blend_dst, c, dst);
+        Col sf = blend_factor(K.blend_src, c, dst);
+        Col df = blend_factor(K.blend_dst, c, dst);
         c = col_clamp(col_add(col_mul(c, sf), col_mul(dst, df)));
     }
     c = col_clamp(c);
@@ -1344,21 +1466,10 @@ __device__ __forceinline__ void edge_values(const RTri *r, int32_t x, int32_t y,
     w2e = (px - fx0) * (fy1 - fy0) - (py - fy0) * (fx1 - fx0);
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
-{
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(v, (unsigned)d, 64);
-        if (lane >= d) v += y;
-    }
-    return v;
-}
-
 constexpr int CHUNK = 64;          // bin entries per coverage pass: one bit each in a pixel's mask
-constexpr int FCAP = 128;          // fragment sources held in LDS per batch
+constexpr int FCAP = 128;          // fragment colours held in LDS per batch (general kernel)
 
-// the fragment-stage part of a record, as staged in LDS
+// the colour-stage part of a record, as staged in LDS
 struct __align__(16) RFrag {
     float4 v[RT_FRAG_VEC4];
 };
@@ -1372,22 +1483,22 @@ struct __align__(16) RCov {
     float inv_area, z0, z1, z2;
 };
 
-// field of the record held as 11 float4 registers
+// field of a record held as float4 registers
 template <int OFF>
 __device__ __forceinline__ float rec_f(const float4 *q)
 {
     constexpr int v = OFF / 16, c = (OFF % 16) / 4;
     return c == 0 ? q[v].x : (c == 1 ? q[v].y : (c == 2 ? q[v].z : q[v].w));
 }
-#define REC_F(q, field, extra) rec_f<(int)offsetof(RTri, field) + (extra)>(q)
+#define REC_F(q, field) rec_f<(int)offsetof(RTri, field)>(q)
 #define REC_I(q, field) __float_as_int(rec_f<(int)offsetof(RTri, field)>(q))
 
 // texels one fragment of a triangle with this LOD reads (the branch structure
-// of sample_texture: taps depend on the triangle, not on the pixel)
-__device__ __forceinline__ uint32_t texel_taps(const mtgl_raster_state *rs, const RasterConsts &K, float lod)
+// of sample_texture: the count depends on the triangle, not on the pixel)
+__device__ __forceinline__ uint32_t texel_taps(const RasterConsts &K, float lod)
 {
     if (!K.textured) return 0u;
-    const uint32_t filter = lod > 0.0f ? rs->min_filter : rs->mag_filter;
+    const uint32_t filter = lod > 0.0f ? K.min_filter : K.mag_filter;
     if (filter == GL_NEAREST_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_NEAREST) {
         if (lod >= 0.5f) return K.L1.px ? (filter == GL_LINEAR_MIPMAP_NEAREST ? 4u : 1u) : 0u;
         return filter == GL_LINEAR_MIPMAP_NEAREST ? 4u : 1u;
@@ -1401,7 +1512,7 @@ __device__ __forceinline__ uint32_t texel_taps(const mtgl_raster_state *rs, cons
 
 template <int R, int I>
 __device__ __forceinline__ void cover_pixel(const float *a0, const float *a1, const float *a2, float b0, float b1,
-                                            float b2, int32_t &mlo, int32_t &mhi, uint32_t &tri_px)
+                                            float b2, int32_t &mlo, int32_t &mhi)
 {
     const bool p = (a0[I] >= b0) & (a1[I] >= b1) & (a2[I] >= b2);
     const uint64_t bal = __builtin_amdgcn_ballot_w64(p);
@@ -1409,81 +1520,131 @@ __device__ __forceinline__ void cover_pixel(const float *a0, const float *a1, co
         mlo = (int32_t)(uint32_t)bal;
         mhi = (int32_t)(uint32_t)(bal >> 32);
     }
-    tri_px += p ? 1u : 0u;
 }
 
 template <int R>
 __device__ __forceinline__ void cover_row(const float *a0, const float *a1, const float *a2, const float *b0,
-                                          const float *b1, const float *b2, int32_t &mlo, int32_t &mhi,
-                                          uint32_t &tri_px)
+                                          const float *b1, const float *b2, int32_t &mlo, int32_t &mhi)
 {
-    cover_pixel<R, 0>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 1>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 2>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 3>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 4>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 5>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 6>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
-    cover_pixel<R, 7>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi, tri_px);
+    cover_pixel<R, 0>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 1>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 2>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 3>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 4>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 5>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 6>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 7>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
 }
 
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+
+// Depth-pass terms of a bin entry from its first 64 record bytes.
+__device__ __forceinline__ RCov make_cov(const float4 *q)
 {
+    RCov cv;
+    const float fx0 = (float)REC_I(q, x0), fy0 = (float)REC_I(q, y0);
+    const float fx1 = (float)REC_I(q, x1), fy1 = (float)REC_I(q, y1);
+    const float fx2 = (float)REC_I(q, x2), fy2 = (float)REC_I(q, y2);
+    cv.fx1 = fx1; cv.fy1 = fy1; cv.d0y = fy2 - fy1; cv.d0x = fx2 - fx1;
+    cv.fx2 = fx2; cv.fy2 = fy2; cv.d1y = fy0 - fy2; cv.d1x = fx0 - fx2;
+    cv.fx0 = fx0; cv.fy0 = fy0; cv.d2y = fy1 - fy0; cv.d2x = fx1 - fx0;
+    cv.inv_area = REC_F(q, inv_area);
+    cv.z0 = REC_F(q, z0); cv.z1 = REC_F(q, z1); cv.z2 = REC_F(q, z2);
+    return cv;
+}
+
+// Coverage of one chunk, one lane per bin entry.  The edge functions are
+// separable, w = A(x) - B(y) with A and B the reference's rounded products,
+// so the lane forms 8 column and 8 row terms per edge and a pixel's test is
+// three compares (A and B are finite integer-valued floats, so fl(A - B) >= 0
+// exactly when A >= B).  One ballot per pixel gives the mask of entries
+// covering it, kept by that pixel's lane.  Negative-area triangles are
+// tested with A and B negated; the bounding box and the framebuffer edge
+// fold in as -inf / +inf.
+__device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, int32_t minY, int32_t maxX, int32_t maxY,
+                                                float area, bool valid, int32_t X0, int32_t Y0, int32_t fb_w,
+                                                int32_t fb_h)
+{
+    const float fx0 = cv.fx0, fy0 = cv.fy0, fx1 = cv.fx1, fy1 = cv.fy1, fx2 = cv.fx2, fy2 = cv.fy2;
+    const float d0y = cv.d0y, d0x = cv.d0x, d1y = cv.d1y, d1x = cv.d1x, d2y = cv.d2y, d2x = cv.d2x;
+    const bool live = valid && (area > 0.0f || area < 0.0f);
+    const float sg = area > 0.0f ? 1.0f : -1.0f;
+    float a0[TILE], a1[TILE], a2[TILE], b0[TILE], b1[TILE], b2[TILE];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
+    for (int i = 0; i < TILE; i++) {
+        const int32_t xi = X0 + i, yi = Y0 + i;
+        const float pxi = (float)xi, pyi = (float)yi;
+        a0[i] = sg * ((pxi - fx1) * d0y);
+        a1[i] = sg * ((pxi - fx2) * d1y);
+        a2[i] = sg * ((pxi - fx0) * d2y);
+        b0[i] = sg * ((pyi - fy1) * d0x);
+        b1[i] = sg * ((pyi - fy2) * d1x);
+        b2[i] = sg * ((pyi - fy0) * d2x);
+        if (!(live && xi >= minX && xi <= maxX && xi < fb_w)) a0[i] = -__builtin_inff();
+        if (!(yi >= minY && yi <= maxY && yi < fb_h)) b0[i] = __builtin_inff();
+    }
+    int32_t mlo = 0, mhi = 0;
+    cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<2>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<3>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<4>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<5>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<6>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    cover_row<7>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    return ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
 }
 
-// K4: one 8x8 tile per block of one wave.  Per chunk of 64 bin entries:
-//  1. coverage, one lane per triangle: the edge functions are separable,
-//     w = A(x) - B(y) with A and B the reference's rounded products, so each
-//     lane forms 8 column and 8 row terms per edge and a pixel's test is
-//     three compares (A and B are finite integers-valued floats, so
-//     fl(A - B) >= 0 exactly when A >= B); one ballot per pixel gives the
-//     mask of triangles covering it, written into that pixel's lane.
-//     Triangles with negative area are tested with A and B negated; the
-//     bounding box and the framebuffer edge fold in as -inf / +inf.
-//  2. depth and stencil in submission order, one pixel per lane, unless the
-//     alpha test (which reads the texture) decides depth writes.
-//  3. colour: when nothing blends, only the last fragment that passed a
-//     pixel can leave its colour, so one fragment per pixel is shaded;
-//     otherwise every passing fragment is shaded, one per lane in batches of
-//     at most FCAP, and blended per pixel in order.
-// Tiles whose bins exceed HEAVY_BIN entries come from a list and occupy the
-// first blocks, so the longest tiles start first.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster(RasterParams P)
+// Depth and stencil of this lane's pixel for the entries in m, in
+// submission order; returns the entries that passed (the ones whose colour
+// step runs).  Valid when no alpha test reads the texture.
+__device__ __forceinline__ uint64_t depth_pass(const RasterConsts &K, const RCov *s_cov, const uint32_t *s_tap,
+                                               uint64_t m, float xf, float yf, PixelState &px,
+                                               unsigned long long &bytes)
 {
-    __shared__ RFrag s_rec[CHUNK];
-    __shared__ RCov s_cov[CHUNK];
-    __shared__ uint32_t s_rid[CHUNK];
-    __shared__ uint32_t s_tap[CHUNK];
-    __shared__ float4 s_col[FCAP];
-    __shared__ float s_dep[FCAP];
-    __shared__ uint16_t s_slot[FCAP];
-    __shared__ uint8_t s_meta[FCAP];
-
-    const mtgl_raster_state *rs = P.rs;
-    const int lane = threadIdx.x;
-
-    uint32_t tile, e0, e1;
-    if (blockIdx.x < P.heavy_cap) {
-        const uint32_t nh = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nheavy);
-        if (blockIdx.x >= nh) return;
-        tile = P.heavy[blockIdx.x];
-        e0 = P.tile_start[tile];
-        e1 = P.tile_end[tile];
-    } else {
-        tile = blockIdx.x - P.heavy_cap;
-        e1 = P.tile_end[tile];
-        if (e1 == 0u) return;
-        e0 = P.tile_start[tile];
-        if (e1 - e0 > HEAVY_BIN) return;   // dispatched from the heavy list
+    uint64_t pass = 0;
+    while (m) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        m &= m - 1;
+        const RCov &cv = s_cov[k];
+        const float w0e = (xf - cv.fx1) * cv.d0y - (yf - cv.fy1) * cv.d0x;
+        const float w1e = (xf - cv.fx2) * cv.d1y - (yf - cv.fy2) * cv.d1x;
+        const float w2e = (xf - cv.fx0) * cv.d2y - (yf - cv.fy0) * cv.d2x;
+        const float bb0 = w0e * cv.inv_area, bb1 = w1e * cv.inv_area, bb2 = w2e * cv.inv_area;
+        const float z = bb0 * cv.z0 + bb1 * cv.z1 + bb2 * cv.z2;
+        float depth;
+        if (K.unit_range) depth = (z + 1.0f) * 0.5f;
+        else depth = (float)((double)((z + 1.0f) * 0.5f) * (K.depth_far - K.depth_near) + K.depth_near);
+        if (depth_stencil_step(K, depth, s_tap[k], true, px, bytes)) pass |= 1ull << k;
     }
-    tile = __builtin_amdgcn_readfirstlane(tile);
-    e0 = __builtin_amdgcn_readfirstlane(e0);
-    e1 = __builtin_amdgcn_readfirstlane(e1);
+    return pass;
+}
 
-    RasterConsts K;
+// The raster kernels run a persistent grid over the list of non-empty
+// tiles: block b takes items b, b + gridDim.x, ...; items [0, nheavy) are the
+// tiles with bins over HEAVY_BIN entries (the front of the list), the rest
+// come from the back, so the longest tiles start first.
+__device__ __forceinline__ void list_counts(const RasterParams &P, uint32_t &nh, uint32_t &it0, uint32_t &nt)
+{
+    nh = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nheavy);
+    nt = nh + __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nlight);
+    nt = min(nt, P.ntiles);
+    nh = min(nh, nt);
+    it0 = P.part == 2u ? nh : 0u;
+    if (P.part == 1u) nt = nh;
+}
+
+__device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &tile, uint32_t &e0,
+                                          uint32_t &e1)
+{
+    tile = it < nh ? P.list[it] : P.list[P.ntiles - 1u - (it - nh)];
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    e0 = __builtin_amdgcn_readfirstlane(P.tile_start[tile]);
+    e1 = __builtin_amdgcn_readfirstlane(P.tile_end[tile]);
+}
+
+__device__ __forceinline__ void raster_consts(const RasterParams &P, RasterConsts &K)
+{
+    const mtgl_raster_state *rs = P.rs;
     const uint32_t flags = rs->flags;
     K.depth_on = flags & MTGL_F_DEPTH_TEST;
     K.stencil_on = flags & MTGL_F_STENCIL_TEST;
@@ -1497,16 +1658,275 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     K.flat = rs->shade_model == GL_FLAT;
     K.phong = rs->shade_model == GL_PHONG;
     K.unit_range = rs->depth_near == 0.0 && rs->depth_far == 1.0;
+    K.two_side = rs->lighting.two_side != 0;
+    K.rep_s = rs->wrap_s == GL_REPEAT;
+    K.rep_t = rs->wrap_t == GL_REPEAT;
+    K.depth_mask = rs->depth_mask != 0;
+    K.depth_write = K.depth_on && K.depth_mask;
+    K.pass_bytes = (K.depth_write ? 4u : 0u) + (K.cmask == 0xFu ? (K.blend_on ? 8u : 4u) : (K.cmask ? 8u : 0u));
+    K.depth_cmp = cmp_mask_of(rs->depth_func);
+    K.stencil_cmp = cmp_mask_of(rs->stencil_func);
+    K.stencil_mask = rs->stencil_mask;
+    K.stencil_writemask = rs->stencil_writemask;
+    K.stencil_fail = rs->stencil_fail;
+    K.stencil_zfail = rs->stencil_zfail;
+    K.stencil_zpass = rs->stencil_zpass;
+    K.stencil_ref = rs->stencil_ref;
+    K.blend_src = rs->blend_src;
+    K.blend_dst = rs->blend_dst;
+    K.alpha_cmp = cmp_mask_of(rs->alpha_func);
+    K.alpha_ref = rs->alpha_ref;
+    K.tex_env_mode = rs->tex_env_mode;
+    K.tex_env_color = col_of(rs->tex_env_color);
+    K.fog_mode = rs->fog_mode;
+    K.fog_density = rs->fog_density;
+    K.fog_start = rs->fog_start;
+    K.fog_end = rs->fog_end;
+    K.fog_color = col_of(rs->fog_color);
+    K.min_filter = rs->min_filter;
+    K.mag_filter = rs->mag_filter;
+    K.depth_near = rs->depth_near;
+    K.depth_far = rs->depth_far;
     K.L0 = {P.tex, P.tex_w, P.tex_h, P.tex_w > 0 && (P.tex_w & (P.tex_w - 1)) == 0 && (P.tex_h & (P.tex_h - 1)) == 0};
     K.L1 = {P.tex1, P.tex1_w, P.tex1_h,
             P.tex1_w > 0 && (P.tex1_w & (P.tex1_w - 1)) == 0 && (P.tex1_h & (P.tex1_h - 1)) == 0};
+}
+
+// Per-lane counts summed over the wave (bytes as two 32-bit halves) and
+// added to this block's own slot: no two blocks of a launch share a slot and
+// launches on the stream are ordered, so plain adds suffice (thousands of
+// same-address atomics would queue at one memory channel and stall every
+// load that maps there).
+__device__ __forceinline__ void add_stats(const RasterParams &P, unsigned long long frags, unsigned long long bytes)
+{
+    const uint32_t f = dpp_wave_sum((uint32_t)frags);
+    const unsigned long long blo = dpp_wave_sum((uint32_t)(bytes & 0xFFFFFFFFull));
+    const unsigned long long bhi = dpp_wave_sum((uint32_t)(bytes >> 32));
+    if (threadIdx.x == 0 && f) {
+        unsigned long long *slot = P.stats + 2u * blockIdx.x;
+        slot[0] += (unsigned long long)f;
+        slot[1] += blo + (bhi << 32);
+    }
+}
+
+// Orders this wave's LDS accesses before and after it.  The raster kernels
+// run one wave per block, whose LDS operations complete in issue order, so
+// no s_barrier (and no wait on outstanding global loads) is needed.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr int DCAP = 512;          // fragment depths held in LDS per batch (depth kernel)
+
+// K4a, the depth kernel of draws without blending, alpha test on a texture or
+// a partial colour mask: when nothing blends, only the last fragment that
+// passes a pixel can leave its colour there, so this kernel runs coverage and
+// the in-order depth/stencil pass, writes depth and stencil, and records per
+// pixel 1 + the record slot of that last fragment (0: none) for K4b.  One 8x8
+// tile per one-wave block; it reads only the first 64 bytes of each record.
+// Per chunk: coverage (cover_chunk); then batches of at most DCAP fragments
+// in pixel-major order: the fragments' window depths one per lane, and per
+// pixel lane the depth/stencil steps in submission order over its own
+// fragments, which only load, compare and select.
+template <bool STENCIL>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_raster_depth(RasterParams P)
+{
+    __shared__ RCov s_cov[CHUNK];
+    __shared__ uint32_t s_rid[CHUNK];
+    __shared__ uint32_t s_tap[CHUNK];
+    __shared__ float s_dep[DCAP];
+    __shared__ uint16_t s_slot[DCAP];   // taps << 12 | entry << 6 | pixel
+
+    RasterConsts K;
+    raster_consts(P, K);
+    K.stencil_on = STENCIL;   // the instantiation matches the draw's state
+    const int lane = threadIdx.x;
+    uint32_t nh, it0, nt;
+    list_counts(P, nh, it0, nt);
+    uint32_t frag_count = 0;
+    unsigned long long frag_bytes = 0;
+    for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
+    uint32_t tile, e0, e1;
+    item_tile(P, it, nh, tile, e0, e1);
+    const int32_t X0 = (int32_t)(tile % (uint32_t)P.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)P.tiles_x) * TILE;
+    const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
+    const bool in_fb = x < P.fb_w && y < P.fb_h;
+    const size_t pix = in_fb ? (size_t)y * P.fb_w + x : 0;
+    PixelState px = {0u, 1.0f, 0u, false, false, false};
+    if (in_fb) {
+        if (K.depth_on) px.d = P.depth[pix];
+        if (K.stencil_on) px.s = P.stencil[pix];
+    }
+    uint32_t writer = 0;
+    // the next chunk's entries are loaded (contiguous, in bin order) while the
+    // current one is processed
+    float4 qn[RT_COV_VEC4];
+    uint32_t ridn = 0;
+    {
+        const bool v = (uint32_t)lane < e1 - e0;
+        const uint32_t e = v ? e0 + lane : e0;
+        ridn = P.vals[e];
+#pragma unroll
+        for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = P.cov[(size_t)e * RT_COV_VEC4 + i];
+    }
+    for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
+        const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
+        const bool valid = (uint32_t)lane < cn;
+        float4 q[RT_COV_VEC4];
+#pragma unroll
+        for (int i = 0; i < RT_COV_VEC4; i++) q[i] = qn[i];
+        const uint32_t rid = ridn;
+        if (c0 + CHUNK < e1) {
+            const bool v = (uint32_t)lane < e1 - (c0 + CHUNK);
+            const uint32_t e = v ? c0 + CHUNK + lane : c0 + CHUNK;
+            ridn = P.vals[e];
+#pragma unroll
+            for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = P.cov[(size_t)e * RT_COV_VEC4 + i];
+        }
+        const RCov cv = make_cov(q);
+        wave_sync();   // the previous chunk's entries are no longer read
+        if (valid) {
+            s_cov[lane] = cv;
+            s_rid[lane] = rid;
+            s_tap[lane] = texel_taps(K, REC_F(q, lod));
+        }
+        const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
+                                       REC_F(q, area), valid, X0, Y0, P.fb_w, P.fb_h);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        frag_count += cnt;
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int32_t)dpp_incl_scan(cnt), 63);
+        uint64_t rem = m;
+        int32_t last = -1;   // entry of this pixel's last passing fragment in the chunk
+        bool first = true;
+        while (total && (first || __builtin_amdgcn_ballot_w64(rem != 0ull))) {
+            // each pixel takes its next L fragments, L the largest quota that
+            // keeps the batch within DCAP (64 always fits)
+            uint32_t L = 64;
+            if (!first || total > (uint32_t)DCAP) {
+                const uint32_t rc = (uint32_t)__popcll(rem);
+                L = 1;
+#pragma unroll
+                for (uint32_t step = 32; step >= 1; step >>= 1) {
+                    const uint32_t sum = (uint32_t)__builtin_amdgcn_readlane((int32_t)dpp_incl_scan(min(rc, L + step)), 63);
+                    if (sum <= (uint32_t)DCAP) L += step;
+                }
+            }
+            first = false;
+            const uint32_t take = min((uint32_t)__popcll(rem), L);
+            const uint32_t ij = dpp_incl_scan(take);
+            const uint32_t off = ij - take;
+            const uint32_t fr = min((uint32_t)__builtin_amdgcn_readlane((int32_t)ij, 63), (uint32_t)DCAP);
+            for (uint32_t o = off, qn_ = 0; qn_ < take; qn_++, o++) {
+                const uint32_t k = (uint32_t)__builtin_ctzll(rem);
+                rem &= rem - 1;
+                if (o < (uint32_t)DCAP) s_slot[o] = (uint16_t)((k << 6) | (uint32_t)lane);
+            }
+            wave_sync();
+            // window depth of every fragment of the batch, one per lane
+            for (uint32_t f0 = 0; f0 < fr; f0 += 64) {
+                const uint32_t f = f0 + (uint32_t)lane;
+                if (f < fr) {
+                    const uint32_t sl = s_slot[f];
+                    const uint32_t k = (sl >> 6) & 63u, j = sl & 63u;
+                    const RCov &c = s_cov[k];
+                    const float fxp = (float)(X0 + (int32_t)(j & 7u)), fyp = (float)(Y0 + (int32_t)(j >> 3));
+                    const float w0e = (fxp - c.fx1) * c.d0y - (fyp - c.fy1) * c.d0x;
+                    const float w1e = (fxp - c.fx2) * c.d1y - (fyp - c.fy2) * c.d1x;
+                    const float w2e = (fxp - c.fx0) * c.d2y - (fyp - c.fy0) * c.d2x;
+                    const float bb0 = w0e * c.inv_area, bb1 = w1e * c.inv_area, bb2 = w2e * c.inv_area;
+                    const float z = bb0 * c.z0 + bb1 * c.z1 + bb2 * c.z2;
+                    float depth;
+                    if (K.unit_range) depth = (z + 1.0f) * 0.5f;
+                    else depth = (float)((double)((z + 1.0f) * 0.5f) * (K.depth_far - K.depth_near) + K.depth_near);
+                    s_dep[f] = depth;
+                    s_slot[f] = (uint16_t)((s_tap[k] << 12) | sl);
+                }
+            }
+            wave_sync();
+            // per pixel, in submission order
+            for (uint32_t qn_ = 0; qn_ < take && off + qn_ < (uint32_t)DCAP; qn_++) {
+                const uint32_t f = off + qn_;
+                const float depth = s_dep[f];
+                const uint32_t sl = s_slot[f];
+                if (depth_stencil_step(K, depth, sl >> 12, true, px, frag_bytes)) last = (int32_t)((sl >> 6) & 63u);
+            }
+            wave_sync();   // slots and depths are reused by the next batch
+        }
+        if (last >= 0) writer = 1u + s_rid[last];
+    }
+    if (in_fb) {
+        if (px.dirty_d) P.depth[pix] = px.d;
+        if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
+        P.writer[pix] = writer;
+    }
+    }
+    add_stats(P, frag_count, frag_bytes);
+}
+
+// K4b: the colour of each pixel's last passing fragment (K4a's record),
+// clamped and packed; one tile per one-wave block, one pixel per lane, the
+// record read from HBM / L2.
+__global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
+{
+    RasterConsts K;
+    raster_consts(P, K);
+    const int lane = threadIdx.x;
+    uint32_t nh, it0, nt;
+    list_counts(P, nh, it0, nt);
+    for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(it < nh ? P.list[it] : P.list[P.ntiles - 1u - (it - nh)]);
+    const int32_t x = (int32_t)(tile % (uint32_t)P.tiles_x) * TILE + (lane & 7);
+    const int32_t y = (int32_t)(tile / (uint32_t)P.tiles_x) * TILE + (lane >> 3);
+    if (x >= P.fb_w || y >= P.fb_h) continue;
+    const size_t pix = (size_t)y * P.fb_w + x;
+    const uint32_t writer = P.writer[pix];
+    if (writer == 0u) continue;
+    const RTri *r = P.rt + (writer - 1u);
+    float w0e, w1e, w2e;
+    edge_values(r, x, y, w0e, w1e, w2e);
+    const float inv = r->inv_area;
+    Col c;
+    float dep;
+    uint32_t meta;
+    fragment_source(P, K, r, writer - 1u, w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
+    P.color[pix] = col_pack(col_clamp(c));
+    }
+}
+
+// K4, general: blending, a partial colour mask, or an alpha test on a
+// texture.  One 8x8 tile per one-wave block.  Per chunk of 64 bin entries:
+// coverage (cover_chunk); unless the alpha test decides depth writes, the
+// in-order depth/stencil pass; then colours for the fragments whose colour
+// step runs (all covered ones under an alpha test on a texture), one
+// fragment per lane in batches of at most FCAP, applied per pixel in
+// submission order.
+__global__ void __launch_bounds__(64) k_raster(RasterParams P)
+{
+    __shared__ RFrag s_rec[CHUNK];
+    __shared__ RCov s_cov[CHUNK];
+    __shared__ uint32_t s_rid[CHUNK];
+    __shared__ uint32_t s_tap[CHUNK];
+    __shared__ float4 s_col[FCAP];
+    __shared__ float s_dep[FCAP];
+    __shared__ uint16_t s_slot[FCAP];
+    __shared__ uint8_t s_meta[FCAP];
+
+    RasterConsts K;
+    raster_consts(P, K);
+    const int lane = threadIdx.x;
     const bool need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
     // the alpha test reads the texture, and its outcome gates the depth write
     const bool alpha_dep = K.alpha_on && K.textured;
-    const bool last_writer = !alpha_dep && !K.blend_on && (K.cmask == 0xFu || K.cmask == 0u);
-
-    const int32_t tx = (int32_t)(tile % (uint32_t)P.tiles_x), ty = (int32_t)(tile / (uint32_t)P.tiles_x);
-    const int32_t X0 = tx * TILE, Y0 = ty * TILE;
+    uint32_t nh, it0, nt;
+    list_counts(P, nh, it0, nt);
+    unsigned long long frag_count = 0, frag_bytes = 0;
+    for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
+    uint32_t tile, e0, e1;
+    item_tile(P, it, nh, tile, e0, e1);
+    const int32_t X0 = (int32_t)(tile % (uint32_t)P.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)P.tiles_x) * TILE;
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
     const float xf = (float)x, yf = (float)y;
     const bool in_fb = x < P.fb_w && y < P.fb_h;
@@ -1518,111 +1938,34 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         if (K.stencil_on) px.s = P.stencil[pix];
     }
 
-    unsigned long long frag_count = 0, frag_bytes = 0;
     for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
         const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
-        __syncthreads();   // the previous chunk's records are no longer read
-        // ---- 1. stage entry `lane` and test it against the 64 pixels
         const bool valid = (uint32_t)lane < cn;
+        wave_sync();   // the previous chunk's records are no longer read
         float4 q[RT_FRAG_VEC4];
 #pragma unroll
         for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t rid = 0;
         if (valid) {
-            const uint32_t rid = P.vals[c0 + lane];
+            rid = P.vals[c0 + lane];
             const float4 *src = reinterpret_cast<const float4 *>(&P.rt[rid]);
 #pragma unroll
             for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = src[i];
 #pragma unroll
             for (int i = 0; i < RT_FRAG_VEC4; i++) s_rec[lane].v[i] = q[i];
-            s_rid[lane] = rid;
         }
-        const float fx0 = (float)REC_I(q, x0), fy0 = (float)REC_I(q, y0);
-        const float fx1 = (float)REC_I(q, x1), fy1 = (float)REC_I(q, y1);
-        const float fx2 = (float)REC_I(q, x2), fy2 = (float)REC_I(q, y2);
-        const int32_t minX = REC_I(q, minX), minY = REC_I(q, minY);
-        const int32_t maxX = REC_I(q, maxX), maxY = REC_I(q, maxY);
-        const float area = REC_F(q, area, 0);
-        const float d0y = fy2 - fy1, d0x = fx2 - fx1;
-        const float d1y = fy0 - fy2, d1x = fx0 - fx2;
-        const float d2y = fy1 - fy0, d2x = fx1 - fx0;
+        const RCov cv = make_cov(q);
         if (valid) {
-            RCov cv;
-            cv.fx1 = fx1; cv.fy1 = fy1; cv.d0y = d0y; cv.d0x = d0x;
-            cv.fx2 = fx2; cv.fy2 = fy2; cv.d1y = d1y; cv.d1x = d1x;
-            cv.fx0 = fx0; cv.fy0 = fy0; cv.d2y = d2y; cv.d2x = d2x;
-            cv.inv_area = REC_F(q, inv_area, 0);
-            cv.z0 = REC_F(q, z0, 0); cv.z1 = REC_F(q, z1, 0); cv.z2 = REC_F(q, z2, 0);
             s_cov[lane] = cv;
-            s_tap[lane] = texel_taps(rs, K, REC_F(q, lod, 0));
+            s_rid[lane] = rid;
+            s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
-        const bool live = valid && (area > 0.0f || area < 0.0f);
-        const float sg = area > 0.0f ? 1.0f : -1.0f;
-        float a0[TILE], a1[TILE], a2[TILE], b0[TILE], b1[TILE], b2[TILE];
-#pragma unroll
-        for (int i = 0; i < TILE; i++) {
-            const int32_t xi = X0 + i, yi = Y0 + i;
-            const float pxi = (float)xi, pyi = (float)yi;
-            a0[i] = sg * ((pxi - fx1) * d0y);
-            a1[i] = sg * ((pxi - fx2) * d1y);
-            a2[i] = sg * ((pxi - fx0) * d2y);
-            b0[i] = sg * ((pyi - fy1) * d0x);
-            b1[i] = sg * ((pyi - fy2) * d1x);
-            b2[i] = sg * ((pyi - fy0) * d2x);
-            if (!(live && xi >= minX && xi <= maxX && xi < P.fb_w)) a0[i] = -__builtin_inff();
-            if (!(yi >= minY && yi <= maxY && yi < P.fb_h)) b0[i] = __builtin_inff();
-        }
-        int32_t mlo = 0, mhi = 0;
-        uint32_t tri_px = 0;
-        cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<2>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<3>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<4>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<5>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<6>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        cover_row<7>(a0, a1, a2, b0, b1, b2, mlo, mhi, tri_px);
-        (void)tri_px;
-        const uint64_t m = ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
+        const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
+                                       REC_F(q, area), valid, X0, Y0, P.fb_w, P.fb_h);
         frag_count += (unsigned long long)__popcll(m);
-        __syncthreads();   // staged records visible to every lane
+        wave_sync();   // staged records visible to every lane
 
-        // ---- 2. depth and stencil in order
-        uint64_t todo = m;   // fragments whose colour step runs
-        if (!alpha_dep) {
-            uint64_t t = m, pass = 0;
-            while (t) {
-                const uint32_t k = (uint32_t)__builtin_ctzll(t);
-                t &= t - 1;
-                const RCov &cv = s_cov[k];
-                const float w0e = (xf - cv.fx1) * cv.d0y - (yf - cv.fy1) * cv.d0x;
-                const float w1e = (xf - cv.fx2) * cv.d1y - (yf - cv.fy2) * cv.d1x;
-                const float w2e = (xf - cv.fx0) * cv.d2y - (yf - cv.fy0) * cv.d2x;
-                const float bb0 = w0e * cv.inv_area, bb1 = w1e * cv.inv_area, bb2 = w2e * cv.inv_area;
-                const float z = bb0 * cv.z0 + bb1 * cv.z1 + bb2 * cv.z2;
-                float depth;
-                if (K.unit_range) depth = (z + 1.0f) * 0.5f;
-                else depth = (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
-                if (depth_stencil_step(rs, K, depth, s_tap[k], true, px, frag_bytes)) pass |= 1ull << k;
-            }
-            todo = pass;
-        }
-
-        // ---- 3. colour
-        if (last_writer) {
-            if (K.cmask != 0u && todo) {
-                const uint32_t k = 63u - (uint32_t)__builtin_clzll(todo);
-                const RTri *r = reinterpret_cast<const RTri *>(&s_rec[k]);
-                float w0e, w1e, w2e;
-                edge_values(r, x, y, w0e, w1e, w2e);
-                const float inv = r->inv_area;
-                Col c;
-                float dep;
-                uint32_t meta;
-                fragment_source(P, K, r, s_rid[k], w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
-                color_step(rs, K, c, px);
-            }
-            continue;
-        }
+        const uint64_t todo = alpha_dep ? m : depth_pass(K, s_cov, s_tap, m, xf, yf, px, frag_bytes);
         // batches: each pixel takes its next L fragments, L the largest quota
         // that keeps the batch within FCAP (L >= 1 fits since 64 <= FCAP)
         uint64_t rem = todo;
@@ -1631,9 +1974,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             uint32_t L = 1;
 #pragma unroll
             for (uint32_t step = 32; step >= 1; step >>= 1)
-                if (wave_sum_u32(min(rc, L + step)) <= (uint32_t)FCAP) L += step;
+                if (dpp_wave_sum(min(rc, L + step)) <= (uint32_t)FCAP) L += step;
             const uint32_t take = min(rc, L);
-            const uint32_t ij = wave_incl_scan(take, lane);
+            const uint32_t ij = dpp_incl_scan(take);
             const uint32_t off = ij - take;
             const uint32_t fr = min((uint32_t)__builtin_amdgcn_readlane((int32_t)ij, 63), (uint32_t)FCAP);
             {
@@ -1645,7 +1988,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                     o++;
                 }
             }
-            __syncthreads();
+            wave_sync();
             for (uint32_t f0 = 0; f0 < fr; f0 += 64) {
                 const uint32_t f = f0 + (uint32_t)lane;
                 if (f < fr) {
@@ -1664,15 +2007,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
                     s_meta[f] = (uint8_t)meta;
                 }
             }
-            __syncthreads();
+            wave_sync();
             for (uint32_t qn = 0; qn < take && off + qn < (uint32_t)FCAP; qn++) {
                 const uint32_t f = off + qn;
                 const float4 cc = s_col[f];
                 const uint32_t meta = s_meta[f];
-                if (!alpha_dep || depth_stencil_step(rs, K, s_dep[f], meta >> 1, (meta & 1u) != 0u, px, frag_bytes))
-                    color_step(rs, K, Col{cc.x, cc.y, cc.z, cc.w}, px);
+                if (!alpha_dep || depth_stencil_step(K, s_dep[f], meta >> 1, (meta & 1u) != 0u, px, frag_bytes))
+                    color_step(K, Col{cc.x, cc.y, cc.z, cc.w}, px);
             }
-            __syncthreads();   // slots and sources are reused by the next batch
+            wave_sync();   // slots and colours are reused by the next batch
         }
     }
     if (in_fb) {
@@ -1680,12 +2023,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         if (px.dirty_d) P.depth[pix] = px.d;
         if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
     }
-    frag_count = wave_sum_u64(frag_count);
-    frag_bytes = wave_sum_u64(frag_bytes);
-    if (lane == 0 && frag_count) {
-        atomicAdd(&P.ctr->fragments, frag_count);
-        atomicAdd(&P.ctr->frag_bytes, frag_bytes);
     }
+    add_stats(P, frag_count, frag_bytes);
 }
 
 // ---------------------------------------------------------------------------
@@ -1762,7 +2101,10 @@ struct mtgl_dev {
     hipEvent_t upload_done = nullptr;
     Arena params, vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
-    Arena keys[2], vals[2], sort_counts, tile_start, tile_end, heavy_tiles;
+    Arena keys[2], vals[2], sort_counts, tile_start, tile_end, tile_list, writer, cov;
+    uint32_t resident[3] = {256, 256, 256};   // blocks of k_raster_depth, k_raster_color, k_raster at once
+    unsigned long long *stat_slots = nullptr;  // 2 x STAT_SLOTS
+    std::vector<unsigned long long> stat_host;
     mtgl_dev_stats stats = {};
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
@@ -1913,6 +2255,16 @@ MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
         if (hipDeviceGetAttribute(&d->cu_count, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess ||
             d->cu_count <= 0)
             d->cu_count = 256;
+        {
+            // resident one-wave blocks of each raster kernel (registers and LDS bound)
+            const void *kern[3] = {(const void *)k_raster_depth<true>, (const void *)k_raster_color,
+                                   (const void *)k_raster};
+            for (int k = 0; k < 3; k++) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
+                d->resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)d->cu_count, STAT_SLOTS);
+            }
+        }
         d->w = width;
         d->h = height;
         d->tiles_x = (width + TILE - 1) / TILE;
@@ -1923,6 +2275,9 @@ MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
         if ((rc = map_err(hipMalloc((void **)&d->depth, n * 4))) != MTGL_OK) break;
         if ((rc = map_err(hipMalloc((void **)&d->stencil, n))) != MTGL_OK) break;
         if ((rc = map_err(hipMalloc((void **)&d->ctr, sizeof(Counters)))) != MTGL_OK) break;
+        if ((rc = map_err(hipMalloc((void **)&d->stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long)))) != MTGL_OK)
+            break;
+        if ((rc = map_err(hipMemset(d->stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long)))) != MTGL_OK) break;
         if ((rc = map_err(hipHostMalloc((void **)&d->ctr_host, sizeof(Counters), hipHostMallocDefault))) != MTGL_OK) break;
         if ((rc = map_err(hipHostMalloc((void **)&d->pairs_host, sizeof(unsigned long long), hipHostMallocDefault))) != MTGL_OK) break;
         if ((rc = map_err(hipEventCreateWithFlags(&d->upload_done, hipEventDisableTiming))) != MTGL_OK) break;
@@ -1951,7 +2306,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     }
     Arena *arenas[] = {&d->params, &d->vhot, &d->vcold, &d->rt, &d->rcold, &d->n_rt, &d->rec_base, &d->tri_tiles, &d->pair_base,
                        &d->scan_tmp, &d->clip_list, &d->keys[0], &d->keys[1], &d->vals[0], &d->vals[1], &d->sort_counts,
-                       &d->tile_start, &d->tile_end, &d->heavy_tiles};
+                       &d->tile_start, &d->tile_end, &d->tile_list, &d->writer, &d->cov};
     for (Arena *a : arenas) if (a->ptr) (void)hipFree(a->ptr);
     for (auto &pr : d->pending_events) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
     for (hipEvent_t e : d->free_events) (void)hipEventDestroy(e);
@@ -1959,6 +2314,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     if (d->depth) (void)hipFree(d->depth);
     if (d->stencil) (void)hipFree(d->stencil);
     if (d->ctr) (void)hipFree(d->ctr);
+    if (d->stat_slots) (void)hipFree(d->stat_slots);
     if (d->ctr_host) (void)hipHostFree(d->ctr_host);
     if (d->pairs_host) (void)hipHostFree(d->pairs_host);
     if (d->pinned) (void)hipHostFree(d->pinned);
@@ -2247,7 +2603,7 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     const uint32_t ntiles = (uint32_t)(d->tiles_x * d->tiles_y);
     if ((rc = arena_reserve(d, d->tile_start, (size_t)ntiles * 4))) return rc;
     if ((rc = arena_reserve(d, d->tile_end, (size_t)ntiles * 4))) return rc;
-    if ((rc = arena_reserve(d, d->heavy_tiles, (size_t)ntiles * 4))) return rc;
+    if ((rc = arena_reserve(d, d->tile_list, (size_t)ntiles * 4))) return rc;
     uint32_t *keys = as<uint32_t>(d->keys[0]), *vals = as<uint32_t>(d->vals[0]);
     uint32_t *keys2 = as<uint32_t>(d->keys[1]), *vals2 = as<uint32_t>(d->vals[1]);
     hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const unsigned long long *)pair_base,
@@ -2267,13 +2623,12 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         uint32_t *tk = keys; keys = keys2; keys2 = tk;
         uint32_t *tv = vals; vals = vals2; vals2 = tv;
     }
-    // every tile with more than HEAVY_BIN entries takes HEAVY_BIN + 1 of the n
-    const uint32_t heavy_cap = std::min<uint32_t>(ntiles, n / (HEAVY_BIN + 1u));
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, d->stream, as<uint32_t>(d->tile_end),
-                       ntiles, 0u);
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const uint32_t *)keys, n, ntiles,
-                       as<uint32_t>(d->tile_start), as<uint32_t>(d->tile_end), as<uint32_t>(d->heavy_tiles), heavy_cap,
-                       d->ctr);
+    if ((rc = arena_reserve(d, d->cov, (size_t)n * RT_COV_VEC4 * sizeof(float4)))) return rc;
+    hipLaunchKernelGGL(k_segments, dim3((n + SEG_THREADS * SEG_ITEMS - 1) / (SEG_THREADS * SEG_ITEMS)),
+                       dim3(SEG_THREADS), 0, d->stream, (const uint32_t *)keys, n, ntiles, as<uint32_t>(d->tile_start),
+                       as<uint32_t>(d->tile_end), as<uint32_t>(d->tile_list), d->ctr);
+    hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, d->stream,
+                       (const uint32_t *)vals, (const RTri *)sp.rt, n, as<float4>(d->cov));
     HIPC(hipGetLastError());
 
     // ---- K4
@@ -2282,13 +2637,24 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     rp.rt = sp.rt;
     rp.rcold = sp.rcold;
     rp.vals = vals;
-    rp.heavy = as<uint32_t>(d->heavy_tiles);
-    rp.heavy_cap = heavy_cap;
+    rp.list = as<uint32_t>(d->tile_list);
+    rp.ntiles = ntiles;
+    rp.cov = as<float4>(d->cov);
     rp.tile_start = as<uint32_t>(d->tile_start);
     rp.tile_end = as<uint32_t>(d->tile_end);
     rp.color = d->color;
     rp.depth = d->depth;
     rp.stencil = d->stencil;
+    // without blending, a partial colour mask or an alpha test on a texture,
+    // only each pixel's last passing fragment leaves its colour: depth kernel
+    // then one colour per pixel
+    const uint32_t cm = rsh->color_mask & 0xFu;
+    const bool last_writer = !(rsh->flags & MTGL_F_BLEND) && (cm == 0xFu || cm == 0u) &&
+                             !((rsh->flags & MTGL_F_ALPHA_TEST) && tx);
+    if (last_writer) {
+        if ((rc = arena_reserve(d, d->writer, (size_t)d->w * (size_t)d->h * 4))) return rc;
+        rp.writer = as<uint32_t>(d->writer);
+    }
     rp.fb_w = d->w;
     rp.fb_h = d->h;
     rp.tiles_x = d->tiles_x;
@@ -2299,14 +2665,28 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     rp.tex1_w = tx ? tx->m1w : 0;
     rp.tex1_h = tx ? tx->m1h : 0;
     rp.ctr = d->ctr;
+    rp.stats = d->stat_slots;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (d->timing) {
         ev0 = take_event(d);
         ev1 = take_event(d);
         if (ev0) HIPC(hipEventRecord(ev0, d->stream));
     }
-    // heavy-list blocks first, then one block per tile (empty and heavy tiles exit)
-    hipLaunchKernelGGL(k_raster, dim3(heavy_cap + ntiles), dim3(64), 0, d->stream, rp);
+    // persistent grids over the tile list: at most one block per non-empty
+    // tile, at most as many as are resident at once
+    // persistent grids over the tile list (longest bins first): at most one
+    // block per non-empty tile, at most as many as are resident at once
+    const uint32_t nlist = std::min<uint32_t>(n, ntiles);
+    if (last_writer) {
+        const dim3 g(std::min<uint32_t>(nlist, d->resident[0]));
+        if (rsh->flags & MTGL_F_STENCIL_TEST) hipLaunchKernelGGL(k_raster_depth<true>, g, dim3(64), 0, d->stream, rp);
+        else hipLaunchKernelGGL(k_raster_depth<false>, g, dim3(64), 0, d->stream, rp);
+        if (cm)
+            hipLaunchKernelGGL(k_raster_color, dim3(std::min<uint32_t>(nlist, d->resident[1])), dim3(64), 0, d->stream,
+                               rp);
+    } else {
+        hipLaunchKernelGGL(k_raster, dim3(std::min<uint32_t>(nlist, d->resident[2])), dim3(64), 0, d->stream, rp);
+    }
     HIPC(hipGetLastError());
     if (ev0 && ev1) {
         HIPC(hipEventRecord(ev1, d->stream));
@@ -2325,13 +2705,19 @@ MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int enable)
 MTGL_API int mtgl_dev_stats_get(mtgl_dev *d, mtgl_dev_stats *out)
 {
     if (!d || !out) return MTGL_ERR_INVALID;
-    HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+    d->stat_host.resize(2 * STAT_SLOTS);
+    HIPC(hipMemcpyAsync(d->stat_host.data(), d->stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, d->stream));
     HIPC(hipStreamSynchronize(d->stream));
     int rc = resolve_timing(d);
     if (rc) return rc;
     *out = d->stats;
-    out->fragments = d->ctr_host->fragments;
-    out->frag_bytes = d->ctr_host->frag_bytes;
+    out->fragments = 0;
+    out->frag_bytes = 0;
+    for (uint32_t i = 0; i < STAT_SLOTS; i++) {
+        out->fragments += d->stat_host[2 * i];
+        out->frag_bytes += d->stat_host[2 * i + 1];
+    }
     return MTGL_OK;
 }
 
@@ -2343,6 +2729,7 @@ MTGL_API int mtgl_dev_stats_reset(mtgl_dev *d)
     if (rc) return rc;
     d->stats = mtgl_dev_stats();
     HIPC(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
+    HIPC(hipMemsetAsync(d->stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long), d->stream));
     HIPC(hipStreamSynchronize(d->stream));
     return MTGL_OK;
 }
