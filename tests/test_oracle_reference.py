@@ -1,0 +1,29 @@
+"""The CPU oracle against the reference renderer itself, where
+oracle/_ref/libmtgl_ref.so was built from the reference's sources: feature
+scenes at seeds the golden fixtures do not hold, plus the API scripts."""
+import json
+
+import pytest
+
+from conftest import plane_diffs
+from mytinygl_amd import scenes as S
+import api_scripts
+
+FEATURE = sorted(n for n in S.FEATURE_SCENES if n not in S.UNDEFINED_IN_REFERENCE)
+
+
+@pytest.mark.parametrize("name", FEATURE)
+@pytest.mark.parametrize("seed", [4, 5])
+def test_scene_vs_reference(oracle, reference, name, seed):
+    W, H = (160, 120) if "small" in name else (96, 72)
+    fn = S.FEATURE_SCENES[name]
+    (a, ea), (b, eb) = S.render(oracle, fn, W, H, seed), S.render(reference, fn, W, H, seed)
+    assert ea == eb
+    assert plane_diffs(a, b) == {"color": 0, "depth": 0, "stencil": 0}
+
+
+@pytest.mark.parametrize("name", sorted(api_scripts.SCRIPTS))
+def test_api_vs_reference(oracle, reference, name):
+    a = json.loads(json.dumps(api_scripts.run_script(oracle, name)))
+    b = json.loads(json.dumps(api_scripts.run_script(reference, name)))
+    assert a == b
