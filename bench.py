@@ -45,7 +45,7 @@ def traffic_from_profile():
     """HBM bytes per fragment-kernel launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
     or None when no summary is present."""
-    path = os.path.join(ROOT, "profiles", "k_raster_textured_sphere_1080p.md")
+    path = os.path.join(ROOT, "profiles", "fragment_stage_textured_sphere_1080p.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -81,6 +81,20 @@ def cpu_baseline(budget_s: float):
     return {"frames": frames, "seconds": el, "kind": kind}
 
 
+def copy_calibration(nbytes: int, reps: int = 5) -> None:
+    """Plain device-to-device copies of nbytes (torch's elementwise copy
+    kernel): the reference the HBM counters FETCH_SIZE / WRITE_SIZE are
+    scaled by, since that copy moves exactly nbytes in and nbytes out."""
+    import torch
+    n = nbytes // 4
+    x = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +102,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU split is a design note, not an implemented path)", file=sys.stderr)
@@ -160,7 +179,8 @@ def main():
                    "fog": "GL_LINEAR 1..3"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profile(),
-                     "kernel": "k_raster", "kernel_ms": round(k_ms, 5), "launches": int(st.raster_launches),
+                     "kernel": "fragment stage (k_raster_depth + k_raster_color)", "kernel_ms": round(k_ms, 5),
+                     "launches": int(st.raster_launches),
                      "bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu_obj,
     }

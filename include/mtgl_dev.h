@@ -204,6 +204,7 @@ typedef struct mtgl_dev mtgl_dev;
  * width x height (reference framebuffer_init, src/framebuffer.h:32-58). */
 int  mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out);
 void mtgl_dev_destroy(mtgl_dev *dev);
+/* Wait for all queued work (reference glFinish, src/gl_api.c:895-899). */
 int  mtgl_dev_finish(mtgl_dev *dev);
 
 /* glClear: fill [x0,x1) x [y0,y1) of the selected planes
@@ -213,7 +214,9 @@ int  mtgl_dev_clear(mtgl_dev *dev, uint32_t buffers, int32_t x0, int32_t y0, int
 
 /* Copy a w x h window whose top-left is framebuffer pixel (x, y) to/from tightly
  * packed host memory (u32 colour, f32 depth or u8 stencil).  The window must lie
- * inside the framebuffer. */
+ * inside the framebuffer.  Replaces the reference's direct framebuffer reads in
+ * glReadPixels (src/gl_api.c:1180-1229) and the framebuffer accessors
+ * (src/framebuffer.h:92-134). */
 int  mtgl_dev_read(mtgl_dev *dev, uint32_t buffer, int32_t x, int32_t y, int32_t w, int32_t h, void *dst);
 int  mtgl_dev_write(mtgl_dev *dev, uint32_t buffer, int32_t x, int32_t y, int32_t w, int32_t h, const void *src);
 
@@ -229,10 +232,13 @@ int  mtgl_dev_buffer_data(mtgl_dev *dev, uint32_t id, int64_t size, const void *
 int  mtgl_dev_buffer_sub_data(mtgl_dev *dev, uint32_t id, int64_t offset, int64_t size, const void *data);
 int  mtgl_dev_buffer_free(mtgl_dev *dev, uint32_t id);
 
-/* Render one draw into the context's framebuffer. */
+/* Render one draw into the context's framebuffer: replaces glEnd's flush_*
+ * dispatch (reference src/gl_api.c:648-659, flush_* at src/raster.c:961-1231)
+ * and the array draws glDrawArrays / glDrawElements (src/gl_api.c:1799-1941). */
 int  mtgl_dev_draw(mtgl_dev *dev, const mtgl_draw *draw);
 
-/* Counters and timing. */
+/* Counters and timing; no counterpart in the reference (fragments passing
+ * coverage, algorithmic bytes of the per-fragment pipeline, fragment-stage time). */
 int  mtgl_dev_set_timing(mtgl_dev *dev, int enable);
 int  mtgl_dev_stats_get(mtgl_dev *dev, mtgl_dev_stats *out);
 int  mtgl_dev_stats_reset(mtgl_dev *dev);
