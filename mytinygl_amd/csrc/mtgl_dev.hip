@@ -720,6 +720,7 @@ __global__ void __launch_bounds__(256) k_setup(SetupParams P)
     P.n_rt[t] = nout;
     P.rec_base[t] = t;
     P.tri_tiles[t] = tiles;
+    if (t == 0) P.tri_tiles[P.ntri] = 0;   // the scan's last element becomes the total
 }
 
 // slow path: Sutherland-Hodgman in the reference's plane order (near, far,
@@ -962,9 +963,6 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *k
 // K3c: tile segments of the sorted pairs
 // ---------------------------------------------------------------------------
 
-constexpr int SEG_THREADS = 256;
-constexpr int SEG_ITEMS = 16;
-
 // Inclusive prefix sum over the 64 lanes of a wave with DPP row shifts and
 // row broadcasts (VALU only): Hillis-Steele within each row of 16 lanes,
 // then row 0's and row 1's totals carried into the rows above.
@@ -984,67 +982,66 @@ __device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v)
     return (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(v), 63);
 }
 
-// Tile ranges of the sorted pair keys and the list of non-empty tiles:
-// tiles whose bins hold more than HEAVY_BIN entries from the front of `list`,
-// the others from its back, so that the raster kernels, which walk the list
-// from the front, start the longest tiles first.  Each thread owns SEG_ITEMS
-// consecutive keys; one atomic per block and list end.
-__global__ void __launch_bounds__(SEG_THREADS) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles,
-                                                          uint32_t *tile_start, uint32_t *tile_end, uint32_t *list,
-                                                          Counters *ctr)
+// Tile ranges of the sorted pair keys, one thread per key: the first and
+// last key of each run write its range; the run's tile is stamped with the
+// draw's generation so no per-draw clear of the tile arrays is needed.
+__global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles,
+                                                  uint32_t *tile_start, uint32_t *tile_end, uint32_t *tile_gen,
+                                                  uint32_t gen)
 {
-    __shared__ uint32_t s_wave[2][SEG_THREADS / 64];
-    __shared__ uint32_t s_base[2];
-    const uint32_t p0 = (blockIdx.x * SEG_THREADS + threadIdx.x) * SEG_ITEMS;
-    uint32_t nh = 0, nl = 0;
-    for (int i = 0; i < SEG_ITEMS; i++) {
-        const uint32_t p = p0 + i;
-        if (p >= n) break;
-        const uint32_t k = keys[p];
-        if (k >= ntiles) continue;
-        if (p == 0 || keys[p - 1] != k) tile_start[k] = p;
-        if (p == n - 1 || keys[p + 1] != k) {
-            tile_end[k] = p + 1;
-            // a bin longer than HEAVY_BIN holds its key HEAVY_BIN places earlier
-            if (p >= HEAVY_BIN && keys[p - HEAVY_BIN] == k) nh++;
-            else nl++;
-        }
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = keys[p];
+    if (k >= ntiles) return;
+    if (p == 0 || keys[p - 1] != k) tile_start[k] = p;
+    if (p == n - 1 || keys[p + 1] != k) {
+        tile_end[k] = p + 1;
+        tile_gen[k] = gen;
     }
+}
+
+// The non-empty tiles of the draw: those whose bins hold more than HEAVY_BIN
+// entries from the front of `list`, the others from its back, so that the
+// raster kernels, which walk the list from the front, start the longest tiles
+// first.  One thread per tile; one atomic per block and list end.
+__global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
+                                                   const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
+                                                   uint32_t *list, Counters *ctr)
+{
+    __shared__ uint32_t s_n[2][4];
+    __shared__ uint32_t s_base[2];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ne = t < ntiles && tile_gen[t] == gen;
+    const bool hv = ne && tile_end[t] - tile_start[t] > HEAVY_BIN;
+    const bool lt = ne && !hv;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t ih = dpp_incl_scan(nh), il = dpp_incl_scan(nl);
-    if (lane == 63) {
-        s_wave[0][wave] = ih;
-        s_wave[1][wave] = il;
+    const uint64_t bh = __ballot(hv), bl = __ballot(lt);
+    if (lane == 0) {
+        s_n[0][wave] = (uint32_t)__popcll(bh);
+        s_n[1][wave] = (uint32_t)__popcll(bl);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t th = 0, tl = 0;
-        for (int w = 0; w < SEG_THREADS / 64; w++) {
-            const uint32_t a = s_wave[0][w], b = s_wave[1][w];
-            s_wave[0][w] = th;
-            s_wave[1][w] = tl;
+        for (int w = 0; w < 4; w++) {
+            const uint32_t a = s_n[0][w], c = s_n[1][w];
+            s_n[0][w] = th;
+            s_n[1][w] = tl;
             th += a;
-            tl += b;
+            tl += c;
         }
         s_base[0] = th ? atomicAdd(&ctr->nheavy, th) : 0u;
         s_base[1] = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
     }
     __syncthreads();
-    if (nh == 0 && nl == 0) return;
-    uint32_t oh = s_base[0] + s_wave[0][wave] + ih - nh;
-    uint32_t ol = s_base[1] + s_wave[1][wave] + il - nl;
-    for (int i = 0; i < SEG_ITEMS; i++) {
-        const uint32_t p = p0 + i;
-        if (p >= n) break;
-        const uint32_t k = keys[p];
-        if (k >= ntiles || (p != n - 1 && keys[p + 1] == k)) continue;
-        if (p >= HEAVY_BIN && keys[p - HEAVY_BIN] == k) {
-            if (oh < ntiles) list[oh] = k;
-            oh++;
-        } else {
-            if (ol < ntiles) list[ntiles - 1u - ol] = k;
-            ol++;
-        }
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (hv) {
+        const uint32_t o = s_base[0] + s_n[0][wave] + (uint32_t)__popcll(bh & below);
+        if (o < ntiles) list[o] = t;
+    }
+    if (lt) {
+        const uint32_t o = s_base[1] + s_n[1][wave] + (uint32_t)__popcll(bl & below);
+        if (o < ntiles) list[ntiles - 1u - o] = t;
     }
 }
 
@@ -2101,7 +2098,8 @@ struct mtgl_dev {
     hipEvent_t upload_done = nullptr;
     Arena params, vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
-    Arena keys[2], vals[2], sort_counts, tile_start, tile_end, tile_list, writer, cov;
+    Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list, writer, cov;
+    uint32_t gen = 0;                 // draw generation stamped into tile_gen
     uint32_t resident[3] = {256, 256, 256};   // blocks of k_raster_depth, k_raster_color, k_raster at once
     unsigned long long *stat_slots = nullptr;  // 2 x STAT_SLOTS
     std::vector<unsigned long long> stat_host;
@@ -2165,15 +2163,16 @@ unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threa
 
 // exclusive scan of n values of T (in may equal out); *total_dev receives the sum
 template <typename T>
-int device_scan(mtgl_dev *d, const T *in, T *out, uint64_t n, T *total_dev)
+int device_scan(mtgl_dev *d, const T *in, T *out, uint64_t n)
 {
+    // exclusive scan of n values; out may alias in
     if (n == 0) return MTGL_OK;
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     int rc = arena_reserve(d, d->scan_tmp, nb * sizeof(T));
     if (rc) return rc;
     T *sums = as<T>(d->scan_tmp);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, d->stream, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, d->stream, sums, nb, total_dev);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, d->stream, sums, nb, (T *)nullptr);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, d->stream, in, n,
                        (const T *)sums, out);
     HIPC(hipGetLastError());
@@ -2305,7 +2304,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
         if (d->buf[i].data) (void)hipFree(d->buf[i].data);
     }
     Arena *arenas[] = {&d->params, &d->vhot, &d->vcold, &d->rt, &d->rcold, &d->n_rt, &d->rec_base, &d->tri_tiles, &d->pair_base,
-                       &d->scan_tmp, &d->clip_list, &d->keys[0], &d->keys[1], &d->vals[0], &d->vals[1], &d->sort_counts,
+                       &d->scan_tmp, &d->clip_list, &d->keys[0], &d->keys[1], &d->vals[0], &d->vals[1], &d->sort_tmp, &d->tile_gen,
                        &d->tile_start, &d->tile_end, &d->tile_list, &d->writer, &d->cov};
     for (Arena *a : arenas) if (a->ptr) (void)hipFree(a->ptr);
     for (auto &pr : d->pending_events) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
@@ -2564,7 +2563,7 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
 
     // ---- K3: pair offsets, total, then expand / sort / segment
     unsigned long long *pair_base = as<unsigned long long>(d->pair_base);
-    if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, ntri, pair_base + ntri))) return rc;
+    if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, (uint64_t)ntri + 1))) return rc;
     HIPC(hipMemcpyAsync(d->pairs_host, pair_base + ntri, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
     HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
     HIPC(hipStreamSynchronize(d->stream));
@@ -2583,7 +2582,7 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         hipLaunchKernelGGL(k_setup, dim3(blocks_for(ntri, 256)), dim3(256), 0, d->stream, sp);
         hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ntri, 64)), dim3(64), 0, d->stream, sp);
         HIPC(hipGetLastError());
-        if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, ntri, pair_base + ntri))) return rc;
+        if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, (uint64_t)ntri + 1))) return rc;
         HIPC(hipMemcpyAsync(d->pairs_host, pair_base + ntri, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
         HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
         HIPC(hipStreamSynchronize(d->stream));
@@ -2604,29 +2603,37 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     if ((rc = arena_reserve(d, d->tile_start, (size_t)ntiles * 4))) return rc;
     if ((rc = arena_reserve(d, d->tile_end, (size_t)ntiles * 4))) return rc;
     if ((rc = arena_reserve(d, d->tile_list, (size_t)ntiles * 4))) return rc;
+    if (d->tile_gen.cap < (size_t)ntiles * 4 || ++d->gen == 0) {
+        if ((rc = arena_reserve(d, d->tile_gen, (size_t)ntiles * 4))) return rc;
+        HIPC(hipMemsetAsync(d->tile_gen.ptr, 0, d->tile_gen.cap, d->stream));
+        d->gen = 1;
+    }
     uint32_t *keys = as<uint32_t>(d->keys[0]), *vals = as<uint32_t>(d->vals[0]);
     uint32_t *keys2 = as<uint32_t>(d->keys[1]), *vals2 = as<uint32_t>(d->vals[1]);
     hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const unsigned long long *)pair_base,
                        ntri, (const uint32_t *)sp.n_rt, (const uint32_t *)sp.rec_base, (const RTri *)sp.rt, (uint64_t)n,
                        d->tiles_x, keys, vals);
+    // stable LSD radix sort by tile: pairs keep submission order within a tile
     int key_bits = 1;
     while ((1u << key_bits) < ntiles) key_bits++;
     const uint32_t nblocks = (n + SORT_TILE - 1) / SORT_TILE;
-    if ((rc = arena_reserve(d, d->sort_counts, (size_t)RADIX * nblocks * 4))) return rc;
-    uint32_t *counts = as<uint32_t>(d->sort_counts);
+    if ((rc = arena_reserve(d, d->sort_tmp, (size_t)RADIX * nblocks * 4))) return rc;
+    uint32_t *counts = as<uint32_t>(d->sort_tmp);
     for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, d->stream, (const uint32_t *)keys, n, shift,
                            nblocks, counts);
-        if ((rc = device_scan<uint32_t>(d, counts, counts, (uint64_t)RADIX * nblocks, nullptr))) return rc;
+        if ((rc = device_scan<uint32_t>(d, counts, counts, (uint64_t)RADIX * nblocks))) return rc;
         hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, d->stream, (const uint32_t *)keys,
                            (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
         uint32_t *tk = keys; keys = keys2; keys2 = tk;
         uint32_t *tv = vals; vals = vals2; vals2 = tv;
     }
     if ((rc = arena_reserve(d, d->cov, (size_t)n * RT_COV_VEC4 * sizeof(float4)))) return rc;
-    hipLaunchKernelGGL(k_segments, dim3((n + SEG_THREADS * SEG_ITEMS - 1) / (SEG_THREADS * SEG_ITEMS)),
-                       dim3(SEG_THREADS), 0, d->stream, (const uint32_t *)keys, n, ntiles, as<uint32_t>(d->tile_start),
-                       as<uint32_t>(d->tile_end), as<uint32_t>(d->tile_list), d->ctr);
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const uint32_t *)keys, n, ntiles,
+                       as<uint32_t>(d->tile_start), as<uint32_t>(d->tile_end), as<uint32_t>(d->tile_gen), d->gen);
+    hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, d->stream,
+                       (const uint32_t *)as<uint32_t>(d->tile_start), (const uint32_t *)as<uint32_t>(d->tile_end),
+                       (const uint32_t *)as<uint32_t>(d->tile_gen), d->gen, ntiles, as<uint32_t>(d->tile_list), d->ctr);
     hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, d->stream,
                        (const uint32_t *)vals, (const RTri *)sp.rt, n, as<float4>(d->cov));
     HIPC(hipGetLastError());
