@@ -13,6 +13,10 @@ counted by the device), frames/s, the roofline of the fragment kernel
 (algorithmic bytes per launch / kernel time from HIP events on the context's
 stream) and a CPU baseline: the reference renderer compiled strict, one core,
 on a bounded sample of the same frame.
+
+"batched": the 512-scene workload (one context per scene) issued per draw and
+replayed from one mtgl_batch (a captured graph of two launches over all
+scenes), with its own bounded CPU sample.
 """
 from __future__ import annotations
 
@@ -81,6 +85,97 @@ def cpu_baseline(budget_s: float):
     return {"frames": frames, "seconds": el, "kind": kind}
 
 
+def batch_bench(launches: int, per_draw_reps: int, cpu_seconds: float):
+    """The 512-scene batched workload: the same frames (one clear and one
+    glDrawElements per context) issued per draw, context after context, and
+    replayed from one batch.  Times are whole 512-scene frame sets with the
+    GPU drained on both sides."""
+    g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
+    specs = S.batch_specs()
+    run = S.BatchRun(g, S.batch_items(specs))
+    try:
+        run.frames_immediate()
+        run.finish()
+        for ctx in run.ctxs:
+            g.stats_reset(ctx)
+        run.frames_immediate()
+        run.finish()
+        frags_scene = [int(g.stats(ctx).fragments) for ctx in run.ctxs]
+        frags = sum(frags_scene)
+        t0 = time.perf_counter()
+        for _ in range(per_draw_reps):
+            run.frames_immediate()
+        run.finish()
+        per_draw_ms = (time.perf_counter() - t0) * 1e3 / per_draw_reps
+        b = run.record(autotune=True)
+        info = b.info()
+        for _ in range(3):
+            b.launch()
+        if b.sync() != 0:
+            raise RuntimeError("batched launch incomplete")
+        b.stats_reset()
+        b.launch()
+        b.sync()
+        frags_batch = int(b.stats().fragments)
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            b.launch()
+        rc = b.sync()
+        batched_ms = (time.perf_counter() - t0) * 1e3 / launches
+        if rc != 0:
+            raise RuntimeError("batched launch incomplete")
+    finally:
+        run.close()
+        g.release()
+    out = {
+        "workload": "512 scenes, one context each: square framebuffers 64..1024 px (log-uniform), grid spheres "
+                    "of 98..48,672 triangles (log-uniform), every other one textured 64..512 px "
+                    "GL_LINEAR_MIPMAP_LINEAR; one clear + one glDrawElements per scene",
+        "scenes": len(specs), "triangles": int(sum(sp["triangles"] for sp in specs)),
+        "pixels": int(sum(sp["w"] * sp["h"] for sp in specs)), "fragments": frags,
+        "fragments_batched": frags_batch,
+        "per_draw_ms": round(per_draw_ms, 3), "batched_ms": round(batched_ms, 3),
+        "speedup": round(per_draw_ms / batched_ms, 2),
+        "mfragments_s_per_draw": round(frags / per_draw_ms / 1e3, 2),
+        "mfragments_s_batched": round(frags / batched_ms / 1e3, 2),
+        "scene_frames_s_batched": round(len(specs) / batched_ms * 1e3, 1),
+        "launch": {"rounds": info.rounds, "heavy_bin": info.heavy_bin, "grid_div": info.grid_div,
+                   "tune_ms": [round(info.tune_ms[i], 3) for i in range(info.tune_candidates)]},
+    }
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = batch_cpu_baseline(specs, frags_scene, cpu_seconds)
+    return out
+
+
+def batch_cpu_baseline(specs, frags_scene, budget_s: float):
+    """The reference renderer (strict build, one thread) on the first scenes of
+    the batched workload until budget_s of frame time has elapsed."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "libmtgl_ref.so")
+    kind = "reference"
+    if not os.path.exists(ref):
+        ref = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+        kind = "port"
+    if not os.path.exists(ref):
+        return None
+    g = GLLibrary(ref, kind if kind == "reference" else "oracle")
+    spent, frags, done = 0.0, 0, 0
+    for sp, fr in zip(specs, frags_scene):
+        ctx = g.create_context(sp["w"], sp["h"])
+        n = S.setup_batch_scene(g, sp)
+        t0 = time.perf_counter()
+        S.frame_batch_scene(g, n)
+        spent += time.perf_counter() - t0
+        g.destroy_context(ctx)
+        g.release()
+        frags += fr
+        done += 1
+        if spent >= budget_s:
+            break
+    return {"value": round(frags / spent / 1e6, 4), "unit": "Mfragments/s", "cores": 1, "kind": kind,
+            "sample": f"scenes 0..{done - 1} of the 512 ({frags} fragments) rendered one after another in "
+                      f"{spent:.1f} s of frame time, one thread"}
+
+
 def copy_calibration(nbytes: int, reps: int = 5) -> None:
     """Plain device-to-device copies of nbytes (torch's elementwise copy
     kernel): the reference the HBM counters FETCH_SIZE / WRITE_SIZE are
@@ -102,6 +197,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
+    ap.add_argument("--batch-launches", type=int, default=50)
+    ap.add_argument("--batch-per-draw", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -159,6 +257,9 @@ def main():
                              f"{cpu['seconds']:.1f} s, one thread ({os.cpu_count()} host CPUs visible)",
                    "frames_per_s": round(cpu_fps, 4)}
 
+    batched = None if args.no_batch else batch_bench(args.batch_launches, args.batch_per_draw,
+                                                      0.0 if args.no_cpu else args.cpu_seconds / 2)
+
     out = {
         "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
         "value": round(mfrag, 3),
@@ -183,6 +284,7 @@ def main():
                      "launches": int(st.raster_launches),
                      "bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu_obj,
+        "batched": batched,
     }
     if cpu_obj:
         out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None

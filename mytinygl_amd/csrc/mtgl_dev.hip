@@ -36,10 +36,12 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
 #include "GL/gl.h"
+#include "mtgl_batch.h"
 #include "mtgl_dev.h"
 
 #define MTGL_API extern "C" __attribute__((visibility("default")))
@@ -56,7 +58,7 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-constexpr uint32_t HEAVY_BIN = 96;      // tiles with longer bins are rasterised first
+constexpr uint32_t HEAVY_BIN = 96;      // default: tiles with longer bins are rasterised first
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
@@ -110,12 +112,13 @@ struct Counters {
     unsigned long long fragments;
     unsigned long long frag_bytes;
     unsigned long long tri_raster;   // fan triangles set up by the current draw
-    unsigned int nheavy;          // non-empty tiles of the current draw with bins over HEAVY_BIN entries
+    unsigned int nheavy;          // non-empty tiles of the current launch with bins over the heavy-bin threshold
     unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
     unsigned int nclip;
     unsigned int noverflow;       // overflow slots requested by clipped triangles
     unsigned int overflow_full;   // a request did not fit: the host grows the pool and reruns
+    unsigned int sticky;          // overflow_full of every launch since the host last cleared it
 };
 
 // attribute stream resolved to a device pointer
@@ -141,28 +144,62 @@ struct VertexParams {
     VCold *out_cold;              // null when no per-fragment lighting needs it
 };
 
+// One scene (one draw into one framebuffer) of a launch.  A launch covers
+// 1..n scenes: the triangles of scene s are global triangles
+// [tri_off, tri_off + ntri), its records the same slots of the record pool,
+// clipped triangles take slots of the shared overflow pool after all scenes'
+// triangles.  Pointers to per-triangle arrays are already offset to the scene.
 struct SetupParams {
     uint32_t mode;
     uint32_t count;
     uint32_t ntri;
+    uint32_t tri_off;             // first global triangle of the scene
+    uint32_t scene;
     int32_t fb_w, fb_h;
     int32_t tiles_x;
     int32_t tex_w, tex_h;         // 0 when the draw is untextured
     const mtgl_raster_state *rs;
     const VHot *vhot;
     const VCold *vcold;
-    RTri *rt;
+    RTri *rt;                     // the scene's own slots
     RCold *rcold;
+    RTri *rt_over;                // the shared overflow pool (after every scene's slots)
+    RCold *rcold_over;
+    uint32_t over_base;           // global slot of rt_over[0]
     uint32_t *n_rt;               // fan triangles kept per input triangle
-    uint32_t *rec_base;           // first record slot per input triangle
+    uint32_t *rec_base;           // first global record slot per input triangle
     unsigned long long *tri_tiles;
-    uint32_t *clip_list;          // triangles that cross a frustum plane
-    uint32_t overflow_cap;        // slots available after the first ntri
+    uint32_t *clip_list;          // global triangles that cross a frustum plane
+    uint32_t overflow_cap;        // slots of the overflow pool
     Counters *ctr;
 };
 
-struct RasterParams {
+// what the raster stage needs of a scene
+enum : uint32_t {
+    KIND_LAST_WRITER = 1u,        // no blending, full or empty colour mask, no alpha test on a texture
+    KIND_STENCIL = 2u,            // stencil test enabled
+    KIND_COLOR = 4u,              // colour written
+};
+
+struct SceneRaster {
     const mtgl_raster_state *rs;
+    uint32_t *color;
+    float *depth;
+    uint8_t *stencil;
+    uint32_t *writer;             // per pixel: 1 + record slot of the last passing fragment
+    const uint32_t *tex;          // level 0 (null = untextured)
+    const uint32_t *tex1;         // level 1 (null = none: samples read as white)
+    int32_t tex_w, tex_h, tex1_w, tex1_h;
+    int32_t fb_w, fb_h, tiles_x;
+    uint32_t tile_off;            // first global tile of the scene
+    uint32_t kind;
+    uint32_t pad_;
+};
+
+struct RasterParams {
+    const SceneRaster *scenes;
+    const uint32_t *tile_pre;      // nscenes + 1 first global tiles
+    uint32_t nscenes;
     const RTri *rt;
     const RCold *rcold;
     const uint32_t *vals;          // sorted pair values (record slot)
@@ -173,14 +210,6 @@ struct RasterParams {
     const float4 *cov;             // first 64 record bytes per pair, in pair order
     const uint32_t *tile_start;
     const uint32_t *tile_end;      // 0 for an empty tile
-    uint32_t *color;
-    float *depth;
-    uint8_t *stencil;
-    uint32_t *writer;              // per pixel: 1 + record slot of the last passing fragment
-    int32_t fb_w, fb_h, tiles_x;
-    const uint32_t *tex;           // level 0 (null = untextured)
-    const uint32_t *tex1;          // level 1 (null = none: samples read as white)
-    int32_t tex_w, tex_h, tex1_w, tex1_h;
     Counters *ctr;
 };
 
@@ -347,6 +376,19 @@ __device__ void fetch(const DArr &a, int32_t index, float *out, int n)
     }
 }
 
+// The scene of a launch that owns item x: the last s with pre[s] <= x (pre
+// holds the n + 1 running starts, pre[0] = 0).  x is uniform over a block or
+// a wave wherever this is called, so the search runs on scalar registers.
+__device__ __forceinline__ uint32_t find_scene(const uint32_t *pre, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;   // pre[lo] <= x < pre[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 __device__ int32_t fetch_index(const VertexParams &P, uint32_t i)
 {
     if (P.index_type == 0) return (int32_t)((uint32_t)P.first + i);
@@ -374,9 +416,12 @@ __device__ __forceinline__ float sanitize(float v, float nan_value)
     return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
 }
 
-__global__ void __launch_bounds__(256) k_vertex(VertexParams P)
+// blocks are dealt to scenes in order: block_pre holds each scene's first block
+__global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const uint32_t *block_pre, uint32_t nscenes)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t sc = __builtin_amdgcn_readfirstlane(find_scene(block_pre, nscenes, blockIdx.x));
+    const VertexParams &P = Ps[sc];
+    const uint32_t i = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
     if (i >= P.count) return;
     int32_t idx = fetch_index(P, i);
     float pos[4], col[4], tc[2], nrm[3];
@@ -662,7 +707,8 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     r.ez0 = A.ez; r.ez1 = B.ez; r.ez2 = C.ez;
     r.lod = lod;
     r.back = back;
-    r.pad0[0] = r.pad0[1] = r.pad0[2] = 0;
+    r.pad0[0] = (int32_t)P.scene;
+    r.pad0[1] = r.pad0[2] = 0;
     r.tx0 = bx0 / TILE; r.ty0 = by0 / TILE; r.tx1 = bx1 / TILE; r.ty1 = by1 / TILE;
     r.ntiles = (r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
     r.pad[0] = r.pad[1] = r.pad[2] = 0;
@@ -683,9 +729,11 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
 // fast path: triangles inside every frustum plane pass through the clipper
 // unchanged (each plane re-emits the vertices in order), so they are set up
 // directly; the others are queued for k_setup_clip
-__global__ void __launch_bounds__(256) k_setup(SetupParams P)
+__global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint32_t *block_pre, uint32_t nscenes)
 {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t sc = __builtin_amdgcn_readfirstlane(find_scene(block_pre, nscenes, blockIdx.x));
+    const SetupParams &P = Ss[sc];
+    const uint32_t t = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
     if (t >= P.ntri) return;
     const mtgl_raster_state *rs = P.rs;
     const bool cold = P.vcold != nullptr;
@@ -711,25 +759,29 @@ __global__ void __launch_bounds__(256) k_setup(SetupParams P)
         if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, &nt)) { nout = 1; tiles = (unsigned long long)nt; }
     } else {
         uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
-        P.clip_list[slot] = t;
+        P.clip_list[slot] = P.tri_off + t;
     }
     // one counter update per wave
     const uint64_t kept = __ballot(nout != 0);
     if (kept && (threadIdx.x & 63) == __builtin_ctzll(__ballot(1)))
         atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
     P.n_rt[t] = nout;
-    P.rec_base[t] = t;
+    P.rec_base[t] = P.tri_off + t;
     P.tri_tiles[t] = tiles;
-    if (t == 0) P.tri_tiles[P.ntri] = 0;   // the scan's last element becomes the total
 }
 
 // slow path: Sutherland-Hodgman in the reference's plane order (near, far,
 // left, right, bottom, top; src/clipping.h:106-126), divide, fan split
-__global__ void __launch_bounds__(64) k_setup_clip(SetupParams P)
+// one thread per queued global triangle; tri_pre holds each scene's first triangle
+__global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const uint32_t *tri_pre, uint32_t nscenes,
+                                                   const uint32_t *clip_list, const Counters *ctr)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *(volatile uint32_t *)&P.ctr->nclip) return;
-    const uint32_t t = P.clip_list[i];
+    if (i >= *(volatile const uint32_t *)&ctr->nclip) return;
+    const uint32_t g = clip_list[i];
+    const uint32_t sc = find_scene(tri_pre, nscenes, g);
+    const SetupParams &P = Ss[sc];
+    const uint32_t t = g - P.tri_off;
     const mtgl_raster_state *rs = P.rs;
     const bool cold = P.vcold != nullptr;
     uint32_t ia, ib, ic;
@@ -755,8 +807,8 @@ __global__ void __launch_bounds__(64) k_setup_clip(SetupParams P)
         }
     }
     if (n >= 3) {
-        RTri *out = P.rt + P.ntri + base;
-        RCold *cout = cold ? P.rcold + P.ntri + base : nullptr;
+        RTri *out = P.rt_over + base;
+        RCold *cout = cold ? P.rcold_over + base : nullptr;
         for (int k = 0; k < n; k++) perspective_divide(b[k]);
         for (int j = 1; j + 1 < n; j++) {
             int nt = 0;
@@ -767,7 +819,7 @@ __global__ void __launch_bounds__(64) k_setup_clip(SetupParams P)
         }
     }
     P.n_rt[t] = nout;
-    P.rec_base[t] = P.ntri + base;
+    P.rec_base[t] = P.over_base + base;
     P.tri_tiles[t] = tiles;
     if (nout) atomicAdd(&P.ctr->tri_raster, (unsigned long long)nout);
 }
@@ -864,12 +916,23 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64
 // K3a: expand triangles into (tile, fan triangle) pairs, in submission order
 // ---------------------------------------------------------------------------
 
+// Pairs [0, npairs) exist; a launch sized for more (a captured graph sized by an
+// earlier run) marks the rest with a key beyond every tile, which sorts last.
 __global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
-                                                const uint32_t *rec_base, const RTri *rt, uint64_t npairs,
-                                                int32_t tiles_x, uint32_t *keys, uint32_t *vals)
+                                                const uint32_t *rec_base, const RTri *rt, uint64_t cap,
+                                                const SceneRaster *scenes, uint32_t *keys, uint32_t *vals,
+                                                Counters *ctr)
 {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= cap) return;
+    const uint64_t npairs = pair_base[ntri];
+    if (p >= npairs) {
+        keys[p] = 0xFFFFFFFFu;
+        vals[p] = 0u;
+        if (p == 0 && npairs > cap) atomicOr(&ctr->overflow_full, 2u);
+        return;
+    }
+    if (p == 0 && npairs > cap) atomicOr(&ctr->overflow_full, 2u);
     // last triangle whose first pair is <= p
     uint32_t lo = 0, hi = ntri;   // invariant: pair_base[lo] <= p < pair_base[hi]
     while (hi - lo > 1) {
@@ -888,10 +951,11 @@ __global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_b
         k -= nt;
     }
     const RTri &q = r[j];
+    const SceneRaster &S = scenes[q.pad0[0]];
     int32_t tw = q.tx1 - q.tx0 + 1;
     int32_t ty = q.ty0 + (int32_t)(k / (uint64_t)tw);
     int32_t tx = q.tx0 + (int32_t)(k % (uint64_t)tw);
-    keys[p] = (uint32_t)(ty * tiles_x + tx);
+    keys[p] = S.tile_off + (uint32_t)(ty * S.tiles_x + tx);
     vals[p] = rb + j;
 }
 
@@ -1000,19 +1064,19 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
     }
 }
 
-// The non-empty tiles of the draw: those whose bins hold more than HEAVY_BIN
+// The non-empty tiles of the launch: those whose bins hold more than heavy_bin
 // entries from the front of `list`, the others from its back, so that the
 // raster kernels, which walk the list from the front, start the longest tiles
 // first.  One thread per tile; one atomic per block and list end.
 __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
                                                    const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
-                                                   uint32_t *list, Counters *ctr)
+                                                   uint32_t heavy_bin, uint32_t *list, Counters *ctr)
 {
     __shared__ uint32_t s_n[2][4];
     __shared__ uint32_t s_base[2];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ne = t < ntiles && tile_gen[t] == gen;
-    const bool hv = ne && tile_end[t] - tile_start[t] > HEAVY_BIN;
+    const bool hv = ne && tile_end[t] - tile_start[t] > heavy_bin;
     const bool lt = ne && !hv;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t bh = __ballot(hv), bl = __ballot(lt);
@@ -1056,15 +1120,17 @@ __global__ void __launch_bounds__(256) k_gather_cov(const uint32_t *vals, const 
     out[i] = reinterpret_cast<const float4 *>(rt + vals[e])[q];
 }
 
-__global__ void k_reset(Counters *ctr)
+__global__ void k_reset(Counters *ctr, unsigned long long *scan_tail)
 {
     if (threadIdx.x == 0) {
+        *scan_tail = 0;   // the pair-offset scan's last element becomes the total
         ctr->tri_raster = 0;
         ctr->nheavy = 0;
         ctr->nlight = 0;
         ctr->unsupported = 0;
         ctr->nclip = 0;
         ctr->noverflow = 0;
+        ctr->sticky |= ctr->overflow_full;
         ctr->overflow_full = 0;
     }
 }
@@ -1172,6 +1238,7 @@ struct TexLevel {
 
 // the draw's fragment state, copied once per block into (scalar) registers
 struct RasterConsts {
+    const mtgl_raster_state *rs;
     bool depth_on, stencil_on, blend_on, fog_on, alpha_on, lighting_on, textured, persp, flat, phong, unit_range;
     bool rep_s, rep_t, depth_mask, two_side, depth_write;
     uint32_t pass_bytes;   // bytes a passing, kept fragment writes: depth and colour
@@ -1285,7 +1352,7 @@ __device__ __forceinline__ void stencil_write(const RasterConsts &K, PixelState 
 __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, const RTri *r, uint32_t rid, float b0,
                                 float b1, float b2, Col &c_out, float &depth_out, uint32_t &meta_out)
 {
-    const mtgl_raster_state *rs = P.rs;
+    const mtgl_raster_state *rs = K.rs;
     const float z = b0 * r->z0 + b1 * r->z1 + b2 * r->z2;
     if (K.unit_range) depth_out = (z + 1.0f) * 0.5f;   // (double)q * 1.0 + 0.0 rounds back to q
     else depth_out = (float)((double)((z + 1.0f) * 0.5f) * (K.depth_far - K.depth_near) + K.depth_near);
@@ -1618,7 +1685,7 @@ __device__ __forceinline__ uint64_t depth_pass(const RasterConsts &K, const RCov
 
 // The raster kernels run a persistent grid over the list of non-empty
 // tiles: block b takes items b, b + gridDim.x, ...; items [0, nheavy) are the
-// tiles with bins over HEAVY_BIN entries (the front of the list), the rest
+// tiles with bins over the heavy-bin threshold (the front of the list), the rest
 // come from the back, so the longest tiles start first.
 __device__ __forceinline__ void list_counts(const RasterParams &P, uint32_t &nh, uint32_t &it0, uint32_t &nt)
 {
@@ -1630,18 +1697,22 @@ __device__ __forceinline__ void list_counts(const RasterParams &P, uint32_t &nh,
     if (P.part == 1u) nt = nh;
 }
 
-__device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &tile, uint32_t &e0,
-                                          uint32_t &e1)
+// list item -> scene, the scene-local tile and the tile's bin [e0, e1)
+__device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &sc, uint32_t &tile,
+                                          uint32_t &e0, uint32_t &e1)
 {
-    tile = it < nh ? P.list[it] : P.list[P.ntiles - 1u - (it - nh)];
-    tile = __builtin_amdgcn_readfirstlane(tile);
-    e0 = __builtin_amdgcn_readfirstlane(P.tile_start[tile]);
-    e1 = __builtin_amdgcn_readfirstlane(P.tile_end[tile]);
+    uint32_t g = it < nh ? P.list[it] : P.list[P.ntiles - 1u - (it - nh)];
+    g = __builtin_amdgcn_readfirstlane(g);
+    e0 = __builtin_amdgcn_readfirstlane(P.tile_start[g]);
+    e1 = __builtin_amdgcn_readfirstlane(P.tile_end[g]);
+    sc = __builtin_amdgcn_readfirstlane(find_scene(P.tile_pre, P.nscenes, g));
+    tile = g - P.tile_pre[sc];
 }
 
-__device__ __forceinline__ void raster_consts(const RasterParams &P, RasterConsts &K)
+__device__ __forceinline__ void raster_consts(const SceneRaster &P, RasterConsts &K)
 {
     const mtgl_raster_state *rs = P.rs;
+    K.rs = rs;
     const uint32_t flags = rs->flags;
     K.depth_on = flags & MTGL_F_DEPTH_TEST;
     K.stencil_on = flags & MTGL_F_STENCIL_TEST;
@@ -1738,24 +1809,32 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     __shared__ uint16_t s_slot[DCAP];   // taps << 12 | entry << 6 | pixel
 
     RasterConsts K;
-    raster_consts(P, K);
-    K.stencil_on = STENCIL;   // the instantiation matches the draw's state
+    uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
     uint32_t frag_count = 0;
     unsigned long long frag_bytes = 0;
     for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
-    uint32_t tile, e0, e1;
-    item_tile(P, it, nh, tile, e0, e1);
-    const int32_t X0 = (int32_t)(tile % (uint32_t)P.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)P.tiles_x) * TILE;
+    uint32_t sc, tile, e0, e1;
+    item_tile(P, it, nh, sc, tile, e0, e1);
+    const SceneRaster &S = P.scenes[sc];
+    const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
+    if (!(kind & KIND_LAST_WRITER) || ((kind & KIND_STENCIL) != 0u) != STENCIL) continue;
+    if (sc != cur) {
+        raster_consts(S, K);
+        K.stencil_on = STENCIL;   // the instantiation matches the scene's state
+        cur = sc;
+    }
+    const int32_t X0 = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE;
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
-    const bool in_fb = x < P.fb_w && y < P.fb_h;
-    const size_t pix = in_fb ? (size_t)y * P.fb_w + x : 0;
+    const int32_t fb_w = S.fb_w, fb_h = S.fb_h;
+    const bool in_fb = x < fb_w && y < fb_h;
+    const size_t pix = in_fb ? (size_t)y * fb_w + x : 0;
     PixelState px = {0u, 1.0f, 0u, false, false, false};
     if (in_fb) {
-        if (K.depth_on) px.d = P.depth[pix];
-        if (K.stencil_on) px.s = P.stencil[pix];
+        if (K.depth_on) px.d = S.depth[pix];
+        if (K.stencil_on) px.s = S.stencil[pix];
     }
     uint32_t writer = 0;
     // the next chunk's entries are loaded (contiguous, in bin order) while the
@@ -1791,7 +1870,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
         const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
-                                       REC_F(q, area), valid, X0, Y0, P.fb_w, P.fb_h);
+                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h);
         const uint32_t cnt = (uint32_t)__popcll(m);
         frag_count += cnt;
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int32_t)dpp_incl_scan(cnt), 63);
@@ -1855,9 +1934,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         if (last >= 0) writer = 1u + s_rid[last];
     }
     if (in_fb) {
-        if (px.dirty_d) P.depth[pix] = px.d;
-        if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
-        P.writer[pix] = writer;
+        if (px.dirty_d) S.depth[pix] = px.d;
+        if (px.dirty_s) S.stencil[pix] = (uint8_t)px.s;
+        S.writer[pix] = writer;
     }
     }
     add_stats(P, frag_count, frag_bytes);
@@ -1869,17 +1948,25 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
 __global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
 {
     RasterConsts K;
-    raster_consts(P, K);
+    uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
     for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(it < nh ? P.list[it] : P.list[P.ntiles - 1u - (it - nh)]);
-    const int32_t x = (int32_t)(tile % (uint32_t)P.tiles_x) * TILE + (lane & 7);
-    const int32_t y = (int32_t)(tile / (uint32_t)P.tiles_x) * TILE + (lane >> 3);
-    if (x >= P.fb_w || y >= P.fb_h) continue;
-    const size_t pix = (size_t)y * P.fb_w + x;
-    const uint32_t writer = P.writer[pix];
+    uint32_t sc, tile, e0, e1;
+    item_tile(P, it, nh, sc, tile, e0, e1);
+    const SceneRaster &S = P.scenes[sc];
+    const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
+    if ((kind & (KIND_LAST_WRITER | KIND_COLOR)) != (KIND_LAST_WRITER | KIND_COLOR)) continue;
+    if (sc != cur) {
+        raster_consts(S, K);
+        cur = sc;
+    }
+    const int32_t x = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE + (lane & 7);
+    const int32_t y = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE + (lane >> 3);
+    if (x >= S.fb_w || y >= S.fb_h) continue;
+    const size_t pix = (size_t)y * S.fb_w + x;
+    const uint32_t writer = S.writer[pix];
     if (writer == 0u) continue;
     const RTri *r = P.rt + (writer - 1u);
     float w0e, w1e, w2e;
@@ -1889,7 +1976,7 @@ __global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
     float dep;
     uint32_t meta;
     fragment_source(P, K, r, writer - 1u, w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
-    P.color[pix] = col_pack(col_clamp(c));
+    S.color[pix] = col_pack(col_clamp(c));
     }
 }
 
@@ -1912,27 +1999,36 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     __shared__ uint8_t s_meta[FCAP];
 
     RasterConsts K;
-    raster_consts(P, K);
+    uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
-    const bool need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
-    // the alpha test reads the texture, and its outcome gates the depth write
-    const bool alpha_dep = K.alpha_on && K.textured;
+    bool need_color = false, alpha_dep = false;
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
     unsigned long long frag_count = 0, frag_bytes = 0;
     for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
-    uint32_t tile, e0, e1;
-    item_tile(P, it, nh, tile, e0, e1);
-    const int32_t X0 = (int32_t)(tile % (uint32_t)P.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)P.tiles_x) * TILE;
+    uint32_t sc, tile, e0, e1;
+    item_tile(P, it, nh, sc, tile, e0, e1);
+    const SceneRaster &S = P.scenes[sc];
+    const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
+    if (kind & KIND_LAST_WRITER) continue;
+    if (sc != cur) {
+        raster_consts(S, K);
+        need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
+        // the alpha test reads the texture, and its outcome gates the depth write
+        alpha_dep = K.alpha_on && K.textured;
+        cur = sc;
+    }
+    const int32_t X0 = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE;
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
     const float xf = (float)x, yf = (float)y;
-    const bool in_fb = x < P.fb_w && y < P.fb_h;
-    const size_t pix = in_fb ? (size_t)y * P.fb_w + x : 0;
+    const int32_t fb_w = S.fb_w, fb_h = S.fb_h;
+    const bool in_fb = x < fb_w && y < fb_h;
+    const size_t pix = in_fb ? (size_t)y * fb_w + x : 0;
     PixelState px = {0u, 1.0f, 0u, false, false, false};
     if (in_fb) {
-        if (need_color) px.c = P.color[pix];
-        if (K.depth_on) px.d = P.depth[pix];
-        if (K.stencil_on) px.s = P.stencil[pix];
+        if (need_color) px.c = S.color[pix];
+        if (K.depth_on) px.d = S.depth[pix];
+        if (K.stencil_on) px.s = S.stencil[pix];
     }
 
     for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
@@ -1958,7 +2054,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
             s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
         const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
-                                       REC_F(q, area), valid, X0, Y0, P.fb_w, P.fb_h);
+                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h);
         frag_count += (unsigned long long)__popcll(m);
         wave_sync();   // staged records visible to every lane
 
@@ -2016,9 +2112,9 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
         }
     }
     if (in_fb) {
-        if (px.dirty_c) P.color[pix] = px.c;
-        if (px.dirty_d) P.depth[pix] = px.d;
-        if (px.dirty_s) P.stencil[pix] = (uint8_t)px.s;
+        if (px.dirty_c) S.color[pix] = px.c;
+        if (px.dirty_d) S.depth[pix] = px.d;
+        if (px.dirty_s) S.stencil[pix] = (uint8_t)px.s;
     }
     }
     add_stats(P, frag_count, frag_bytes);
@@ -2054,11 +2150,50 @@ __global__ void __launch_bounds__(256) k_mip1(const uint32_t *src, int32_t w, ui
     dst[i] = col_pack(col_scale(k, 0.25f));
 }
 
+// A clear of part of one framebuffer, as the batched clear sees it.
+struct ClearDesc {
+    uint32_t *color;
+    float *depth;
+    uint8_t *stencil;
+    int32_t fb_w, x0, y0, w, h;
+    uint32_t cv;
+    float dv;
+    uint32_t sv;
+};
+
+// The clears of many framebuffers in one launch; block_pre holds each
+// clear's first block (n + 1 entries).
+__global__ void __launch_bounds__(256) k_clear_multi(const ClearDesc *cd, const uint32_t *block_pre, uint32_t n)
+{
+    const uint32_t sc = __builtin_amdgcn_readfirstlane(find_scene(block_pre, n, blockIdx.x));
+    const ClearDesc &c = cd[sc];
+    const uint64_t i = (uint64_t)(blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)c.w * (uint64_t)c.h) return;
+    const int32_t y = c.y0 + (int32_t)(i / (uint64_t)c.w), x = c.x0 + (int32_t)(i % (uint64_t)c.w);
+    const size_t p = (size_t)y * c.fb_w + x;
+    if (c.color) c.color[p] = c.cv;
+    if (c.depth) c.depth[p] = c.dv;
+    if (c.stencil) c.stencil[p] = (uint8_t)c.sv;
+}
+
+// Fill kernel, used where a memset node cannot be (graph capture).
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t n, uint32_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 }  // namespace
 
 // ===========================================================================
 // host side
 // ===========================================================================
+//
+// A launch runs the pipeline over 1..n scenes (one draw into one framebuffer
+// each).  mtgl_dev_draw is a launch of one scene on the context's stream; a
+// batch records the clears and draws of many contexts and runs round r (the
+// r-th command of every context) as one launch, then replays all rounds from
+// one captured hipGraph.
 
 namespace {
 
@@ -2078,30 +2213,25 @@ struct Arena {
     size_t cap = 0;
 };
 
-}  // namespace
-
-struct mtgl_dev {
-    int device = 0;
-    int cu_count = 256;
-    int32_t w = 0, h = 0, tiles_x = 0, tiles_y = 0;
+// Stream, pipeline buffers and counters shared by the launches of one owner
+// (a context for its own draws, a batch for its rounds).
+struct Exec {
     hipStream_t stream = nullptr;
-    uint32_t *color = nullptr;
-    float *depth = nullptr;
-    uint8_t *stencil = nullptr;
-    DevTex tex[MTGL_MAX_OBJECTS + 1];
-    DevBuf buf[MTGL_MAX_OBJECTS + 1];
-    Counters *ctr = nullptr;           // device
-    Counters *ctr_host = nullptr;      // pinned mirror for readback
+    int cu_count = 256;
+    uint32_t resident[4] = {256, 256, 256, 256};   // blocks at once: depth<stencil>, depth<no stencil>, colour, general
+    Counters *ctr = nullptr;                       // device
+    Counters *ctr_host = nullptr;                  // pinned mirror for readback
     unsigned long long *pairs_host = nullptr;
-    uint8_t *pinned = nullptr;         // staging for per-draw uploads
+    uint8_t *pinned = nullptr;                     // staging for uploads
     size_t pinned_cap = 0;
     hipEvent_t upload_done = nullptr;
-    Arena params, vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
+    Arena vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
-    Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list, writer, cov;
-    uint32_t gen = 0;                 // draw generation stamped into tile_gen
-    uint32_t resident[3] = {256, 256, 256};   // blocks of k_raster_depth, k_raster_color, k_raster at once
-    unsigned long long *stat_slots = nullptr;  // 2 x STAT_SLOTS
+    Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list, cov;
+    uint32_t gen = 0;                              // launch generation stamped into tile_gen
+    uint32_t heavy_bin = HEAVY_BIN;                // bins longer than this go to the front of the tile list
+    uint32_t grid_div = 1;                         // raster grids: resident blocks / grid_div
+    unsigned long long *stat_slots = nullptr;      // 2 x STAT_SLOTS
     std::vector<unsigned long long> stat_host;
     mtgl_dev_stats stats = {};
     bool timing = false;
@@ -2109,7 +2239,46 @@ struct mtgl_dev {
     std::vector<hipEvent_t> free_events;
 };
 
+// Device tables of one launch and what its kernels are launched with.  A
+// batch keeps one per round so that a captured graph replays the same launch.
+struct Launch {
+    Arena params;                  // raster/vertex states, staged bytes, scene tables
+    uint32_t ns = 0;               // scenes with triangles
+    uint32_t tri_total = 0, tiles_total = 0;
+    uint32_t vblocks = 0, sblocks = 0;
+    bool any_cold = false;
+    bool unsupported = false;
+    uint32_t kinds = 0;            // bit (1 << kind) for each raster kind present
+    uint64_t pairs_cap = 0;        // pairs of the sizing run (frozen replays use it)
+    const VertexParams *vp = nullptr;
+    const uint32_t *vblock_pre = nullptr;
+    const SetupParams *sp = nullptr;
+    const uint32_t *sblock_pre = nullptr;
+    const uint32_t *tri_pre = nullptr;
+    const SceneRaster *sr = nullptr;
+    const uint32_t *tile_pre = nullptr;
+};
+
+}  // namespace
+
+struct mtgl_dev {
+    int device = 0;
+    int32_t w = 0, h = 0, tiles_x = 0, tiles_y = 0;
+    uint32_t *color = nullptr;
+    float *depth = nullptr;
+    uint8_t *stencil = nullptr;
+    Arena writer;                  // per pixel: last passing fragment (depth kernel)
+    DevTex tex[MTGL_MAX_OBJECTS + 1];
+    DevBuf buf[MTGL_MAX_OBJECTS + 1];
+    Exec ex;
+    Launch one;                    // tables of the context's latest draw
+    mtgl_batch *pending = nullptr; // batch launched into this framebuffer, not yet joined
+    bool dirty = false;            // work queued on ex.stream since the last batch launch joined it
+};
+
 namespace {
+
+constexpr int RC_RETRY_CLIP = 1000;   // internal: the clip pool was too small
 
 int map_err(hipError_t e)
 {
@@ -2128,12 +2297,18 @@ int map_err(hipError_t e)
         }                                                                                           \
     } while (0)
 
-int arena_reserve(mtgl_dev *d, Arena &a, size_t bytes)
+// Grow a to at least bytes.  A frozen launch (a replay, or one being
+// captured) may not allocate: it runs on buffers its sizing run sized.
+int arena_reserve(Exec &x, Arena &a, size_t bytes, bool frozen = false)
 {
     if (bytes <= a.cap) return MTGL_OK;
+    if (frozen) {
+        fprintf(stderr, "libmtgl: a replayed launch needs %zu bytes, its sizing run reserved %zu\n", bytes, a.cap);
+        return MTGL_ERR_INVALID;
+    }
     size_t cap = a.cap ? a.cap : 4096;
     while (cap < bytes) cap *= 2;
-    HIPC(hipStreamSynchronize(d->stream));
+    HIPC(hipStreamSynchronize(x.stream));
     if (a.ptr) HIPC(hipFree(a.ptr));
     a.ptr = nullptr;
     a.cap = 0;
@@ -2142,45 +2317,52 @@ int arena_reserve(mtgl_dev *d, Arena &a, size_t bytes)
     return MTGL_OK;
 }
 
+void arena_free(Arena &a)
+{
+    if (a.ptr) (void)hipFree(a.ptr);
+    a = Arena();
+}
+
 template <typename T>
 T *as(Arena &a) { return reinterpret_cast<T *>(a.ptr); }
 
-int pinned_reserve(mtgl_dev *d, size_t bytes)
+int pinned_reserve(Exec &x, size_t bytes)
 {
-    if (d->upload_done) HIPC(hipEventSynchronize(d->upload_done));
-    if (bytes <= d->pinned_cap) return MTGL_OK;
-    size_t cap = d->pinned_cap ? d->pinned_cap : (1u << 20);
+    if (x.upload_done) HIPC(hipEventSynchronize(x.upload_done));
+    if (bytes <= x.pinned_cap) return MTGL_OK;
+    size_t cap = x.pinned_cap ? x.pinned_cap : (64u << 10);
     while (cap < bytes) cap *= 2;
-    if (d->pinned) HIPC(hipHostFree(d->pinned));
-    d->pinned = nullptr;
-    d->pinned_cap = 0;
-    HIPC(hipHostMalloc((void **)&d->pinned, cap, hipHostMallocDefault));
-    d->pinned_cap = cap;
+    if (x.pinned) HIPC(hipHostFree(x.pinned));
+    x.pinned = nullptr;
+    x.pinned_cap = 0;
+    HIPC(hipHostMalloc((void **)&x.pinned, cap, hipHostMallocDefault));
+    x.pinned_cap = cap;
     return MTGL_OK;
 }
 
 unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
-// exclusive scan of n values of T (in may equal out); *total_dev receives the sum
+size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// exclusive scan of n values of T; out may alias in
 template <typename T>
-int device_scan(mtgl_dev *d, const T *in, T *out, uint64_t n)
+int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen)
 {
-    // exclusive scan of n values; out may alias in
     if (n == 0) return MTGL_OK;
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    int rc = arena_reserve(d, d->scan_tmp, nb * sizeof(T));
+    int rc = arena_reserve(x, x.scan_tmp, nb * sizeof(T), frozen);
     if (rc) return rc;
-    T *sums = as<T>(d->scan_tmp);
-    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, d->stream, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, d->stream, sums, nb, (T *)nullptr);
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, d->stream, in, n,
+    T *sums = as<T>(x.scan_tmp);
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, x.stream, sums, nb, (T *)nullptr);
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n,
                        (const T *)sums, out);
     HIPC(hipGetLastError());
     return MTGL_OK;
 }
 
 // resolve an mtgl_array to a device stream
-DArr resolve(mtgl_dev *d, const mtgl_array &a, const uint8_t *staged_dev, uint64_t staged_bytes)
+DArr resolve(const mtgl_dev *d, const mtgl_array &a, const uint8_t *staged_dev, uint64_t staged_bytes)
 {
     DArr r = {};
     r.enabled = a.enabled;
@@ -2199,11 +2381,11 @@ DArr resolve(mtgl_dev *d, const mtgl_array &a, const uint8_t *staged_dev, uint64
     return r;
 }
 
-hipEvent_t take_event(mtgl_dev *d)
+hipEvent_t take_event(Exec &x)
 {
-    if (!d->free_events.empty()) {
-        hipEvent_t e = d->free_events.back();
-        d->free_events.pop_back();
+    if (!x.free_events.empty()) {
+        hipEvent_t e = x.free_events.back();
+        x.free_events.pop_back();
         return e;
     }
     hipEvent_t e = nullptr;
@@ -2211,19 +2393,812 @@ hipEvent_t take_event(mtgl_dev *d)
     return e;
 }
 
-int resolve_timing(mtgl_dev *d)
+int resolve_timing(Exec &x)
 {
-    for (auto &pr : d->pending_events) {
+    for (auto &pr : x.pending_events) {
         HIPC(hipEventSynchronize(pr.second));
         float ms = 0.0f;
         HIPC(hipEventElapsedTime(&ms, pr.first, pr.second));
-        d->stats.raster_ms += ms;
-        d->stats.raster_launches++;
-        d->free_events.push_back(pr.first);
-        d->free_events.push_back(pr.second);
+        x.stats.raster_ms += ms;
+        x.stats.raster_launches++;
+        x.free_events.push_back(pr.first);
+        x.free_events.push_back(pr.second);
     }
-    d->pending_events.clear();
+    x.pending_events.clear();
     return MTGL_OK;
+}
+
+int exec_init(Exec &x, int device)
+{
+    if (hipDeviceGetAttribute(&x.cu_count, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        x.cu_count <= 0)
+        x.cu_count = 256;
+    // resident one-wave blocks of each raster kernel (registers and LDS bound)
+    const void *kern[4] = {(const void *)k_raster_depth<true>, (const void *)k_raster_depth<false>,
+                           (const void *)k_raster_color, (const void *)k_raster};
+    for (int k = 0; k < 4; k++) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
+        x.resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)x.cu_count, STAT_SLOTS);
+    }
+    HIPC(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    HIPC(hipMalloc((void **)&x.ctr, sizeof(Counters)));
+    HIPC(hipMalloc((void **)&x.stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long)));
+    HIPC(hipHostMalloc((void **)&x.ctr_host, sizeof(Counters), hipHostMallocDefault));
+    HIPC(hipHostMalloc((void **)&x.pairs_host, sizeof(unsigned long long), hipHostMallocDefault));
+    HIPC(hipEventCreateWithFlags(&x.upload_done, hipEventDisableTiming));
+    HIPC(hipMemsetAsync(x.stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long), x.stream));
+    HIPC(hipMemsetAsync(x.ctr, 0, sizeof(Counters), x.stream));
+    HIPC(hipStreamSynchronize(x.stream));
+    return MTGL_OK;
+}
+
+void exec_free(Exec &x)
+{
+    if (x.stream) (void)hipStreamSynchronize(x.stream);
+    Arena *arenas[] = {&x.vhot, &x.vcold, &x.rt, &x.rcold, &x.n_rt, &x.rec_base, &x.tri_tiles, &x.pair_base,
+                       &x.scan_tmp, &x.clip_list, &x.keys[0], &x.keys[1], &x.vals[0], &x.vals[1], &x.sort_tmp,
+                       &x.tile_gen, &x.tile_start, &x.tile_end, &x.tile_list, &x.cov};
+    for (Arena *a : arenas) arena_free(*a);
+    for (auto &pr : x.pending_events) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    for (hipEvent_t e : x.free_events) (void)hipEventDestroy(e);
+    x.pending_events.clear();
+    x.free_events.clear();
+    if (x.ctr) (void)hipFree(x.ctr);
+    if (x.stat_slots) (void)hipFree(x.stat_slots);
+    if (x.ctr_host) (void)hipHostFree(x.ctr_host);
+    if (x.pairs_host) (void)hipHostFree(x.pairs_host);
+    if (x.pinned) (void)hipHostFree(x.pinned);
+    if (x.upload_done) (void)hipEventDestroy(x.upload_done);
+    if (x.stream) (void)hipStreamDestroy(x.stream);
+    x.ctr = nullptr;
+    x.stat_slots = nullptr;
+    x.ctr_host = nullptr;
+    x.pairs_host = nullptr;
+    x.pinned = nullptr;
+    x.upload_done = nullptr;
+    x.stream = nullptr;
+}
+
+int exec_stats(Exec &x, mtgl_dev_stats *out)
+{
+    x.stat_host.resize(2 * STAT_SLOTS);
+    HIPC(hipMemcpyAsync(x.stat_host.data(), x.stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, x.stream));
+    HIPC(hipStreamSynchronize(x.stream));
+    int rc = resolve_timing(x);
+    if (rc) return rc;
+    *out = x.stats;
+    out->fragments = 0;
+    out->frag_bytes = 0;
+    for (uint32_t i = 0; i < STAT_SLOTS; i++) {
+        out->fragments += x.stat_host[2 * i];
+        out->frag_bytes += x.stat_host[2 * i + 1];
+    }
+    return MTGL_OK;
+}
+
+int exec_stats_reset(Exec &x)
+{
+    HIPC(hipStreamSynchronize(x.stream));
+    int rc = resolve_timing(x);
+    if (rc) return rc;
+    x.stats = mtgl_dev_stats();
+    HIPC(hipMemsetAsync(x.ctr, 0, sizeof(Counters), x.stream));
+    HIPC(hipMemsetAsync(x.stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long), x.stream));
+    HIPC(hipStreamSynchronize(x.stream));
+    return MTGL_OK;
+}
+
+bool draw_mode_ok(uint32_t mode)
+{
+    switch (mode) {
+    case GL_TRIANGLES: case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_QUADS: case GL_QUAD_STRIP: case GL_POLYGON:
+        return true;
+    default:
+        return false;
+    }
+}
+
+const DevTex *bound_texture(const mtgl_dev *d, const mtgl_raster_state *rs)
+{
+    return (rs->texture && rs->texture <= MTGL_MAX_OBJECTS && d->tex[rs->texture].px) ? &d->tex[rs->texture] : nullptr;
+}
+
+uint32_t raster_kind(const mtgl_raster_state *rs, const DevTex *tx)
+{
+    const uint32_t cm = rs->color_mask & 0xFu;
+    // without blending, a partial colour mask or an alpha test on a texture,
+    // only each pixel's last passing fragment leaves its colour
+    const bool lw = !(rs->flags & MTGL_F_BLEND) && (cm == 0xFu || cm == 0u) && !((rs->flags & MTGL_F_ALPHA_TEST) && tx);
+    uint32_t k = lw ? KIND_LAST_WRITER : 0u;
+    if (rs->flags & MTGL_F_STENCIL_TEST) k |= KIND_STENCIL;
+    if (cm) k |= KIND_COLOR;
+    return k;
+}
+
+bool needs_cold(const mtgl_raster_state *rs)
+{
+    return (rs->flags & MTGL_F_LIGHTING) && (rs->shade_model == GL_PHONG || rs->lighting.two_side);
+}
+
+// one draw of one context, as a launch sees it
+struct SceneIn {
+    mtgl_dev *dev;
+    const mtgl_draw *dr;
+};
+
+// Vertex, triangle and tile totals of a set of scenes and the buffers they need.
+struct Sizes {
+    uint64_t vtx = 0, tri = 0, tiles = 0;
+    bool cold = false;
+};
+
+Sizes scene_sizes(const SceneIn *in, uint32_t n)
+{
+    Sizes z;
+    for (uint32_t i = 0; i < n; i++) {
+        const mtgl_draw *dr = in[i].dr;
+        if (!draw_mode_ok(dr->mode)) continue;
+        const uint32_t ntri = tri_count(dr->mode, dr->count);
+        if (ntri == 0) continue;
+        z.vtx += dr->count;
+        z.tri += ntri;
+        z.tiles += (uint64_t)in[i].dev->tiles_x * (uint64_t)in[i].dev->tiles_y;
+        z.cold |= needs_cold(dr->raster);
+    }
+    return z;
+}
+
+int reserve_scene_buffers(Exec &x, const Sizes &z)
+{
+    int rc;
+    const size_t t = (size_t)z.tri;
+    if ((rc = arena_reserve(x, x.vhot, (size_t)z.vtx * sizeof(VHot)))) return rc;
+    if (z.cold && (rc = arena_reserve(x, x.vcold, (size_t)z.vtx * sizeof(VCold)))) return rc;
+    // record pool: one slot per triangle plus an overflow pool for clipped ones
+    if (x.overflow_cap < 4096u) x.overflow_cap = 4096u;
+    x.overflow_cap = std::max<uint32_t>(x.overflow_cap, std::min<uint32_t>((uint32_t)(z.tri / 8u), 1u << 24));
+    if ((rc = arena_reserve(x, x.rt, (t + x.overflow_cap) * sizeof(RTri)))) return rc;
+    if (z.cold && (rc = arena_reserve(x, x.rcold, (t + x.overflow_cap) * sizeof(RCold)))) return rc;
+    if ((rc = arena_reserve(x, x.n_rt, t * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(x, x.rec_base, t * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(x, x.tri_tiles, (t + 1) * sizeof(unsigned long long)))) return rc;
+    if ((rc = arena_reserve(x, x.pair_base, (t + 1) * sizeof(unsigned long long)))) return rc;
+    if ((rc = arena_reserve(x, x.clip_list, t * sizeof(uint32_t)))) return rc;
+    return MTGL_OK;
+}
+
+// Upload the scenes' raster and vertex states, staged client arrays and the
+// scene tables into L.params, after sizing the vertex and record buffers.
+// Scenes without triangles or with an unsupported primitive are left out.
+// The tables hold pointers into x's buffers: a later growth of those buffers
+// needs the tables prepared again.
+int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
+{
+    struct Plan {
+        const SceneIn *s;
+        uint32_t ntri;
+        bool cold;
+        const DevTex *tx;
+        size_t off_rs, off_vs, off_st;
+    };
+    std::vector<Plan> plan;
+    plan.reserve(n);
+    size_t bytes = 0;
+    const uint64_t pairs_cap = L.pairs_cap;
+    Arena params = L.params;
+    L = Launch();
+    L.params = params;
+    L.pairs_cap = pairs_cap;
+    for (uint32_t i = 0; i < n; i++) {
+        const mtgl_draw *dr = in[i].dr;
+        if (!draw_mode_ok(dr->mode)) {
+            L.unsupported = true;
+            continue;
+        }
+        const uint32_t ntri = tri_count(dr->mode, dr->count);
+        if (ntri == 0) continue;
+        Plan p;
+        p.s = &in[i];
+        p.ntri = ntri;
+        p.cold = needs_cold(dr->raster);
+        p.tx = bound_texture(in[i].dev, dr->raster);
+        p.off_rs = bytes;
+        bytes += up256(sizeof(mtgl_raster_state));
+        p.off_vs = bytes;
+        bytes += up256((size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+        p.off_st = bytes;
+        bytes += up256((size_t)dr->staged_bytes);
+        plan.push_back(p);
+    }
+    const uint32_t ns = (uint32_t)plan.size();
+    L.ns = ns;
+    if (ns == 0) return MTGL_OK;
+    const size_t off_vp = bytes;
+    bytes += up256(ns * sizeof(VertexParams));
+    const size_t off_sp = bytes;
+    bytes += up256(ns * sizeof(SetupParams));
+    const size_t off_sr = bytes;
+    bytes += up256(ns * sizeof(SceneRaster));
+    const size_t off_pre = bytes;
+    bytes += up256(4 * (size_t)(ns + 1) * sizeof(uint32_t));
+
+    const Sizes z = scene_sizes(in, n);
+    uint64_t vb = 0, sb = 0;
+    for (const Plan &p : plan) {
+        vb += blocks_for(p.s->dr->count, 256);
+        sb += blocks_for(p.ntri, 256);
+    }
+    if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles > 0x7FFFFFFFull || vb > 0x7FFFFFFFull)
+        return MTGL_ERR_OUT_OF_MEMORY;
+    L.tri_total = (uint32_t)z.tri;
+    L.tiles_total = (uint32_t)z.tiles;
+    L.vblocks = (uint32_t)vb;
+    L.sblocks = (uint32_t)sb;
+    L.any_cold = z.cold;
+
+    int rc;
+    if ((rc = reserve_scene_buffers(x, z))) return rc;
+    for (const Plan &p : plan) {
+        if (raster_kind(p.s->dr->raster, p.tx) & KIND_LAST_WRITER) {
+            mtgl_dev *d = p.s->dev;
+            if ((rc = arena_reserve(x, d->writer, (size_t)d->w * (size_t)d->h * 4))) return rc;
+        }
+    }
+    if ((rc = arena_reserve(x, L.params, bytes))) return rc;
+    if ((rc = pinned_reserve(x, bytes))) return rc;
+
+    const uint32_t ttot = L.tri_total;
+    uint8_t *host = x.pinned;
+    const uint8_t *dev = (const uint8_t *)L.params.ptr;
+    VertexParams *vp = reinterpret_cast<VertexParams *>(host + off_vp);
+    SetupParams *sp = reinterpret_cast<SetupParams *>(host + off_sp);
+    SceneRaster *sr = reinterpret_cast<SceneRaster *>(host + off_sr);
+    uint32_t *pre = reinterpret_cast<uint32_t *>(host + off_pre);
+    uint32_t *vbp = pre, *sbp = pre + (ns + 1), *tp = pre + 2 * (ns + 1), *tlp = pre + 3 * (ns + 1);
+    uint32_t voff = 0, toff = 0, tileoff = 0, vbo = 0, sbo = 0;
+    for (uint32_t s = 0; s < ns; s++) {
+        const Plan &p = plan[s];
+        mtgl_dev *d = p.s->dev;
+        const mtgl_draw *dr = p.s->dr;
+        memcpy(host + p.off_rs, dr->raster, sizeof(mtgl_raster_state));
+        memcpy(host + p.off_vs, dr->vstates, (size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+        if (dr->staged_bytes) memcpy(host + p.off_st, dr->staged, (size_t)dr->staged_bytes);
+        const mtgl_raster_state *rs_dev = (const mtgl_raster_state *)(dev + p.off_rs);
+        const uint8_t *staged_dev = dev + p.off_st;
+        const uint64_t stb = dr->staged_bytes;
+
+        VertexParams v = {};
+        v.count = dr->count;
+        v.first = dr->first;
+        v.index_type = dr->index_type;
+        v.n_vstates = dr->n_vstates;
+        v.indices = resolve(d, dr->indices, staged_dev, stb);
+        v.indices.enabled = dr->index_type != 0;
+        v.position = resolve(d, dr->position, staged_dev, stb);
+        v.color = resolve(d, dr->color, staged_dev, stb);
+        v.texcoord = resolve(d, dr->texcoord, staged_dev, stb);
+        v.normal = resolve(d, dr->normal, staged_dev, stb);
+        v.state_index = resolve(d, dr->state_index, staged_dev, stb);
+        memcpy(v.cur_color, dr->cur_color, sizeof(v.cur_color));
+        memcpy(v.cur_texcoord, dr->cur_texcoord, sizeof(v.cur_texcoord));
+        memcpy(v.cur_normal, dr->cur_normal, sizeof(v.cur_normal));
+        v.vstates = (const mtgl_vertex_state *)(dev + p.off_vs);
+        v.out_hot = as<VHot>(x.vhot) + voff;
+        v.out_cold = p.cold ? as<VCold>(x.vcold) + voff : nullptr;
+        vp[s] = v;
+
+        SetupParams q = {};
+        q.mode = dr->mode;
+        q.count = dr->count;
+        q.ntri = p.ntri;
+        q.tri_off = toff;
+        q.scene = s;
+        q.fb_w = d->w;
+        q.fb_h = d->h;
+        q.tiles_x = d->tiles_x;
+        q.tex_w = p.tx ? p.tx->w : 0;
+        q.tex_h = p.tx ? p.tx->h : 0;
+        q.rs = rs_dev;
+        q.vhot = as<VHot>(x.vhot) + voff;
+        q.vcold = p.cold ? as<VCold>(x.vcold) + voff : nullptr;
+        q.rt = as<RTri>(x.rt) + toff;
+        q.rcold = L.any_cold ? as<RCold>(x.rcold) + toff : nullptr;
+        q.rt_over = as<RTri>(x.rt) + ttot;
+        q.rcold_over = L.any_cold ? as<RCold>(x.rcold) + ttot : nullptr;
+        q.over_base = ttot;
+        q.n_rt = as<uint32_t>(x.n_rt) + toff;
+        q.rec_base = as<uint32_t>(x.rec_base) + toff;
+        q.tri_tiles = as<unsigned long long>(x.tri_tiles) + toff;
+        q.clip_list = as<uint32_t>(x.clip_list);
+        q.overflow_cap = x.overflow_cap;
+        q.ctr = x.ctr;
+        sp[s] = q;
+
+        SceneRaster r = {};
+        r.rs = rs_dev;
+        r.color = d->color;
+        r.depth = d->depth;
+        r.stencil = d->stencil;
+        r.kind = raster_kind(dr->raster, p.tx);
+        r.writer = (r.kind & KIND_LAST_WRITER) ? as<uint32_t>(d->writer) : nullptr;
+        r.tex = p.tx ? p.tx->px : nullptr;
+        r.tex1 = p.tx ? p.tx->m1 : nullptr;
+        r.tex_w = p.tx ? p.tx->w : 0;
+        r.tex_h = p.tx ? p.tx->h : 0;
+        r.tex1_w = p.tx ? p.tx->m1w : 0;
+        r.tex1_h = p.tx ? p.tx->m1h : 0;
+        r.fb_w = d->w;
+        r.fb_h = d->h;
+        r.tiles_x = d->tiles_x;
+        r.tile_off = tileoff;
+        sr[s] = r;
+        L.kinds |= 1u << r.kind;
+
+        vbp[s] = vbo;
+        sbp[s] = sbo;
+        tp[s] = toff;
+        tlp[s] = tileoff;
+        voff += dr->count;
+        toff += p.ntri;
+        tileoff += (uint32_t)(d->tiles_x * d->tiles_y);
+        vbo += blocks_for(dr->count, 256);
+        sbo += blocks_for(p.ntri, 256);
+    }
+    vbp[ns] = vbo;
+    sbp[ns] = sbo;
+    tp[ns] = toff;
+    tlp[ns] = tileoff;
+    HIPC(hipMemcpyAsync(L.params.ptr, host, bytes, hipMemcpyHostToDevice, x.stream));
+    HIPC(hipEventRecord(x.upload_done, x.stream));
+    L.vp = (const VertexParams *)(dev + off_vp);
+    L.sp = (const SetupParams *)(dev + off_sp);
+    L.sr = (const SceneRaster *)(dev + off_sr);
+    const uint32_t *pre_dev = (const uint32_t *)(dev + off_pre);
+    L.vblock_pre = pre_dev;
+    L.sblock_pre = pre_dev + (ns + 1);
+    L.tri_pre = pre_dev + 2 * (ns + 1);
+    L.tile_pre = pre_dev + 3 * (ns + 1);
+    return MTGL_OK;
+}
+
+// The pipeline over the scenes of L.  A sizing run (frozen false) reads the
+// pair total back and sizes the binning buffers; a frozen run (a replay, or
+// one being captured into a graph) allocates, synchronises and reads back
+// nothing and takes the pair total of the sizing run.
+int run_launch(Exec &x, Launch &L, bool frozen)
+{
+    if (L.ns == 0) return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+    int rc;
+    const uint32_t ttot = L.tri_total;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, x.stream, x.ctr, as<unsigned long long>(x.tri_tiles) + ttot);
+    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(256), 0, x.stream, L.vp, L.vblock_pre, L.ns);
+    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(256), 0, x.stream, L.sp, L.sblock_pre, L.ns);
+    hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
+                       (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
+    HIPC(hipGetLastError());
+
+    // ---- K3: pair offsets and total, then expand / sort / segment
+    unsigned long long *pair_base = as<unsigned long long>(x.pair_base);
+    if ((rc = device_scan<unsigned long long>(x, as<unsigned long long>(x.tri_tiles), pair_base, (uint64_t)ttot + 1,
+                                              frozen)))
+        return rc;
+    uint64_t npairs;
+    if (!frozen) {
+        HIPC(hipMemcpyAsync(x.pairs_host, pair_base + ttot, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            x.stream));
+        HIPC(hipMemcpyAsync(x.ctr_host, x.ctr, sizeof(Counters), hipMemcpyDeviceToHost, x.stream));
+        HIPC(hipStreamSynchronize(x.stream));
+        // the clipped triangles asked for more slots than the pool held:
+        // nothing downstream ran; the caller grows the pool and runs again
+        if (x.ctr_host->overflow_full & 1u) return RC_RETRY_CLIP;
+        npairs = *x.pairs_host;
+        L.unsupported |= x.ctr_host->unsupported != 0;
+        x.stats.triangles_raster += x.ctr_host->tri_raster;
+        if (npairs > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+        L.pairs_cap = npairs;
+        x.stats.bin_pairs += npairs;
+    } else {
+        npairs = L.pairs_cap;
+    }
+    if (npairs == 0) return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+    const uint32_t n = (uint32_t)npairs;
+    for (int k = 0; k < 2; k++) {
+        if ((rc = arena_reserve(x, x.keys[k], (size_t)n * 4, frozen))) return rc;
+        if ((rc = arena_reserve(x, x.vals[k], (size_t)n * 4, frozen))) return rc;
+    }
+    const uint32_t ntiles = L.tiles_total;
+    if ((rc = arena_reserve(x, x.tile_start, (size_t)ntiles * 4, frozen))) return rc;
+    if ((rc = arena_reserve(x, x.tile_end, (size_t)ntiles * 4, frozen))) return rc;
+    if ((rc = arena_reserve(x, x.tile_list, (size_t)ntiles * 4, frozen))) return rc;
+    if (!frozen) {
+        if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
+            if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
+            HIPC(hipMemsetAsync(x.tile_gen.ptr, 0, x.tile_gen.cap, x.stream));
+            x.gen = 0;
+        }
+        x.gen++;
+    } else {
+        // replays of different rounds share one generation: clear the stamps
+        if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4, true))) return rc;
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream, as<uint32_t>(x.tile_gen),
+                           ntiles, 0u);
+    }
+    uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
+    uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
+    hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, (const unsigned long long *)pair_base,
+                       ttot, (const uint32_t *)as<uint32_t>(x.n_rt), (const uint32_t *)as<uint32_t>(x.rec_base),
+                       (const RTri *)as<RTri>(x.rt), (uint64_t)n, L.sr, keys, vals, x.ctr);
+    // stable LSD radix sort by tile: pairs keep submission order within a tile;
+    // the all-ones key of unused capacity stays above every tile
+    int key_bits = 1;
+    while ((1u << key_bits) <= ntiles) key_bits++;
+    const uint32_t nblocks = (n + SORT_TILE - 1) / SORT_TILE;
+    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)RADIX * nblocks * 4, frozen))) return rc;
+    uint32_t *counts = as<uint32_t>(x.sort_tmp);
+    for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, n, shift,
+                           nblocks, counts);
+        if ((rc = device_scan<uint32_t>(x, counts, counts, (uint64_t)RADIX * nblocks, frozen))) return rc;
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
+                           (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
+        std::swap(keys, keys2);
+        std::swap(vals, vals2);
+    }
+    if ((rc = arena_reserve(x, x.cov, (size_t)n * RT_COV_VEC4 * sizeof(float4), frozen))) return rc;
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, n, ntiles,
+                       as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen), x.gen);
+    hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
+                       (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
+                       (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin,
+                       as<uint32_t>(x.tile_list), x.ctr);
+    hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, x.stream,
+                       (const uint32_t *)vals, (const RTri *)as<RTri>(x.rt), n, as<float4>(x.cov));
+    HIPC(hipGetLastError());
+
+    // ---- K4
+    RasterParams rp = {};
+    rp.scenes = L.sr;
+    rp.tile_pre = L.tile_pre;
+    rp.nscenes = L.ns;
+    rp.rt = as<RTri>(x.rt);
+    rp.rcold = L.any_cold ? as<RCold>(x.rcold) : nullptr;
+    rp.vals = vals;
+    rp.list = as<uint32_t>(x.tile_list);
+    rp.ntiles = ntiles;
+    rp.part = 0;
+    rp.cov = as<float4>(x.cov);
+    rp.tile_start = as<uint32_t>(x.tile_start);
+    rp.tile_end = as<uint32_t>(x.tile_end);
+    rp.ctr = x.ctr;
+    rp.stats = x.stat_slots;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (x.timing && !frozen) {
+        ev0 = take_event(x);
+        ev1 = take_event(x);
+        if (ev0) HIPC(hipEventRecord(ev0, x.stream));
+    }
+    // persistent grids over the tile list (longest bins first): at most one
+    // block per non-empty tile, at most as many as are resident at once
+    const uint32_t nlist = std::min<uint32_t>(n, ntiles);
+    auto grid = [&](int k) { return dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nlist, x.resident[k] / x.grid_div))); };
+    const uint32_t LW = KIND_LAST_WRITER, ST = KIND_STENCIL, CO = KIND_COLOR;
+    const uint32_t lw_stencil = (1u << (LW | ST)) | (1u << (LW | ST | CO));
+    const uint32_t lw_plain = (1u << LW) | (1u << (LW | CO));
+    const uint32_t lw_color = (1u << (LW | CO)) | (1u << (LW | ST | CO));
+    const uint32_t general = (1u << 0) | (1u << ST) | (1u << CO) | (1u << (ST | CO));
+    if (L.kinds & lw_stencil) hipLaunchKernelGGL(k_raster_depth<true>, grid(0), dim3(64), 0, x.stream, rp);
+    if (L.kinds & lw_plain) hipLaunchKernelGGL(k_raster_depth<false>, grid(1), dim3(64), 0, x.stream, rp);
+    if (L.kinds & lw_color) hipLaunchKernelGGL(k_raster_color, grid(2), dim3(64), 0, x.stream, rp);
+    if (L.kinds & general) hipLaunchKernelGGL(k_raster, grid(3), dim3(64), 0, x.stream, rp);
+    HIPC(hipGetLastError());
+    if (ev0 && ev1) {
+        HIPC(hipEventRecord(ev1, x.stream));
+        x.pending_events.emplace_back(ev0, ev1);
+    }
+    return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+}
+
+// prepare + sizing run, growing the clip overflow pool once when it was too small
+int execute_scenes(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
+{
+    for (int attempt = 0; attempt < 3; attempt++) {
+        int rc = prepare_launch(x, L, in, n);
+        if (rc) return rc;
+        rc = run_launch(x, L, false);
+        if (rc != RC_RETRY_CLIP) return rc;
+        x.overflow_cap = std::max<uint32_t>(x.ctr_host->noverflow, x.overflow_cap * 2u);
+    }
+    return MTGL_ERR_DEVICE;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct BatchCmd {
+    mtgl_dev *dev = nullptr;
+    bool is_clear = false;
+    ClearDesc clear = {};
+    mtgl_draw draw = {};                   // its pointers point at the copies below
+    mtgl_raster_state raster = {};
+    std::vector<mtgl_vertex_state> vstates;
+    std::vector<uint8_t> staged;
+};
+
+struct BatchRound {
+    std::vector<SceneIn> scenes;           // draws of the round, at most one per context
+    std::vector<ClearDesc> clears;
+    Launch launch;
+    Arena clear_params;
+    uint32_t clear_blocks = 0;
+    const ClearDesc *clear_dev = nullptr;
+    const uint32_t *clear_pre = nullptr;
+};
+
+}  // namespace
+
+struct mtgl_batch {
+    int device = 0;
+    Exec ex;
+    std::vector<BatchCmd *> cmds;
+    std::vector<mtgl_dev *> devs;          // contexts in first-use order
+    std::vector<BatchRound> rounds;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipEvent_t done = nullptr;             // end of the latest launch
+    hipEvent_t join = nullptr;
+    bool recorded = false;                 // mtgl_batch_end ran
+    bool ready = false;                    // a graph is instantiated and every context is alive
+    mtgl_batch_info_t info = {};
+};
+
+namespace {
+
+thread_local mtgl_batch *g_recording = nullptr;
+std::mutex g_batches_mu;
+std::vector<mtgl_batch *> g_batches;   // live batches that hold contexts
+
+// A context's buffers or objects are about to be reallocated or freed: the
+// graphs of the batches that use it hold stale pointers and stop launching.
+void invalidate_batches(mtgl_dev *d)
+{
+    std::lock_guard<std::mutex> lk(g_batches_mu);
+    for (mtgl_batch *b : g_batches) {
+        for (mtgl_dev *e : b->devs) {
+            if (e == d) {
+                if (b->ready) fprintf(stderr, "libmtgl: a context of a batch changed its objects; the batch must be rebuilt\n");
+                b->ready = false;
+                break;
+            }
+        }
+    }
+}
+
+// Order the context's next work after the batch launch that rendered into it.
+int dev_join(mtgl_dev *d)
+{
+    if (d->pending) {
+        HIPC(hipStreamWaitEvent(d->ex.stream, d->pending->done, 0));
+        d->pending = nullptr;
+    }
+    return MTGL_OK;
+}
+
+void batch_note_dev(mtgl_batch *b, mtgl_dev *d)
+{
+    for (mtgl_dev *e : b->devs)
+        if (e == d) return;
+    b->devs.push_back(d);
+}
+
+// Round r holds the r-th recorded command of every context, so each
+// context's commands keep their order and a round touches each framebuffer
+// at most once.
+void batch_plan(mtgl_batch *b)
+{
+    b->rounds.clear();
+    std::vector<std::vector<BatchCmd *>> per_dev(b->devs.size());
+    for (BatchCmd *c : b->cmds) {
+        size_t i = 0;
+        while (b->devs[i] != c->dev) i++;
+        per_dev[i].push_back(c);
+    }
+    size_t nr = 0;
+    for (auto &v : per_dev) nr = std::max(nr, v.size());
+    b->rounds.resize(nr);
+    for (size_t r = 0; r < nr; r++) {
+        for (auto &v : per_dev) {
+            if (r >= v.size()) continue;
+            BatchCmd *c = v[r];
+            if (c->is_clear) b->rounds[r].clears.push_back(c->clear);
+            else b->rounds[r].scenes.push_back({c->dev, &c->draw});
+        }
+    }
+}
+
+int batch_upload_clears(Exec &x, BatchRound &R)
+{
+    const uint32_t n = (uint32_t)R.clears.size();
+    if (n == 0) return MTGL_OK;
+    const size_t b_cd = up256(n * sizeof(ClearDesc)), bytes = b_cd + up256((n + 1) * sizeof(uint32_t));
+    int rc;
+    if ((rc = arena_reserve(x, R.clear_params, bytes))) return rc;
+    if ((rc = pinned_reserve(x, bytes))) return rc;
+    memcpy(x.pinned, R.clears.data(), n * sizeof(ClearDesc));
+    uint32_t *pre = reinterpret_cast<uint32_t *>(x.pinned + b_cd);
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        pre[i] = (uint32_t)acc;
+        acc += blocks_for((uint64_t)R.clears[i].w * (uint64_t)R.clears[i].h, 256);
+    }
+    if (acc > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    pre[n] = (uint32_t)acc;
+    R.clear_blocks = (uint32_t)acc;
+    HIPC(hipMemcpyAsync(R.clear_params.ptr, x.pinned, bytes, hipMemcpyHostToDevice, x.stream));
+    HIPC(hipEventRecord(x.upload_done, x.stream));
+    R.clear_dev = (const ClearDesc *)R.clear_params.ptr;
+    R.clear_pre = (const uint32_t *)((const uint8_t *)R.clear_params.ptr + b_cd);
+    return MTGL_OK;
+}
+
+// Every round in order.  The sizing run (frozen false) uploads the tables
+// and sizes the buffers; frozen runs replay exactly those launches.
+int batch_run(mtgl_batch *b, bool frozen)
+{
+    Exec &x = b->ex;
+    for (BatchRound &R : b->rounds) {
+        int rc;
+        if (!R.clears.empty()) {
+            if (!frozen && (rc = batch_upload_clears(x, R))) return rc;
+            if (R.clear_blocks)
+                hipLaunchKernelGGL(k_clear_multi, dim3(R.clear_blocks), dim3(256), 0, x.stream, R.clear_dev,
+                                   R.clear_pre, (uint32_t)R.clears.size());
+            HIPC(hipGetLastError());
+        }
+        if (R.scenes.empty()) continue;
+        rc = frozen ? run_launch(x, R.launch, true)
+                    : execute_scenes(x, R.launch, R.scenes.data(), (uint32_t)R.scenes.size());
+        if (rc && rc != MTGL_ERR_UNSUPPORTED) return rc;
+    }
+    return MTGL_OK;
+}
+
+// Save (restore false) or restore the colour, depth and stencil planes of
+// the batch's contexts to or from mem, on the batch stream.
+void planes_copy(mtgl_batch *b, uint8_t *mem, bool restore)
+{
+    for (mtgl_dev *d : b->devs) {
+        const size_t n = (size_t)d->w * d->h;
+        uint8_t *c = mem, *z = mem + 4 * n, *st = mem + 8 * n;
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        if (!restore) {
+            (void)hipMemcpyAsync(c, d->color, 4 * n, k, b->ex.stream);
+            (void)hipMemcpyAsync(z, d->depth, 4 * n, k, b->ex.stream);
+            (void)hipMemcpyAsync(st, d->stencil, n, k, b->ex.stream);
+        } else {
+            (void)hipMemcpyAsync(d->color, c, 4 * n, k, b->ex.stream);
+            (void)hipMemcpyAsync(d->depth, z, 4 * n, k, b->ex.stream);
+            (void)hipMemcpyAsync(d->stencil, st, n, k, b->ex.stream);
+        }
+        mem += up256(9 * n);
+    }
+}
+
+// The largest round fixes the sizes of the shared buffers up front, so that
+// the sizing run seldom has to grow them between rounds.
+int batch_reserve(mtgl_batch *b)
+{
+    Sizes m;
+    for (BatchRound &R : b->rounds) {
+        const Sizes z = scene_sizes(R.scenes.data(), (uint32_t)R.scenes.size());
+        m.vtx = std::max(m.vtx, z.vtx);
+        m.tri = std::max(m.tri, z.tri);
+        m.cold |= z.cold;
+    }
+    if (m.vtx > 0x7FFFFFFFull || m.tri > 0x0FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    return m.tri ? reserve_scene_buffers(b->ex, m) : MTGL_OK;
+}
+
+int batch_build_inner(mtgl_batch *b, int autotune)
+{
+    Exec &x = b->ex;
+    int rc;
+    if ((rc = batch_reserve(b))) return rc;
+    // sizing run: every round in order, reading its pair total back
+    if ((rc = batch_run(b, false))) return rc;
+    HIPC(hipStreamSynchronize(x.stream));
+    // a later round may have grown a buffer that an earlier round's tables
+    // point into: prepare every round's tables again, at the final sizes
+    for (BatchRound &R : b->rounds) {
+        if (R.scenes.empty()) continue;
+        if ((rc = prepare_launch(x, R.launch, R.scenes.data(), (uint32_t)R.scenes.size()))) return rc;
+    }
+    HIPC(hipStreamSynchronize(x.stream));
+    b->info.heavy_bin = x.heavy_bin;
+    b->info.grid_div = x.grid_div;
+    if (autotune) {
+        // the heavy-bin threshold of the tile list and the raster grid size:
+        // one warm and three timed frozen replays of the batch per candidate
+        static const uint32_t cand_bin[] = {32u, 96u, 256u, 0xFFFFFFFFu, 96u, 96u};
+        static const uint32_t cand_div[] = {1u, 1u, 1u, 1u, 2u, 4u};
+        const int ncand = (int)(sizeof(cand_bin) / sizeof(cand_bin[0]));
+        static_assert(sizeof(cand_bin) / sizeof(cand_bin[0]) <= MTGL_BATCH_TUNE_MAX, "tune table");
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        float best = -1.0f;
+        for (int c = 0; c < ncand && !rc; c++) {
+            x.heavy_bin = cand_bin[c];
+            x.grid_div = cand_div[c];
+            if ((rc = batch_run(b, true))) break;
+            if ((rc = map_err(hipEventRecord(e0, x.stream)))) break;
+            for (int rep = 0; rep < 3 && !rc; rep++) rc = batch_run(b, true);
+            if (rc || (rc = map_err(hipEventRecord(e1, x.stream))) || (rc = map_err(hipEventSynchronize(e1)))) break;
+            float ms = 0.0f;
+            if ((rc = map_err(hipEventElapsedTime(&ms, e0, e1)))) break;
+            b->info.tune_ms[c] = ms / 3.0;
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                b->info.heavy_bin = cand_bin[c];
+                b->info.grid_div = cand_div[c];
+            }
+        }
+        b->info.tune_candidates = (uint32_t)ncand;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        x.heavy_bin = b->info.heavy_bin;
+        x.grid_div = b->info.grid_div;
+    }
+    if (b->rounds.empty()) return MTGL_OK;   // nothing recorded: launches do nothing
+    // capture the replay
+    HIPC(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_run(b, true);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(x.stream, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : map_err(e);
+    }
+    b->graph = g;
+    HIPC(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    return MTGL_OK;
+}
+
+// The sizing run and the tuning replays render into the contexts'
+// framebuffers: their planes are saved first and restored after, so that
+// building a batch leaves every framebuffer as it was.
+int batch_build(mtgl_batch *b, int autotune)
+{
+    batch_plan(b);
+    for (mtgl_dev *d : b->devs) {
+        int rc = dev_join(d);
+        if (rc) return rc;
+        HIPC(hipStreamSynchronize(d->ex.stream));
+        d->dirty = false;
+    }
+    size_t total = 0;
+    for (mtgl_dev *d : b->devs) total += up256((size_t)d->w * d->h * 9);
+    void *snap = nullptr;
+    HIPC(hipMalloc(&snap, std::max<size_t>(total, 256)));
+    planes_copy(b, (uint8_t *)snap, false);
+    const int rc = batch_build_inner(b, autotune);
+    planes_copy(b, (uint8_t *)snap, true);
+    const hipError_t e = hipStreamSynchronize(b->ex.stream);
+    (void)hipFree(snap);
+    if (rc) return rc;
+    if (e != hipSuccess) return map_err(e);
+    // counters count launches only from here on
+    return exec_stats_reset(b->ex);
 }
 
 }  // namespace
@@ -2240,6 +3215,7 @@ MTGL_API const char *mtgl_dev_backend(void)
 MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
 {
     if (!out || width <= 0 || height <= 0 || width > MTGL_MAX_FB_DIM || height > MTGL_MAX_FB_DIM) return MTGL_ERR_INVALID;
+    *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         fprintf(stderr, "libmtgl: no HIP device available\n");
@@ -2247,44 +3223,22 @@ MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
     }
     mtgl_dev *d = new (std::nothrow) mtgl_dev();
     if (!d) return MTGL_ERR_OUT_OF_MEMORY;
-    *out = nullptr;
     int rc = MTGL_OK;
     do {
         if (hipGetDevice(&d->device) != hipSuccess) { rc = MTGL_ERR_DEVICE; break; }
-        if (hipDeviceGetAttribute(&d->cu_count, hipDeviceAttributeMultiprocessorCount, d->device) != hipSuccess ||
-            d->cu_count <= 0)
-            d->cu_count = 256;
-        {
-            // resident one-wave blocks of each raster kernel (registers and LDS bound)
-            const void *kern[3] = {(const void *)k_raster_depth<true>, (const void *)k_raster_color,
-                                   (const void *)k_raster};
-            for (int k = 0; k < 3; k++) {
-                int nb = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
-                d->resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)d->cu_count, STAT_SLOTS);
-            }
-        }
+        if ((rc = exec_init(d->ex, d->device)) != MTGL_OK) break;
         d->w = width;
         d->h = height;
         d->tiles_x = (width + TILE - 1) / TILE;
         d->tiles_y = (height + TILE - 1) / TILE;
-        size_t n = (size_t)width * (size_t)height;
-        if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) { rc = MTGL_ERR_DEVICE; break; }
+        const size_t n = (size_t)width * (size_t)height;
         if ((rc = map_err(hipMalloc((void **)&d->color, n * 4))) != MTGL_OK) break;
         if ((rc = map_err(hipMalloc((void **)&d->depth, n * 4))) != MTGL_OK) break;
         if ((rc = map_err(hipMalloc((void **)&d->stencil, n))) != MTGL_OK) break;
-        if ((rc = map_err(hipMalloc((void **)&d->ctr, sizeof(Counters)))) != MTGL_OK) break;
-        if ((rc = map_err(hipMalloc((void **)&d->stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long)))) != MTGL_OK)
-            break;
-        if ((rc = map_err(hipMemset(d->stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long)))) != MTGL_OK) break;
-        if ((rc = map_err(hipHostMalloc((void **)&d->ctr_host, sizeof(Counters), hipHostMallocDefault))) != MTGL_OK) break;
-        if ((rc = map_err(hipHostMalloc((void **)&d->pairs_host, sizeof(unsigned long long), hipHostMallocDefault))) != MTGL_OK) break;
-        if ((rc = map_err(hipEventCreateWithFlags(&d->upload_done, hipEventDisableTiming))) != MTGL_OK) break;
-        if ((rc = map_err(hipMemsetAsync(d->color, 0, n * 4, d->stream))) != MTGL_OK) break;
-        if ((rc = map_err(hipMemsetAsync(d->depth, 0, n * 4, d->stream))) != MTGL_OK) break;
-        if ((rc = map_err(hipMemsetAsync(d->stencil, 0, n, d->stream))) != MTGL_OK) break;
-        if ((rc = map_err(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream))) != MTGL_OK) break;
-        if ((rc = map_err(hipStreamSynchronize(d->stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->color, 0, n * 4, d->ex.stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->depth, 0, n * 4, d->ex.stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipMemsetAsync(d->stencil, 0, n, d->ex.stream))) != MTGL_OK) break;
+        if ((rc = map_err(hipStreamSynchronize(d->ex.stream))) != MTGL_OK) break;
     } while (0);
     if (rc != MTGL_OK) {
         mtgl_dev_destroy(d);
@@ -2297,35 +3251,37 @@ MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
 MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
 {
     if (!d) return;
-    if (d->stream) (void)hipStreamSynchronize(d->stream);
+    invalidate_batches(d);
+    {
+        // batches forget the context: their graphs no longer launch
+        std::lock_guard<std::mutex> lk(g_batches_mu);
+        for (mtgl_batch *b : g_batches) {
+            auto it = std::find(b->devs.begin(), b->devs.end(), d);
+            if (it != b->devs.end()) *it = nullptr;
+        }
+    }
+    if (d->pending) (void)hipStreamSynchronize(d->pending->ex.stream);
+    if (d->ex.stream) (void)hipStreamSynchronize(d->ex.stream);
     for (int i = 0; i <= MTGL_MAX_OBJECTS; i++) {
         if (d->tex[i].px) (void)hipFree(d->tex[i].px);
         if (d->tex[i].m1) (void)hipFree(d->tex[i].m1);
         if (d->buf[i].data) (void)hipFree(d->buf[i].data);
     }
-    Arena *arenas[] = {&d->params, &d->vhot, &d->vcold, &d->rt, &d->rcold, &d->n_rt, &d->rec_base, &d->tri_tiles, &d->pair_base,
-                       &d->scan_tmp, &d->clip_list, &d->keys[0], &d->keys[1], &d->vals[0], &d->vals[1], &d->sort_tmp, &d->tile_gen,
-                       &d->tile_start, &d->tile_end, &d->tile_list, &d->writer, &d->cov};
-    for (Arena *a : arenas) if (a->ptr) (void)hipFree(a->ptr);
-    for (auto &pr : d->pending_events) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
-    for (hipEvent_t e : d->free_events) (void)hipEventDestroy(e);
+    arena_free(d->writer);
+    arena_free(d->one.params);
+    exec_free(d->ex);
     if (d->color) (void)hipFree(d->color);
     if (d->depth) (void)hipFree(d->depth);
     if (d->stencil) (void)hipFree(d->stencil);
-    if (d->ctr) (void)hipFree(d->ctr);
-    if (d->stat_slots) (void)hipFree(d->stat_slots);
-    if (d->ctr_host) (void)hipHostFree(d->ctr_host);
-    if (d->pairs_host) (void)hipHostFree(d->pairs_host);
-    if (d->pinned) (void)hipHostFree(d->pinned);
-    if (d->upload_done) (void)hipEventDestroy(d->upload_done);
-    if (d->stream) (void)hipStreamDestroy(d->stream);
     delete d;
 }
 
 MTGL_API int mtgl_dev_finish(mtgl_dev *d)
 {
     if (!d) return MTGL_ERR_INVALID;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     return MTGL_OK;
 }
 
@@ -2338,12 +3294,35 @@ MTGL_API int mtgl_dev_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y
     if (x1 > d->w) x1 = d->w;
     if (y1 > d->h) y1 = d->h;
     if (x0 >= x1 || y0 >= y1 || !(buffers & (MTGL_BUF_COLOR | MTGL_BUF_DEPTH | MTGL_BUF_STENCIL))) return MTGL_OK;
-    int32_t w = x1 - x0, h = y1 - y0;
-    uint64_t n = (uint64_t)w * (uint64_t)h;
-    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream,
-                       (buffers & MTGL_BUF_COLOR) ? d->color : nullptr, (buffers & MTGL_BUF_DEPTH) ? d->depth : nullptr,
-                       (buffers & MTGL_BUF_STENCIL) ? d->stencil : nullptr, d->w, x0, y0, w, h, color, depth, stencil);
+    ClearDesc c;
+    c.color = (buffers & MTGL_BUF_COLOR) ? d->color : nullptr;
+    c.depth = (buffers & MTGL_BUF_DEPTH) ? d->depth : nullptr;
+    c.stencil = (buffers & MTGL_BUF_STENCIL) ? d->stencil : nullptr;
+    c.fb_w = d->w;
+    c.x0 = x0;
+    c.y0 = y0;
+    c.w = x1 - x0;
+    c.h = y1 - y0;
+    c.cv = color;
+    c.dv = depth;
+    c.sv = stencil;
+    if (g_recording) {
+        BatchCmd *cmd = new (std::nothrow) BatchCmd();
+        if (!cmd) return MTGL_ERR_OUT_OF_MEMORY;
+        cmd->dev = d;
+        cmd->is_clear = true;
+        cmd->clear = c;
+        g_recording->cmds.push_back(cmd);
+        batch_note_dev(g_recording, d);
+        return MTGL_OK;
+    }
+    int rc = dev_join(d);
+    if (rc) return rc;
+    const uint64_t n = (uint64_t)c.w * (uint64_t)c.h;
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->ex.stream, c.color, c.depth, c.stencil, d->w,
+                       x0, y0, c.w, c.h, color, depth, stencil);
     HIPC(hipGetLastError());
+    d->dirty = true;
     return MTGL_OK;
 }
 
@@ -2362,7 +3341,9 @@ MTGL_API int mtgl_dev_read(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, in
     if (!d || !dst || !plane_of(d, which, &base, &esz)) return MTGL_ERR_INVALID;
     if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
     if (w == 0 || h == 0) return MTGL_OK;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy2D(dst, (size_t)w * esz, base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, (size_t)w * esz,
                      (size_t)h, hipMemcpyDeviceToHost));
     return MTGL_OK;
@@ -2375,7 +3356,9 @@ MTGL_API int mtgl_dev_write(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, i
     if (!d || !src || !plane_of(d, which, &base, &esz)) return MTGL_ERR_INVALID;
     if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
     if (w == 0 || h == 0) return MTGL_OK;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy2D(base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, src, (size_t)w * esz, (size_t)w * esz,
                      (size_t)h, hipMemcpyHostToDevice));
     return MTGL_OK;
@@ -2386,7 +3369,10 @@ MTGL_API int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, 
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS || w <= 0 || h <= 0 || w > MTGL_MAX_TEX_SIZE || h > MTGL_MAX_TEX_SIZE ||
         !texels)
         return MTGL_ERR_INVALID;
-    size_t n = (size_t)w * (size_t)h;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    invalidate_batches(d);
+    const size_t n = (size_t)w * (size_t)h;
     uint32_t *px = nullptr, *m1 = nullptr;
     int32_t mw = 0, mh = 0;
     HIPC(hipMalloc((void **)&px, n * 4));
@@ -2394,27 +3380,38 @@ MTGL_API int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, 
         mw = w / 2;
         mh = h / 2;
         hipError_t e = hipMalloc((void **)&m1, (size_t)mw * mh * 4);
-        if (e != hipSuccess) { (void)hipFree(px); return map_err(e); }
+        if (e != hipSuccess) {
+            (void)hipFree(px);
+            return map_err(e);
+        }
     }
     HIPC(hipMemcpy(px, texels, n * 4, hipMemcpyHostToDevice));
     if (m1) {
-        hipLaunchKernelGGL(k_mip1, dim3(blocks_for((uint64_t)mw * mh, 256)), dim3(256), 0, d->stream, px, w, m1, mw, mh);
+        hipLaunchKernelGGL(k_mip1, dim3(blocks_for((uint64_t)mw * mh, 256)), dim3(256), 0, d->ex.stream, px, w, m1, mw,
+                           mh);
         HIPC(hipGetLastError());
     }
-    HIPC(hipStreamSynchronize(d->stream));
+    HIPC(hipStreamSynchronize(d->ex.stream));
     DevTex &t = d->tex[id];
     if (t.px) HIPC(hipFree(t.px));
     if (t.m1) HIPC(hipFree(t.m1));
-    t.px = px; t.w = w; t.h = h;
-    t.m1 = m1; t.m1w = mw; t.m1h = mh;
+    t.px = px;
+    t.w = w;
+    t.h = h;
+    t.m1 = m1;
+    t.m1w = mw;
+    t.m1h = mh;
     return MTGL_OK;
 }
 
 MTGL_API int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
 {
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     DevTex &t = d->tex[id];
+    if (t.px || t.m1) invalidate_batches(d);
     if (t.px) HIPC(hipFree(t.px));
     if (t.m1) HIPC(hipFree(t.m1));
     t = DevTex();
@@ -2424,8 +3421,11 @@ MTGL_API int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
 MTGL_API int mtgl_dev_buffer_data(mtgl_dev *d, uint32_t id, int64_t size, const void *data)
 {
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     DevBuf &b = d->buf[id];
+    if (b.data) invalidate_batches(d);
     if (size <= 0) {
         if (b.data) HIPC(hipFree(b.data));
         b = DevBuf();
@@ -2434,7 +3434,10 @@ MTGL_API int mtgl_dev_buffer_data(mtgl_dev *d, uint32_t id, int64_t size, const 
     uint8_t *p = nullptr;
     HIPC(hipMalloc((void **)&p, (size_t)size));
     hipError_t e = data ? hipMemcpy(p, data, (size_t)size, hipMemcpyHostToDevice) : hipMemset(p, 0, (size_t)size);
-    if (e != hipSuccess) { (void)hipFree(p); return map_err(e); }
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return map_err(e);
+    }
     if (b.data) HIPC(hipFree(b.data));
     b.data = p;
     b.size = size;
@@ -2446,7 +3449,9 @@ MTGL_API int mtgl_dev_buffer_sub_data(mtgl_dev *d, uint32_t id, int64_t offset, 
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS || !data || offset < 0 || size < 0) return MTGL_ERR_INVALID;
     DevBuf &b = d->buf[id];
     if (!b.data || offset + size > b.size) return MTGL_ERR_INVALID;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy(b.data + offset, data, (size_t)size, hipMemcpyHostToDevice));
     return MTGL_OK;
 }
@@ -2454,8 +3459,11 @@ MTGL_API int mtgl_dev_buffer_sub_data(mtgl_dev *d, uint32_t id, int64_t offset, 
 MTGL_API int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
 {
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
-    HIPC(hipStreamSynchronize(d->stream));
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
     DevBuf &b = d->buf[id];
+    if (b.data) invalidate_batches(d);
     if (b.data) HIPC(hipFree(b.data));
     b = DevBuf();
     return MTGL_OK;
@@ -2464,279 +3472,190 @@ MTGL_API int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
 MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
 {
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
-    switch (dr->mode) {
-    case GL_TRIANGLES: case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_QUADS: case GL_QUAD_STRIP: case GL_POLYGON:
-        break;
-    default:
-        return MTGL_ERR_UNSUPPORTED;
+    if (!draw_mode_ok(dr->mode)) return MTGL_ERR_UNSUPPORTED;
+    const uint32_t ntri = tri_count(dr->mode, dr->count);
+    if (g_recording) {
+        BatchCmd *cmd = new (std::nothrow) BatchCmd();
+        if (!cmd) return MTGL_ERR_OUT_OF_MEMORY;
+        cmd->dev = d;
+        cmd->draw = *dr;
+        cmd->raster = *dr->raster;
+        cmd->vstates.assign(dr->vstates, dr->vstates + dr->n_vstates);
+        if (dr->staged_bytes) {
+            const uint8_t *st = (const uint8_t *)dr->staged;
+            cmd->staged.assign(st, st + dr->staged_bytes);
+        }
+        cmd->draw.raster = &cmd->raster;
+        cmd->draw.vstates = cmd->vstates.data();
+        cmd->draw.staged = cmd->staged.empty() ? nullptr : cmd->staged.data();
+        g_recording->cmds.push_back(cmd);
+        batch_note_dev(g_recording, d);
+        return MTGL_OK;
     }
-    d->stats.draws++;
-    const uint32_t count = dr->count;
-    const uint32_t ntri = tri_count(dr->mode, count);
+    d->ex.stats.draws++;
     if (ntri == 0) return MTGL_OK;
-    d->stats.triangles_in += ntri;
-    const mtgl_raster_state *rsh = dr->raster;
-    const bool need_cold = (rsh->flags & MTGL_F_LIGHTING) && (rsh->shade_model == GL_PHONG || rsh->lighting.two_side);
-
-    // ---- upload raster state, vertex states and staged bytes in one copy
-    const size_t rs_bytes = (sizeof(mtgl_raster_state) + 255) & ~(size_t)255;
-    const size_t vs_bytes = ((size_t)dr->n_vstates * sizeof(mtgl_vertex_state) + 255) & ~(size_t)255;
-    const size_t st_bytes = (size_t)dr->staged_bytes;
-    const size_t total = rs_bytes + vs_bytes + ((st_bytes + 255) & ~(size_t)255);
-    int rc;
-    if ((rc = pinned_reserve(d, total))) return rc;
-    if ((rc = arena_reserve(d, d->params, total))) return rc;
-    memcpy(d->pinned, rsh, sizeof(mtgl_raster_state));
-    memcpy(d->pinned + rs_bytes, dr->vstates, (size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
-    if (st_bytes) memcpy(d->pinned + rs_bytes + vs_bytes, dr->staged, st_bytes);
-    HIPC(hipMemcpyAsync(d->params.ptr, d->pinned, total, hipMemcpyHostToDevice, d->stream));
-    HIPC(hipEventRecord(d->upload_done, d->stream));
-    const uint8_t *base = (const uint8_t *)d->params.ptr;
-    const mtgl_raster_state *rs_dev = (const mtgl_raster_state *)base;
-    const mtgl_vertex_state *vs_dev = (const mtgl_vertex_state *)(base + rs_bytes);
-    const uint8_t *staged_dev = base + rs_bytes + vs_bytes;
-
-    // ---- buffers
-    if ((rc = arena_reserve(d, d->vhot, (size_t)count * sizeof(VHot)))) return rc;
-    if (need_cold && (rc = arena_reserve(d, d->vcold, (size_t)count * sizeof(VCold)))) return rc;
-    // record pool: one slot per triangle plus an overflow pool for clipped ones
-    if (d->overflow_cap < 4096u) d->overflow_cap = 4096u;
-    uint32_t want_overflow = std::max<uint32_t>(d->overflow_cap, std::min<uint32_t>(ntri / 8u, 1u << 24));
-    d->overflow_cap = want_overflow;
-    if ((rc = arena_reserve(d, d->rt, ((size_t)ntri + d->overflow_cap) * sizeof(RTri)))) return rc;
-    if (need_cold && (rc = arena_reserve(d, d->rcold, ((size_t)ntri + d->overflow_cap) * sizeof(RCold)))) return rc;
-    if ((rc = arena_reserve(d, d->n_rt, (size_t)ntri * sizeof(uint32_t)))) return rc;
-    if ((rc = arena_reserve(d, d->rec_base, (size_t)ntri * sizeof(uint32_t)))) return rc;
-    if ((rc = arena_reserve(d, d->tri_tiles, (size_t)(ntri + 1) * sizeof(unsigned long long)))) return rc;
-    if ((rc = arena_reserve(d, d->pair_base, (size_t)(ntri + 1) * sizeof(unsigned long long)))) return rc;
-    if ((rc = arena_reserve(d, d->clip_list, (size_t)ntri * sizeof(uint32_t)))) return rc;
-
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, d->ctr);
-
-    // ---- K1
-    VertexParams vp = {};
-    vp.count = count;
-    vp.first = dr->first;
-    vp.index_type = dr->index_type;
-    vp.n_vstates = dr->n_vstates;
-    vp.indices = resolve(d, dr->indices, staged_dev, st_bytes);
-    vp.indices.enabled = dr->index_type != 0;
-    vp.position = resolve(d, dr->position, staged_dev, st_bytes);
-    vp.color = resolve(d, dr->color, staged_dev, st_bytes);
-    vp.texcoord = resolve(d, dr->texcoord, staged_dev, st_bytes);
-    vp.normal = resolve(d, dr->normal, staged_dev, st_bytes);
-    vp.state_index = resolve(d, dr->state_index, staged_dev, st_bytes);
-    memcpy(vp.cur_color, dr->cur_color, sizeof(vp.cur_color));
-    memcpy(vp.cur_texcoord, dr->cur_texcoord, sizeof(vp.cur_texcoord));
-    memcpy(vp.cur_normal, dr->cur_normal, sizeof(vp.cur_normal));
-    vp.vstates = vs_dev;
-    vp.out_hot = as<VHot>(d->vhot);
-    vp.out_cold = need_cold ? as<VCold>(d->vcold) : nullptr;
-    hipLaunchKernelGGL(k_vertex, dim3(blocks_for(count, 256)), dim3(256), 0, d->stream, vp);
-
-    // ---- K2
-    const DevTex *tx = (rsh->texture && rsh->texture <= MTGL_MAX_OBJECTS && d->tex[rsh->texture].px) ? &d->tex[rsh->texture]
-                                                                                                    : nullptr;
-    SetupParams sp = {};
-    sp.mode = dr->mode;
-    sp.count = count;
-    sp.ntri = ntri;
-    sp.fb_w = d->w;
-    sp.fb_h = d->h;
-    sp.tiles_x = d->tiles_x;
-    sp.tex_w = tx ? tx->w : 0;
-    sp.tex_h = tx ? tx->h : 0;
-    sp.rs = rs_dev;
-    sp.vhot = as<VHot>(d->vhot);
-    sp.vcold = need_cold ? as<VCold>(d->vcold) : nullptr;
-    sp.rt = as<RTri>(d->rt);
-    sp.rcold = need_cold ? as<RCold>(d->rcold) : nullptr;
-    sp.n_rt = as<uint32_t>(d->n_rt);
-    sp.rec_base = as<uint32_t>(d->rec_base);
-    sp.tri_tiles = as<unsigned long long>(d->tri_tiles);
-    sp.clip_list = as<uint32_t>(d->clip_list);
-    sp.overflow_cap = d->overflow_cap;
-    sp.ctr = d->ctr;
-    hipLaunchKernelGGL(k_setup, dim3(blocks_for(ntri, 256)), dim3(256), 0, d->stream, sp);
-    hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ntri, 64)), dim3(64), 0, d->stream, sp);
-    HIPC(hipGetLastError());
-
-    // ---- K3: pair offsets, total, then expand / sort / segment
-    unsigned long long *pair_base = as<unsigned long long>(d->pair_base);
-    if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, (uint64_t)ntri + 1))) return rc;
-    HIPC(hipMemcpyAsync(d->pairs_host, pair_base + ntri, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
-    HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
-    HIPC(hipStreamSynchronize(d->stream));
-    if (d->ctr_host->overflow_full) {
-        // the clipped triangles asked for more slots than the pool held: grow it
-        // to the total they requested and set them up again (nothing downstream ran)
-        uint32_t need = d->ctr_host->noverflow;
-        d->overflow_cap = need;
-        sp.overflow_cap = need;
-        if ((rc = arena_reserve(d, d->rt, ((size_t)ntri + need) * sizeof(RTri)))) return rc;
-        if (need_cold && (rc = arena_reserve(d, d->rcold, ((size_t)ntri + need) * sizeof(RCold)))) return rc;
-        sp.rt = as<RTri>(d->rt);
-        sp.rcold = need_cold ? as<RCold>(d->rcold) : nullptr;
-        // the fast path wrote its records into the old pool when it was reallocated
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, d->stream, d->ctr);
-        hipLaunchKernelGGL(k_setup, dim3(blocks_for(ntri, 256)), dim3(256), 0, d->stream, sp);
-        hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ntri, 64)), dim3(64), 0, d->stream, sp);
-        HIPC(hipGetLastError());
-        if ((rc = device_scan<unsigned long long>(d, sp.tri_tiles, pair_base, (uint64_t)ntri + 1))) return rc;
-        HIPC(hipMemcpyAsync(d->pairs_host, pair_base + ntri, sizeof(unsigned long long), hipMemcpyDeviceToHost, d->stream));
-        HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
-        HIPC(hipStreamSynchronize(d->stream));
-        if (d->ctr_host->overflow_full) return MTGL_ERR_DEVICE;
-    }
-    const uint64_t npairs = *d->pairs_host;
-    const int unsupported = d->ctr_host->unsupported != 0;
-    d->stats.triangles_raster += d->ctr_host->tri_raster;
-    if (npairs == 0) return unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
-    if (npairs > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
-    d->stats.bin_pairs += npairs;
-    const uint32_t n = (uint32_t)npairs;
-    for (int k = 0; k < 2; k++) {
-        if ((rc = arena_reserve(d, d->keys[k], (size_t)n * 4))) return rc;
-        if ((rc = arena_reserve(d, d->vals[k], (size_t)n * 4))) return rc;
-    }
-    const uint32_t ntiles = (uint32_t)(d->tiles_x * d->tiles_y);
-    if ((rc = arena_reserve(d, d->tile_start, (size_t)ntiles * 4))) return rc;
-    if ((rc = arena_reserve(d, d->tile_end, (size_t)ntiles * 4))) return rc;
-    if ((rc = arena_reserve(d, d->tile_list, (size_t)ntiles * 4))) return rc;
-    if (d->tile_gen.cap < (size_t)ntiles * 4 || ++d->gen == 0) {
-        if ((rc = arena_reserve(d, d->tile_gen, (size_t)ntiles * 4))) return rc;
-        HIPC(hipMemsetAsync(d->tile_gen.ptr, 0, d->tile_gen.cap, d->stream));
-        d->gen = 1;
-    }
-    uint32_t *keys = as<uint32_t>(d->keys[0]), *vals = as<uint32_t>(d->vals[0]);
-    uint32_t *keys2 = as<uint32_t>(d->keys[1]), *vals2 = as<uint32_t>(d->vals[1]);
-    hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const unsigned long long *)pair_base,
-                       ntri, (const uint32_t *)sp.n_rt, (const uint32_t *)sp.rec_base, (const RTri *)sp.rt, (uint64_t)n,
-                       d->tiles_x, keys, vals);
-    // stable LSD radix sort by tile: pairs keep submission order within a tile
-    int key_bits = 1;
-    while ((1u << key_bits) < ntiles) key_bits++;
-    const uint32_t nblocks = (n + SORT_TILE - 1) / SORT_TILE;
-    if ((rc = arena_reserve(d, d->sort_tmp, (size_t)RADIX * nblocks * 4))) return rc;
-    uint32_t *counts = as<uint32_t>(d->sort_tmp);
-    for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, d->stream, (const uint32_t *)keys, n, shift,
-                           nblocks, counts);
-        if ((rc = device_scan<uint32_t>(d, counts, counts, (uint64_t)RADIX * nblocks))) return rc;
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, d->stream, (const uint32_t *)keys,
-                           (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
-        uint32_t *tk = keys; keys = keys2; keys2 = tk;
-        uint32_t *tv = vals; vals = vals2; vals2 = tv;
-    }
-    if ((rc = arena_reserve(d, d->cov, (size_t)n * RT_COV_VEC4 * sizeof(float4)))) return rc;
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, d->stream, (const uint32_t *)keys, n, ntiles,
-                       as<uint32_t>(d->tile_start), as<uint32_t>(d->tile_end), as<uint32_t>(d->tile_gen), d->gen);
-    hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, d->stream,
-                       (const uint32_t *)as<uint32_t>(d->tile_start), (const uint32_t *)as<uint32_t>(d->tile_end),
-                       (const uint32_t *)as<uint32_t>(d->tile_gen), d->gen, ntiles, as<uint32_t>(d->tile_list), d->ctr);
-    hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, d->stream,
-                       (const uint32_t *)vals, (const RTri *)sp.rt, n, as<float4>(d->cov));
-    HIPC(hipGetLastError());
-
-    // ---- K4
-    RasterParams rp = {};
-    rp.rs = rs_dev;
-    rp.rt = sp.rt;
-    rp.rcold = sp.rcold;
-    rp.vals = vals;
-    rp.list = as<uint32_t>(d->tile_list);
-    rp.ntiles = ntiles;
-    rp.cov = as<float4>(d->cov);
-    rp.tile_start = as<uint32_t>(d->tile_start);
-    rp.tile_end = as<uint32_t>(d->tile_end);
-    rp.color = d->color;
-    rp.depth = d->depth;
-    rp.stencil = d->stencil;
-    // without blending, a partial colour mask or an alpha test on a texture,
-    // only each pixel's last passing fragment leaves its colour: depth kernel
-    // then one colour per pixel
-    const uint32_t cm = rsh->color_mask & 0xFu;
-    const bool last_writer = !(rsh->flags & MTGL_F_BLEND) && (cm == 0xFu || cm == 0u) &&
-                             !((rsh->flags & MTGL_F_ALPHA_TEST) && tx);
-    if (last_writer) {
-        if ((rc = arena_reserve(d, d->writer, (size_t)d->w * (size_t)d->h * 4))) return rc;
-        rp.writer = as<uint32_t>(d->writer);
-    }
-    rp.fb_w = d->w;
-    rp.fb_h = d->h;
-    rp.tiles_x = d->tiles_x;
-    rp.tex = tx ? tx->px : nullptr;
-    rp.tex1 = tx ? tx->m1 : nullptr;
-    rp.tex_w = tx ? tx->w : 0;
-    rp.tex_h = tx ? tx->h : 0;
-    rp.tex1_w = tx ? tx->m1w : 0;
-    rp.tex1_h = tx ? tx->m1h : 0;
-    rp.ctr = d->ctr;
-    rp.stats = d->stat_slots;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (d->timing) {
-        ev0 = take_event(d);
-        ev1 = take_event(d);
-        if (ev0) HIPC(hipEventRecord(ev0, d->stream));
-    }
-    // persistent grids over the tile list: at most one block per non-empty
-    // tile, at most as many as are resident at once
-    // persistent grids over the tile list (longest bins first): at most one
-    // block per non-empty tile, at most as many as are resident at once
-    const uint32_t nlist = std::min<uint32_t>(n, ntiles);
-    if (last_writer) {
-        const dim3 g(std::min<uint32_t>(nlist, d->resident[0]));
-        if (rsh->flags & MTGL_F_STENCIL_TEST) hipLaunchKernelGGL(k_raster_depth<true>, g, dim3(64), 0, d->stream, rp);
-        else hipLaunchKernelGGL(k_raster_depth<false>, g, dim3(64), 0, d->stream, rp);
-        if (cm)
-            hipLaunchKernelGGL(k_raster_color, dim3(std::min<uint32_t>(nlist, d->resident[1])), dim3(64), 0, d->stream,
-                               rp);
-    } else {
-        hipLaunchKernelGGL(k_raster, dim3(std::min<uint32_t>(nlist, d->resident[2])), dim3(64), 0, d->stream, rp);
-    }
-    HIPC(hipGetLastError());
-    if (ev0 && ev1) {
-        HIPC(hipEventRecord(ev1, d->stream));
-        d->pending_events.emplace_back(ev0, ev1);
-    }
-    return unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
+    d->ex.stats.triangles_in += ntri;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    if (d->ex.pending_events.size() > 4096) (void)resolve_timing(d->ex);
+    SceneIn in = {d, dr};
+    d->one.pairs_cap = 0;
+    rc = execute_scenes(d->ex, d->one, &in, 1);
+    d->dirty = true;
+    return rc;
 }
 
 MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int enable)
 {
     if (!d) return MTGL_ERR_INVALID;
-    d->timing = enable != 0;
+    d->ex.timing = enable != 0;
     return MTGL_OK;
 }
 
 MTGL_API int mtgl_dev_stats_get(mtgl_dev *d, mtgl_dev_stats *out)
 {
     if (!d || !out) return MTGL_ERR_INVALID;
-    d->stat_host.resize(2 * STAT_SLOTS);
-    HIPC(hipMemcpyAsync(d->stat_host.data(), d->stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long),
-                        hipMemcpyDeviceToHost, d->stream));
-    HIPC(hipStreamSynchronize(d->stream));
-    int rc = resolve_timing(d);
-    if (rc) return rc;
-    *out = d->stats;
-    out->fragments = 0;
-    out->frag_bytes = 0;
-    for (uint32_t i = 0; i < STAT_SLOTS; i++) {
-        out->fragments += d->stat_host[2 * i];
-        out->frag_bytes += d->stat_host[2 * i + 1];
-    }
-    return MTGL_OK;
+    return exec_stats(d->ex, out);
 }
 
 MTGL_API int mtgl_dev_stats_reset(mtgl_dev *d)
 {
     if (!d) return MTGL_ERR_INVALID;
-    HIPC(hipStreamSynchronize(d->stream));
-    int rc = resolve_timing(d);
-    if (rc) return rc;
-    d->stats = mtgl_dev_stats();
-    HIPC(hipMemsetAsync(d->ctr, 0, sizeof(Counters), d->stream));
-    HIPC(hipMemsetAsync(d->stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long), d->stream));
-    HIPC(hipStreamSynchronize(d->stream));
+    return exec_stats_reset(d->ex);
+}
+
+// ---- batches
+
+MTGL_API int mtgl_batch_create(mtgl_batch **out)
+{
+    if (!out) return MTGL_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MTGL_ERR_DEVICE;
+    mtgl_batch *b = new (std::nothrow) mtgl_batch();
+    if (!b) return MTGL_ERR_OUT_OF_MEMORY;
+    int rc = MTGL_OK;
+    if (hipGetDevice(&b->device) != hipSuccess) rc = MTGL_ERR_DEVICE;
+    if (!rc) rc = exec_init(b->ex, b->device);
+    if (!rc) rc = map_err(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    if (!rc) rc = map_err(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
+    if (rc) {
+        mtgl_batch_destroy(b);
+        return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_batches_mu);
+        g_batches.push_back(b);
+    }
+    *out = b;
     return MTGL_OK;
+}
+
+MTGL_API void mtgl_batch_destroy(mtgl_batch *b)
+{
+    if (!b) return;
+    if (g_recording == b) g_recording = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_batches_mu);
+        g_batches.erase(std::remove(g_batches.begin(), g_batches.end(), b), g_batches.end());
+    }
+    if (b->ex.stream) (void)hipStreamSynchronize(b->ex.stream);
+    for (mtgl_dev *d : b->devs)
+        if (d && d->pending == b) d->pending = nullptr;   // the launch has finished
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (BatchRound &R : b->rounds) {
+        arena_free(R.launch.params);
+        arena_free(R.clear_params);
+    }
+    for (BatchCmd *c : b->cmds) delete c;
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->join) (void)hipEventDestroy(b->join);
+    exec_free(b->ex);
+    delete b;
+}
+
+MTGL_API int mtgl_batch_begin(mtgl_batch *b)
+{
+    if (!b || g_recording || b->recorded || !b->cmds.empty()) return MTGL_ERR_INVALID;
+    g_recording = b;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_batch_end(mtgl_batch *b, int autotune)
+{
+    if (!b || g_recording != b) return MTGL_ERR_INVALID;
+    g_recording = nullptr;
+    b->recorded = true;
+    b->info.contexts = (uint32_t)b->devs.size();
+    b->info.commands = (uint32_t)b->cmds.size();
+    for (mtgl_dev *d : b->devs)
+        if (!d) return MTGL_ERR_INVALID;   // a context was destroyed while recording
+    int rc = batch_build(b, autotune);
+    b->info.rounds = (uint32_t)b->rounds.size();
+    uint64_t scenes = 0;
+    for (BatchRound &R : b->rounds) scenes += R.launch.ns;
+    b->info.scenes = (uint32_t)scenes;
+    if (rc) return rc;
+    b->ready = true;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_batch_launch(mtgl_batch *b)
+{
+    if (!b || !b->ready) return MTGL_ERR_INVALID;
+    Exec &x = b->ex;
+    // after the contexts' queued work ...
+    for (mtgl_dev *d : b->devs) {
+        if (d->pending && d->pending != b) {
+            HIPC(hipStreamWaitEvent(x.stream, d->pending->done, 0));
+        }
+        if (d->dirty) {
+            HIPC(hipEventRecord(b->join, d->ex.stream));
+            HIPC(hipStreamWaitEvent(x.stream, b->join, 0));
+            d->dirty = false;
+        }
+    }
+    if (b->graph_exec) HIPC(hipGraphLaunch(b->graph_exec, x.stream));
+    HIPC(hipEventRecord(b->done, x.stream));
+    // ... and before their next work (joined lazily)
+    for (mtgl_dev *d : b->devs) d->pending = b;
+    b->info.launches++;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_batch_sync(mtgl_batch *b)
+{
+    if (!b) return MTGL_ERR_INVALID;
+    HIPC(hipStreamSynchronize(b->ex.stream));
+    Counters *c = b->ex.ctr_host;
+    HIPC(hipMemcpy(c, b->ex.ctr, sizeof(Counters), hipMemcpyDeviceToHost));
+    const unsigned int over = c->sticky | c->overflow_full;
+    if (over) {
+        // a replay produced more clipped triangles or pairs than the sizing
+        // run (buffer objects changed since): its output is incomplete
+        HIPC(hipMemsetAsync(b->ex.ctr, 0, sizeof(Counters), b->ex.stream));
+        HIPC(hipStreamSynchronize(b->ex.stream));
+        return MTGL_ERR_OUT_OF_MEMORY;
+    }
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_batch_info(mtgl_batch *b, mtgl_batch_info_t *out)
+{
+    if (!b || !out) return MTGL_ERR_INVALID;
+    *out = b->info;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_batch_stats_get(mtgl_batch *b, mtgl_dev_stats *out)
+{
+    if (!b || !out) return MTGL_ERR_INVALID;
+    return exec_stats(b->ex, out);
+}
+
+MTGL_API int mtgl_batch_stats_reset(mtgl_batch *b)
+{
+    if (!b) return MTGL_ERR_INVALID;
+    return exec_stats_reset(b->ex);
 }

@@ -97,6 +97,63 @@ class MtglDevStats(C.Structure):
     ]
 
 
+class MtglBatchInfo(C.Structure):
+    """mtgl_batch_info_t of include/mtgl_batch.h."""
+    _fields_ = [
+        ("contexts", C.c_uint32), ("commands", C.c_uint32), ("rounds", C.c_uint32), ("scenes", C.c_uint32),
+        ("launches", C.c_uint32), ("heavy_bin", C.c_uint32), ("grid_div", C.c_uint32),
+        ("tune_candidates", C.c_uint32), ("tune_ms", C.c_double * 8),
+    ]
+
+
+class Batch:
+    """An mtgl_batch of include/mtgl_batch.h: record() returns a context manager
+    under which this thread's clears and draws are recorded."""
+
+    def __init__(self, gl: "GLLibrary"):
+        self.gl = gl
+        lib = gl.lib
+        h = C.c_void_p()
+        rc = lib.mtgl_batch_create(C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"{gl.kind}: mtgl_batch_create failed ({rc})")
+        self.h = h.value
+
+    def begin(self) -> None:
+        self._check(self.gl.lib.mtgl_batch_begin(self.h), "begin")
+
+    def end(self, autotune: bool = True) -> None:
+        self._check(self.gl.lib.mtgl_batch_end(self.h, 1 if autotune else 0), "end")
+
+    def launch(self) -> None:
+        self._check(self.gl.lib.mtgl_batch_launch(self.h), "launch")
+
+    def sync(self) -> int:
+        return self.gl.lib.mtgl_batch_sync(self.h)
+
+    def info(self) -> MtglBatchInfo:
+        i = MtglBatchInfo()
+        self._check(self.gl.lib.mtgl_batch_info(self.h, C.byref(i)), "info")
+        return i
+
+    def stats(self) -> MtglDevStats:
+        s = MtglDevStats()
+        self._check(self.gl.lib.mtgl_batch_stats_get(self.h, C.byref(s)), "stats")
+        return s
+
+    def stats_reset(self) -> None:
+        self._check(self.gl.lib.mtgl_batch_stats_reset(self.h), "stats_reset")
+
+    def destroy(self) -> None:
+        if self.h:
+            self.gl.lib.mtgl_batch_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise RuntimeError(f"{self.gl.kind}: mtgl_batch_{what} failed ({rc})")
+
+
 def _ptr(a) -> Optional[int]:
     """Pointer argument: None, an int address, bytes, a ctypes object or a numpy array."""
     if a is None:
@@ -138,6 +195,17 @@ class GLLibrary:
             self.lib.mtgl_dev_set_timing.argtypes = [C.c_void_p, C.c_int]
             self.lib.mtgl_dev_finish.argtypes = [C.c_void_p]
             self.lib.mtgl_dev_backend.restype = C.c_char_p
+            vp = C.c_void_p
+            for name, args in (("mtgl_batch_create", [C.POINTER(vp)]), ("mtgl_batch_begin", [vp]),
+                               ("mtgl_batch_end", [vp, C.c_int]), ("mtgl_batch_launch", [vp]),
+                               ("mtgl_batch_sync", [vp]), ("mtgl_batch_info", [vp, C.POINTER(MtglBatchInfo)]),
+                               ("mtgl_batch_stats_get", [vp, C.POINTER(MtglDevStats)]),
+                               ("mtgl_batch_stats_reset", [vp])):
+                fn = getattr(self.lib, name)
+                fn.argtypes = args
+                fn.restype = C.c_int
+            self.lib.mtgl_batch_destroy.argtypes = [vp]
+            self.lib.mtgl_batch_destroy.restype = None
         self._read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         self._read.restype = C.c_int
 
@@ -188,6 +256,9 @@ class GLLibrary:
 
     def finish(self, ctx: int) -> None:
         self.lib.mtgl_dev_finish(self.device(ctx))
+
+    def batch(self) -> Batch:
+        return Batch(self)
 
     def backend(self) -> str:
         if self.kind == "reference":

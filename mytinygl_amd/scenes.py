@@ -17,6 +17,10 @@ use:
 * ``overdraw`` - 200k triangles with log-uniform areas of 1..1e4 px, blended
   SRC_ALPHA / ONE_MINUS_SRC_ALPHA, depth LESS, stencil GREATER 8 / INCR,
   back-face culling, 3840x2160.
+* ``batch`` - 512 independent scenes, one context each: square framebuffers
+  with log-uniform sides 64..1024, grid spheres of log-uniform 100..50k
+  triangles, every other one textured (power-of-two sides 64..512,
+  GL_LINEAR_MIPMAP_LINEAR), one clear and one glDrawElements per frame.
 
 The remaining scenes are small feature scenes that together reach every branch
 of the per-vertex, clipping and per-fragment paths (reference src/raster.c,
@@ -752,6 +756,164 @@ def scene_textured_sphere_small(g, W, H, seed):
 
 def scene_overdraw_small(g, W, H, seed):
     draw_overdraw(g, setup_overdraw(g, W, H, seed, ntri=3000, max_log_area=3.0))
+
+
+# ---------------------------------------------------------------------------
+# batched workload: many small independent scenes, one context each
+# ---------------------------------------------------------------------------
+
+def batch_specs(n: int = 512, seed: int = SEED_BASE + 5, fb=(64, 1024), tris=(100, 50_000), tex=(64, 512)):
+    """n scene descriptions: square framebuffer side log-uniform in fb; a
+    lat-long grid sphere of 2 g^2 triangles with 2 g^2 log-uniform in tris;
+    every other scene textured with a power-of-two texture whose side is
+    log-uniform in tex; random yaw and distance."""
+    rng = _rng(seed)
+    lt = int(round(math.log2(tex[0]))), int(round(math.log2(tex[1])))
+    out = []
+    for i in range(n):
+        side = int(round(math.exp(rng.uniform(math.log(fb[0]), math.log(fb[1])))))
+        t = math.exp(rng.uniform(math.log(tris[0]), math.log(tris[1])))
+        grid = max(4, int(round(math.sqrt(t / 2.0))))
+        out.append({
+            "w": side, "h": side, "grid": grid, "triangles": 2 * grid * grid,
+            "textured": i % 2 == 0, "tex": 2 ** int(rng.integers(lt[0], lt[1] + 1)),
+            "seed": int(seed * 1000 + i), "yaw": float(rng.uniform(0.0, 360.0)),
+            "dist": float(rng.uniform(2.2, 3.5)), "clear": [float(c) for c in rng.random(3)],
+        })
+    return out
+
+
+_TEX_CACHE = {}
+
+
+def _batch_texture(size: int) -> np.ndarray:
+    if size not in _TEX_CACHE:
+        _TEX_CACHE[size] = np.ascontiguousarray(value_noise_rgba(size, SEED_BASE + 6 + size))
+    return _TEX_CACHE[size]
+
+
+def setup_batch_scene(g, spec):
+    """State, texture and buffer objects of one batched scene on the current
+    context; returns the index count of its draw."""
+    W, H = spec["w"], spec["h"]
+    verts, idx = sphere_grid(spec["grid"], spec["seed"])
+    g.glViewport(0, 0, W, H)
+    g.glClearColor(spec["clear"][0], spec["clear"][1], spec["clear"][2], 1.0)
+    _perspective(g, W, H, dist=-spec["dist"])
+    g.glRotatef(spec["yaw"], 0.0, 1.0, 0.0)
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.fv(1, 1, 1, 0))
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glHint(E["GL_PERSPECTIVE_CORRECTION_HINT"], E["GL_NICEST"])
+    if spec["textured"]:
+        tex = _batch_texture(spec["tex"])
+        ids = np.zeros(1, np.uint32)
+        g.glGenTextures(1, ids)
+        g.glBindTexture(E["GL_TEXTURE_2D"], int(ids[0]))
+        g.glTexImage2D(E["GL_TEXTURE_2D"], 0, E["GL_RGBA"], spec["tex"], spec["tex"], 0, E["GL_RGBA"],
+                       E["GL_UNSIGNED_BYTE"], g.keep(tex))
+        g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E["GL_LINEAR_MIPMAP_LINEAR"])
+        g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"], E["GL_LINEAR"])
+        g.glEnable(E["GL_TEXTURE_2D"])
+    else:
+        g.glEnable(E["GL_COLOR_MATERIAL"])
+        g.glColor4f(0.8, 0.6, 0.3, 1.0)
+    bufs = np.zeros(2, np.uint32)
+    g.glGenBuffers(2, bufs)
+    g.glBindBuffer(E["GL_ARRAY_BUFFER"], int(bufs[0]))
+    g.glBufferData(E["GL_ARRAY_BUFFER"], verts.nbytes, g.keep(verts), E["GL_STATIC_DRAW"])
+    g.glBindBuffer(E["GL_ELEMENT_ARRAY_BUFFER"], int(bufs[1]))
+    g.glBufferData(E["GL_ELEMENT_ARRAY_BUFFER"], idx.nbytes, g.keep(idx), E["GL_STATIC_DRAW"])
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_NORMAL_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 32, 0)
+    g.glNormalPointer(E["GL_FLOAT"], 32, 12)
+    if spec["textured"]:
+        g.glEnableClientState(E["GL_TEXTURE_COORD_ARRAY"])
+        g.glTexCoordPointer(2, E["GL_FLOAT"], 32, 24)
+    return len(idx)
+
+
+def frame_batch_scene(g, n):
+    """One frame of a batched scene: clear colour and depth, draw the sphere."""
+    g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"])
+    g.glDrawElements(E["GL_TRIANGLES"], n, E["GL_UNSIGNED_INT"], 0)
+
+
+class BatchRun:
+    """Contexts of a batched workload on one GLLibrary.  items are
+    (W, H, prepare, frame): prepare(g) runs at once on the new context and
+    returns what frame(g, state) needs; frame issues one frame's GL calls."""
+
+    def __init__(self, g, items):
+        self.g = g
+        self.items = items
+        self.ctxs, self.states = [], []
+        for W, H, prepare, _ in items:
+            ctx = g.create_context(W, H)
+            self.ctxs.append(ctx)
+            self.states.append(prepare(g) if prepare else None)
+        self.batch = None
+
+    def frames_immediate(self):
+        """One frame of every context through the ordinary per-draw path."""
+        for ctx, (_, _, _, frame), st in zip(self.ctxs, self.items, self.states):
+            self.g.gl_make_current(ctx)
+            frame(self.g, st)
+
+    def record(self, autotune=True):
+        """Record one frame of every context into a new batch and build it."""
+        b = self.g.batch()
+        b.begin()
+        try:
+            self.frames_immediate()
+        finally:
+            b.end(autotune)
+        self.batch = b
+        return b
+
+    def finish(self):
+        for ctx in self.ctxs:
+            self.g.gl_make_current(ctx)
+            self.g.glFinish()
+
+    def read(self, i):
+        W, H = self.items[i][0], self.items[i][1]
+        return self.g.read_framebuffer(self.ctxs[i], W, H)
+
+    def close(self):
+        if self.batch is not None:
+            self.batch.destroy()
+            self.batch = None
+        for ctx in self.ctxs:
+            self.g.destroy_context(ctx)
+        self.ctxs = []
+        self.g.release()
+
+
+def batch_items(specs):
+    """BatchRun items of batch_specs() scenes."""
+    return [(sp["w"], sp["h"], (lambda g, sp=sp: setup_batch_scene(g, sp)), frame_batch_scene) for sp in specs]
+
+
+def feature_items(names, seed, size=(96, 72)):
+    """BatchRun items of feature scenes: each frame clears every plane, then
+    runs the scene (state calls take effect at once, clears and draws are
+    what a batch records)."""
+    def frame_of(fn, W, H):
+        def frame(g, _):
+            g.glClearColor(0.0, 0.0, 0.0, 1.0)
+            g.glClearDepth(1.0)
+            g.glClearStencil(0)
+            g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"] | E["GL_STENCIL_BUFFER_BIT"])
+            fn(g, W, H, seed)
+        return frame
+    out = []
+    for name in names:
+        W, H = (160, 120) if "small" in name else size
+        out.append((W, H, None, frame_of(FEATURE_SCENES[name], W, H)))
+    return out
 
 
 # scenes whose reference behaviour is undefined (reads outside a buffer); they

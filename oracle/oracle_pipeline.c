@@ -29,6 +29,7 @@
 #include <math.h>
 
 #include "GL/gl.h"
+#include "mtgl_batch.h"
 #include "mtgl_dev.h"
 
 #define OR_MAX_CLIP 12   /* reference MAX_CLIP_VERTS, src/clipping.h:16 */
@@ -899,10 +900,16 @@ void mtgl_dev_destroy(mtgl_dev *d)
 
 int mtgl_dev_finish(mtgl_dev *d) { return d ? MTGL_OK : MTGL_ERR_INVALID; }
 
+static int or_record_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                           uint32_t color, float depth, uint8_t stencil);
+static int or_record_draw(mtgl_dev *d, const mtgl_draw *dr);
+static __thread mtgl_batch *or_recording;
+
 int mtgl_dev_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
                    uint32_t color, float depth, uint8_t stencil)
 {
     if (!d) return MTGL_ERR_INVALID;
+    if (or_recording) return or_record_clear(d, buffers, x0, y0, x1, y1, color, depth, stencil);
     if (x0 < 0) x0 = 0;
     if (y0 < 0) y0 = 0;
     if (x1 > d->w) x1 = d->w;
@@ -1036,6 +1043,7 @@ int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     default:
         return MTGL_ERR_UNSUPPORTED;
     }
+    if (or_recording) return or_record_draw(d, dr);
     d->stats.draws++;
     if (n == 0) return MTGL_OK;
     or_vtx *vt = (or_vtx *)malloc((size_t)n * sizeof(or_vtx));
@@ -1117,3 +1125,203 @@ int mtgl_dev_stats_reset(mtgl_dev *d)
 }
 
 const char *mtgl_dev_backend(void) { return "oracle:cpu (strict C restatement)"; }
+
+/* ------------------------------------------------------------------------- */
+/* batches: the semantic restatement of include/mtgl_batch.h.  Recording copies */
+/* each clear and draw; a launch executes them in recorded order, which keeps   */
+/* every context's order (the only order the batch promises).                   */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    mtgl_dev *dev;
+    int is_clear;
+    uint32_t buffers;
+    int32_t x0, y0, x1, y1;
+    uint32_t color;
+    float depth;
+    uint8_t stencil;
+    mtgl_draw draw;
+    mtgl_raster_state raster;
+    mtgl_vertex_state *vstates;
+    uint8_t *staged;
+} or_cmd;
+
+struct mtgl_batch {
+    or_cmd *cmds;
+    size_t n, cap;
+    mtgl_dev **devs;
+    size_t ndevs;
+    int recorded, ready;
+    mtgl_batch_info_t info;
+    mtgl_dev_stats stats;
+};
+
+/* triangles the assembly loops of mtgl_dev_draw above produce for n vertices */
+static uint32_t or_tri_count(uint32_t mode, uint32_t n)
+{
+    switch (mode) {
+    case GL_TRIANGLES: return n / 3;
+    case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_POLYGON: return n >= 3 ? n - 2 : 0;
+    case GL_QUADS: return (n / 4) * 2;
+    case GL_QUAD_STRIP: return n >= 4 ? ((n - 4) / 2 + 1) * 2 : 0;
+    default: return 0;
+    }
+}
+
+static or_cmd *or_batch_push(mtgl_batch *b, mtgl_dev *d)
+{
+    if (b->n == b->cap) {
+        size_t cap = b->cap ? 2 * b->cap : 64;
+        or_cmd *c = (or_cmd *)realloc(b->cmds, cap * sizeof(or_cmd));
+        if (!c) return NULL;
+        b->cmds = c;
+        b->cap = cap;
+    }
+    size_t i;
+    for (i = 0; i < b->ndevs; i++)
+        if (b->devs[i] == d) break;
+    if (i == b->ndevs) {
+        mtgl_dev **nd = (mtgl_dev **)realloc(b->devs, (b->ndevs + 1) * sizeof(mtgl_dev *));
+        if (!nd) return NULL;
+        b->devs = nd;
+        b->devs[b->ndevs++] = d;
+    }
+    or_cmd *c = &b->cmds[b->n++];
+    memset(c, 0, sizeof(*c));
+    c->dev = d;
+    return c;
+}
+
+static int or_record_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
+                           uint32_t color, float depth, uint8_t stencil)
+{
+    or_cmd *c = or_batch_push(or_recording, d);
+    if (!c) return MTGL_ERR_OUT_OF_MEMORY;
+    c->is_clear = 1;
+    c->buffers = buffers;
+    c->x0 = x0;
+    c->y0 = y0;
+    c->x1 = x1;
+    c->y1 = y1;
+    c->color = color;
+    c->depth = depth;
+    c->stencil = stencil;
+    return MTGL_OK;
+}
+
+static int or_record_draw(mtgl_dev *d, const mtgl_draw *dr)
+{
+    or_cmd *c = or_batch_push(or_recording, d);
+    if (!c) return MTGL_ERR_OUT_OF_MEMORY;
+    c->draw = *dr;
+    c->raster = *dr->raster;
+    c->vstates = (mtgl_vertex_state *)malloc((size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+    if (!c->vstates) return MTGL_ERR_OUT_OF_MEMORY;
+    memcpy(c->vstates, dr->vstates, (size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+    if (dr->staged_bytes) {
+        c->staged = (uint8_t *)malloc((size_t)dr->staged_bytes);
+        if (!c->staged) return MTGL_ERR_OUT_OF_MEMORY;
+        memcpy(c->staged, dr->staged, (size_t)dr->staged_bytes);
+    }
+    c->draw.raster = NULL;   /* set at launch: the command array may move */
+    c->draw.vstates = c->vstates;
+    c->draw.staged = c->staged;
+    return MTGL_OK;
+}
+
+int mtgl_batch_create(mtgl_batch **out)
+{
+    if (!out) return MTGL_ERR_INVALID;
+    *out = (mtgl_batch *)calloc(1, sizeof(mtgl_batch));
+    return *out ? MTGL_OK : MTGL_ERR_OUT_OF_MEMORY;
+}
+
+void mtgl_batch_destroy(mtgl_batch *b)
+{
+    if (!b) return;
+    if (or_recording == b) or_recording = NULL;
+    for (size_t i = 0; i < b->n; i++) {
+        free(b->cmds[i].vstates);
+        free(b->cmds[i].staged);
+    }
+    free(b->cmds);
+    free(b->devs);
+    free(b);
+}
+
+int mtgl_batch_begin(mtgl_batch *b)
+{
+    if (!b || or_recording || b->recorded || b->n) return MTGL_ERR_INVALID;
+    or_recording = b;
+    return MTGL_OK;
+}
+
+int mtgl_batch_end(mtgl_batch *b, int autotune)
+{
+    (void)autotune;
+    if (!b || or_recording != b) return MTGL_ERR_INVALID;
+    or_recording = NULL;
+    b->recorded = 1;
+    b->ready = 1;
+    b->info.contexts = (uint32_t)b->ndevs;
+    b->info.commands = (uint32_t)b->n;
+    /* rounds: the most commands of one context */
+    uint32_t rounds = 0, scenes = 0;
+    for (size_t i = 0; i < b->ndevs; i++) {
+        uint32_t k = 0;
+        for (size_t j = 0; j < b->n; j++)
+            if (b->cmds[j].dev == b->devs[i]) k++;
+        if (k > rounds) rounds = k;
+    }
+    for (size_t j = 0; j < b->n; j++)
+        if (!b->cmds[j].is_clear && or_tri_count(b->cmds[j].draw.mode, b->cmds[j].draw.count)) scenes++;
+    b->info.rounds = rounds;
+    b->info.scenes = scenes;
+    return MTGL_OK;
+}
+
+int mtgl_batch_launch(mtgl_batch *b)
+{
+    if (!b || !b->ready) return MTGL_ERR_INVALID;
+    for (size_t j = 0; j < b->n; j++) {
+        or_cmd *c = &b->cmds[j];
+        if (c->is_clear) {
+            mtgl_dev_clear(c->dev, c->buffers, c->x0, c->y0, c->x1, c->y1, c->color, c->depth, c->stencil);
+        } else {
+            mtgl_dev_stats s0 = c->dev->stats;
+            c->draw.raster = &c->raster;   /* the command array may have moved since recording */
+            (void)mtgl_dev_draw(c->dev, &c->draw);
+            const mtgl_dev_stats *s1 = &c->dev->stats;
+            b->stats.fragments += s1->fragments - s0.fragments;
+            b->stats.frag_bytes += s1->frag_bytes - s0.frag_bytes;
+            b->stats.triangles_in += s1->triangles_in - s0.triangles_in;
+            b->stats.triangles_raster += s1->triangles_raster - s0.triangles_raster;
+            b->stats.draws += s1->draws - s0.draws;
+        }
+    }
+    b->info.launches++;
+    return MTGL_OK;
+}
+
+int mtgl_batch_sync(mtgl_batch *b) { return b ? MTGL_OK : MTGL_ERR_INVALID; }
+
+int mtgl_batch_info(mtgl_batch *b, mtgl_batch_info_t *out)
+{
+    if (!b || !out) return MTGL_ERR_INVALID;
+    *out = b->info;
+    return MTGL_OK;
+}
+
+int mtgl_batch_stats_get(mtgl_batch *b, mtgl_dev_stats *out)
+{
+    if (!b || !out) return MTGL_ERR_INVALID;
+    *out = b->stats;
+    return MTGL_OK;
+}
+
+int mtgl_batch_stats_reset(mtgl_batch *b)
+{
+    if (!b) return MTGL_ERR_INVALID;
+    memset(&b->stats, 0, sizeof(b->stats));
+    return MTGL_OK;
+}
