@@ -200,6 +200,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-scene batched leg")
     ap.add_argument("--batch-launches", type=int, default=50)
     ap.add_argument("--batch-per-draw", type=int, default=5)
+    ap.add_argument("--only-batch", action="store_true", help="run only the batched leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -210,6 +211,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU split is a design note, not an implemented path)", file=sys.stderr)
         sys.exit(2)
+
+    if args.only_batch:
+        print(json.dumps(batch_bench(args.batch_launches, args.batch_per_draw, 0.0)), flush=True)
+        return
 
     g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
     ctx = g.create_context(W, H)

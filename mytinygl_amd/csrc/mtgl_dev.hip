@@ -79,7 +79,7 @@ struct __align__(16) VCold {
     float pad[2];
 };
 
-// fan triangle after setup (208 B); an unclipped triangle t owns slot t, a
+// fan triangle after setup (176 B); an unclipped triangle t owns slot t, a
 // clipped one owns a run of slots from the overflow pool after the first ntri
 struct __align__(16) RTri {
     int32_t x0, y0, x1, y1, x2, y2;
@@ -92,15 +92,22 @@ struct __align__(16) RTri {
     float uv0[2], uv1[2], uv2[2];
     float ez0, ez1, ez2;
     int32_t back;
-    int32_t pad0[3];                     // the first 176 bytes are what colour computation reads
-    int32_t tx0, ty0, tx1, ty1;          // tile box (framebuffer clamped)
-    int32_t ntiles;
-    int32_t pad[3];
+    int32_t pad0[3];                     // all 176 bytes are what colour computation reads
 };
-static_assert(sizeof(RTri) == 208, "RTri layout");
+static_assert(sizeof(RTri) == 176, "RTri layout");
+
+// what the binner reads of a record (16 B, same slot as the record): the
+// framebuffer-clamped tile box as the launch-wide key of its first tile, its
+// width, its tile count and the row pitch of the record's scene
+struct __align__(16) RBox {
+    uint32_t key0;                       // scene tile offset + ty0 * tiles_x + tx0
+    uint32_t tw;                         // tiles per box row
+    uint32_t ntiles;
+    uint32_t tiles_x;
+};
 constexpr int RT_FRAG_VEC4 = 11;         // 176 bytes = 11 x 16 B staged in LDS per bin entry
 constexpr int RT_COV_VEC4 = 4;           // 64 bytes: vertices, box, area, 1/area, z, lod
-static_assert(offsetof(RTri, lod) == 60 && offsetof(RTri, pad0) == 164, "RTri layout");
+static_assert(offsetof(RTri, lod) == 60, "RTri layout");
 
 struct __align__(16) RCold {
     float ep[9];
@@ -119,6 +126,7 @@ struct Counters {
     unsigned int noverflow;       // overflow slots requested by clipped triangles
     unsigned int overflow_full;   // a request did not fit: the host grows the pool and reruns
     unsigned int sticky;          // overflow_full of every launch since the host last cleared it
+    unsigned int nbig;            // triangles queued for the one-wave pair expansion
 };
 
 // attribute stream resolved to a device pointer
@@ -155,6 +163,7 @@ struct SetupParams {
     uint32_t ntri;
     uint32_t tri_off;             // first global triangle of the scene
     uint32_t scene;
+    uint32_t tile_off;            // first global tile of the scene
     int32_t fb_w, fb_h;
     int32_t tiles_x;
     int32_t tex_w, tex_h;         // 0 when the draw is untextured
@@ -163,8 +172,10 @@ struct SetupParams {
     const VCold *vcold;
     RTri *rt;                     // the scene's own slots
     RCold *rcold;
+    RBox *box;
     RTri *rt_over;                // the shared overflow pool (after every scene's slots)
     RCold *rcold_over;
+    RBox *box_over;
     uint32_t over_base;           // global slot of rt_over[0]
     uint32_t *n_rt;               // fan triangles kept per input triangle
     uint32_t *rec_base;           // first global record slot per input triangle
@@ -199,6 +210,7 @@ struct SceneRaster {
 struct RasterParams {
     const SceneRaster *scenes;
     const uint32_t *tile_pre;      // nscenes + 1 first global tiles
+    const uint32_t *tile_scene;    // global tile -> scene (nscenes > 1)
     uint32_t nscenes;
     const RTri *rt;
     const RCold *rcold;
@@ -377,8 +389,7 @@ __device__ void fetch(const DArr &a, int32_t index, float *out, int n)
 }
 
 // The scene of a launch that owns item x: the last s with pre[s] <= x (pre
-// holds the n + 1 running starts, pre[0] = 0).  x is uniform over a block or
-// a wave wherever this is called, so the search runs on scalar registers.
+// holds the n + 1 running starts, pre[0] = 0).
 __device__ __forceinline__ uint32_t find_scene(const uint32_t *pre, uint32_t n, uint32_t x)
 {
     uint32_t lo = 0, hi = n;   // pre[lo] <= x < pre[hi]
@@ -387,6 +398,26 @@ __device__ __forceinline__ uint32_t find_scene(const uint32_t *pre, uint32_t n, 
         if (pre[mid] <= x) lo = mid; else hi = mid;
     }
     return lo;
+}
+
+// Scene of block, tile or item x when a launch has several scenes: a lookup
+// in a map the launch built once (k_scene_maps), one scalar load instead of a
+// chain of dependent ones.
+__device__ __forceinline__ uint32_t scene_of(const uint32_t *map, uint32_t nscenes, uint32_t x)
+{
+    return nscenes == 1u ? 0u : __builtin_amdgcn_readfirstlane(map[x]);
+}
+
+// The three maps of a multi-scene launch, one block per scene: vertex block,
+// setup block and tile -> scene.
+__global__ void __launch_bounds__(256) k_scene_maps(const uint32_t *vblock_pre, const uint32_t *sblock_pre,
+                                                    const uint32_t *tile_pre, uint32_t *vmap, uint32_t *smap,
+                                                    uint32_t *tmap)
+{
+    const uint32_t s = blockIdx.x;
+    for (uint32_t i = vblock_pre[s] + threadIdx.x; i < vblock_pre[s + 1]; i += blockDim.x) vmap[i] = s;
+    for (uint32_t i = sblock_pre[s] + threadIdx.x; i < sblock_pre[s + 1]; i += blockDim.x) smap[i] = s;
+    for (uint32_t i = tile_pre[s] + threadIdx.x; i < tile_pre[s + 1]; i += blockDim.x) tmap[i] = s;
 }
 
 __device__ int32_t fetch_index(const VertexParams &P, uint32_t i)
@@ -417,9 +448,10 @@ __device__ __forceinline__ float sanitize(float v, float nan_value)
 }
 
 // blocks are dealt to scenes in order: block_pre holds each scene's first block
-__global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const uint32_t *block_pre, uint32_t nscenes)
+__global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const uint32_t *block_pre,
+                                                const uint32_t *block_scene, uint32_t nscenes)
 {
-    const uint32_t sc = __builtin_amdgcn_readfirstlane(find_scene(block_pre, nscenes, blockIdx.x));
+    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     const VertexParams &P = Ps[sc];
     const uint32_t i = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
     if (i >= P.count) return;
@@ -632,7 +664,7 @@ __device__ __forceinline__ void perspective_divide(CV &v)
 // one fan triangle (c[0], c[j], c[j+1]) after the divide; returns 1 when a
 // record was written
 __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, const CV &A, const CV &B, const CV &C,
-                         bool cold, RTri *outp, RCold *coldp, int *tiles_out)
+                         bool cold, RTri *outp, RCold *coldp, RBox *boxp, int *tiles_out)
 {
     const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
     const CV *v[3] = {&A, &B, &C};
@@ -707,12 +739,15 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     r.ez0 = A.ez; r.ez1 = B.ez; r.ez2 = C.ez;
     r.lod = lod;
     r.back = back;
-    r.pad0[0] = (int32_t)P.scene;
-    r.pad0[1] = r.pad0[2] = 0;
-    r.tx0 = bx0 / TILE; r.ty0 = by0 / TILE; r.tx1 = bx1 / TILE; r.ty1 = by1 / TILE;
-    r.ntiles = (r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1);
-    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    r.pad0[0] = r.pad0[1] = r.pad0[2] = 0;
     *outp = r;
+    const int32_t tx0 = bx0 / TILE, ty0 = by0 / TILE, tx1 = bx1 / TILE, ty1 = by1 / TILE;
+    RBox bx;
+    bx.key0 = P.tile_off + (uint32_t)(ty0 * P.tiles_x + tx0);
+    bx.tw = (uint32_t)(tx1 - tx0 + 1);
+    bx.ntiles = bx.tw * (uint32_t)(ty1 - ty0 + 1);
+    bx.tiles_x = (uint32_t)P.tiles_x;
+    *boxp = bx;
     if (cold) {
         RCold q;
         for (int k = 0; k < 3; k++) {
@@ -722,16 +757,17 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
         q.pad[0] = q.pad[1] = 0.0f;
         *coldp = q;
     }
-    *tiles_out = r.ntiles;
+    *tiles_out = (int)bx.ntiles;
     return 1;
 }
 
 // fast path: triangles inside every frustum plane pass through the clipper
 // unchanged (each plane re-emits the vertices in order), so they are set up
 // directly; the others are queued for k_setup_clip
-__global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint32_t *block_pre, uint32_t nscenes)
+__global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint32_t *block_pre,
+                                               const uint32_t *block_scene, uint32_t nscenes)
 {
-    const uint32_t sc = __builtin_amdgcn_readfirstlane(find_scene(block_pre, nscenes, blockIdx.x));
+    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     const SetupParams &P = Ss[sc];
     const uint32_t t = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
     if (t >= P.ntri) return;
@@ -756,7 +792,10 @@ __global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint
         int nt = 0;
         RTri *out = P.rt + t;
         RCold *cout = cold ? P.rcold + t : nullptr;
-        if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, &nt)) { nout = 1; tiles = (unsigned long long)nt; }
+        if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, P.box + t, &nt)) {
+            nout = 1;
+            tiles = (unsigned long long)nt;
+        }
     } else {
         uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
         P.clip_list[slot] = P.tri_off + t;
@@ -812,7 +851,8 @@ __global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const 
         for (int k = 0; k < n; k++) perspective_divide(b[k]);
         for (int j = 1; j + 1 < n; j++) {
             int nt = 0;
-            if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr, &nt)) {
+            if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr,
+                          P.box_over + base + nout, &nt)) {
                 nout++;
                 tiles += (unsigned long long)nt;
             }
@@ -916,47 +956,93 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64
 // K3a: expand triangles into (tile, fan triangle) pairs, in submission order
 // ---------------------------------------------------------------------------
 
-// Pairs [0, npairs) exist; a launch sized for more (a captured graph sized by an
-// earlier run) marks the rest with a key beyond every tile, which sorts last.
-__global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
-                                                const uint32_t *rec_base, const RTri *rt, uint64_t cap,
-                                                const SceneRaster *scenes, uint32_t *keys, uint32_t *vals,
-                                                Counters *ctr)
+constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
+
+// (tile, record) pairs of fan record j of a triangle: tiles k0 <= k < k1 of
+// its tile box (row-major), pair p0 + k onwards; pairs at or beyond cap are
+// dropped (the host learns of it through the overflow flag).
+__device__ __forceinline__ void expand_record(const RBox &q, uint32_t rec, uint64_t p0, uint32_t k0, uint32_t k1,
+                                              uint32_t step, uint64_t cap, uint32_t *keys, uint32_t *vals)
 {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= cap) return;
-    const uint64_t npairs = pair_base[ntri];
-    if (p >= npairs) {
-        keys[p] = 0xFFFFFFFFu;
-        vals[p] = 0u;
-        if (p == 0 && npairs > cap) atomicOr(&ctr->overflow_full, 2u);
+    const uint32_t tw = q.tw, base = q.key0, pitch = q.tiles_x;
+    uint32_t ky = k0 / tw, kx = k0 % tw;
+    const uint32_t sy = step / tw, sx = step % tw;
+    for (uint32_t k = k0; k < k1; k += step) {
+        const uint64_t p = p0 + k;
+        if (p >= cap) break;
+        keys[p] = base + ky * pitch + kx;
+        vals[p] = rec;
+        kx += sx;
+        ky += sy;
+        if (kx >= tw) {
+            kx -= tw;
+            ky++;
+        }
+    }
+}
+
+// Pairs of every triangle, in (triangle, fan record, tile) order from the
+// triangle's offset pair_base[t]: one thread per triangle writes them when
+// there are at most EXPAND_SMALL, longer triangles are queued for
+// k_expand_big (one wave each).
+__global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
+                                                const uint32_t *rec_base, const RBox *box, uint64_t cap,
+                                                uint32_t *keys, uint32_t *vals,
+                                                uint32_t *big, Counters *ctr)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0 && pair_base[ntri] > cap) atomicOr(&ctr->overflow_full, 2u);
+    uint64_t p0 = 0, np = 0;
+    if (t < ntri) {
+        p0 = pair_base[t];
+        np = pair_base[t + 1] - p0;
+    }
+    const bool is_big = np > EXPAND_SMALL;
+    // one queue slot reservation per wave
+    const uint64_t bal = __ballot(is_big);
+    uint32_t base = 0;
+    const int lane = threadIdx.x & 63;
+    if (bal && lane == __builtin_ctzll(bal)) base = atomicAdd(&ctr->nbig, (uint32_t)__popcll(bal));
+    base = __shfl(base, __builtin_ctzll(bal ? bal : 1ull));
+    if (is_big) {
+        big[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = t;
         return;
     }
-    if (p == 0 && npairs > cap) atomicOr(&ctr->overflow_full, 2u);
-    // last triangle whose first pair is <= p
-    uint32_t lo = 0, hi = ntri;   // invariant: pair_base[lo] <= p < pair_base[hi]
-    while (hi - lo > 1) {
-        uint32_t mid = lo + (hi - lo) / 2;
-        if (pair_base[mid] <= p) lo = mid; else hi = mid;
+    if (np == 0) return;
+    const uint32_t nr = n_rt[t], rb = rec_base[t];
+    for (uint32_t j = 0; j < nr; j++) {
+        const RBox q = box[rb + j];
+        expand_record(q, rb + j, p0, 0u, q.ntiles, 1u, cap, keys, vals);
+        p0 += q.ntiles;
     }
-    uint32_t t = lo;
-    uint64_t k = p - pair_base[t];
-    uint32_t nr = n_rt[t];
-    const uint32_t rb = rec_base[t];
-    const RTri *r = rt + rb;
-    uint32_t j = 0;
-    for (; j + 1 < nr; j++) {
-        uint64_t nt = (uint64_t)r[j].ntiles;
-        if (k < nt) break;
-        k -= nt;
+}
+
+// The queued triangles, one wave each (lanes stride over each record's tile
+// box), then the tail: capacity the pair total does not fill (a replay sized
+// by an earlier run of the same launch) gets a key beyond every tile, which
+// sorts last.
+__global__ void __launch_bounds__(64) k_expand_big(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
+                                                   const uint32_t *rec_base, const RBox *box, uint64_t cap,
+                                                   uint32_t *keys, uint32_t *vals,
+                                                   const uint32_t *big, const Counters *ctr)
+{
+    const uint32_t lane = threadIdx.x;
+    const uint32_t nbig = __builtin_amdgcn_readfirstlane(*(volatile const uint32_t *)&ctr->nbig);
+    for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const uint32_t t = __builtin_amdgcn_readfirstlane(big[b]);
+        uint64_t p0 = pair_base[t];
+        const uint32_t nr = n_rt[t], rb = rec_base[t];
+        for (uint32_t j = 0; j < nr; j++) {
+            const RBox q = box[rb + j];
+            expand_record(q, rb + j, p0, lane, q.ntiles, 64u, cap, keys, vals);
+            p0 += q.ntiles;
+        }
     }
-    const RTri &q = r[j];
-    const SceneRaster &S = scenes[q.pad0[0]];
-    int32_t tw = q.tx1 - q.tx0 + 1;
-    int32_t ty = q.ty0 + (int32_t)(k / (uint64_t)tw);
-    int32_t tx = q.tx0 + (int32_t)(k % (uint64_t)tw);
-    keys[p] = S.tile_off + (uint32_t)(ty * S.tiles_x + tx);
-    vals[p] = rb + j;
+    const uint64_t np = pair_base[ntri];
+    for (uint64_t p = np + (uint64_t)blockIdx.x * 64u + lane; p < cap; p += (uint64_t)gridDim.x * 64u) {
+        keys[p] = 0xFFFFFFFFu;
+        vals[p] = 0u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,6 +1216,7 @@ __global__ void k_reset(Counters *ctr, unsigned long long *scan_tail)
         ctr->unsupported = 0;
         ctr->nclip = 0;
         ctr->noverflow = 0;
+        ctr->nbig = 0;
         ctr->sticky |= ctr->overflow_full;
         ctr->overflow_full = 0;
     }
@@ -1705,7 +1792,7 @@ __device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, ui
     g = __builtin_amdgcn_readfirstlane(g);
     e0 = __builtin_amdgcn_readfirstlane(P.tile_start[g]);
     e1 = __builtin_amdgcn_readfirstlane(P.tile_end[g]);
-    sc = __builtin_amdgcn_readfirstlane(find_scene(P.tile_pre, P.nscenes, g));
+    sc = scene_of(P.tile_scene, P.nscenes, g);
     tile = g - P.tile_pre[sc];
 }
 
@@ -2161,19 +2248,37 @@ struct ClearDesc {
     uint32_t sv;
 };
 
-// The clears of many framebuffers in one launch; block_pre holds each
-// clear's first block (n + 1 entries).
-__global__ void __launch_bounds__(256) k_clear_multi(const ClearDesc *cd, const uint32_t *block_pre, uint32_t n)
+constexpr uint32_t CLEAR_BLOCK_PX = 4096;   // pixels per block of the batched clear
+
+// The clears of many framebuffers in one launch: block b clears pixels
+// [CLEAR_BLOCK_PX * (b - block_pre[c]), +CLEAR_BLOCK_PX) of clear c =
+// block_clear[b] (row-major over the clear's rectangle).
+__global__ void __launch_bounds__(256) k_clear_multi(const ClearDesc *cd, const uint32_t *block_pre,
+                                                     const uint32_t *block_clear)
 {
-    const uint32_t sc = __builtin_amdgcn_readfirstlane(find_scene(block_pre, n, blockIdx.x));
+    const uint32_t sc = __builtin_amdgcn_readfirstlane(block_clear[blockIdx.x]);
     const ClearDesc &c = cd[sc];
-    const uint64_t i = (uint64_t)(blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)c.w * (uint64_t)c.h) return;
-    const int32_t y = c.y0 + (int32_t)(i / (uint64_t)c.w), x = c.x0 + (int32_t)(i % (uint64_t)c.w);
-    const size_t p = (size_t)y * c.fb_w + x;
-    if (c.color) c.color[p] = c.cv;
-    if (c.depth) c.depth[p] = c.dv;
-    if (c.stencil) c.stencil[p] = (uint8_t)c.sv;
+    // framebuffer sides are at most 16384, so a rectangle has < 2^28 pixels
+    const uint32_t w = (uint32_t)c.w, n = w * (uint32_t)c.h;
+    const uint32_t base = (blockIdx.x - block_pre[sc]) * CLEAR_BLOCK_PX;
+    const bool full_rows = c.x0 == 0 && c.w == c.fb_w;
+    uint32_t *color = c.color;
+    float *depth = c.depth;
+    uint8_t *stencil = c.stencil;
+    for (uint32_t k = threadIdx.x; k < CLEAR_BLOCK_PX; k += blockDim.x) {
+        const uint32_t i = base + k;
+        if (i >= n) break;
+        size_t p;
+        if (full_rows) {
+            p = (size_t)c.y0 * (size_t)c.fb_w + i;
+        } else {
+            const uint32_t y = i / w;
+            p = (size_t)(c.y0 + (int32_t)y) * (size_t)c.fb_w + (size_t)(c.x0 + (int32_t)(i - y * w));
+        }
+        if (color) color[p] = c.cv;
+        if (depth) depth[p] = c.dv;
+        if (stencil) stencil[p] = (uint8_t)c.sv;
+    }
 }
 
 // Fill kernel, used where a memset node cannot be (graph capture).
@@ -2225,7 +2330,7 @@ struct Exec {
     uint8_t *pinned = nullptr;                     // staging for uploads
     size_t pinned_cap = 0;
     hipEvent_t upload_done = nullptr;
-    Arena vhot, vcold, rt, rcold, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
+    Arena vhot, vcold, rt, rcold, box, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
     Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list, cov;
     uint32_t gen = 0;                              // launch generation stamped into tile_gen
@@ -2257,6 +2362,8 @@ struct Launch {
     const uint32_t *tri_pre = nullptr;
     const SceneRaster *sr = nullptr;
     const uint32_t *tile_pre = nullptr;
+    Arena maps;                    // block/tile -> scene maps of a launch with several scenes
+    const uint32_t *vmap = nullptr, *smap = nullptr, *tmap = nullptr;
 };
 
 }  // namespace
@@ -2436,7 +2543,7 @@ int exec_init(Exec &x, int device)
 void exec_free(Exec &x)
 {
     if (x.stream) (void)hipStreamSynchronize(x.stream);
-    Arena *arenas[] = {&x.vhot, &x.vcold, &x.rt, &x.rcold, &x.n_rt, &x.rec_base, &x.tri_tiles, &x.pair_base,
+    Arena *arenas[] = {&x.vhot, &x.vcold, &x.rt, &x.rcold, &x.box, &x.n_rt, &x.rec_base, &x.tri_tiles, &x.pair_base,
                        &x.scan_tmp, &x.clip_list, &x.keys[0], &x.keys[1], &x.vals[0], &x.vals[1], &x.sort_tmp,
                        &x.tile_gen, &x.tile_start, &x.tile_end, &x.tile_list, &x.cov};
     for (Arena *a : arenas) arena_free(*a);
@@ -2564,6 +2671,7 @@ int reserve_scene_buffers(Exec &x, const Sizes &z)
     x.overflow_cap = std::max<uint32_t>(x.overflow_cap, std::min<uint32_t>((uint32_t)(z.tri / 8u), 1u << 24));
     if ((rc = arena_reserve(x, x.rt, (t + x.overflow_cap) * sizeof(RTri)))) return rc;
     if (z.cold && (rc = arena_reserve(x, x.rcold, (t + x.overflow_cap) * sizeof(RCold)))) return rc;
+    if ((rc = arena_reserve(x, x.box, (t + x.overflow_cap) * sizeof(RBox)))) return rc;
     if ((rc = arena_reserve(x, x.n_rt, t * sizeof(uint32_t)))) return rc;
     if ((rc = arena_reserve(x, x.rec_base, t * sizeof(uint32_t)))) return rc;
     if ((rc = arena_reserve(x, x.tri_tiles, (t + 1) * sizeof(unsigned long long)))) return rc;
@@ -2590,9 +2698,10 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     plan.reserve(n);
     size_t bytes = 0;
     const uint64_t pairs_cap = L.pairs_cap;
-    Arena params = L.params;
+    const Arena params = L.params, maps = L.maps;
     L = Launch();
     L.params = params;
+    L.maps = maps;
     L.pairs_cap = pairs_cap;
     for (uint32_t i = 0; i < n; i++) {
         const mtgl_draw *dr = in[i].dr;
@@ -2698,6 +2807,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         q.ntri = p.ntri;
         q.tri_off = toff;
         q.scene = s;
+        q.tile_off = tileoff;
         q.fb_w = d->w;
         q.fb_h = d->h;
         q.tiles_x = d->tiles_x;
@@ -2708,8 +2818,10 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         q.vcold = p.cold ? as<VCold>(x.vcold) + voff : nullptr;
         q.rt = as<RTri>(x.rt) + toff;
         q.rcold = L.any_cold ? as<RCold>(x.rcold) + toff : nullptr;
+        q.box = as<RBox>(x.box) + toff;
         q.rt_over = as<RTri>(x.rt) + ttot;
         q.rcold_over = L.any_cold ? as<RCold>(x.rcold) + ttot : nullptr;
+        q.box_over = as<RBox>(x.box) + ttot;
         q.over_base = ttot;
         q.n_rt = as<uint32_t>(x.n_rt) + toff;
         q.rec_base = as<uint32_t>(x.rec_base) + toff;
@@ -2763,6 +2875,17 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     L.sblock_pre = pre_dev + (ns + 1);
     L.tri_pre = pre_dev + 2 * (ns + 1);
     L.tile_pre = pre_dev + 3 * (ns + 1);
+    if (ns > 1) {
+        const size_t nmap = (size_t)L.vblocks + L.sblocks + L.tiles_total;
+        if ((rc = arena_reserve(x, L.maps, nmap * 4))) return rc;
+        uint32_t *m = as<uint32_t>(L.maps);
+        L.vmap = m;
+        L.smap = m + L.vblocks;
+        L.tmap = m + L.vblocks + L.sblocks;
+        hipLaunchKernelGGL(k_scene_maps, dim3(ns), dim3(256), 0, x.stream, L.vblock_pre, L.sblock_pre, L.tile_pre,
+                           m, m + L.vblocks, m + L.vblocks + L.sblocks);
+        HIPC(hipGetLastError());
+    }
     return MTGL_OK;
 }
 
@@ -2776,8 +2899,8 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     int rc;
     const uint32_t ttot = L.tri_total;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, x.stream, x.ctr, as<unsigned long long>(x.tri_tiles) + ttot);
-    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(256), 0, x.stream, L.vp, L.vblock_pre, L.ns);
-    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(256), 0, x.stream, L.sp, L.sblock_pre, L.ns);
+    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(256), 0, x.stream, L.vp, L.vblock_pre, L.vmap, L.ns);
+    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(256), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
     hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
                        (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
     HIPC(hipGetLastError());
@@ -2830,9 +2953,15 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     }
     uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
     uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
-    hipLaunchKernelGGL(k_expand, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, (const unsigned long long *)pair_base,
-                       ttot, (const uint32_t *)as<uint32_t>(x.n_rt), (const uint32_t *)as<uint32_t>(x.rec_base),
-                       (const RTri *)as<RTri>(x.rt), (uint64_t)n, L.sr, keys, vals, x.ctr);
+    // the clip queue has been consumed: it holds the long triangles now
+    hipLaunchKernelGGL(k_expand, dim3(blocks_for(ttot, 256)), dim3(256), 0, x.stream,
+                       (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
+                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
+                       keys, vals, as<uint32_t>(x.clip_list), x.ctr);
+    hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
+                       x.stream, (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
+                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
+                       keys, vals, (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
     // stable LSD radix sort by tile: pairs keep submission order within a tile;
     // the all-ones key of unused capacity stays above every tile
     int key_bits = 1;
@@ -2864,6 +2993,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     RasterParams rp = {};
     rp.scenes = L.sr;
     rp.tile_pre = L.tile_pre;
+    rp.tile_scene = L.tmap;
     rp.nscenes = L.ns;
     rp.rt = as<RTri>(x.rt);
     rp.rcold = L.any_cold ? as<RCold>(x.rcold) : nullptr;
@@ -2942,6 +3072,7 @@ struct BatchRound {
     uint32_t clear_blocks = 0;
     const ClearDesc *clear_dev = nullptr;
     const uint32_t *clear_pre = nullptr;
+    const uint32_t *clear_map = nullptr;   // block -> clear
 };
 
 }  // namespace
@@ -3029,24 +3160,31 @@ int batch_upload_clears(Exec &x, BatchRound &R)
 {
     const uint32_t n = (uint32_t)R.clears.size();
     if (n == 0) return MTGL_OK;
-    const size_t b_cd = up256(n * sizeof(ClearDesc)), bytes = b_cd + up256((n + 1) * sizeof(uint32_t));
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) acc += (uint64_t)R.clears[i].w * R.clears[i].h / CLEAR_BLOCK_PX + 1;
+    if (acc > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    const size_t b_cd = up256(n * sizeof(ClearDesc)), b_pre = up256((n + 1) * sizeof(uint32_t));
+    const size_t bytes = b_cd + b_pre + up256(acc * sizeof(uint32_t));
     int rc;
     if ((rc = arena_reserve(x, R.clear_params, bytes))) return rc;
     if ((rc = pinned_reserve(x, bytes))) return rc;
     memcpy(x.pinned, R.clears.data(), n * sizeof(ClearDesc));
     uint32_t *pre = reinterpret_cast<uint32_t *>(x.pinned + b_cd);
-    uint64_t acc = 0;
+    uint32_t *map = reinterpret_cast<uint32_t *>(x.pinned + b_cd + b_pre);
+    uint32_t nb = 0;
     for (uint32_t i = 0; i < n; i++) {
-        pre[i] = (uint32_t)acc;
-        acc += blocks_for((uint64_t)R.clears[i].w * (uint64_t)R.clears[i].h, 256);
+        pre[i] = nb;
+        const uint32_t k = (uint32_t)blocks_for((uint64_t)R.clears[i].w * (uint64_t)R.clears[i].h, CLEAR_BLOCK_PX);
+        for (uint32_t j = 0; j < k; j++) map[nb + j] = i;
+        nb += k;
     }
-    if (acc > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
-    pre[n] = (uint32_t)acc;
-    R.clear_blocks = (uint32_t)acc;
+    pre[n] = nb;
+    R.clear_blocks = nb;
     HIPC(hipMemcpyAsync(R.clear_params.ptr, x.pinned, bytes, hipMemcpyHostToDevice, x.stream));
     HIPC(hipEventRecord(x.upload_done, x.stream));
     R.clear_dev = (const ClearDesc *)R.clear_params.ptr;
     R.clear_pre = (const uint32_t *)((const uint8_t *)R.clear_params.ptr + b_cd);
+    R.clear_map = (const uint32_t *)((const uint8_t *)R.clear_params.ptr + b_cd + b_pre);
     return MTGL_OK;
 }
 
@@ -3061,7 +3199,7 @@ int batch_run(mtgl_batch *b, bool frozen)
             if (!frozen && (rc = batch_upload_clears(x, R))) return rc;
             if (R.clear_blocks)
                 hipLaunchKernelGGL(k_clear_multi, dim3(R.clear_blocks), dim3(256), 0, x.stream, R.clear_dev,
-                                   R.clear_pre, (uint32_t)R.clears.size());
+                                   R.clear_pre, R.clear_map);
             HIPC(hipGetLastError());
         }
         if (R.scenes.empty()) continue;
@@ -3269,6 +3407,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     }
     arena_free(d->writer);
     arena_free(d->one.params);
+    arena_free(d->one.maps);
     exec_free(d->ex);
     if (d->color) (void)hipFree(d->color);
     if (d->depth) (void)hipFree(d->depth);
@@ -3566,6 +3705,7 @@ MTGL_API void mtgl_batch_destroy(mtgl_batch *b)
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (BatchRound &R : b->rounds) {
         arena_free(R.launch.params);
+        arena_free(R.launch.maps);
         arena_free(R.clear_params);
     }
     for (BatchCmd *c : b->cmds) delete c;
