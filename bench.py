@@ -17,6 +17,9 @@ on a bounded sample of the same frame.
 "batched": the 512-scene workload (one context per scene) issued per draw and
 replayed from one mtgl_batch (a captured graph of two launches over all
 scenes), with its own bounded CPU sample.
+
+"differentiable": forward + L2 loss + adjoint over 64 captured views
+(mytinygl_amd/diff.py), with the oracle's CPU restatement timed on two views.
 """
 from __future__ import annotations
 
@@ -28,6 +31,13 @@ import sys
 import time
 
 import numpy as np
+
+try:
+    # torch's HIP runtime must be the process's first: loaded after libmtgl.so
+    # it finds no GPU, loaded before it serves both
+    import torch  # noqa: F401
+except ImportError:
+    torch = None
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -176,6 +186,93 @@ def batch_cpu_baseline(specs, frags_scene, budget_s: float):
                       f"{spent:.1f} s of frame time, one thread"}
 
 
+def diff_bench(steps: int, warmup: int, cpu_views: int):
+    """The differentiable workload: 64 views of 256x256 of a 100,352-triangle
+    grid sphere, 1024x1024 fp32 RGBA texture, 8 lights.  One step = forward
+    render of every view, L2 loss against target renders, backward (adjoint)
+    to texels, vertex colours and light colours, through the
+    torch.autograd.Function of mytinygl_amd/diff.py."""
+    import torch
+    from mytinygl_amd import diff as D
+    g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
+    P = D.diff_params()
+    t0 = time.perf_counter()
+    V = D.DiffViews(g, P, 256, 256)
+    capture_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+
+    def t(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+
+    T = D.perturbed(P, seed=13)
+    with torch.no_grad():
+        target = D.render(t(T["vcol"]), t(T["light_rgb"]), t(T["tex"]), V)
+    vc, lr, tx = (t(P[k]).requires_grad_() for k in ("vcol", "light_rgb", "tex"))
+
+    def step():
+        for p in (vc, lr, tx):
+            p.grad = None
+        img = D.render(vc, lr, tx, V)
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) * 1e3 / steps
+    with torch.no_grad():
+        for _ in range(warmup):
+            D.render(vc, lr, tx, V)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            D.render(vc, lr, tx, V)
+        torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3 / steps
+    covered = int(sum((V.gbuffer(v)["tri"] >= 0).sum() for v in range(V.nviews)))
+    V.close()
+    g.release()
+    out = {
+        "workload": "64 views of 256x256 (Fibonacci sphere, distance 3), 224x224 grid sphere (100,352 triangles), "
+                    "1024x1024 fp32 RGBA texture, 8 directional lights, bilinear; L2 loss; adjoint to texels, "
+                    "vertex colours and light colours",
+        "views": 64, "triangles": len(P["idx"]) // 3, "covered_pixels": covered,
+        "capture_s": round(capture_s, 3), "step_ms": round(step_ms, 3), "forward_ms": round(fwd_ms, 3),
+        "views_per_s": round(64 / step_ms * 1e3, 1),
+        "mpixels_s": round(covered / step_ms / 1e3, 2),
+    }
+    if cpu_views > 0:
+        out["cpu_baseline"] = diff_cpu_baseline(P, T, cpu_views)
+        if out["cpu_baseline"]:
+            out["speedup_vs_cpu"] = round(out["cpu_baseline"]["step_ms"] / step_ms, 1)
+    return out
+
+
+def diff_cpu_baseline(P, T, nviews: int):
+    """The oracle's differentiable path (CPU restatement, one thread): forward,
+    loss gradient and backward over the first nviews views, scaled to 64."""
+    from mytinygl_amd import diff as D
+    lib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+    if not os.path.exists(lib):
+        return None
+    g = GLLibrary(lib, "oracle")
+    Ps = dict(P, eyes=P["eyes"][:nviews])
+    V = D.DiffViews(g, Ps, 256, 256)
+    target = V.forward_np(T["vcol"], T["light_rgb"], T["tex"])
+    t0 = time.perf_counter()
+    img = V.forward_np(P["vcol"], P["light_rgb"], P["tex"])
+    V.backward_np(P["vcol"], P["light_rgb"], P["tex"], 2.0 * (img - target))
+    sec = time.perf_counter() - t0
+    V.close()
+    g.release()
+    return {"step_ms": round(sec * 1e3 * 64 / nviews, 1), "unit": "ms per 64-view step", "cores": 1, "kind": "port",
+            "sample": f"{nviews} of the 64 views, forward + backward in {sec:.2f} s, one thread, scaled to 64"}
+
+
 def copy_calibration(nbytes: int, reps: int = 5) -> None:
     """Plain device-to-device copies of nbytes (torch's elementwise copy
     kernel): the reference the HBM counters FETCH_SIZE / WRITE_SIZE are
@@ -201,6 +298,9 @@ def main():
     ap.add_argument("--batch-launches", type=int, default=50)
     ap.add_argument("--batch-per-draw", type=int, default=5)
     ap.add_argument("--only-batch", action="store_true", help="run only the batched leg (profiling)")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
+    ap.add_argument("--only-diff", action="store_true", help="run only the differentiable leg (profiling)")
+    ap.add_argument("--diff-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -214,6 +314,9 @@ def main():
 
     if args.only_batch:
         print(json.dumps(batch_bench(args.batch_launches, args.batch_per_draw, 0.0)), flush=True)
+        return
+    if args.only_diff:
+        print(json.dumps(diff_bench(args.diff_steps, 2, 0)), flush=True)
         return
 
     g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
@@ -264,6 +367,7 @@ def main():
 
     batched = None if args.no_batch else batch_bench(args.batch_launches, args.batch_per_draw,
                                                       0.0 if args.no_cpu else args.cpu_seconds / 2)
+    differentiable = None if args.no_diff else diff_bench(args.diff_steps, 2, 0 if args.no_cpu else 2)
 
     out = {
         "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
@@ -290,6 +394,7 @@ def main():
                      "bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu_obj,
         "batched": batched,
+        "differentiable": differentiable,
     }
     if cpu_obj:
         out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None

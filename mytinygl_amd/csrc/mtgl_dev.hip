@@ -43,6 +43,7 @@
 #include "GL/gl.h"
 #include "mtgl_batch.h"
 #include "mtgl_dev.h"
+#include "mtgl_diff.h"
 
 #define MTGL_API extern "C" __attribute__((visibility("default")))
 
@@ -302,9 +303,14 @@ __device__ __forceinline__ void xform(const float *m, float x, float y, float z,
     o[3] = m[3] * x + m[7] * y + m[11] * z + m[15] * w;
 }
 
-// compute_lighting (reference src/lighting.h:53-142)
-__device__ Col compute_lighting(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
-                                Col m_amb, Col m_dif, Col m_spec, Col m_emi, float shininess)
+// compute_lighting (reference src/lighting.h:53-142) before its final clamp.
+// OVERRIDE (differentiable shading): the lights' diffuse rgb come from
+// dif_rgb (3 per light) and each light's att * N.L where it adds diffuse
+// light (else 0) is stored in ndl_out.
+template <bool OVERRIDE>
+__device__ Col lighting_sum(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
+                            Col m_amb, Col m_dif, Col m_spec, Col m_emi, float shininess, const float *dif_rgb,
+                            float *ndl_out)
 {
     Col result = col_add(m_emi, col_mul(m_amb, col_of(L->model_ambient)));
     result.a = m_dif.a;
@@ -312,6 +318,7 @@ __device__ Col compute_lighting(const mtgl_lighting *L, uint32_t flags, const fl
     if (flags & MTGL_F_NORMALIZE) normalize3(N);
     for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
         const mtgl_light *lt = &L->lights[i];
+        if (OVERRIDE) ndl_out[i] = 0.0f;
         if (!lt->enabled) continue;
         float Ld[3];
         float att = 1.0f;
@@ -339,7 +346,14 @@ __device__ Col compute_lighting(const mtgl_lighting *L, uint32_t flags, const fl
         result = col_add(result, col_scale(col_mul(m_amb, col_of(lt->ambient)), att));
         float ndl = dot3(N, Ld);
         if (ndl > 0.0f) {
-            result = col_add(result, col_scale(col_mul(m_dif, col_of(lt->diffuse)), att * ndl));
+            Col ld = col_of(lt->diffuse);
+            if (OVERRIDE) {
+                ld.r = dif_rgb[3 * i];
+                ld.g = dif_rgb[3 * i + 1];
+                ld.b = dif_rgb[3 * i + 2];
+                ndl_out[i] = att * ndl;
+            }
+            result = col_add(result, col_scale(col_mul(m_dif, ld), att * ndl));
             if (shininess > 0.0f) {
                 float V[3] = {0.0f, 0.0f, 1.0f};
                 if (L->local_viewer) {
@@ -356,7 +370,14 @@ __device__ Col compute_lighting(const mtgl_lighting *L, uint32_t flags, const fl
             }
         }
     }
-    return col_clamp(result);
+    return result;
+}
+
+__device__ Col compute_lighting(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
+                                Col m_amb, Col m_dif, Col m_spec, Col m_emi, float shininess)
+{
+    return col_clamp(lighting_sum<false>(L, flags, eye_pos, eye_normal, m_amb, m_dif, m_spec, m_emi, shininess,
+                                         nullptr, nullptr));
 }
 
 // ---------------------------------------------------------------------------
@@ -1351,27 +1372,42 @@ __device__ __forceinline__ int32_t wrap_coord(int32_t x, int32_t w, bool repeat,
 
 // one level: bilinear with wrapped taps (reference bilinear_filter,
 // get_texel_wrapped) or nearest with a clamped index
-__device__ uint32_t sample_level(const TexLevel &L, bool rep_s, bool rep_t, float u, float v, bool linear, int &taps)
+// the four texels (00, 10, 01, 11) and fractions of a bilinear tap at (u, v)
+__device__ __forceinline__ void bilinear_taps(const TexLevel &L, bool rep_s, bool rep_t, float u, float v, size_t *at,
+                                              float &fx, float &fy)
 {
     float tx = u * (float)L.w - 0.5f;
     float ty = v * (float)L.h - 0.5f;
+    int32_t x0 = cvtt(floorf(tx));
+    int32_t y0 = cvtt(floorf(ty));
+    int32_t x1 = x0 + 1, y1 = y0 + 1;
+    fx = tx - (float)x0;
+    fy = ty - (float)y0;
+    int32_t wx0 = wrap_coord(x0, L.w, rep_s, L.pow2), wx1 = wrap_coord(x1, L.w, rep_s, L.pow2);
+    int32_t wy0 = wrap_coord(y0, L.h, rep_t, L.pow2), wy1 = wrap_coord(y1, L.h, rep_t, L.pow2);
+    at[0] = (size_t)wy0 * L.w + wx0;
+    at[1] = (size_t)wy0 * L.w + wx1;
+    at[2] = (size_t)wy1 * L.w + wx0;
+    at[3] = (size_t)wy1 * L.w + wx1;
+}
+
+__device__ uint32_t sample_level(const TexLevel &L, bool rep_s, bool rep_t, float u, float v, bool linear, int &taps)
+{
     if (linear) {
-        int32_t x0 = cvtt(floorf(tx));
-        int32_t y0 = cvtt(floorf(ty));
-        int32_t x1 = x0 + 1, y1 = y0 + 1;
-        float fx = tx - (float)x0;
-        float fy = ty - (float)y0;
-        int32_t wx0 = wrap_coord(x0, L.w, rep_s, L.pow2), wx1 = wrap_coord(x1, L.w, rep_s, L.pow2);
-        int32_t wy0 = wrap_coord(y0, L.h, rep_t, L.pow2), wy1 = wrap_coord(y1, L.h, rep_t, L.pow2);
-        uint32_t c00 = L.px[(size_t)wy0 * L.w + wx0];
-        uint32_t c10 = L.px[(size_t)wy0 * L.w + wx1];
-        uint32_t c01 = L.px[(size_t)wy1 * L.w + wx0];
-        uint32_t c11 = L.px[(size_t)wy1 * L.w + wx1];
+        size_t at[4];
+        float fx, fy;
+        bilinear_taps(L, rep_s, rep_t, u, v, at, fx, fy);
+        uint32_t c00 = L.px[at[0]];
+        uint32_t c10 = L.px[at[1]];
+        uint32_t c01 = L.px[at[2]];
+        uint32_t c11 = L.px[at[3]];
         taps += 4;
         Col top = col_lerp(col_unpack(c00), col_unpack(c10), fx);
         Col bot = col_lerp(col_unpack(c01), col_unpack(c11), fx);
         return col_pack(col_lerp(top, bot, fy));
     }
+    const float tx = u * (float)L.w - 0.5f;
+    const float ty = v * (float)L.h - 0.5f;
     int32_t x = cvtt(floorf(tx + 0.5f));
     int32_t y = cvtt(floorf(ty + 0.5f));
     if (x < 0) x = 0;
@@ -1383,14 +1419,20 @@ __device__ uint32_t sample_level(const TexLevel &L, bool rep_s, bool rep_t, floa
 }
 
 // reference texture_sample_lod (src/textures.c:457-557)
-__device__ uint32_t sample_texture(const RasterConsts &K, const TexLevel &L0, const TexLevel &L1, float u, float v,
-                                   float lod, int &taps)
+// texture coordinate wrap of texture_sample_lod: fraction for GL_REPEAT, else clamp to [0, 1]
+__device__ __forceinline__ void wrap_uv(bool rep_s, bool rep_t, float &u, float &v)
 {
-    const bool rep_s = K.rep_s, rep_t = K.rep_t;
     if (rep_s) { u = u - (float)cvtt(u); if (u < 0) u += 1.0f; }
     else { if (u < 0.0f) u = 0.0f; if (u > 1.0f) u = 1.0f; }
     if (rep_t) { v = v - (float)cvtt(v); if (v < 0) v += 1.0f; }
     else { if (v < 0.0f) v = 0.0f; if (v > 1.0f) v = 1.0f; }
+}
+
+__device__ uint32_t sample_texture(const RasterConsts &K, const TexLevel &L0, const TexLevel &L1, float u, float v,
+                                   float lod, int &taps)
+{
+    const bool rep_s = K.rep_s, rep_t = K.rep_t;
+    wrap_uv(rep_s, rep_t, u, v);
     uint32_t filter = lod > 0.0f ? K.min_filter : K.mag_filter;
     if (filter == GL_NEAREST_MIPMAP_NEAREST || filter == GL_LINEAR_MIPMAP_NEAREST) {
         if (lod >= 0.5f) {
@@ -1427,6 +1469,26 @@ __device__ __forceinline__ void stencil_write(const RasterConsts &K, PixelState 
     uint32_t m8 = K.stencil_writemask & 0xFFu;
     px.s = ((px.s & ~m8) | (nv & m8)) & 0xFFu;
     px.dirty_s = true;
+}
+
+// texture coordinates of a fragment (src/raster.c:618-633): perspective-correct
+// unless the hint is GL_FASTEST
+__device__ __forceinline__ void texcoord_at(const RTri *r, bool persp, float b0, float b1, float b2, float &u, float &v)
+{
+    if (persp) {
+        const float u0w = r->uv0[0] * r->w0, v0w = r->uv0[1] * r->w0;
+        const float u1w = r->uv1[0] * r->w1, v1w = r->uv1[1] * r->w1;
+        const float u2w = r->uv2[0] * r->w2, v2w = r->uv2[1] * r->w2;
+        float uow = b0 * u0w + b1 * u1w + b2 * u2w;
+        float vow = b0 * v0w + b1 * v1w + b2 * v2w;
+        float oow = b0 * r->w0 + b1 * r->w1 + b2 * r->w2;
+        float w = 1.0f / oow;
+        u = uow * w;
+        v = vow * w;
+    } else {
+        u = b0 * r->uv0[0] + b1 * r->uv1[0] + b2 * r->uv2[0];
+        v = b0 * r->uv0[1] + b1 * r->uv1[1] + b2 * r->uv2[1];
+    }
 }
 
 // The part of one fragment's pipeline (src/raster.c:546-721) that does not
@@ -1474,20 +1536,7 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
     }
     if (K.textured) {
         float u, v;
-        if (K.persp) {
-            const float u0w = r->uv0[0] * r->w0, v0w = r->uv0[1] * r->w0;
-            const float u1w = r->uv1[0] * r->w1, v1w = r->uv1[1] * r->w1;
-            const float u2w = r->uv2[0] * r->w2, v2w = r->uv2[1] * r->w2;
-            float uow = b0 * u0w + b1 * u1w + b2 * u2w;
-            float vow = b0 * v0w + b1 * v1w + b2 * v2w;
-            float oow = b0 * r->w0 + b1 * r->w1 + b2 * r->w2;
-            float w = 1.0f / oow;
-            u = uow * w;
-            v = vow * w;
-        } else {
-            u = b0 * r->uv0[0] + b1 * r->uv1[0] + b2 * r->uv2[0];
-            v = b0 * r->uv0[1] + b1 * r->uv1[1] + b2 * r->uv2[1];
-        }
+        texcoord_at(r, K.persp, b0, b1, b2, u, v);
         int taps = 0;
         Col tc = col_unpack(sample_texture(K, K.L0, K.L1, u, v, r->lod, taps));
         if (K.alpha_on && !cmp_f(K.alpha_cmp, tc.a, K.alpha_ref)) {
@@ -2288,6 +2337,314 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint32_t n, uint3
     if (i < n) p[i] = v;
 }
 
+
+// ---------------------------------------------------------------------------
+// differentiable shading of captured views (include/mtgl_diff.h)
+// ---------------------------------------------------------------------------
+
+struct GPx {
+    int32_t tri;
+    float b[3];
+    uint32_t texel[4];
+    float fx, fy;
+};
+static_assert(sizeof(GPx) == sizeof(mtgl_gbuffer_px), "G-buffer pixel layout");
+
+// one captured view as the kernels see it
+struct DiffView {
+    const GPx *gb;                 // w * h
+    const uint32_t *elems;         // 3 per triangle: element of each vertex
+    const float *pos;              // 3 per element, object space
+    const float *nrm;              // 3 per element
+    const mtgl_vertex_state *vs;
+    uint32_t ntri;
+    int32_t w, h;
+    uint32_t pad;
+};
+
+// Per pixel of a context's latest draw (a single-scene launch of a
+// last-writer kind): the triangle of the fragment that left its colour, that
+// fragment's colour weights and its bilinear texture footprint, computed as
+// k_raster_color and sample_texture compute them.  Tiles the launch did not
+// list (tile_gen != gen) and pixels no fragment reached get tri = -1 and
+// their colour; fragments of clipped triangles (overflow slots) get -2.
+__global__ void __launch_bounds__(256) k_gbuffer(const SceneRaster *scene, const RTri *rt, const uint32_t *writer,
+                                                 const uint32_t *tile_gen, uint32_t gen, uint32_t over_base,
+                                                 GPx *out, uint32_t *nclipped)
+{
+    const SceneRaster &S = scene[0];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)S.fb_w * (uint32_t)S.fb_h) return;
+    const int32_t x = (int32_t)(p % (uint32_t)S.fb_w), y = (int32_t)(p / (uint32_t)S.fb_w);
+    const uint32_t tile = (uint32_t)(y / TILE) * (uint32_t)S.tiles_x + (uint32_t)(x / TILE);
+    const uint32_t w = tile_gen[tile] == gen ? writer[p] : 0u;
+    GPx g = {};
+    if (w == 0u) {
+        g.tri = -1;
+        g.texel[0] = S.color[p];
+        out[p] = g;
+        return;
+    }
+    const uint32_t rec = w - 1u;
+    if (rec >= over_base) {
+        g.tri = -2;
+        out[p] = g;
+        atomicAdd(nclipped, 1u);
+        return;
+    }
+    RasterConsts K;
+    raster_consts(S, K);
+    const RTri *r = rt + rec;
+    float w0e, w1e, w2e;
+    edge_values(r, x, y, w0e, w1e, w2e);
+    const float inv = r->inv_area;
+    const float b0 = w0e * inv, b1 = w1e * inv, b2 = w2e * inv;
+    float u, v;
+    texcoord_at(r, K.persp, b0, b1, b2, u, v);
+    wrap_uv(K.rep_s, K.rep_t, u, v);
+    size_t at[4];
+    float fx, fy;
+    bilinear_taps(K.L0, K.rep_s, K.rep_t, u, v, at, fx, fy);
+    g.tri = (int32_t)rec;
+    g.b[0] = b0;
+    g.b[1] = b1;
+    g.b[2] = b2;
+    for (int k = 0; k < 4; k++) g.texel[k] = (uint32_t)at[k];
+    g.fx = fx;
+    g.fy = fy;
+    out[p] = g;
+}
+
+// The draw's index stream, positions and normals as dense arrays.
+__global__ void __launch_bounds__(256) k_diff_geom(const VertexParams *vp, uint32_t ntri, uint32_t nvert,
+                                                   uint32_t *elems, float *pos, float *nrm)
+{
+    const VertexParams &P = *vp;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3u * ntri) elems[i] = (uint32_t)fetch_index(P, i);
+    if (i < nvert) {
+        float p4[4], n3[3];
+        fetch(P.position, (int32_t)i, p4, 4);
+        if (P.position.comps == 2) p4[2] = 0.0f;
+        if (P.normal.enabled) fetch(P.normal, (int32_t)i, n3, 3);
+        else { n3[0] = P.cur_normal[0]; n3[1] = P.cur_normal[1]; n3[2] = P.cur_normal[2]; }
+        for (int k = 0; k < 3; k++) {
+            pos[3 * i + k] = p4[k];
+            nrm[3 * i + k] = n3[k];
+        }
+    }
+}
+
+// eye-space position and normal of element e, and the material of its
+// lighting with the vertex colour through GL_COLOR_MATERIAL, as k_vertex
+// computes them
+struct DiffVertex {
+    float ep[3], en[3];
+    Col vc;                       // sanitised vertex colour
+    Col amb, dif, spe, emi;
+};
+
+__device__ __forceinline__ void diff_vertex(const DiffView &D, uint32_t e, const float *vcol, DiffVertex &o)
+{
+    const mtgl_vertex_state *vs = D.vs;
+    const float *pp = D.pos + 3u * e, *nn = D.nrm + 3u * e;
+    float eye[4];
+    xform(vs->modelview, pp[0], pp[1], pp[2], 1.0f, eye);
+    float en4[4];
+    xform(vs->normal_matrix, nn[0], nn[1], nn[2], 0.0f, en4);
+    o.en[0] = en4[0]; o.en[1] = en4[1]; o.en[2] = en4[2];
+    normalize3(o.en);
+    o.ep[0] = eye[0]; o.ep[1] = eye[1]; o.ep[2] = eye[2];
+    const float *c = vcol + 4u * e;
+    o.vc = {sanitize(c[0], 0.0f), sanitize(c[1], 0.0f), sanitize(c[2], 0.0f), sanitize(c[3], 1.0f)};
+    const mtgl_material &f = vs->lighting.front;
+    o.amb = col_of(f.ambient); o.dif = col_of(f.diffuse); o.spe = col_of(f.specular); o.emi = col_of(f.emission);
+    const Col k = col_clamp(o.vc);
+    const uint32_t mode = vs->color_material_mode;
+    if (mode == GL_AMBIENT || mode == GL_AMBIENT_AND_DIFFUSE) o.amb = k;
+    if (mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE) o.dif = k;
+}
+
+// lit colour of every (view, element): grid (elements, views)
+__global__ void __launch_bounds__(256) k_diff_lit(const DiffView *views, uint32_t nvert, const float *vcol,
+                                                  const float *light_rgb, float4 *lit)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nvert) return;
+    const DiffView &D = views[blockIdx.y];
+    DiffVertex q;
+    diff_vertex(D, e, vcol, q);
+    float ndl[MTGL_MAX_LIGHTS];
+    const Col r = col_clamp(lighting_sum<true>(&D.vs->lighting, D.vs->flags, q.ep, q.en, q.amb, q.dif, q.spe, q.emi,
+                                               D.vs->lighting.front.shininess, light_rgb, ndl));
+    lit[(size_t)blockIdx.y * nvert + e] = make_float4(r.r, r.g, r.b, r.a);
+}
+
+__device__ __forceinline__ Col col4(float4 v) { return {v.x, v.y, v.z, v.w}; }
+
+// colour and texture sample of a covered pixel (GL_MODULATE operands)
+__device__ __forceinline__ void diff_pixel(const GPx &g, const uint32_t *el, const float4 *litv, uint32_t nvert,
+                                           const float4 *tex, Col &c, Col &tc)
+{
+    Col l[3];
+    for (int k = 0; k < 3; k++) {
+        const uint32_t e = el[k];
+        l[k] = e < nvert ? col4(litv[e]) : Col{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    const float b0 = g.b[0], b1 = g.b[1], b2 = g.b[2];
+    c.r = l[0].r * b0 + l[1].r * b1 + l[2].r * b2;
+    c.g = l[0].g * b0 + l[1].g * b1 + l[2].g * b2;
+    c.b = l[0].b * b0 + l[1].b * b1 + l[2].b * b2;
+    c.a = l[0].a * b0 + l[1].a * b1 + l[2].a * b2;
+    const Col top = col_lerp(col4(tex[g.texel[0]]), col4(tex[g.texel[1]]), g.fx);
+    const Col bot = col_lerp(col4(tex[g.texel[2]]), col4(tex[g.texel[3]]), g.fx);
+    tc = col_lerp(top, bot, g.fy);
+}
+
+// images: grid (pixels, views)
+__global__ void __launch_bounds__(256) k_diff_shade(const DiffView *views, uint32_t nvert, const float4 *lit,
+                                                    const float4 *tex, float4 *images)
+{
+    const DiffView &D = views[blockIdx.y];
+    const uint32_t npx = (uint32_t)D.w * (uint32_t)D.h;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx) return;
+    const GPx g = D.gb[p];
+    Col out;
+    if (g.tri < 0) {
+        out = col_unpack(g.texel[0]);
+    } else {
+        Col c, tc;
+        diff_pixel(g, D.elems + 3u * (uint32_t)g.tri, lit + (size_t)blockIdx.y * nvert, nvert, tex, c, tc);
+        out = col_clamp(col_mul(c, tc));
+    }
+    images[(size_t)blockIdx.y * npx + p] = make_float4(out.r, out.g, out.b, out.a);
+}
+
+__device__ __forceinline__ float clamp_grad(float v, float g) { return (v >= 0.0f && v <= 1.0f) ? g : 0.0f; }
+
+// adjoint of k_diff_shade: dL/dlit per (view, element) into dlit, dL/dtex;
+// grid (pixels, views)
+__global__ void __launch_bounds__(256) k_diff_shade_bwd(const DiffView *views, uint32_t nvert, const float4 *lit,
+                                                        const float4 *tex, const float4 *grad, float *dlit,
+                                                        float *g_tex)
+{
+    const DiffView &D = views[blockIdx.y];
+    const uint32_t npx = (uint32_t)D.w * (uint32_t)D.h;
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx) return;
+    const GPx g = D.gb[p];
+    if (g.tri < 0) return;
+    const uint32_t *el = D.elems + 3u * (uint32_t)g.tri;
+    Col c, tc;
+    diff_pixel(g, el, lit + (size_t)blockIdx.y * nvert, nvert, tex, c, tc);
+    const Col prod = col_mul(c, tc);
+    const float4 gi = grad[(size_t)blockIdx.y * npx + p];
+    const Col gm = {clamp_grad(prod.r, gi.x), clamp_grad(prod.g, gi.y), clamp_grad(prod.b, gi.z),
+                    clamp_grad(prod.a, gi.w)};
+    const Col dc = col_mul(gm, tc), dt = col_mul(gm, c);
+    for (int k = 0; k < 3; k++) {
+        const uint32_t e = el[k];
+        if (e >= nvert) continue;
+        float *d = dlit + 4u * ((size_t)blockIdx.y * nvert + e);
+        const float b = g.b[k];
+        unsafeAtomicAdd(d + 0, dc.r * b);
+        unsafeAtomicAdd(d + 1, dc.g * b);
+        unsafeAtomicAdd(d + 2, dc.b * b);
+        unsafeAtomicAdd(d + 3, dc.a * b);
+    }
+    const float fx = g.fx, fy = g.fy;
+    const float wt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+    for (int k = 0; k < 4; k++) {
+        float *d = g_tex + 4u * (size_t)g.texel[k];
+        unsafeAtomicAdd(d + 0, dt.r * wt[k]);
+        unsafeAtomicAdd(d + 1, dt.g * wt[k]);
+        unsafeAtomicAdd(d + 2, dt.b * wt[k]);
+        unsafeAtomicAdd(d + 3, dt.a * wt[k]);
+    }
+}
+
+// adjoint of k_diff_lit: one thread per element over every view; dL/dvcol
+// written, dL/dlight_rgb reduced per block and added
+__global__ void __launch_bounds__(256) k_diff_lit_bwd(const DiffView *views, uint32_t nviews, uint32_t nvert,
+                                                      const float *vcol, const float *light_rgb, const float *dlit,
+                                                      float *g_vcol, float *g_light)
+{
+    __shared__ float s_gl[3 * MTGL_MAX_LIGHTS];
+    if (threadIdx.x < 3 * MTGL_MAX_LIGHTS) s_gl[threadIdx.x] = 0.0f;
+    __syncthreads();
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nvert) {
+        Col gv = {0.0f, 0.0f, 0.0f, 0.0f};
+        float gl[3 * MTGL_MAX_LIGHTS];
+        for (int i = 0; i < 3 * MTGL_MAX_LIGHTS; i++) gl[i] = 0.0f;
+        bool any_light = false;
+        for (uint32_t v = 0; v < nviews; v++) {
+            const float *dl4 = dlit + 4u * ((size_t)v * nvert + e);
+            const Col dl = {dl4[0], dl4[1], dl4[2], dl4[3]};
+            if (dl.r == 0.0f && dl.g == 0.0f && dl.b == 0.0f && dl.a == 0.0f) continue;
+            const DiffView &D = views[v];
+            const mtgl_vertex_state *vs = D.vs;
+            const mtgl_lighting &L = vs->lighting;
+            DiffVertex q;
+            diff_vertex(D, e, vcol, q);
+            float ndl[MTGL_MAX_LIGHTS];
+            const Col r = lighting_sum<true>(&L, vs->flags, q.ep, q.en, q.amb, q.dif, q.spe, q.emi,
+                                             L.front.shininess, light_rgb, ndl);
+            const Col g = {clamp_grad(r.r, dl.r), clamp_grad(r.g, dl.g), clamp_grad(r.b, dl.b), clamp_grad(r.a, dl.a)};
+            const uint32_t mode = vs->color_material_mode;
+            const bool amb_c = mode == GL_AMBIENT || mode == GL_AMBIENT_AND_DIFFUSE;
+            const bool dif_c = mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE;
+            // dr/dk for the material colour k = clamp(vcol) (att = 1: directional lights)
+            Col dk = {0.0f, 0.0f, 0.0f, dif_c ? 1.0f : 0.0f};
+            if (amb_c) {
+                dk.r += L.model_ambient.r;
+                dk.g += L.model_ambient.g;
+                dk.b += L.model_ambient.b;
+            }
+            for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
+                const mtgl_light &lt = L.lights[i];
+                if (!lt.enabled) continue;
+                if (amb_c) {
+                    dk.r += lt.ambient.r;
+                    dk.g += lt.ambient.g;
+                    dk.b += lt.ambient.b;
+                    dk.a += lt.ambient.a;
+                }
+                const float n = ndl[i];
+                if (n > 0.0f) {
+                    if (dif_c) {
+                        dk.r += light_rgb[3 * i] * n;
+                        dk.g += light_rgb[3 * i + 1] * n;
+                        dk.b += light_rgb[3 * i + 2] * n;
+                        dk.a += lt.diffuse.a * n;
+                    }
+                    gl[3 * i] += g.r * q.dif.r * n;
+                    gl[3 * i + 1] += g.g * q.dif.g * n;
+                    gl[3 * i + 2] += g.b * q.dif.b * n;
+                    any_light = true;
+                }
+            }
+            const Col gk = col_mul(g, dk);
+            gv.r += clamp_grad(q.vc.r, gk.r);
+            gv.g += clamp_grad(q.vc.g, gk.g);
+            gv.b += clamp_grad(q.vc.b, gk.b);
+            gv.a += clamp_grad(q.vc.a, gk.a);
+        }
+        float *o = g_vcol + 4u * e;
+        o[0] = gv.r;
+        o[1] = gv.g;
+        o[2] = gv.b;
+        o[3] = gv.a;
+        if (any_light)
+            for (int i = 0; i < 3 * MTGL_MAX_LIGHTS; i++)
+                if (gl[i] != 0.0f) atomicAdd(&s_gl[i], gl[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * MTGL_MAX_LIGHTS && s_gl[threadIdx.x] != 0.0f)
+        unsafeAtomicAdd(g_light + threadIdx.x, s_gl[threadIdx.x]);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2368,6 +2725,19 @@ struct Launch {
 
 }  // namespace
 
+// what a differentiable capture needs to know of a context's latest draw
+struct LastDraw {
+    bool valid = false;
+    uint32_t mode = 0, ntri = 0, n_vstates = 0;
+    bool color_array = false;
+    mtgl_raster_state rs = {};
+    mtgl_vertex_state vs = {};
+    uint32_t kind = 0;
+    uint32_t gen = 0;              // tile generation of its launch
+    uint32_t over_base = 0;        // first overflow record slot of its launch
+    int32_t tex_w = 0, tex_h = 0;
+};
+
 struct mtgl_dev {
     int device = 0;
     int32_t w = 0, h = 0, tiles_x = 0, tiles_y = 0;
@@ -2379,6 +2749,7 @@ struct mtgl_dev {
     DevBuf buf[MTGL_MAX_OBJECTS + 1];
     Exec ex;
     Launch one;                    // tables of the context's latest draw
+    LastDraw last;
     mtgl_batch *pending = nullptr; // batch launched into this framebuffer, not yet joined
     bool dirty = false;            // work queued on ex.stream since the last batch launch joined it
 };
@@ -3639,8 +4010,25 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     if (d->ex.pending_events.size() > 4096) (void)resolve_timing(d->ex);
     SceneIn in = {d, dr};
     d->one.pairs_cap = 0;
+    d->last.valid = false;
     rc = execute_scenes(d->ex, d->one, &in, 1);
     d->dirty = true;
+    if (rc == MTGL_OK && d->one.ns == 1) {
+        LastDraw &l = d->last;
+        const DevTex *tx = bound_texture(d, dr->raster);
+        l.valid = true;
+        l.mode = dr->mode;
+        l.ntri = ntri;
+        l.n_vstates = dr->n_vstates;
+        l.color_array = dr->color.enabled != 0;
+        l.rs = *dr->raster;
+        l.vs = dr->vstates[0];
+        l.kind = raster_kind(dr->raster, tx);
+        l.gen = d->ex.gen;
+        l.over_base = d->one.tri_total;
+        l.tex_w = tx ? tx->w : 0;
+        l.tex_h = tx ? tx->h : 0;
+    }
     return rc;
 }
 
@@ -3798,4 +4186,247 @@ MTGL_API int mtgl_batch_stats_reset(mtgl_batch *b)
 {
     if (!b) return MTGL_ERR_INVALID;
     return exec_stats_reset(b->ex);
+}
+
+// ---- differentiable shading
+
+namespace {
+
+struct DiffCapture {
+    Arena gb, elems, pos, nrm, vs;
+    uint32_t ntri = 0;
+};
+
+// the capture conditions of include/mtgl_diff.h
+bool diff_capturable(const LastDraw &l, int32_t tex_w, int32_t tex_h)
+{
+    const mtgl_raster_state &rs = l.rs;
+    const mtgl_vertex_state &vs = l.vs;
+    if (!l.valid || l.mode != GL_TRIANGLES || l.n_vstates != 1 || !l.color_array) return false;
+    if ((l.kind & (KIND_LAST_WRITER | KIND_COLOR)) != (KIND_LAST_WRITER | KIND_COLOR) || (rs.color_mask & 0xFu) != 0xFu)
+        return false;
+    if (rs.flags & (MTGL_F_BLEND | MTGL_F_ALPHA_TEST | MTGL_F_FOG)) return false;
+    if (!rs.texture || l.tex_w != tex_w || l.tex_h != tex_h) return false;
+    if (rs.min_filter != GL_LINEAR || rs.mag_filter != GL_LINEAR || rs.tex_env_mode != GL_MODULATE) return false;
+    if (rs.shade_model == GL_FLAT || rs.shade_model == GL_PHONG || rs.lighting.two_side) return false;
+    if (!(vs.flags & MTGL_F_LIGHTING) || !(vs.flags & MTGL_F_COLOR_MATERIAL) || vs.shade_model == GL_PHONG) return false;
+    if (vs.color_material_face != GL_FRONT && vs.color_material_face != GL_FRONT_AND_BACK) return false;
+    const uint32_t m = vs.color_material_mode;
+    if (m != GL_AMBIENT && m != GL_DIFFUSE && m != GL_AMBIENT_AND_DIFFUSE) return false;
+    for (int i = 0; i < MTGL_MAX_LIGHTS; i++)
+        if (vs.lighting.lights[i].enabled && vs.lighting.lights[i].position[3] != 0.0f) return false;
+    return true;
+}
+
+}  // namespace
+
+struct mtgl_diff {
+    int device = 0;
+    uint32_t nvert = 0;
+    int32_t tex_w = 0, tex_h = 0, w = 0, h = 0;
+    std::vector<DiffCapture> views;
+    Arena table, lit, counter;
+    bool table_ok = false;
+};
+
+namespace {
+
+int diff_table(mtgl_diff *df, hipStream_t st)
+{
+    if (df->table_ok) return MTGL_OK;
+    std::vector<DiffView> t(df->views.size());
+    for (size_t i = 0; i < t.size(); i++) {
+        DiffCapture &c = df->views[i];
+        t[i].gb = as<GPx>(c.gb);
+        t[i].elems = as<uint32_t>(c.elems);
+        t[i].pos = as<float>(c.pos);
+        t[i].nrm = as<float>(c.nrm);
+        t[i].vs = as<mtgl_vertex_state>(c.vs);
+        t[i].ntri = c.ntri;
+        t[i].w = df->w;
+        t[i].h = df->h;
+        t[i].pad = 0;
+    }
+    if (df->table.cap < t.size() * sizeof(DiffView)) {
+        if (df->table.ptr) HIPC(hipFree(df->table.ptr));
+        df->table = Arena();
+        HIPC(hipMalloc(&df->table.ptr, std::max<size_t>(t.size() * sizeof(DiffView), 256)));
+        df->table.cap = std::max<size_t>(t.size() * sizeof(DiffView), 256);
+    }
+    HIPC(hipMemcpy(df->table.ptr, t.data(), t.size() * sizeof(DiffView), hipMemcpyHostToDevice));
+    const size_t lit_bytes = t.size() * (size_t)df->nvert * sizeof(float4);
+    if (df->lit.cap < lit_bytes) {
+        if (df->lit.ptr) HIPC(hipFree(df->lit.ptr));
+        df->lit = Arena();
+        HIPC(hipMalloc(&df->lit.ptr, lit_bytes));
+        df->lit.cap = lit_bytes;
+    }
+    (void)st;
+    df->table_ok = true;
+    return MTGL_OK;
+}
+
+int fill_zero(void *p, size_t floats, hipStream_t st)
+{
+    for (size_t off = 0; off < floats; off += 0x40000000u) {
+        const uint32_t n = (uint32_t)std::min<size_t>(floats - off, 0x40000000u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, (uint32_t *)p + off, n, 0u);
+    }
+    HIPC(hipGetLastError());
+    return MTGL_OK;
+}
+
+}  // namespace
+
+MTGL_API int mtgl_diff_create(uint32_t vertices, int32_t tex_w, int32_t tex_h, mtgl_diff **out)
+{
+    if (!out || vertices == 0 || tex_w <= 0 || tex_h <= 0 || tex_w > MTGL_MAX_TEX_SIZE || tex_h > MTGL_MAX_TEX_SIZE)
+        return MTGL_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MTGL_ERR_DEVICE;
+    mtgl_diff *df = new (std::nothrow) mtgl_diff();
+    if (!df) return MTGL_ERR_OUT_OF_MEMORY;
+    if (hipGetDevice(&df->device) != hipSuccess || hipMalloc(&df->counter.ptr, 256) != hipSuccess) {
+        delete df;
+        return MTGL_ERR_DEVICE;
+    }
+    df->counter.cap = 256;
+    df->nvert = vertices;
+    df->tex_w = tex_w;
+    df->tex_h = tex_h;
+    *out = df;
+    return MTGL_OK;
+}
+
+MTGL_API void mtgl_diff_destroy(mtgl_diff *df)
+{
+    if (!df) return;
+    (void)hipDeviceSynchronize();
+    for (DiffCapture &c : df->views) {
+        arena_free(c.gb);
+        arena_free(c.elems);
+        arena_free(c.pos);
+        arena_free(c.nrm);
+        arena_free(c.vs);
+    }
+    arena_free(df->table);
+    arena_free(df->lit);
+    arena_free(df->counter);
+    delete df;
+}
+
+MTGL_API int mtgl_diff_add_view(mtgl_diff *df, mtgl_dev *d)
+{
+    if (!df || !d) return MTGL_ERR_INVALID;
+    if (!diff_capturable(d->last, df->tex_w, df->tex_h)) return MTGL_ERR_UNSUPPORTED;
+    if (!df->views.empty() && (d->w != df->w || d->h != df->h)) return MTGL_ERR_INVALID;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    Exec &x = d->ex;
+    const LastDraw &l = d->last;
+    DiffCapture c;
+    c.ntri = l.ntri;
+    const size_t npx = (size_t)d->w * d->h;
+    do {
+        if ((rc = map_err(hipMalloc(&c.gb.ptr, npx * sizeof(GPx))))) break;
+        if ((rc = map_err(hipMalloc(&c.elems.ptr, std::max<size_t>(3u * (size_t)l.ntri, 1) * 4)))) break;
+        if ((rc = map_err(hipMalloc(&c.pos.ptr, 3u * (size_t)df->nvert * 4)))) break;
+        if ((rc = map_err(hipMalloc(&c.nrm.ptr, 3u * (size_t)df->nvert * 4)))) break;
+        if ((rc = map_err(hipMalloc(&c.vs.ptr, sizeof(mtgl_vertex_state))))) break;
+        if ((rc = map_err(hipMemcpyAsync(c.vs.ptr, &l.vs, sizeof(mtgl_vertex_state), hipMemcpyHostToDevice,
+                                         x.stream))))
+            break;
+        if ((rc = map_err(hipMemsetAsync(df->counter.ptr, 0, 4, x.stream)))) break;
+        hipLaunchKernelGGL(k_gbuffer, dim3(blocks_for(npx, 256)), dim3(256), 0, x.stream, d->one.sr,
+                           (const RTri *)as<RTri>(x.rt), (const uint32_t *)as<uint32_t>(d->writer),
+                           (const uint32_t *)as<uint32_t>(x.tile_gen), l.gen, l.over_base, as<GPx>(c.gb),
+                           as<uint32_t>(df->counter));
+        const uint64_t n = std::max<uint64_t>(3u * (uint64_t)l.ntri, df->nvert);
+        hipLaunchKernelGGL(k_diff_geom, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, d->one.vp, l.ntri,
+                           df->nvert, as<uint32_t>(c.elems), as<float>(c.pos), as<float>(c.nrm));
+        if ((rc = map_err(hipGetLastError()))) break;
+        uint32_t clipped = 0;
+        if ((rc = map_err(hipMemcpyAsync(&clipped, df->counter.ptr, 4, hipMemcpyDeviceToHost, x.stream)))) break;
+        if ((rc = map_err(hipStreamSynchronize(x.stream)))) break;
+        if (clipped) rc = MTGL_ERR_UNSUPPORTED;   // a triangle crossed a frustum plane
+    } while (0);
+    if (rc) {
+        arena_free(c.gb);
+        arena_free(c.elems);
+        arena_free(c.pos);
+        arena_free(c.nrm);
+        arena_free(c.vs);
+        return rc;
+    }
+    c.gb.cap = npx * sizeof(GPx);
+    df->w = d->w;
+    df->h = d->h;
+    df->views.push_back(c);
+    df->table_ok = false;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_diff_info(mtgl_diff *df, mtgl_diff_info_t *out)
+{
+    if (!df || !out) return MTGL_ERR_INVALID;
+    out->views = (uint32_t)df->views.size();
+    out->width = df->w;
+    out->height = df->h;
+    out->vertices = df->nvert;
+    out->tex_w = df->tex_w;
+    out->tex_h = df->tex_h;
+    out->work_floats = (uint64_t)df->views.size() * df->nvert * 4u;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_diff_gbuffer(mtgl_diff *df, uint32_t view, mtgl_gbuffer_px *host_out)
+{
+    if (!df || !host_out || view >= df->views.size()) return MTGL_ERR_INVALID;
+    HIPC(hipMemcpy(host_out, df->views[view].gb.ptr, (size_t)df->w * df->h * sizeof(GPx), hipMemcpyDeviceToHost));
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_diff_forward(mtgl_diff *df, const float *vcol, const float *light_rgb, const float *tex,
+                               float *images, void *stream)
+{
+    if (!df || !vcol || !light_rgb || !tex || !images || df->views.empty()) return MTGL_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = diff_table(df, st);
+    if (rc) return rc;
+    const uint32_t nv = (uint32_t)df->views.size(), npx = (uint32_t)df->w * (uint32_t)df->h;
+    hipLaunchKernelGGL(k_diff_lit, dim3(blocks_for(df->nvert, 256), nv), dim3(256), 0, st,
+                       (const DiffView *)df->table.ptr, df->nvert, vcol, light_rgb, as<float4>(df->lit));
+    hipLaunchKernelGGL(k_diff_shade, dim3(blocks_for(npx, 256), nv), dim3(256), 0, st,
+                       (const DiffView *)df->table.ptr, df->nvert, (const float4 *)as<float4>(df->lit),
+                       (const float4 *)tex, (float4 *)images);
+    HIPC(hipGetLastError());
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_diff_backward(mtgl_diff *df, const float *vcol, const float *light_rgb, const float *tex,
+                                const float *grad_images, float *g_vcol, float *g_light_rgb, float *g_tex,
+                                float *work, void *stream)
+{
+    if (!df || !vcol || !light_rgb || !tex || !grad_images || !g_vcol || !g_light_rgb || !g_tex || !work ||
+        df->views.empty())
+        return MTGL_ERR_INVALID;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = diff_table(df, st);
+    if (rc) return rc;
+    const uint32_t nv = (uint32_t)df->views.size(), npx = (uint32_t)df->w * (uint32_t)df->h;
+    const DiffView *tab = (const DiffView *)df->table.ptr;
+    // the lit colours of the current parameters (the forward's, recomputed)
+    hipLaunchKernelGGL(k_diff_lit, dim3(blocks_for(df->nvert, 256), nv), dim3(256), 0, st, tab, df->nvert, vcol,
+                       light_rgb, as<float4>(df->lit));
+    if ((rc = fill_zero(work, (size_t)nv * df->nvert * 4u, st))) return rc;
+    if ((rc = fill_zero(g_tex, (size_t)df->tex_w * df->tex_h * 4u, st))) return rc;
+    if ((rc = fill_zero(g_light_rgb, 3u * MTGL_MAX_LIGHTS, st))) return rc;
+    hipLaunchKernelGGL(k_diff_shade_bwd, dim3(blocks_for(npx, 256), nv), dim3(256), 0, st, tab, df->nvert,
+                       (const float4 *)as<float4>(df->lit), (const float4 *)tex, (const float4 *)grad_images, work,
+                       g_tex);
+    hipLaunchKernelGGL(k_diff_lit_bwd, dim3(blocks_for(df->nvert, 256)), dim3(256), 0, st, tab, nv, df->nvert, vcol,
+                       light_rgb, (const float *)work, g_vcol, g_light_rgb);
+    HIPC(hipGetLastError());
+    return MTGL_OK;
 }

@@ -31,6 +31,7 @@
 #include "GL/gl.h"
 #include "mtgl_batch.h"
 #include "mtgl_dev.h"
+#include "mtgl_diff.h"
 
 #define OR_MAX_CLIP 12   /* reference MAX_CLIP_VERTS, src/clipping.h:16 */
 
@@ -60,6 +61,14 @@ typedef struct {
     int64_t size;
 } or_buf;
 
+/* last colour-writing fragment of a pixel, kept for draws a differentiable
+ * capture accepts (stamp = the draw's) */
+typedef struct {
+    uint32_t stamp;
+    int32_t tri;            /* -2: the fragment's triangle crossed a frustum plane */
+    float b0, b1, b2, u, v;
+} or_gbrec;
+
 struct mtgl_dev {
     int32_t w, h;
     uint32_t *color;
@@ -68,6 +77,18 @@ struct mtgl_dev {
     or_tex tex[MTGL_MAX_OBJECTS + 1];
     or_buf buf[MTGL_MAX_OBJECTS + 1];
     mtgl_dev_stats stats;
+    /* differentiable capture of the latest draw */
+    int last_valid;
+    mtgl_draw last_dr;
+    mtgl_raster_state last_rs;
+    mtgl_vertex_state last_vs;
+    uint8_t *last_staged;
+    uint32_t last_ntri;
+    or_gbrec *gbr;
+    uint32_t gb_stamp;
+    int gb_on;
+    int32_t cur_tri, tri_counter;
+    int cur_clipped;
 };
 
 /* ------------------------------------------------------------------------- */
@@ -170,8 +191,11 @@ static void or_normal_matrix(const float *m, float *r)
 /* lighting (reference compute_lighting, src/lighting.h:53-142)               */
 /* ------------------------------------------------------------------------- */
 
-static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
-                       const mtgl_material *mat)
+/* compute_lighting before its final clamp; with dif_rgb the lights' diffuse
+ * rgb are taken from it (3 per light) and att * N.L of each light that adds
+ * diffuse light is stored in ndl_out (else 0) - the differentiable form */
+static or_col or_light_sum(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
+                           const mtgl_material *mat, const float *dif_rgb, float *ndl_out)
 {
     or_col result = or_add(or_from(mat->emission), or_mul(or_from(mat->ambient), or_from(L->model_ambient)));
     result.a = mat->diffuse.a;
@@ -179,6 +203,7 @@ static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_
     if (flags & MTGL_F_NORMALIZE) or_normalize3(N);
     for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
         const mtgl_light *lt = &L->lights[i];
+        if (ndl_out) ndl_out[i] = 0.0f;
         if (!lt->enabled) continue;
         float Ld[3];
         float att = 1.0f;
@@ -208,7 +233,10 @@ static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_
         result = or_add(result, or_scale(or_mul(or_from(mat->ambient), or_from(lt->ambient)), att));
         float ndl = or_dot3(N, Ld);
         if (ndl > 0.0f) {
-            result = or_add(result, or_scale(or_mul(or_from(mat->diffuse), or_from(lt->diffuse)), att * ndl));
+            or_col ld = or_from(lt->diffuse);
+            if (dif_rgb) { ld.r = dif_rgb[3 * i]; ld.g = dif_rgb[3 * i + 1]; ld.b = dif_rgb[3 * i + 2]; }
+            if (ndl_out) ndl_out[i] = att * ndl;
+            result = or_add(result, or_scale(or_mul(or_from(mat->diffuse), ld), att * ndl));
             if (mat->shininess > 0.0f) {
                 float V[3] = { 0.0f, 0.0f, 1.0f };
                 if (L->local_viewer) {
@@ -225,7 +253,13 @@ static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_
             }
         }
     }
-    return or_clamp(result);
+    return result;
+}
+
+static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
+                       const mtgl_material *mat)
+{
+    return or_clamp(or_light_sum(L, flags, eye_pos, eye_normal, mat, NULL, NULL));
 }
 
 /* ------------------------------------------------------------------------- */
@@ -747,8 +781,8 @@ static void or_raster(mtgl_dev *dev, const mtgl_raster_state *rs, const or_rv *v
                     c = or_light(&rs->lighting, rs->flags, ep, en, mat);
                 }
             }
+            float u = 0.0f, v = 0.0f;
             if (tex) {
-                float u, v;
                 if (persp) {
                     float uow = b0 * u0w + b1 * u1w + b2 * u2w;
                     float vow = b0 * v0w + b1 * v1w + b2 * v2w;
@@ -810,6 +844,13 @@ static void or_raster(mtgl_dev *dev, const mtgl_raster_state *rs, const or_rv *v
             c = or_clamp(c);
             if (cmask) dev->stats.frag_bytes += (blend_on || cmask != 0xF) ? 8 : 4;
             or_write_color(dev, cmask, x, y, c);
+            if (dev->gb_on) {
+                or_gbrec *g = &dev->gbr[(size_t)y * dev->w + x];
+                g->stamp = dev->gb_stamp;
+                g->tri = dev->cur_clipped ? -2 : dev->cur_tri;
+                g->b0 = b0; g->b1 = b1; g->b2 = b2;
+                g->u = u; g->v = v;
+            }
         }
     }
 }
@@ -825,6 +866,13 @@ static void or_render_triangle(mtgl_dev *dev, const mtgl_raster_state *rs, const
     or_vtx tri[3] = { *a, *b, *c };
     or_vtx cl[OR_MAX_CLIP];
     dev->stats.triangles_in++;
+    /* for a differentiable capture: the triangle's index in the draw, and
+     * whether any vertex lies outside a frustum plane */
+    dev->cur_tri = dev->tri_counter++;
+    dev->cur_clipped = 0;
+    for (int k = 0; k < 3; k++)
+        for (int pl = 0; pl < 6; pl++)
+            if (!(or_plane(&tri[k].pos, pl) >= 0.0f)) dev->cur_clipped = 1;
     int n = or_clip_triangle(tri, cl);
     if (n < 3) return;
     for (int j = 0; j < n; j++) {
@@ -887,6 +935,8 @@ int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
 void mtgl_dev_destroy(mtgl_dev *d)
 {
     if (!d) return;
+    free(d->gbr);
+    free(d->last_staged);
     for (int i = 0; i <= MTGL_MAX_OBJECTS; i++) {
         free(d->tex[i].px);
         free(d->tex[i].m1);
@@ -902,6 +952,7 @@ int mtgl_dev_finish(mtgl_dev *d) { return d ? MTGL_OK : MTGL_ERR_INVALID; }
 
 static int or_record_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
                            uint32_t color, float depth, uint8_t stencil);
+static void or_diff_prepare(mtgl_dev *d, const mtgl_draw *dr);
 static int or_record_draw(mtgl_dev *d, const mtgl_draw *dr);
 static __thread mtgl_batch *or_recording;
 
@@ -1044,6 +1095,7 @@ int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         return MTGL_ERR_UNSUPPORTED;
     }
     if (or_recording) return or_record_draw(d, dr);
+    or_diff_prepare(d, dr);
     d->stats.draws++;
     if (n == 0) return MTGL_OK;
     or_vtx *vt = (or_vtx *)malloc((size_t)n * sizeof(or_vtx));
@@ -1323,5 +1375,384 @@ int mtgl_batch_stats_reset(mtgl_batch *b)
 {
     if (!b) return MTGL_ERR_INVALID;
     memset(&b->stats, 0, sizeof(b->stats));
+    return MTGL_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* differentiable shading: the restatement of include/mtgl_diff.h, and the    */
+/* CPU reference gradient (adjoint accumulated in double)                     */
+/* ------------------------------------------------------------------------- */
+
+/* the capture conditions of include/mtgl_diff.h, from the draw's own state */
+static int or_diff_qualifies(const mtgl_dev *d, const mtgl_draw *dr)
+{
+    const mtgl_raster_state *rs = dr->raster;
+    const mtgl_vertex_state *vs = &dr->vstates[0];
+    if (dr->mode != GL_TRIANGLES || dr->n_vstates != 1 || !dr->color.enabled) return 0;
+    if ((rs->color_mask & 0xFu) != 0xFu) return 0;
+    if (rs->flags & (MTGL_F_BLEND | MTGL_F_ALPHA_TEST | MTGL_F_FOG)) return 0;
+    if (!rs->texture || rs->texture > MTGL_MAX_OBJECTS || !d->tex[rs->texture].px) return 0;
+    if (rs->min_filter != GL_LINEAR || rs->mag_filter != GL_LINEAR || rs->tex_env_mode != GL_MODULATE) return 0;
+    if (rs->shade_model == GL_FLAT || rs->shade_model == GL_PHONG || rs->lighting.two_side) return 0;
+    if (!(vs->flags & MTGL_F_LIGHTING) || !(vs->flags & MTGL_F_COLOR_MATERIAL) || vs->shade_model == GL_PHONG) return 0;
+    if (vs->color_material_face != GL_FRONT && vs->color_material_face != GL_FRONT_AND_BACK) return 0;
+    if (vs->color_material_mode != GL_AMBIENT && vs->color_material_mode != GL_DIFFUSE &&
+        vs->color_material_mode != GL_AMBIENT_AND_DIFFUSE)
+        return 0;
+    for (int i = 0; i < MTGL_MAX_LIGHTS; i++)
+        if (vs->lighting.lights[i].enabled && vs->lighting.lights[i].position[3] != 0.0f) return 0;
+    return 1;
+}
+
+/* before a draw: keep a copy of it and turn per-pixel recording on when a
+ * capture could take it */
+static void or_diff_prepare(mtgl_dev *d, const mtgl_draw *dr)
+{
+    d->tri_counter = 0;
+    d->last_valid = 0;
+    d->gb_on = 0;
+    if (!or_diff_qualifies(d, dr)) return;
+    if (!d->gbr) {
+        d->gbr = (or_gbrec *)calloc((size_t)d->w * d->h, sizeof(or_gbrec));
+        if (!d->gbr) return;
+    }
+    free(d->last_staged);
+    d->last_staged = NULL;
+    if (dr->staged_bytes) {
+        d->last_staged = (uint8_t *)malloc((size_t)dr->staged_bytes);
+        if (!d->last_staged) return;
+        memcpy(d->last_staged, dr->staged, (size_t)dr->staged_bytes);
+    }
+    d->last_dr = *dr;
+    d->last_rs = *dr->raster;
+    d->last_vs = dr->vstates[0];
+    d->last_dr.raster = &d->last_rs;
+    d->last_dr.vstates = &d->last_vs;
+    d->last_dr.staged = d->last_staged;
+    d->last_ntri = dr->count / 3;
+    if (++d->gb_stamp == 0) {
+        memset(d->gbr, 0, (size_t)d->w * d->h * sizeof(or_gbrec));
+        d->gb_stamp = 1;
+    }
+    d->gb_on = 1;
+    d->last_valid = 1;
+}
+
+typedef struct {
+    mtgl_gbuffer_px *gb;
+    uint32_t *elems;
+    float *pos, *nrm;
+    mtgl_vertex_state vs;
+    uint32_t ntri;
+} or_dview;
+
+struct mtgl_diff {
+    uint32_t nvert;
+    int32_t tex_w, tex_h, w, h;
+    uint32_t nviews;
+    or_dview *views;
+};
+
+int mtgl_diff_create(uint32_t vertices, int32_t tex_w, int32_t tex_h, mtgl_diff **out)
+{
+    if (!out || vertices == 0 || tex_w <= 0 || tex_h <= 0 || tex_w > MTGL_MAX_TEX_SIZE || tex_h > MTGL_MAX_TEX_SIZE)
+        return MTGL_ERR_INVALID;
+    mtgl_diff *df = (mtgl_diff *)calloc(1, sizeof(mtgl_diff));
+    if (!df) return MTGL_ERR_OUT_OF_MEMORY;
+    df->nvert = vertices;
+    df->tex_w = tex_w;
+    df->tex_h = tex_h;
+    *out = df;
+    return MTGL_OK;
+}
+
+void mtgl_diff_destroy(mtgl_diff *df)
+{
+    if (!df) return;
+    for (uint32_t i = 0; i < df->nviews; i++) {
+        free(df->views[i].gb);
+        free(df->views[i].elems);
+        free(df->views[i].pos);
+        free(df->views[i].nrm);
+    }
+    free(df->views);
+    free(df);
+}
+
+int mtgl_diff_add_view(mtgl_diff *df, mtgl_dev *d)
+{
+    if (!df || !d) return MTGL_ERR_INVALID;
+    if (!d->last_valid || !d->gbr) return MTGL_ERR_UNSUPPORTED;
+    const or_tex *t = &d->tex[d->last_rs.texture];
+    if (!t->px || t->w != df->tex_w || t->h != df->tex_h) return MTGL_ERR_UNSUPPORTED;
+    if (df->nviews && (d->w != df->w || d->h != df->h)) return MTGL_ERR_INVALID;
+    const size_t npx = (size_t)d->w * d->h;
+    or_dview v;
+    memset(&v, 0, sizeof(v));
+    v.ntri = d->last_ntri;
+    v.vs = d->last_vs;
+    v.gb = (mtgl_gbuffer_px *)calloc(npx, sizeof(mtgl_gbuffer_px));
+    v.elems = (uint32_t *)malloc((3 * (size_t)v.ntri + 1) * sizeof(uint32_t));
+    v.pos = (float *)malloc(3 * (size_t)df->nvert * sizeof(float));
+    v.nrm = (float *)malloc(3 * (size_t)df->nvert * sizeof(float));
+    or_dview *nv = (or_dview *)realloc(df->views, (df->nviews + 1) * sizeof(or_dview));
+    if (!v.gb || !v.elems || !v.pos || !v.nrm || !nv) {
+        free(v.gb); free(v.elems); free(v.pos); free(v.nrm);
+        if (nv) df->views = nv;
+        return MTGL_ERR_OUT_OF_MEMORY;
+    }
+    df->views = nv;
+    const mtgl_raster_state *rs = &d->last_rs;
+    for (size_t p = 0; p < npx; p++) {
+        const or_gbrec *r = &d->gbr[p];
+        mtgl_gbuffer_px *g = &v.gb[p];
+        if (r->stamp != d->gb_stamp) {
+            g->tri = -1;
+            g->texel[0] = d->color[p];
+            continue;
+        }
+        if (r->tri == -2) {
+            free(v.gb); free(v.elems); free(v.pos); free(v.nrm);
+            return MTGL_ERR_UNSUPPORTED;
+        }
+        g->tri = r->tri;
+        g->b[0] = r->b0; g->b[1] = r->b1; g->b[2] = r->b2;
+        /* the wrap of or_sample and the bilinear taps of or_sample_level */
+        float u = r->u, vv = r->v;
+        if (rs->wrap_s == GL_REPEAT) { u = u - (float)or_cvtt(u); if (u < 0) u += 1.0f; }
+        else { if (u < 0.0f) u = 0.0f; if (u > 1.0f) u = 1.0f; }
+        if (rs->wrap_t == GL_REPEAT) { vv = vv - (float)or_cvtt(vv); if (vv < 0) vv += 1.0f; }
+        else { if (vv < 0.0f) vv = 0.0f; if (vv > 1.0f) vv = 1.0f; }
+        float tx = u * t->w - 0.5f, ty = vv * t->h - 0.5f;
+        int32_t x0 = or_cvtt(floorf(tx)), y0 = or_cvtt(floorf(ty));
+        g->fx = tx - x0;
+        g->fy = ty - y0;
+        int32_t xs[2] = { x0, x0 + 1 }, ys[2] = { y0, y0 + 1 };
+        for (int k = 0; k < 2; k++) {
+            if (rs->wrap_s == GL_REPEAT) xs[k] = ((xs[k] % t->w) + t->w) % t->w;
+            else { if (xs[k] < 0) xs[k] = 0; else if (xs[k] >= t->w) xs[k] = t->w - 1; }
+            if (rs->wrap_t == GL_REPEAT) ys[k] = ((ys[k] % t->h) + t->h) % t->h;
+            else { if (ys[k] < 0) ys[k] = 0; else if (ys[k] >= t->h) ys[k] = t->h - 1; }
+        }
+        g->texel[0] = (uint32_t)(ys[0] * t->w + xs[0]);
+        g->texel[1] = (uint32_t)(ys[0] * t->w + xs[1]);
+        g->texel[2] = (uint32_t)(ys[1] * t->w + xs[0]);
+        g->texel[3] = (uint32_t)(ys[1] * t->w + xs[1]);
+    }
+    for (uint32_t i = 0; i < 3 * v.ntri; i++) v.elems[i] = (uint32_t)or_index(d, &d->last_dr, i);
+    for (uint32_t e = 0; e < df->nvert; e++) {
+        float p4[4], n3[3];
+        or_fetch(d, &d->last_dr, &d->last_dr.position, (int32_t)e, p4, 4);
+        if (d->last_dr.position.size == 2) p4[2] = 0.0f;
+        if (d->last_dr.normal.enabled) or_fetch(d, &d->last_dr, &d->last_dr.normal, (int32_t)e, n3, 3);
+        else memcpy(n3, d->last_dr.cur_normal, sizeof(n3));
+        memcpy(v.pos + 3 * (size_t)e, p4, 3 * sizeof(float));
+        memcpy(v.nrm + 3 * (size_t)e, n3, 3 * sizeof(float));
+    }
+    df->views[df->nviews++] = v;
+    df->w = d->w;
+    df->h = d->h;
+    return MTGL_OK;
+}
+
+int mtgl_diff_info(mtgl_diff *df, mtgl_diff_info_t *out)
+{
+    if (!df || !out) return MTGL_ERR_INVALID;
+    out->views = df->nviews;
+    out->width = df->w;
+    out->height = df->h;
+    out->vertices = df->nvert;
+    out->tex_w = df->tex_w;
+    out->tex_h = df->tex_h;
+    out->work_floats = (uint64_t)df->nviews * df->nvert * 4u;
+    return MTGL_OK;
+}
+
+int mtgl_diff_gbuffer(mtgl_diff *df, uint32_t view, mtgl_gbuffer_px *out)
+{
+    if (!df || !out || view >= df->nviews) return MTGL_ERR_INVALID;
+    memcpy(out, df->views[view].gb, (size_t)df->w * df->h * sizeof(mtgl_gbuffer_px));
+    return MTGL_OK;
+}
+
+/* eye-space vertex, sanitised colour and material as or_emit_vertex forms them */
+typedef struct {
+    float ep[3], en[3];
+    or_col vc;
+    mtgl_material mat;
+} or_dvtx;
+
+static void or_diff_vertex(const or_dview *v, uint32_t e, const float *vcol, or_dvtx *o)
+{
+    const mtgl_vertex_state *vs = &v->vs;
+    const float *pp = v->pos + 3 * (size_t)e, *nn = v->nrm + 3 * (size_t)e;
+    or_v4 eye = or_xform(vs->modelview, (or_v4){ pp[0], pp[1], pp[2], 1.0f });
+    o->ep[0] = eye.x; o->ep[1] = eye.y; o->ep[2] = eye.z;
+    float nm[16];
+    or_normal_matrix(vs->modelview, nm);
+    or_v4 n4 = or_xform(nm, (or_v4){ nn[0], nn[1], nn[2], 0.0f });
+    o->en[0] = n4.x; o->en[1] = n4.y; o->en[2] = n4.z;
+    or_normalize3(o->en);
+    const float *c = vcol + 4 * (size_t)e;
+    o->vc = (or_col){ or_color_component(c[0], 0.0f), or_color_component(c[1], 0.0f), or_color_component(c[2], 0.0f),
+                      or_color_component(c[3], 1.0f) };
+    o->mat = vs->lighting.front;
+    or_col k = or_clamp(o->vc);
+    mtgl_color kk = { k.r, k.g, k.b, k.a };
+    if (vs->color_material_mode == GL_AMBIENT || vs->color_material_mode == GL_AMBIENT_AND_DIFFUSE) o->mat.ambient = kk;
+    if (vs->color_material_mode == GL_DIFFUSE || vs->color_material_mode == GL_AMBIENT_AND_DIFFUSE) o->mat.diffuse = kk;
+}
+
+static or_col or_diff_lit(const or_dview *v, uint32_t e, const float *vcol, const float *light_rgb)
+{
+    or_dvtx q;
+    or_diff_vertex(v, e, vcol, &q);
+    return or_clamp(or_light_sum(&v->vs.lighting, v->vs.flags, q.ep, q.en, &q.mat, light_rgb, NULL));
+}
+
+static or_col or_tex4(const float *tex, uint32_t i)
+{
+    const float *t = tex + 4 * (size_t)i;
+    or_col c = { t[0], t[1], t[2], t[3] };
+    return c;
+}
+
+static void or_diff_pixel(const mtgl_gbuffer_px *g, const uint32_t *el, const or_col *lit, uint32_t nvert,
+                          const float *tex, or_col *c, or_col *tc)
+{
+    or_col l[3];
+    for (int k = 0; k < 3; k++) {
+        or_col z = { 0.0f, 0.0f, 0.0f, 0.0f };
+        l[k] = el[k] < nvert ? lit[el[k]] : z;
+    }
+    float b0 = g->b[0], b1 = g->b[1], b2 = g->b[2];
+    c->r = l[0].r * b0 + l[1].r * b1 + l[2].r * b2;
+    c->g = l[0].g * b0 + l[1].g * b1 + l[2].g * b2;
+    c->b = l[0].b * b0 + l[1].b * b1 + l[2].b * b2;
+    c->a = l[0].a * b0 + l[1].a * b1 + l[2].a * b2;
+    or_col top = or_lerp(or_tex4(tex, g->texel[0]), or_tex4(tex, g->texel[1]), g->fx);
+    or_col bot = or_lerp(or_tex4(tex, g->texel[2]), or_tex4(tex, g->texel[3]), g->fx);
+    *tc = or_lerp(top, bot, g->fy);
+}
+
+int mtgl_diff_forward(mtgl_diff *df, const float *vcol, const float *light_rgb, const float *tex, float *images,
+                      void *stream)
+{
+    (void)stream;
+    if (!df || !vcol || !light_rgb || !tex || !images || !df->nviews) return MTGL_ERR_INVALID;
+    or_col *lit = (or_col *)malloc((size_t)df->nvert * sizeof(or_col));
+    if (!lit) return MTGL_ERR_OUT_OF_MEMORY;
+    const size_t npx = (size_t)df->w * df->h;
+    for (uint32_t vi = 0; vi < df->nviews; vi++) {
+        const or_dview *v = &df->views[vi];
+        for (uint32_t e = 0; e < df->nvert; e++) lit[e] = or_diff_lit(v, e, vcol, light_rgb);
+        float *img = images + 4 * npx * vi;
+        for (size_t p = 0; p < npx; p++) {
+            const mtgl_gbuffer_px *g = &v->gb[p];
+            or_col o;
+            if (g->tri < 0) {
+                o = or_unpack(g->texel[0]);
+            } else {
+                or_col c, tc;
+                or_diff_pixel(g, v->elems + 3 * (size_t)g->tri, lit, df->nvert, tex, &c, &tc);
+                o = or_clamp(or_mul(c, tc));
+            }
+            img[4 * p] = o.r; img[4 * p + 1] = o.g; img[4 * p + 2] = o.b; img[4 * p + 3] = o.a;
+        }
+    }
+    free(lit);
+    return MTGL_OK;
+}
+
+static float or_clamp_grad(float v, float g) { return (v >= 0.0f && v <= 1.0f) ? g : 0.0f; }
+
+int mtgl_diff_backward(mtgl_diff *df, const float *vcol, const float *light_rgb, const float *tex,
+                       const float *grad_images, float *g_vcol, float *g_light_rgb, float *g_tex, float *work,
+                       void *stream)
+{
+    (void)stream;
+    (void)work;
+    if (!df || !vcol || !light_rgb || !tex || !grad_images || !g_vcol || !g_light_rgb || !g_tex || !df->nviews)
+        return MTGL_ERR_INVALID;
+    const size_t npx = (size_t)df->w * df->h, ntex = (size_t)df->tex_w * df->tex_h;
+    or_col *lit = (or_col *)malloc((size_t)df->nvert * sizeof(or_col));
+    double *dlit = (double *)malloc((size_t)df->nvert * 4 * sizeof(double));
+    double *gt = (double *)calloc(ntex * 4, sizeof(double));
+    double *gv = (double *)calloc((size_t)df->nvert * 4, sizeof(double));
+    double gl[3 * MTGL_MAX_LIGHTS] = { 0 };
+    if (!lit || !dlit || !gt || !gv) {
+        free(lit); free(dlit); free(gt); free(gv);
+        return MTGL_ERR_OUT_OF_MEMORY;
+    }
+    for (uint32_t vi = 0; vi < df->nviews; vi++) {
+        const or_dview *v = &df->views[vi];
+        for (uint32_t e = 0; e < df->nvert; e++) lit[e] = or_diff_lit(v, e, vcol, light_rgb);
+        memset(dlit, 0, (size_t)df->nvert * 4 * sizeof(double));
+        const float *gimg = grad_images + 4 * npx * vi;
+        /* adjoint of the pixel: clamp, modulate, colour weights, bilinear taps */
+        for (size_t p = 0; p < npx; p++) {
+            const mtgl_gbuffer_px *g = &v->gb[p];
+            if (g->tri < 0) continue;
+            const uint32_t *el = v->elems + 3 * (size_t)g->tri;
+            or_col c, tc;
+            or_diff_pixel(g, el, lit, df->nvert, tex, &c, &tc);
+            or_col prod = or_mul(c, tc);
+            or_col gm = { or_clamp_grad(prod.r, gimg[4 * p]), or_clamp_grad(prod.g, gimg[4 * p + 1]),
+                          or_clamp_grad(prod.b, gimg[4 * p + 2]), or_clamp_grad(prod.a, gimg[4 * p + 3]) };
+            or_col dc = or_mul(gm, tc), dt = or_mul(gm, c);
+            for (int k = 0; k < 3; k++) {
+                if (el[k] >= df->nvert) continue;
+                double *d = dlit + 4 * (size_t)el[k];
+                float b = g->b[k];
+                d[0] += dc.r * b; d[1] += dc.g * b; d[2] += dc.b * b; d[3] += dc.a * b;
+            }
+            float fx = g->fx, fy = g->fy;
+            float wt[4] = { (1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy };
+            for (int k = 0; k < 4; k++) {
+                double *d = gt + 4 * (size_t)g->texel[k];
+                d[0] += dt.r * wt[k]; d[1] += dt.g * wt[k]; d[2] += dt.b * wt[k]; d[3] += dt.a * wt[k];
+            }
+        }
+        /* adjoint of the lighting: clamp, material colour = clamp(vertex colour), light diffuse */
+        const mtgl_lighting *L = &v->vs.lighting;
+        const uint32_t mode = v->vs.color_material_mode;
+        const int amb_c = mode == GL_AMBIENT || mode == GL_AMBIENT_AND_DIFFUSE;
+        const int dif_c = mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE;
+        for (uint32_t e = 0; e < df->nvert; e++) {
+            const double *dl = dlit + 4 * (size_t)e;
+            if (dl[0] == 0.0 && dl[1] == 0.0 && dl[2] == 0.0 && dl[3] == 0.0) continue;
+            or_dvtx q;
+            or_diff_vertex(v, e, vcol, &q);
+            float ndl[MTGL_MAX_LIGHTS];
+            or_col r = or_light_sum(L, v->vs.flags, q.ep, q.en, &q.mat, light_rgb, ndl);
+            double g[4] = { or_clamp_grad(r.r, 1.0f) * dl[0], or_clamp_grad(r.g, 1.0f) * dl[1],
+                            or_clamp_grad(r.b, 1.0f) * dl[2], or_clamp_grad(r.a, 1.0f) * dl[3] };
+            double dk[4] = { 0.0, 0.0, 0.0, dif_c ? 1.0 : 0.0 };
+            if (amb_c) { dk[0] += L->model_ambient.r; dk[1] += L->model_ambient.g; dk[2] += L->model_ambient.b; }
+            for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
+                const mtgl_light *lt = &L->lights[i];
+                if (!lt->enabled) continue;
+                if (amb_c) { dk[0] += lt->ambient.r; dk[1] += lt->ambient.g; dk[2] += lt->ambient.b; dk[3] += lt->ambient.a; }
+                if (ndl[i] > 0.0f) {
+                    if (dif_c) {
+                        dk[0] += (double)light_rgb[3 * i] * ndl[i];
+                        dk[1] += (double)light_rgb[3 * i + 1] * ndl[i];
+                        dk[2] += (double)light_rgb[3 * i + 2] * ndl[i];
+                        dk[3] += (double)lt->diffuse.a * ndl[i];
+                    }
+                    gl[3 * i] += g[0] * q.mat.diffuse.r * ndl[i];
+                    gl[3 * i + 1] += g[1] * q.mat.diffuse.g * ndl[i];
+                    gl[3 * i + 2] += g[2] * q.mat.diffuse.b * ndl[i];
+                }
+            }
+            const float vc[4] = { q.vc.r, q.vc.g, q.vc.b, q.vc.a };
+            for (int ch = 0; ch < 4; ch++)
+                if (vc[ch] >= 0.0f && vc[ch] <= 1.0f) gv[4 * (size_t)e + ch] += g[ch] * dk[ch];
+        }
+    }
+    for (size_t i = 0; i < ntex * 4; i++) g_tex[i] = (float)gt[i];
+    for (size_t i = 0; i < (size_t)df->nvert * 4; i++) g_vcol[i] = (float)gv[i];
+    for (int i = 0; i < 3 * MTGL_MAX_LIGHTS; i++) g_light_rgb[i] = (float)gl[i];
+    free(lit); free(dlit); free(gt); free(gv);
     return MTGL_OK;
 }
