@@ -59,7 +59,8 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-constexpr uint32_t HEAVY_BIN = 96;      // default: tiles with longer bins are rasterised first
+constexpr uint32_t HEAVY_BIN = 96;      // default: tiles with longer bins are rasterised first, in quarters
+constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | quarter << 28 (0: whole tile, 1..4)
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
@@ -216,8 +217,9 @@ struct RasterParams {
     const RTri *rt;
     const RCold *rcold;
     const uint32_t *vals;          // sorted pair values (record slot)
-    const uint32_t *list;          // non-empty tiles: nheavy from the front, nlight from the back
+    const uint32_t *list;          // work items: nheavy from the front, nlight from the back
     uint32_t ntiles;
+    uint32_t list_cap;             // items the list holds (4 per tile)
     uint32_t part;                 // list items of this launch: 0 all, 1 the heavy ones, 2 the others
     unsigned long long *stats;     // per block: fragments, algorithmic bytes (summed when read)
     const float4 *cov;             // first 64 record bytes per pair, in pair order
@@ -1171,13 +1173,14 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
     }
 }
 
-// The non-empty tiles of the launch: those whose bins hold more than heavy_bin
-// entries from the front of `list`, the others from its back, so that the
-// raster kernels, which walk the list from the front, start the longest tiles
+// The work items of the launch: four quarters (two pixel rows each) of every
+// tile whose bin holds more than heavy_bin entries, from the front of
+// `list`, and every other non-empty tile whole, from its back, so that the
+// raster kernels, which walk the list from the front, start the longest work
 // first.  One thread per tile; one atomic per block and list end.
 __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
                                                    const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
-                                                   uint32_t heavy_bin, uint32_t *list, Counters *ctr)
+                                                   uint32_t heavy_bin, uint32_t *list, uint32_t list_cap, Counters *ctr)
 {
     __shared__ uint32_t s_n[2][4];
     __shared__ uint32_t s_base[2];
@@ -1201,18 +1204,19 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
             th += a;
             tl += c;
         }
-        s_base[0] = th ? atomicAdd(&ctr->nheavy, th) : 0u;
+        s_base[0] = th ? atomicAdd(&ctr->nheavy, 4u * th) : 0u;
         s_base[1] = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
     if (hv) {
-        const uint32_t o = s_base[0] + s_n[0][wave] + (uint32_t)__popcll(bh & below);
-        if (o < ntiles) list[o] = t;
+        const uint32_t o = s_base[0] + 4u * (s_n[0][wave] + (uint32_t)__popcll(bh & below));
+        for (uint32_t q = 0; q < 4u; q++)
+            if (o + q < list_cap) list[o + q] = t | ((q + 1u) << ITEM_TILE_BITS);
     }
     if (lt) {
         const uint32_t o = s_base[1] + s_n[1][wave] + (uint32_t)__popcll(bl & below);
-        if (o < ntiles) list[ntiles - 1u - o] = t;
+        if (o < list_cap) list[list_cap - 1u - o] = t;
     }
 }
 
@@ -1762,7 +1766,7 @@ __device__ __forceinline__ RCov make_cov(const float4 *q)
 // fold in as -inf / +inf.
 __device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, int32_t minY, int32_t maxX, int32_t maxY,
                                                 float area, bool valid, int32_t X0, int32_t Y0, int32_t fb_w,
-                                                int32_t fb_h)
+                                                int32_t fb_h, int32_t r0, int32_t r1)
 {
     const float fx0 = cv.fx0, fy0 = cv.fy0, fx1 = cv.fx1, fy1 = cv.fy1, fx2 = cv.fx2, fy2 = cv.fy2;
     const float d0y = cv.d0y, d0x = cv.d0x, d1y = cv.d1y, d1x = cv.d1x, d2y = cv.d2y, d2x = cv.d2x;
@@ -1782,15 +1786,16 @@ __device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, in
         if (!(live && xi >= minX && xi <= maxX && xi < fb_w)) a0[i] = -__builtin_inff();
         if (!(yi >= minY && yi <= maxY && yi < fb_h)) b0[i] = __builtin_inff();
     }
+    // only the rows [r0, r1) of the item (a quarter of a heavy tile, or all)
     int32_t mlo = 0, mhi = 0;
-    cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<2>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<3>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<4>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<5>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<6>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    cover_row<7>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 0 && 0 < r1) cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 1 && 1 < r1) cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 2 && 2 < r1) cover_row<2>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 3 && 3 < r1) cover_row<3>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 4 && 4 < r1) cover_row<4>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 5 && 5 < r1) cover_row<5>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 6 && 6 < r1) cover_row<6>(a0, a1, a2, b0, b1, b2, mlo, mhi);
+    if (r0 <= 7 && 7 < r1) cover_row<7>(a0, a1, a2, b0, b1, b2, mlo, mhi);
     return ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
 }
 
@@ -1819,26 +1824,40 @@ __device__ __forceinline__ uint64_t depth_pass(const RasterConsts &K, const RCov
     return pass;
 }
 
-// The raster kernels run a persistent grid over the list of non-empty
-// tiles: block b takes items b, b + gridDim.x, ...; items [0, nheavy) are the
-// tiles with bins over the heavy-bin threshold (the front of the list), the rest
-// come from the back, so the longest tiles start first.
+// The raster kernels run a persistent grid over a list of work items: items
+// [0, nheavy) are quarters (two pixel rows) of the tiles whose bins hold more
+// than the heavy-bin threshold, four per tile, at the front of the list; the
+// other items are whole non-empty tiles, from the back of the list, so the
+// longest items start first.  Pass k of block b takes item k * gridDim.x + b
+// (even k) or k * gridDim.x + gridDim.x - 1 - b (odd k).  Grids hold up to
+// three times the resident blocks, so while items last the dispatcher hands
+// each freed slot the next block: a slot busy with a long item takes no
+// other.
 __device__ __forceinline__ void list_counts(const RasterParams &P, uint32_t &nh, uint32_t &it0, uint32_t &nt)
 {
     nh = __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nheavy);
     nt = nh + __builtin_amdgcn_readfirstlane(*(volatile uint32_t *)&P.ctr->nlight);
-    nt = min(nt, P.ntiles);
+    nt = min(nt, P.list_cap);
     nh = min(nh, nt);
     it0 = P.part == 2u ? nh : 0u;
     if (P.part == 1u) nt = nh;
 }
 
-// list item -> scene, the scene-local tile and the tile's bin [e0, e1)
-__device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &sc, uint32_t &tile,
-                                          uint32_t &e0, uint32_t &e1)
+__device__ __forceinline__ uint32_t list_item(uint32_t it0, uint32_t k, uint32_t b, uint32_t G)
 {
-    uint32_t g = it < nh ? P.list[it] : P.list[P.ntiles - 1u - (it - nh)];
+    return it0 + k * G + ((k & 1u) ? G - 1u - b : b);
+}
+
+
+// list item -> scene, the scene-local tile, the quarter (0 whole tile, q = 1..4
+// pixel rows 2q-2, 2q-1) and the tile's bin [e0, e1)
+__device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &sc, uint32_t &tile,
+                                          uint32_t &e0, uint32_t &e1, uint32_t &quarter)
+{
+    uint32_t g = it < nh ? P.list[it] : P.list[P.list_cap - 1u - (it - nh)];
     g = __builtin_amdgcn_readfirstlane(g);
+    quarter = g >> ITEM_TILE_BITS;
+    g &= (1u << ITEM_TILE_BITS) - 1u;
     e0 = __builtin_amdgcn_readfirstlane(P.tile_start[g]);
     e1 = __builtin_amdgcn_readfirstlane(P.tile_end[g]);
     sc = scene_of(P.tile_scene, P.nscenes, g);
@@ -1906,7 +1925,7 @@ __device__ __forceinline__ void add_stats(const RasterParams &P, unsigned long l
     const uint32_t f = dpp_wave_sum((uint32_t)frags);
     const unsigned long long blo = dpp_wave_sum((uint32_t)(bytes & 0xFFFFFFFFull));
     const unsigned long long bhi = dpp_wave_sum((uint32_t)(bytes >> 32));
-    if (threadIdx.x == 0 && f) {
+    if (threadIdx.x == 0 && f && blockIdx.x < STAT_SLOTS) {
         unsigned long long *slot = P.stats + 2u * blockIdx.x;
         slot[0] += (unsigned long long)f;
         slot[1] += blo + (bhi << 32);
@@ -1951,12 +1970,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     list_counts(P, nh, it0, nt);
     uint32_t frag_count = 0;
     unsigned long long frag_bytes = 0;
-    for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
-    uint32_t sc, tile, e0, e1;
-    item_tile(P, it, nh, sc, tile, e0, e1);
+    for (uint32_t pass = 0;; pass++) {
+    const uint32_t it = list_item(it0, pass, blockIdx.x, gridDim.x);
+    if (it >= nt) break;
+    uint32_t sc, tile, e0, e1, quarter;
+    item_tile(P, it, nh, sc, tile, e0, e1, quarter);
     const SceneRaster &S = P.scenes[sc];
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (!(kind & KIND_LAST_WRITER) || ((kind & KIND_STENCIL) != 0u) != STENCIL) continue;
+    const int32_t r0 = quarter ? 2 * (int32_t)quarter - 2 : 0, r1 = quarter ? r0 + 2 : TILE;
     if (sc != cur) {
         raster_consts(S, K);
         K.stencil_on = STENCIL;   // the instantiation matches the scene's state
@@ -1965,7 +1987,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const int32_t X0 = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE;
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
     const int32_t fb_w = S.fb_w, fb_h = S.fb_h;
-    const bool in_fb = x < fb_w && y < fb_h;
+    const bool in_fb = (lane >> 3) >= r0 && (lane >> 3) < r1 && x < fb_w && y < fb_h;   // this item's pixel
     const size_t pix = in_fb ? (size_t)y * fb_w + x : 0;
     PixelState px = {0u, 1.0f, 0u, false, false, false};
     if (in_fb) {
@@ -2006,7 +2028,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
         const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
-                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h);
+                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1);
         const uint32_t cnt = (uint32_t)__popcll(m);
         frag_count += cnt;
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int32_t)dpp_incl_scan(cnt), 63);
@@ -2088,9 +2110,11 @@ __global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
     const int lane = threadIdx.x;
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
-    for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
-    uint32_t sc, tile, e0, e1;
-    item_tile(P, it, nh, sc, tile, e0, e1);
+    for (uint32_t pass = 0;; pass++) {
+    const uint32_t it = list_item(it0, pass, blockIdx.x, gridDim.x);
+    if (it >= nt) break;
+    uint32_t sc, tile, e0, e1, quarter;
+    item_tile(P, it, nh, sc, tile, e0, e1, quarter);
     const SceneRaster &S = P.scenes[sc];
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if ((kind & (KIND_LAST_WRITER | KIND_COLOR)) != (KIND_LAST_WRITER | KIND_COLOR)) continue;
@@ -2098,6 +2122,8 @@ __global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
         raster_consts(S, K);
         cur = sc;
     }
+    const int32_t r0 = quarter ? 2 * (int32_t)quarter - 2 : 0, r1 = quarter ? r0 + 2 : TILE;
+    if ((lane >> 3) < r0 || (lane >> 3) >= r1) continue;   // not this item's pixel
     const int32_t x = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE + (lane & 7);
     const int32_t y = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE + (lane >> 3);
     if (x >= S.fb_w || y >= S.fb_h) continue;
@@ -2141,12 +2167,15 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
     unsigned long long frag_count = 0, frag_bytes = 0;
-    for (uint32_t it = it0 + blockIdx.x; it < nt; it += gridDim.x) {
-    uint32_t sc, tile, e0, e1;
-    item_tile(P, it, nh, sc, tile, e0, e1);
+    for (uint32_t pass = 0;; pass++) {
+    const uint32_t it = list_item(it0, pass, blockIdx.x, gridDim.x);
+    if (it >= nt) break;
+    uint32_t sc, tile, e0, e1, quarter;
+    item_tile(P, it, nh, sc, tile, e0, e1, quarter);
     const SceneRaster &S = P.scenes[sc];
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (kind & KIND_LAST_WRITER) continue;
+    const int32_t r0 = quarter ? 2 * (int32_t)quarter - 2 : 0, r1 = quarter ? r0 + 2 : TILE;
     if (sc != cur) {
         raster_consts(S, K);
         need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
@@ -2158,7 +2187,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
     const float xf = (float)x, yf = (float)y;
     const int32_t fb_w = S.fb_w, fb_h = S.fb_h;
-    const bool in_fb = x < fb_w && y < fb_h;
+    const bool in_fb = (lane >> 3) >= r0 && (lane >> 3) < r1 && x < fb_w && y < fb_h;   // this item's pixel
     const size_t pix = in_fb ? (size_t)y * fb_w + x : 0;
     PixelState px = {0u, 1.0f, 0u, false, false, false};
     if (in_fb) {
@@ -2190,7 +2219,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
             s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
         const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
-                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h);
+                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1);
         frag_count += (unsigned long long)__popcll(m);
         wave_sync();   // staged records visible to every lane
 
@@ -3113,7 +3142,8 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         vb += blocks_for(p.s->dr->count, 256);
         sb += blocks_for(p.ntri, 256);
     }
-    if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles > 0x7FFFFFFFull || vb > 0x7FFFFFFFull)
+    if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / 4u ||
+        vb > 0x7FFFFFFFull)
         return MTGL_ERR_OUT_OF_MEMORY;
     L.tri_total = (uint32_t)z.tri;
     L.tiles_total = (uint32_t)z.tiles;
@@ -3308,7 +3338,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     const uint32_t ntiles = L.tiles_total;
     if ((rc = arena_reserve(x, x.tile_start, (size_t)ntiles * 4, frozen))) return rc;
     if ((rc = arena_reserve(x, x.tile_end, (size_t)ntiles * 4, frozen))) return rc;
-    if ((rc = arena_reserve(x, x.tile_list, (size_t)ntiles * 4, frozen))) return rc;
+    if ((rc = arena_reserve(x, x.tile_list, (size_t)ntiles * 16, frozen))) return rc;   // up to 4 items per tile
     if (!frozen) {
         if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
             if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
@@ -3355,7 +3385,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
                        (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin,
-                       as<uint32_t>(x.tile_list), x.ctr);
+                       as<uint32_t>(x.tile_list), 4u * ntiles, x.ctr);
     hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)vals, (const RTri *)as<RTri>(x.rt), n, as<float4>(x.cov));
     HIPC(hipGetLastError());
@@ -3371,6 +3401,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     rp.vals = vals;
     rp.list = as<uint32_t>(x.tile_list);
     rp.ntiles = ntiles;
+    rp.list_cap = 4u * ntiles;
     rp.part = 0;
     rp.cov = as<float4>(x.cov);
     rp.tile_start = as<uint32_t>(x.tile_start);
@@ -3383,10 +3414,14 @@ int run_launch(Exec &x, Launch &L, bool frozen)
         ev1 = take_event(x);
         if (ev0) HIPC(hipEventRecord(ev0, x.stream));
     }
-    // persistent grids over the tile list (longest bins first): at most one
-    // block per non-empty tile, at most as many as are resident at once
-    const uint32_t nlist = std::min<uint32_t>(n, ntiles);
-    auto grid = [&](int k) { return dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nlist, x.resident[k] / x.grid_div))); };
+    // persistent grids over the work list (longest items first): at most one
+    // block per item, at most as many as are resident at once
+    const uint32_t nlist = (uint32_t)std::min<uint64_t>((uint64_t)n, 4ull * ntiles);
+    auto grid = [&](int k) {
+        // statistics take one slot per block: grids stay within STAT_SLOTS
+        const uint32_t g = std::min<uint32_t>(3u * x.resident[k] / x.grid_div, STAT_SLOTS);
+        return dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nlist, g)));
+    };
     const uint32_t LW = KIND_LAST_WRITER, ST = KIND_STENCIL, CO = KIND_COLOR;
     const uint32_t lw_stencil = (1u << (LW | ST)) | (1u << (LW | ST | CO));
     const uint32_t lw_plain = (1u << LW) | (1u << (LW | CO));
