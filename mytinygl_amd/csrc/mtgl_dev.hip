@@ -59,7 +59,7 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-constexpr uint32_t HEAVY_BIN = 96;      // default: tiles with longer bins are rasterised first, in quarters
+constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in quarters
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | quarter << 28 (0: whole tile, 1..4)
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
@@ -217,7 +217,7 @@ struct RasterParams {
     const RTri *rt;
     const RCold *rcold;
     const uint32_t *vals;          // sorted pair values (record slot)
-    const uint32_t *list;          // work items: nheavy from the front, nlight from the back
+    const uint4 *list;             // work items (tile | quarter << 28, e0, e1, scene): nheavy from the front, nlight from the back
     uint32_t ntiles;
     uint32_t list_cap;             // items the list holds (4 per tile)
     uint32_t part;                 // list items of this launch: 0 all, 1 the heavy ones, 2 the others
@@ -1180,7 +1180,9 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
 // first.  One thread per tile; one atomic per block and list end.
 __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
                                                    const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
-                                                   uint32_t heavy_bin, uint32_t *list, uint32_t list_cap, Counters *ctr)
+                                                   uint32_t heavy_bin, const uint32_t *tile_scene,
+                                                   const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
+                                                   uint32_t list_cap, Counters *ctr)
 {
     __shared__ uint32_t s_n[2][4];
     __shared__ uint32_t s_base[2];
@@ -1209,14 +1211,20 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
+    // an item: scene-local tile | quarter << ITEM_TILE_BITS, bin start, bin end, scene
+    uint4 item = make_uint4(0u, 0u, 0u, 0u);
+    if (ne) {
+        const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
+        item = make_uint4(t - tile_pre[sc], tile_start[t], tile_end[t], sc);
+    }
     if (hv) {
         const uint32_t o = s_base[0] + 4u * (s_n[0][wave] + (uint32_t)__popcll(bh & below));
         for (uint32_t q = 0; q < 4u; q++)
-            if (o + q < list_cap) list[o + q] = t | ((q + 1u) << ITEM_TILE_BITS);
+            if (o + q < list_cap) list[o + q] = make_uint4(item.x | ((q + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
     }
     if (lt) {
         const uint32_t o = s_base[1] + s_n[1][wave] + (uint32_t)__popcll(bl & below);
-        if (o < list_cap) list[list_cap - 1u - o] = t;
+        if (o < list_cap) list[list_cap - 1u - o] = item;
     }
 }
 
@@ -1589,8 +1597,9 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
 // First half: stencil test, depth test, stencil update, the texel bytes the
 // fragment read, the alpha-test discard (kept == false), depth write and the
 // colour bytes.  Returns whether the colour step runs.
+template <typename Bytes>
 __device__ __forceinline__ bool depth_stencil_step(const RasterConsts &K, float depth,
-                                                   uint32_t taps, bool kept, PixelState &px, unsigned long long &bytes)
+                                                   uint32_t taps, bool kept, PixelState &px, Bytes &bytes)
 {
     if (!K.stencil_on) {
         // no stencil: the depth test alone, written to select rather than branch
@@ -1851,17 +1860,26 @@ __device__ __forceinline__ uint32_t list_item(uint32_t it0, uint32_t k, uint32_t
 
 // list item -> scene, the scene-local tile, the quarter (0 whole tile, q = 1..4
 // pixel rows 2q-2, 2q-1) and the tile's bin [e0, e1)
+__device__ __forceinline__ uint4 load_item(const RasterParams &P, uint32_t it, uint32_t nh)
+{
+    return P.list[it < nh ? it : P.list_cap - 1u - (it - nh)];
+}
+
+__device__ __forceinline__ void decode_item(uint4 v, uint32_t &sc, uint32_t &tile, uint32_t &e0, uint32_t &e1,
+                                            uint32_t &quarter)
+{
+    const uint32_t g = __builtin_amdgcn_readfirstlane(v.x);
+    quarter = g >> ITEM_TILE_BITS;
+    tile = g & ((1u << ITEM_TILE_BITS) - 1u);
+    e0 = __builtin_amdgcn_readfirstlane(v.y);
+    e1 = __builtin_amdgcn_readfirstlane(v.z);
+    sc = __builtin_amdgcn_readfirstlane(v.w);
+}
+
 __device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &sc, uint32_t &tile,
                                           uint32_t &e0, uint32_t &e1, uint32_t &quarter)
 {
-    uint32_t g = it < nh ? P.list[it] : P.list[P.list_cap - 1u - (it - nh)];
-    g = __builtin_amdgcn_readfirstlane(g);
-    quarter = g >> ITEM_TILE_BITS;
-    g &= (1u << ITEM_TILE_BITS) - 1u;
-    e0 = __builtin_amdgcn_readfirstlane(P.tile_start[g]);
-    e1 = __builtin_amdgcn_readfirstlane(P.tile_end[g]);
-    sc = scene_of(P.tile_scene, P.nscenes, g);
-    tile = g - P.tile_pre[sc];
+    decode_item(load_item(P, it, nh), sc, tile, e0, e1, quarter);
 }
 
 __device__ __forceinline__ void raster_consts(const SceneRaster &P, RasterConsts &K)
@@ -1970,11 +1988,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     list_counts(P, nh, it0, nt);
     uint32_t frag_count = 0;
     unsigned long long frag_bytes = 0;
-    for (uint32_t pass = 0;; pass++) {
-    const uint32_t it = list_item(it0, pass, blockIdx.x, gridDim.x);
-    if (it >= nt) break;
+    // the next item's header is loaded while the current item is processed
+    uint32_t it = list_item(it0, 0u, blockIdx.x, gridDim.x);
+    uint4 hdr = it < nt ? load_item(P, it, nh) : make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t pass = 0; it < nt; pass++) {
+    const uint32_t it_next = list_item(it0, pass + 1u, blockIdx.x, gridDim.x);
+    const uint4 hdr_next = it_next < nt ? load_item(P, it_next, nh) : make_uint4(0u, 0u, 0u, 0u);
     uint32_t sc, tile, e0, e1, quarter;
-    item_tile(P, it, nh, sc, tile, e0, e1, quarter);
+    decode_item(hdr, sc, tile, e0, e1, quarter);
+    hdr = hdr_next;
+    it = it_next;
     const SceneRaster &S = P.scenes[sc];
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (!(kind & KIND_LAST_WRITER) || ((kind & KIND_STENCIL) != 0u) != STENCIL) continue;
@@ -2080,13 +2103,30 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 }
             }
             wave_sync();
-            // per pixel, in submission order
-            for (uint32_t qn_ = 0; qn_ < take && off + qn_ < (uint32_t)DCAP; qn_++) {
+            // per pixel, in submission order; the LDS reads of four steps are
+            // issued together, ahead of the dependent compares
+            const uint32_t nst = off < (uint32_t)DCAP ? min(take, (uint32_t)DCAP - off) : 0u;
+            uint32_t bytes32 = 0;
+            uint32_t qn_ = 0;
+            for (; qn_ + 4u <= nst; qn_ += 4u) {
+                float dd[4];
+                uint32_t ss[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    dd[u] = s_dep[off + qn_ + u];
+                    ss[u] = s_slot[off + qn_ + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (depth_stencil_step(K, dd[u], ss[u] >> 12, true, px, bytes32)) last = (int32_t)((ss[u] >> 6) & 63u);
+            }
+            for (; qn_ < nst; qn_++) {
                 const uint32_t f = off + qn_;
                 const float depth = s_dep[f];
                 const uint32_t sl = s_slot[f];
-                if (depth_stencil_step(K, depth, sl >> 12, true, px, frag_bytes)) last = (int32_t)((sl >> 6) & 63u);
+                if (depth_stencil_step(K, depth, sl >> 12, true, px, bytes32)) last = (int32_t)((sl >> 6) & 63u);
             }
+            frag_bytes += bytes32;   // a batch gives a pixel at most 64 fragments of at most 50 bytes
             wave_sync();   // slots and depths are reused by the next batch
         }
         if (last >= 0) writer = 1u + s_rid[last];
@@ -3338,7 +3378,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     const uint32_t ntiles = L.tiles_total;
     if ((rc = arena_reserve(x, x.tile_start, (size_t)ntiles * 4, frozen))) return rc;
     if ((rc = arena_reserve(x, x.tile_end, (size_t)ntiles * 4, frozen))) return rc;
-    if ((rc = arena_reserve(x, x.tile_list, (size_t)ntiles * 16, frozen))) return rc;   // up to 4 items per tile
+    if ((rc = arena_reserve(x, x.tile_list, (size_t)ntiles * 4 * sizeof(uint4), frozen))) return rc;   // up to 4 items per tile
     if (!frozen) {
         if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
             if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
@@ -3384,8 +3424,8 @@ int run_launch(Exec &x, Launch &L, bool frozen)
                        as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen), x.gen);
     hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
-                       (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin,
-                       as<uint32_t>(x.tile_list), 4u * ntiles, x.ctr);
+                       (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
+                       L.ns, as<uint4>(x.tile_list), 4u * ntiles, x.ctr);
     hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)vals, (const RTri *)as<RTri>(x.rt), n, as<float4>(x.cov));
     HIPC(hipGetLastError());
@@ -3399,7 +3439,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     rp.rt = as<RTri>(x.rt);
     rp.rcold = L.any_cold ? as<RCold>(x.rcold) : nullptr;
     rp.vals = vals;
-    rp.list = as<uint32_t>(x.tile_list);
+    rp.list = as<uint4>(x.tile_list);
     rp.ntiles = ntiles;
     rp.list_cap = 4u * ntiles;
     rp.part = 0;
@@ -3672,8 +3712,8 @@ int batch_build_inner(mtgl_batch *b, int autotune)
     if (autotune) {
         // the heavy-bin threshold of the tile list and the raster grid size:
         // one warm and three timed frozen replays of the batch per candidate
-        static const uint32_t cand_bin[] = {32u, 96u, 256u, 0xFFFFFFFFu, 96u, 96u};
-        static const uint32_t cand_div[] = {1u, 1u, 1u, 1u, 2u, 4u};
+        static const uint32_t cand_bin[] = {96u, 192u, 320u, 0xFFFFFFFFu, 192u, 192u};
+        static const uint32_t cand_div[] = {1u, 1u, 1u, 1u, 2u, 3u};
         const int ncand = (int)(sizeof(cand_bin) / sizeof(cand_bin[0]));
         static_assert(sizeof(cand_bin) / sizeof(cand_bin[0]) <= MTGL_BATCH_TUNE_MAX, "tune table");
         hipEvent_t e0 = nullptr, e1 = nullptr;
