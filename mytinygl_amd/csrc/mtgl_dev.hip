@@ -129,6 +129,7 @@ struct Counters {
     unsigned int overflow_full;   // a request did not fit: the host grows the pool and reruns
     unsigned int sticky;          // overflow_full of every launch since the host last cleared it
     unsigned int nbig;            // triangles queued for the one-wave pair expansion
+    unsigned long long npairs;    // (tile, record) pairs of the launch, written by the pair-offset scan
 };
 
 // attribute stream resolved to a device pointer
@@ -471,9 +472,14 @@ __device__ __forceinline__ float sanitize(float v, float nan_value)
 }
 
 // blocks are dealt to scenes in order: block_pre holds each scene's first block
+__device__ __forceinline__ void reset_counters(Counters *ctr, unsigned long long *scan_tail);
+
+// (block 0 also resets the launch's counters and the pair-offset scan's tail)
 __global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const uint32_t *block_pre,
-                                                const uint32_t *block_scene, uint32_t nscenes)
+                                                const uint32_t *block_scene, uint32_t nscenes, Counters *ctr,
+                                                unsigned long long *scan_tail)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) reset_counters(ctr, scan_tail);
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     const VertexParams &P = Ps[sc];
     const uint32_t i = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
@@ -1239,20 +1245,19 @@ __global__ void __launch_bounds__(256) k_gather_cov(const uint32_t *vals, const 
     out[i] = reinterpret_cast<const float4 *>(rt + vals[e])[q];
 }
 
-__global__ void k_reset(Counters *ctr, unsigned long long *scan_tail)
+// per-launch counters (k_vertex's first thread, before any other kernel of the launch)
+__device__ __forceinline__ void reset_counters(Counters *ctr, unsigned long long *scan_tail)
 {
-    if (threadIdx.x == 0) {
-        *scan_tail = 0;   // the pair-offset scan's last element becomes the total
-        ctr->tri_raster = 0;
-        ctr->nheavy = 0;
-        ctr->nlight = 0;
-        ctr->unsupported = 0;
-        ctr->nclip = 0;
-        ctr->noverflow = 0;
-        ctr->nbig = 0;
-        ctr->sticky |= ctr->overflow_full;
-        ctr->overflow_full = 0;
-    }
+    *scan_tail = 0;   // the pair-offset scan's last element becomes the total
+    ctr->tri_raster = 0;
+    ctr->nheavy = 0;
+    ctr->nlight = 0;
+    ctr->unsupported = 0;
+    ctr->nclip = 0;
+    ctr->noverflow = 0;
+    ctr->nbig = 0;
+    ctr->sticky |= ctr->overflow_full;
+    ctr->overflow_full = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2752,7 +2757,6 @@ struct Exec {
     uint32_t resident[4] = {256, 256, 256, 256};   // blocks at once: depth<stencil>, depth<no stencil>, colour, general
     Counters *ctr = nullptr;                       // device
     Counters *ctr_host = nullptr;                  // pinned mirror for readback
-    unsigned long long *pairs_host = nullptr;
     uint8_t *pinned = nullptr;                     // staging for uploads
     size_t pinned_cap = 0;
     hipEvent_t upload_done = nullptr;
@@ -2891,9 +2895,10 @@ unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threa
 
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// exclusive scan of n values of T; out may alias in
+// exclusive scan of n values of T; out may alias in; *total_out (when
+// given) receives the sum
 template <typename T>
-int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen)
+int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen, T *total_out = nullptr)
 {
     if (n == 0) return MTGL_OK;
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -2901,7 +2906,7 @@ int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen)
     if (rc) return rc;
     T *sums = as<T>(x.scan_tmp);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n, sums);
-    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, x.stream, sums, nb, (T *)nullptr);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, x.stream, sums, nb, total_out);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n,
                        (const T *)sums, out);
     HIPC(hipGetLastError());
@@ -2972,7 +2977,6 @@ int exec_init(Exec &x, int device)
     HIPC(hipMalloc((void **)&x.ctr, sizeof(Counters)));
     HIPC(hipMalloc((void **)&x.stat_slots, 2 * STAT_SLOTS * sizeof(unsigned long long)));
     HIPC(hipHostMalloc((void **)&x.ctr_host, sizeof(Counters), hipHostMallocDefault));
-    HIPC(hipHostMalloc((void **)&x.pairs_host, sizeof(unsigned long long), hipHostMallocDefault));
     HIPC(hipEventCreateWithFlags(&x.upload_done, hipEventDisableTiming));
     HIPC(hipMemsetAsync(x.stat_slots, 0, 2 * STAT_SLOTS * sizeof(unsigned long long), x.stream));
     HIPC(hipMemsetAsync(x.ctr, 0, sizeof(Counters), x.stream));
@@ -2997,14 +3001,12 @@ void exec_free(Exec &x)
     if (x.ctr) (void)hipFree(x.ctr);
     if (x.stat_slots) (void)hipFree(x.stat_slots);
     if (x.ctr_host) (void)hipHostFree(x.ctr_host);
-    if (x.pairs_host) (void)hipHostFree(x.pairs_host);
     if (x.pinned) (void)hipHostFree(x.pinned);
     if (x.upload_done) (void)hipEventDestroy(x.upload_done);
     if (x.stream) (void)hipStreamDestroy(x.stream);
     x.ctr = nullptr;
     x.stat_slots = nullptr;
     x.ctr_host = nullptr;
-    x.pairs_host = nullptr;
     x.pinned = nullptr;
     x.upload_done = nullptr;
     x.stream = nullptr;
@@ -3339,8 +3341,8 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     if (L.ns == 0) return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
     int rc;
     const uint32_t ttot = L.tri_total;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, x.stream, x.ctr, as<unsigned long long>(x.tri_tiles) + ttot);
-    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(256), 0, x.stream, L.vp, L.vblock_pre, L.vmap, L.ns);
+    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(256), 0, x.stream, L.vp, L.vblock_pre, L.vmap, L.ns, x.ctr,
+                       as<unsigned long long>(x.tri_tiles) + ttot);
     hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(256), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
     hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
                        (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
@@ -3349,18 +3351,16 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     // ---- K3: pair offsets and total, then expand / sort / segment
     unsigned long long *pair_base = as<unsigned long long>(x.pair_base);
     if ((rc = device_scan<unsigned long long>(x, as<unsigned long long>(x.tri_tiles), pair_base, (uint64_t)ttot + 1,
-                                              frozen)))
+                                              frozen, &x.ctr->npairs)))
         return rc;
     uint64_t npairs;
     if (!frozen) {
-        HIPC(hipMemcpyAsync(x.pairs_host, pair_base + ttot, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                            x.stream));
         HIPC(hipMemcpyAsync(x.ctr_host, x.ctr, sizeof(Counters), hipMemcpyDeviceToHost, x.stream));
         HIPC(hipStreamSynchronize(x.stream));
         // the clipped triangles asked for more slots than the pool held:
         // nothing downstream ran; the caller grows the pool and runs again
         if (x.ctr_host->overflow_full & 1u) return RC_RETRY_CLIP;
-        npairs = *x.pairs_host;
+        npairs = x.ctr_host->npairs;
         L.unsupported |= x.ctr_host->unsupported != 0;
         x.stats.triangles_raster += x.ctr_host->tri_raster;
         if (npairs > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
