@@ -67,7 +67,7 @@ typedef struct {
     int32_t  width, height;
     uint32_t vertices;     /* elements addressed by vcol */
     int32_t  tex_w, tex_h;
-    uint64_t work_floats;  /* floats of the backward workspace (views * vertices * 4) */
+    uint64_t work_floats;  /* floats of the backward workspace (views * vertices * 16) */
 } mtgl_diff_info_t;
 
 int  mtgl_diff_create(uint32_t vertices, int32_t tex_w, int32_t tex_h, mtgl_diff **out);

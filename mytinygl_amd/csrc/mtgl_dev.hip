@@ -310,10 +310,14 @@ __device__ __forceinline__ void xform(const float *m, float x, float y, float z,
 // OVERRIDE (differentiable shading): the lights' diffuse rgb come from
 // dif_rgb (3 per light) and each light's att * N.L where it adds diffuse
 // light (else 0) is stored in ndl_out.
+// OVERRIDE: light i's diffuse rgb is dif_rgb[3i..3i+2] and ndl_out[i]
+// receives att * N.L (0 where the light does not reach); ld_pre, if given,
+// holds the normalised directions of the directional lights (normalize3 of
+// their positions, the same arithmetic hoisted out of a per-vertex loop)
 template <bool OVERRIDE>
 __device__ Col lighting_sum(const mtgl_lighting *L, uint32_t flags, const float *eye_pos, const float *eye_normal,
                             Col m_amb, Col m_dif, Col m_spec, Col m_emi, float shininess, const float *dif_rgb,
-                            float *ndl_out)
+                            float *ndl_out, const float *ld_pre = nullptr)
 {
     Col result = col_add(m_emi, col_mul(m_amb, col_of(L->model_ambient)));
     result.a = m_dif.a;
@@ -326,8 +330,12 @@ __device__ Col lighting_sum(const mtgl_lighting *L, uint32_t flags, const float 
         float Ld[3];
         float att = 1.0f;
         if (lt->position[3] == 0.0f) {
-            Ld[0] = lt->position[0]; Ld[1] = lt->position[1]; Ld[2] = lt->position[2];
-            normalize3(Ld);
+            if (ld_pre) {
+                Ld[0] = ld_pre[3 * i]; Ld[1] = ld_pre[3 * i + 1]; Ld[2] = ld_pre[3 * i + 2];
+            } else {
+                Ld[0] = lt->position[0]; Ld[1] = lt->position[1]; Ld[2] = lt->position[2];
+                normalize3(Ld);
+            }
         } else {
             float to[3] = {lt->position[0] - eye_pos[0], lt->position[1] - eye_pos[1], lt->position[2] - eye_pos[2]};
             float dist = sqrtf(dot3(to, to));
@@ -2539,19 +2547,37 @@ __device__ __forceinline__ void diff_vertex(const DiffView &D, uint32_t e, const
     if (mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE) o.dif = k;
 }
 
-// lit colour of every (view, element): grid (elements, views)
+// lit colour of every (view, element): grid (elements, views).  With aux
+// (the backward's), also the unclamped sum and each light's att * N.L, in
+// three planes of views * nvert float4 each, for k_diff_lit_bwd.
 __global__ void __launch_bounds__(256) k_diff_lit(const DiffView *views, uint32_t nvert, const float *vcol,
-                                                  const float *light_rgb, float4 *lit)
+                                                  const float *light_rgb, float4 *lit, float4 *aux)
 {
+    __shared__ float s_ld[3 * MTGL_MAX_LIGHTS];
+    const DiffView &D = views[blockIdx.y];
+    if (threadIdx.x < MTGL_MAX_LIGHTS) {
+        const mtgl_light &lt = D.vs->lighting.lights[threadIdx.x];
+        float Ld[3] = {lt.position[0], lt.position[1], lt.position[2]};
+        normalize3(Ld);
+        for (int k = 0; k < 3; k++) s_ld[3 * threadIdx.x + k] = Ld[k];
+    }
+    __syncthreads();
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nvert) return;
-    const DiffView &D = views[blockIdx.y];
     DiffVertex q;
     diff_vertex(D, e, vcol, q);
     float ndl[MTGL_MAX_LIGHTS];
-    const Col r = col_clamp(lighting_sum<true>(&D.vs->lighting, D.vs->flags, q.ep, q.en, q.amb, q.dif, q.spe, q.emi,
-                                               D.vs->lighting.front.shininess, light_rgb, ndl));
-    lit[(size_t)blockIdx.y * nvert + e] = make_float4(r.r, r.g, r.b, r.a);
+    const Col u = lighting_sum<true>(&D.vs->lighting, D.vs->flags, q.ep, q.en, q.amb, q.dif, q.spe, q.emi,
+                                     D.vs->lighting.front.shininess, light_rgb, ndl, s_ld);
+    const Col r = col_clamp(u);
+    const size_t i = (size_t)blockIdx.y * nvert + e;
+    lit[i] = make_float4(r.r, r.g, r.b, r.a);
+    if (aux) {
+        const size_t plane = (size_t)gridDim.y * nvert;
+        aux[i] = make_float4(u.r, u.g, u.b, u.a);
+        aux[plane + i] = make_float4(ndl[0], ndl[1], ndl[2], ndl[3]);
+        aux[2 * plane + i] = make_float4(ndl[4], ndl[5], ndl[6], ndl[7]);
+    }
 }
 
 __device__ __forceinline__ Col col4(float4 v) { return {v.x, v.y, v.z, v.w}; }
@@ -2597,78 +2623,85 @@ __global__ void __launch_bounds__(256) k_diff_shade(const DiffView *views, uint3
 
 __device__ __forceinline__ float clamp_grad(float v, float g) { return (v >= 0.0f && v <= 1.0f) ? g : 0.0f; }
 
-// adjoint of k_diff_shade: dL/dlit per (view, element) into dlit, dL/dtex;
-// grid (pixels, views)
-__global__ void __launch_bounds__(256) k_diff_shade_bwd(const DiffView *views, uint32_t nvert, const float4 *lit,
-                                                        const float4 *tex, const float4 *grad, float *dlit,
+// adjoint of k_diff_shade: dL/dlit per (view, element) into dlit, dL/dtex.
+// Channels are independent through the shading, so a thread takes one
+// (pixel, channel): the four lanes of a pixel add to the four consecutive
+// floats of each target together, one L2 request per target and instruction.
+// Grid (4 x pixels, views).
+__global__ void __launch_bounds__(256) k_diff_shade_bwd(const DiffView *views, uint32_t nvert, const float *lit,
+                                                        const float *tex, const float *grad, float *dlit,
                                                         float *g_tex)
 {
     const DiffView &D = views[blockIdx.y];
     const uint32_t npx = (uint32_t)D.w * (uint32_t)D.h;
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t p = t >> 2, ch = t & 3u;
     if (p >= npx) return;
     const GPx g = D.gb[p];
     if (g.tri < 0) return;
     const uint32_t *el = D.elems + 3u * (uint32_t)g.tri;
-    Col c, tc;
-    diff_pixel(g, el, lit + (size_t)blockIdx.y * nvert, nvert, tex, c, tc);
-    const Col prod = col_mul(c, tc);
-    const float4 gi = grad[(size_t)blockIdx.y * npx + p];
-    const Col gm = {clamp_grad(prod.r, gi.x), clamp_grad(prod.g, gi.y), clamp_grad(prod.b, gi.z),
-                    clamp_grad(prod.a, gi.w)};
-    const Col dc = col_mul(gm, tc), dt = col_mul(gm, c);
+    const float *litv = lit + 4u * (size_t)blockIdx.y * nvert;
+    // the operations of diff_pixel, one channel
+    uint32_t e[3];
+    float l[3];
     for (int k = 0; k < 3; k++) {
-        const uint32_t e = el[k];
-        if (e >= nvert) continue;
-        float *d = dlit + 4u * ((size_t)blockIdx.y * nvert + e);
-        const float b = g.b[k];
-        unsafeAtomicAdd(d + 0, dc.r * b);
-        unsafeAtomicAdd(d + 1, dc.g * b);
-        unsafeAtomicAdd(d + 2, dc.b * b);
-        unsafeAtomicAdd(d + 3, dc.a * b);
+        e[k] = el[k];
+        l[k] = e[k] < nvert ? litv[4u * e[k] + ch] : 0.0f;
     }
+    const float c = l[0] * g.b[0] + l[1] * g.b[1] + l[2] * g.b[2];
     const float fx = g.fx, fy = g.fy;
+    const float t0 = tex[4u * (size_t)g.texel[0] + ch], t1 = tex[4u * (size_t)g.texel[1] + ch];
+    const float t2 = tex[4u * (size_t)g.texel[2] + ch], t3 = tex[4u * (size_t)g.texel[3] + ch];
+    const float top = t0 * (1.0f - fx) + t1 * fx, bot = t2 * (1.0f - fx) + t3 * fx;
+    const float tc = top * (1.0f - fy) + bot * fy;
+    const float gm = clamp_grad(c * tc, grad[4u * ((size_t)blockIdx.y * npx + p) + ch]);
+    const float dc = gm * tc, dt = gm * c;
+    for (int k = 0; k < 3; k++)
+        if (e[k] < nvert) unsafeAtomicAdd(dlit + 4u * ((size_t)blockIdx.y * nvert + e[k]) + ch, dc * g.b[k]);
     const float wt[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
-    for (int k = 0; k < 4; k++) {
-        float *d = g_tex + 4u * (size_t)g.texel[k];
-        unsafeAtomicAdd(d + 0, dt.r * wt[k]);
-        unsafeAtomicAdd(d + 1, dt.g * wt[k]);
-        unsafeAtomicAdd(d + 2, dt.b * wt[k]);
-        unsafeAtomicAdd(d + 3, dt.a * wt[k]);
-    }
+    for (int k = 0; k < 4; k++) unsafeAtomicAdd(g_tex + 4u * (size_t)g.texel[k] + ch, dt * wt[k]);
 }
 
-// adjoint of k_diff_lit: one thread per element over every view; dL/dvcol
-// written, dL/dlight_rgb reduced per block and added
-__global__ void __launch_bounds__(256) k_diff_lit_bwd(const DiffView *views, uint32_t nviews, uint32_t nvert,
-                                                      const float *vcol, const float *light_rgb, const float *dlit,
-                                                      float *g_vcol, float *g_light)
+// adjoint of k_diff_lit from the unclamped sums and N.L terms the
+// backward's k_diff_lit stored (aux): a block holds DIFF_BWD_E elements x
+// DIFF_BWD_G view groups (thread (x, y) takes element x over views y,
+// y + DIFF_BWD_G, ...); dL/dvcol is summed over the groups in LDS and
+// written, dL/dlight_rgb is reduced per block and added
+constexpr int DIFF_BWD_E = 64, DIFF_BWD_G = 8;
+__global__ void __launch_bounds__(DIFF_BWD_E * DIFF_BWD_G) k_diff_lit_bwd(const DiffView *views, uint32_t nviews,
+                                                                          uint32_t nvert, const float *vcol,
+                                                                          const float *light_rgb, const float *dlit,
+                                                                          const float4 *aux, float *g_vcol,
+                                                                          float *g_light)
 {
     __shared__ float s_gl[3 * MTGL_MAX_LIGHTS];
-    if (threadIdx.x < 3 * MTGL_MAX_LIGHTS) s_gl[threadIdx.x] = 0.0f;
+    __shared__ float4 s_gv[DIFF_BWD_G][DIFF_BWD_E];
+    const uint32_t tid = threadIdx.y * DIFF_BWD_E + threadIdx.x;
+    if (tid < 3 * MTGL_MAX_LIGHTS) s_gl[tid] = 0.0f;
     __syncthreads();
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t e = blockIdx.x * DIFF_BWD_E + threadIdx.x;
+    const size_t plane = (size_t)nviews * nvert;
+    Col gv = {0.0f, 0.0f, 0.0f, 0.0f};
     if (e < nvert) {
-        Col gv = {0.0f, 0.0f, 0.0f, 0.0f};
         float gl[3 * MTGL_MAX_LIGHTS];
         for (int i = 0; i < 3 * MTGL_MAX_LIGHTS; i++) gl[i] = 0.0f;
         bool any_light = false;
-        for (uint32_t v = 0; v < nviews; v++) {
-            const float *dl4 = dlit + 4u * ((size_t)v * nvert + e);
-            const Col dl = {dl4[0], dl4[1], dl4[2], dl4[3]};
-            if (dl.r == 0.0f && dl.g == 0.0f && dl.b == 0.0f && dl.a == 0.0f) continue;
-            const DiffView &D = views[v];
-            const mtgl_vertex_state *vs = D.vs;
+        const float *c = vcol + 4u * e;
+        const Col vc = {sanitize(c[0], 0.0f), sanitize(c[1], 0.0f), sanitize(c[2], 0.0f), sanitize(c[3], 1.0f)};
+        const Col k = col_clamp(vc);
+        for (uint32_t v = threadIdx.y; v < nviews; v += DIFF_BWD_G) {
+            const size_t i = (size_t)v * nvert + e;
+            const float4 dl = reinterpret_cast<const float4 *>(dlit)[i];
+            if (dl.x == 0.0f && dl.y == 0.0f && dl.z == 0.0f && dl.w == 0.0f) continue;
+            const float4 u = aux[i], n03 = aux[plane + i], n47 = aux[2 * plane + i];
+            const float ndl[MTGL_MAX_LIGHTS] = {n03.x, n03.y, n03.z, n03.w, n47.x, n47.y, n47.z, n47.w};
+            const mtgl_vertex_state *vs = views[v].vs;
             const mtgl_lighting &L = vs->lighting;
-            DiffVertex q;
-            diff_vertex(D, e, vcol, q);
-            float ndl[MTGL_MAX_LIGHTS];
-            const Col r = lighting_sum<true>(&L, vs->flags, q.ep, q.en, q.amb, q.dif, q.spe, q.emi,
-                                             L.front.shininess, light_rgb, ndl);
-            const Col g = {clamp_grad(r.r, dl.r), clamp_grad(r.g, dl.g), clamp_grad(r.b, dl.b), clamp_grad(r.a, dl.a)};
+            const Col g = {clamp_grad(u.x, dl.x), clamp_grad(u.y, dl.y), clamp_grad(u.z, dl.z), clamp_grad(u.w, dl.w)};
             const uint32_t mode = vs->color_material_mode;
             const bool amb_c = mode == GL_AMBIENT || mode == GL_AMBIENT_AND_DIFFUSE;
             const bool dif_c = mode == GL_DIFFUSE || mode == GL_AMBIENT_AND_DIFFUSE;
+            const Col dif = dif_c ? k : col_of(L.front.diffuse);
             // dr/dk for the material colour k = clamp(vcol) (att = 1: directional lights)
             Col dk = {0.0f, 0.0f, 0.0f, dif_c ? 1.0f : 0.0f};
             if (amb_c) {
@@ -2676,8 +2709,8 @@ __global__ void __launch_bounds__(256) k_diff_lit_bwd(const DiffView *views, uin
                 dk.g += L.model_ambient.g;
                 dk.b += L.model_ambient.b;
             }
-            for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
-                const mtgl_light &lt = L.lights[i];
+            for (int li = 0; li < MTGL_MAX_LIGHTS; li++) {
+                const mtgl_light &lt = L.lights[li];
                 if (!lt.enabled) continue;
                 if (amb_c) {
                     dk.r += lt.ambient.r;
@@ -2685,38 +2718,44 @@ __global__ void __launch_bounds__(256) k_diff_lit_bwd(const DiffView *views, uin
                     dk.b += lt.ambient.b;
                     dk.a += lt.ambient.a;
                 }
-                const float n = ndl[i];
+                const float n = ndl[li];
                 if (n > 0.0f) {
                     if (dif_c) {
-                        dk.r += light_rgb[3 * i] * n;
-                        dk.g += light_rgb[3 * i + 1] * n;
-                        dk.b += light_rgb[3 * i + 2] * n;
+                        dk.r += light_rgb[3 * li] * n;
+                        dk.g += light_rgb[3 * li + 1] * n;
+                        dk.b += light_rgb[3 * li + 2] * n;
                         dk.a += lt.diffuse.a * n;
                     }
-                    gl[3 * i] += g.r * q.dif.r * n;
-                    gl[3 * i + 1] += g.g * q.dif.g * n;
-                    gl[3 * i + 2] += g.b * q.dif.b * n;
+                    gl[3 * li] += g.r * dif.r * n;
+                    gl[3 * li + 1] += g.g * dif.g * n;
+                    gl[3 * li + 2] += g.b * dif.b * n;
                     any_light = true;
                 }
             }
             const Col gk = col_mul(g, dk);
-            gv.r += clamp_grad(q.vc.r, gk.r);
-            gv.g += clamp_grad(q.vc.g, gk.g);
-            gv.b += clamp_grad(q.vc.b, gk.b);
-            gv.a += clamp_grad(q.vc.a, gk.a);
+            gv.r += clamp_grad(vc.r, gk.r);
+            gv.g += clamp_grad(vc.g, gk.g);
+            gv.b += clamp_grad(vc.b, gk.b);
+            gv.a += clamp_grad(vc.a, gk.a);
         }
-        float *o = g_vcol + 4u * e;
-        o[0] = gv.r;
-        o[1] = gv.g;
-        o[2] = gv.b;
-        o[3] = gv.a;
         if (any_light)
             for (int i = 0; i < 3 * MTGL_MAX_LIGHTS; i++)
                 if (gl[i] != 0.0f) atomicAdd(&s_gl[i], gl[i]);
     }
+    s_gv[threadIdx.y][threadIdx.x] = make_float4(gv.r, gv.g, gv.b, gv.a);
     __syncthreads();
-    if (threadIdx.x < 3 * MTGL_MAX_LIGHTS && s_gl[threadIdx.x] != 0.0f)
-        unsafeAtomicAdd(g_light + threadIdx.x, s_gl[threadIdx.x]);
+    if (threadIdx.y == 0 && e < nvert) {
+        float4 t = s_gv[0][threadIdx.x];
+        for (int g = 1; g < DIFF_BWD_G; g++) {
+            const float4 u = s_gv[g][threadIdx.x];
+            t.x += u.x;
+            t.y += u.y;
+            t.z += u.z;
+            t.w += u.w;
+        }
+        reinterpret_cast<float4 *>(g_vcol)[e] = t;
+    }
+    if (tid < 3 * MTGL_MAX_LIGHTS && s_gl[tid] != 0.0f) unsafeAtomicAdd(g_light + tid, s_gl[tid]);
 }
 
 }  // namespace
@@ -4451,7 +4490,7 @@ MTGL_API int mtgl_diff_info(mtgl_diff *df, mtgl_diff_info_t *out)
     out->vertices = df->nvert;
     out->tex_w = df->tex_w;
     out->tex_h = df->tex_h;
-    out->work_floats = (uint64_t)df->views.size() * df->nvert * 4u;
+    out->work_floats = (uint64_t)df->views.size() * df->nvert * 16u;
     return MTGL_OK;
 }
 
@@ -4471,7 +4510,8 @@ MTGL_API int mtgl_diff_forward(mtgl_diff *df, const float *vcol, const float *li
     if (rc) return rc;
     const uint32_t nv = (uint32_t)df->views.size(), npx = (uint32_t)df->w * (uint32_t)df->h;
     hipLaunchKernelGGL(k_diff_lit, dim3(blocks_for(df->nvert, 256), nv), dim3(256), 0, st,
-                       (const DiffView *)df->table.ptr, df->nvert, vcol, light_rgb, as<float4>(df->lit));
+                       (const DiffView *)df->table.ptr, df->nvert, vcol, light_rgb, as<float4>(df->lit),
+                       (float4 *)nullptr);
     hipLaunchKernelGGL(k_diff_shade, dim3(blocks_for(npx, 256), nv), dim3(256), 0, st,
                        (const DiffView *)df->table.ptr, df->nvert, (const float4 *)as<float4>(df->lit),
                        (const float4 *)tex, (float4 *)images);
@@ -4491,17 +4531,21 @@ MTGL_API int mtgl_diff_backward(mtgl_diff *df, const float *vcol, const float *l
     if (rc) return rc;
     const uint32_t nv = (uint32_t)df->views.size(), npx = (uint32_t)df->w * (uint32_t)df->h;
     const DiffView *tab = (const DiffView *)df->table.ptr;
+    // work: dL/dlit (views * nvert float4), then the unclamped lit sums and
+    // the N.L terms of the current parameters (three planes of the same size)
+    const size_t plane = (size_t)nv * df->nvert;
+    float4 *aux = reinterpret_cast<float4 *>(work) + plane;
     // the lit colours of the current parameters (the forward's, recomputed)
     hipLaunchKernelGGL(k_diff_lit, dim3(blocks_for(df->nvert, 256), nv), dim3(256), 0, st, tab, df->nvert, vcol,
-                       light_rgb, as<float4>(df->lit));
-    if ((rc = fill_zero(work, (size_t)nv * df->nvert * 4u, st))) return rc;
+                       light_rgb, as<float4>(df->lit), aux);
+    if ((rc = fill_zero(work, plane * 4u, st))) return rc;
     if ((rc = fill_zero(g_tex, (size_t)df->tex_w * df->tex_h * 4u, st))) return rc;
     if ((rc = fill_zero(g_light_rgb, 3u * MTGL_MAX_LIGHTS, st))) return rc;
-    hipLaunchKernelGGL(k_diff_shade_bwd, dim3(blocks_for(npx, 256), nv), dim3(256), 0, st, tab, df->nvert,
-                       (const float4 *)as<float4>(df->lit), (const float4 *)tex, (const float4 *)grad_images, work,
-                       g_tex);
-    hipLaunchKernelGGL(k_diff_lit_bwd, dim3(blocks_for(df->nvert, 256)), dim3(256), 0, st, tab, nv, df->nvert, vcol,
-                       light_rgb, (const float *)work, g_vcol, g_light_rgb);
+    hipLaunchKernelGGL(k_diff_shade_bwd, dim3(blocks_for(4ull * npx, 256), nv), dim3(256), 0, st, tab, df->nvert,
+                       (const float *)as<float>(df->lit), tex, grad_images, work, g_tex);
+    hipLaunchKernelGGL(k_diff_lit_bwd, dim3(blocks_for(df->nvert, DIFF_BWD_E)), dim3(DIFF_BWD_E, DIFF_BWD_G), 0, st,
+                       tab, nv, df->nvert, vcol, light_rgb, (const float *)work, (const float4 *)aux, g_vcol,
+                       g_light_rgb);
     HIPC(hipGetLastError());
     return MTGL_OK;
 }

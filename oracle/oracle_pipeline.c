@@ -1564,7 +1564,7 @@ int mtgl_diff_info(mtgl_diff *df, mtgl_diff_info_t *out)
     out->vertices = df->nvert;
     out->tex_w = df->tex_w;
     out->tex_h = df->tex_h;
-    out->work_floats = (uint64_t)df->nviews * df->nvert * 4u;
+    out->work_floats = (uint64_t)df->nviews * df->nvert * 16u;
     return MTGL_OK;
 }
 
