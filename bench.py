@@ -20,6 +20,9 @@ scenes), with its own bounded CPU sample.
 
 "differentiable": forward + L2 loss + adjoint over 64 captured views
 (mytinygl_amd/diff.py), with the oracle's CPU restatement timed on two views.
+
+"draw_pixels": glDrawPixels of a 1080p RGBA image with depth test and
+blending, per call and per kernel, with the oracle timed on five calls.
 """
 from __future__ import annotations
 
@@ -273,6 +276,78 @@ def diff_cpu_baseline(P, T, nviews: int):
             "sample": f"{nviews} of the 64 views, forward + backward in {sec:.2f} s, one thread, scaled to 64"}
 
 
+def pixels_bench(reps: int, cpu_reps: int):
+    """glDrawPixels of a 1920x1080 GL_RGBA image over a 1080p framebuffer with
+    the depth test (GL_LEQUAL, depth writes on) and blending (GL_SRC_ALPHA,
+    GL_ONE_MINUS_SRC_ALPHA) enabled.  Each call copies the host image to the
+    device (8.3 MB over PCIe) and runs k_draw_pixels; the kernel's own time
+    comes from events on the context's stream.  Algorithmic bytes per pixel:
+    4 source + 4 depth read + 4 depth write + 4 colour read + 4 colour write."""
+    rng = np.random.default_rng(S.SEED_BASE + 8)
+    img = rng.integers(0, 256, size=W * H * 4, dtype=np.uint8)
+
+    def setup(g):
+        g.glMatrixMode(E["GL_PROJECTION"])
+        g.glLoadIdentity()
+        g.glOrtho(0.0, float(W), 0.0, float(H), -1.0, 1.0)
+        g.glMatrixMode(E["GL_MODELVIEW"])
+        g.glLoadIdentity()
+        g.glRasterPos2i(0, 0)
+        g.glEnable(E["GL_DEPTH_TEST"])
+        g.glDepthFunc(E["GL_LEQUAL"])
+        g.glEnable(E["GL_BLEND"])
+        g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE_MINUS_SRC_ALPHA"])
+        g.glClearColor(0.2, 0.3, 0.4, 1.0)
+        g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"])
+
+    g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
+    ctx = g.create_context(W, H)
+    setup(g)
+    ptr = g.keep(img)
+    for _ in range(3):
+        g.glDrawPixels(W, H, E["GL_RGBA"], E["GL_UNSIGNED_BYTE"], ptr)
+    g.glFinish()
+    g.stats_reset(ctx)
+    g.set_timing(ctx, True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.glDrawPixels(W, H, E["GL_RGBA"], E["GL_UNSIGNED_BYTE"], ptr)
+    g.glFinish()
+    sec = time.perf_counter() - t0
+    st = g.stats(ctx)
+    g.set_timing(ctx, False)
+    g.destroy_context(ctx)
+    g.release()
+    k_ms = st.raster_ms / max(1, st.raster_launches)
+    px = W * H
+    achieved = 20.0 * px / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    out = {"workload": "glDrawPixels 1920x1080 GL_RGBA, depth test GL_LEQUAL with writes, blend SRC_ALPHA / "
+                       "ONE_MINUS_SRC_ALPHA, 1080p framebuffer",
+           "calls": reps, "ms_per_call": round(sec * 1e3 / reps, 4),
+           "mpixels_s_per_call": round(px * reps / sec / 1e6, 1),
+           "kernel_ms": round(k_ms, 5), "kernel_launches": int(st.raster_launches),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_launch": 20 * px,
+                        "kernel": "k_draw_pixels"},
+           "note": "ms_per_call includes the host-to-device copy of the image (pageable host memory -> pinned "
+                   "staging -> HBM); the kernel reads an image already in HBM"}
+    lib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+    if cpu_reps > 0 and os.path.exists(lib):
+        o = GLLibrary(lib, "oracle")
+        octx = o.create_context(W, H)
+        setup(o)
+        optr = o.keep(img)
+        t0 = time.perf_counter()
+        for _ in range(cpu_reps):
+            o.glDrawPixels(W, H, E["GL_RGBA"], E["GL_UNSIGNED_BYTE"], optr)
+        csec = time.perf_counter() - t0
+        o.destroy_context(octx)
+        o.release()
+        out["cpu_baseline"] = {"ms_per_call": round(csec * 1e3 / cpu_reps, 3), "cores": 1, "kind": "port",
+                               "sample": f"{cpu_reps} calls of the same workload on the oracle, one thread"}
+    return out
+
+
 def copy_calibration(nbytes: int, reps: int = 5) -> None:
     """Plain device-to-device copies of nbytes (torch's elementwise copy
     kernel): the reference the HBM counters FETCH_SIZE / WRITE_SIZE are
@@ -301,6 +376,9 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--only-diff", action="store_true", help="run only the differentiable leg (profiling)")
     ap.add_argument("--diff-steps", type=int, default=10)
+    ap.add_argument("--no-pixels", action="store_true", help="skip the glDrawPixels leg")
+    ap.add_argument("--only-pixels", action="store_true", help="run only the glDrawPixels leg (profiling)")
+    ap.add_argument("--pixels-calls", type=int, default=50)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -317,6 +395,9 @@ def main():
         return
     if args.only_diff:
         print(json.dumps(diff_bench(args.diff_steps, 2, 0)), flush=True)
+        return
+    if args.only_pixels:
+        print(json.dumps(pixels_bench(args.pixels_calls, 0)), flush=True)
         return
 
     g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
@@ -368,6 +449,7 @@ def main():
     batched = None if args.no_batch else batch_bench(args.batch_launches, args.batch_per_draw,
                                                       0.0 if args.no_cpu else args.cpu_seconds / 2)
     differentiable = None if args.no_diff else diff_bench(args.diff_steps, 2, 0 if args.no_cpu else 2)
+    pixels = None if args.no_pixels else pixels_bench(args.pixels_calls, 0 if args.no_cpu else 5)
 
     out = {
         "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
@@ -395,6 +477,7 @@ def main():
         "cpu_baseline": cpu_obj,
         "batched": batched,
         "differentiable": differentiable,
+        "draw_pixels": pixels,
     }
     if cpu_obj:
         out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None

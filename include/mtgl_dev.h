@@ -192,7 +192,8 @@ typedef struct {
     uint64_t triangles_raster;  /* fan triangles reaching setup */
     uint64_t bin_pairs;         /* (tile, triangle) pairs binned */
     uint64_t draws;
-    double   raster_ms;         /* summed duration of the fragment kernel launches,
+    double   raster_ms;         /* summed duration of the fragment kernel launches
+                                   (the draws' fragment stages and glDrawPixels),
                                    measured with events on the context stream when
                                    timing is enabled */
     uint64_t raster_launches;
@@ -236,6 +237,30 @@ int  mtgl_dev_buffer_free(mtgl_dev *dev, uint32_t id);
  * dispatch (reference src/gl_api.c:648-659, flush_* at src/raster.c:961-1231)
  * and the array draws glDrawArrays / glDrawElements (src/gl_api.c:1799-1941). */
 int  mtgl_dev_draw(mtgl_dev *dev, const mtgl_draw *draw);
+
+/* glDrawPixels: width x height GL_UNSIGNED_BYTE pixels of `format` (GL_RGBA,
+ * GL_RGB, GL_LUMINANCE or GL_LUMINANCE_ALPHA; rows bottom-up, tightly packed),
+ * pixel (col, row) landing on window (x + col, y + row) in GL convention (y up
+ * from the bottom row).  Alpha test, depth test (incoming depth 0) and
+ * blending follow `flags` (MTGL_F_ALPHA_TEST, MTGL_F_DEPTH_TEST, MTGL_F_BLEND;
+ * other bits are ignored); no scissor, stencil, fog or colour mask applies
+ * (reference glDrawPixels, src/gl_api.c:1284-1373, with its own compare and
+ * blend-factor helpers :1231-1282).  `pixels` is host memory, copied by the
+ * call. */
+typedef struct {
+    int32_t  x, y;              /* raster position (reference raster_pos_x / _y) */
+    int32_t  width, height;
+    uint32_t format;
+    uint32_t flags;
+    uint32_t alpha_func;
+    float    alpha_ref;
+    uint32_t depth_func;
+    uint32_t depth_mask;
+    uint32_t blend_src, blend_dst;
+    const void *pixels;
+} mtgl_pixels;
+
+int  mtgl_dev_draw_pixels(mtgl_dev *dev, const mtgl_pixels *pixels);
 
 /* Counters and timing; no counterpart in the reference (fragments passing
  * coverage, algorithmic bytes of the per-fragment pipeline, fragment-stage time). */

@@ -37,7 +37,8 @@ void gls_report_device_error(GLState *c, int rc, const char *what)
     if (rc == MTGL_OK) return;
     if (rc == MTGL_ERR_UNSUPPORTED) {
         if (!(c->warned & 1u)) {
-            fprintf(stderr, "libmtgl: %s is not implemented on the device pipeline; skipped\n", what);
+            fprintf(stderr, "libmtgl: %s: the device pipeline refused this call (unsupported state); skipped\n",
+                    what);
             c->warned |= 1u;
         }
         return;
@@ -1109,12 +1110,31 @@ MTGL_EXPORT void glReadPixels(GLint x, GLint y, GLsizei width, GLsizei height, G
     free(row);
 }
 
+/* Pixels at the raster position with alpha test, depth test and blending
+ * (reference src/gl_api.c:1284-1373).  Formats other than the four the
+ * reference reads draw nothing, as there. */
 MTGL_EXPORT void glDrawPixels(GLsizei width, GLsizei height, GLenum format, GLenum type, const GLvoid *pixels)
 {
     CTX_OR_RETURN();
     if (type != GL_UNSIGNED_BYTE || !pixels || !c->raster_valid) return;
-    (void)width; (void)height; (void)format;
-    gls_report_device_error(c, MTGL_ERR_UNSUPPORTED, "glDrawPixels");
+    if (width <= 0 || height <= 0) return;
+    if (format != GL_RGBA && format != GL_RGB && format != GL_LUMINANCE && format != GL_LUMINANCE_ALPHA) return;
+    mtgl_pixels p;
+    memset(&p, 0, sizeof(p));
+    p.x = c->raster_x;
+    p.y = c->raster_y;
+    p.width = width;
+    p.height = height;
+    p.format = format;
+    p.flags = c->flags & (MTGL_F_ALPHA_TEST | MTGL_F_DEPTH_TEST | MTGL_F_BLEND);
+    p.alpha_func = c->alpha_func;
+    p.alpha_ref = c->alpha_ref;
+    p.depth_func = c->depth_func;
+    p.depth_mask = c->depth_mask ? 1u : 0u;
+    p.blend_src = c->blend_src;
+    p.blend_dst = c->blend_dst;
+    p.pixels = pixels;
+    gls_report_device_error(c, mtgl_dev_draw_pixels(c->dev, &p), "glDrawPixels");
 }
 
 /* window position of a raster position (reference src/gl_api.c:1375-1424) */

@@ -2342,6 +2342,447 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
 // K5 clear, K6 level-1 box filter
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// K8: points, lines and glPolygonMode LINE / POINT (reference
+// src/raster.c:65-296, :776-898, :1019-1196).  One thread per primitive
+// (point, segment or assembled triangle) walks the reference's loops twice:
+// first counting its fragments, then, at its offset in the scanned counts,
+// writing each fragment's pixel, depth and colour in the reference's order.
+// A stable sort by pixel keeps that order per pixel; one thread per pixel
+// then runs depth test, blend, depth write and masked colour write.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t PRIM_NO_PIXEL = 0xFFFFFFFFu;   // fragment key outside the framebuffer or scissor, or discarded
+constexpr uint64_t PRIM_MAX_FRAGS = 1ull << 30;   // fragments one primitive may produce
+
+struct PrimParams {
+    SceneRaster S;                     // state, texture, framebuffer size
+    uint32_t mode, count, units;
+    uint32_t tri;                      // 1: units are assembled triangles (glPolygonMode LINE / POINT)
+    const VHot *vhot;
+    const VCold *vcold;                // null when no per-vertex lighting reads it
+    unsigned long long *counts;        // count pass: fragments per unit
+    const unsigned long long *base;    // emit pass: first fragment of each unit
+    uint32_t *keys, *vals;
+    float *dep;
+    float4 *col;
+    Counters *ctr;
+};
+
+__device__ __forceinline__ void prim_load(const PrimParams &P, uint32_t v, CV &o)
+{
+    const float4 *h = reinterpret_cast<const float4 *>(&P.vhot[v]);
+    const float4 a = h[0], b = h[1], c = h[2];
+    o.x = a.x; o.y = a.y; o.z = a.z; o.w = a.w;
+    o.r = b.x; o.g = b.y; o.b = b.z; o.a = b.w;
+    o.s = c.x; o.t = c.y; o.ez = c.z;
+    for (int k = 0; k < 3; k++) {
+        o.ep[k] = P.vcold ? P.vcold[v].ep[k] : 0.0f;
+        o.en[k] = P.vcold ? P.vcold[v].en[k] : 0.0f;
+    }
+}
+
+// (z + 1) * 0.5 * (far - near) + near in double, stored as float
+__device__ __forceinline__ float prim_depth(const RasterConsts &K, float z)
+{
+    return (float)((double)((z + 1.0f) * 0.5f) * (K.depth_far - K.depth_near) + K.depth_near);
+}
+
+// fog of lines and points: coordinate -eye_z (the triangles use +eye_z),
+// rgb towards the fragment, alpha of the fog colour
+__device__ __forceinline__ Col prim_fog(const RasterConsts &K, Col c, float fog_coord)
+{
+    float f = 1.0f;
+    if (K.fog_mode == GL_LINEAR) {
+        if (K.fog_end != K.fog_start) f = (K.fog_end - fog_coord) / (K.fog_end - K.fog_start);
+    } else if (K.fog_mode == GL_EXP) {
+        f = expf(-K.fog_density * fog_coord);
+    } else if (K.fog_mode == GL_EXP2) {
+        const float d = K.fog_density * fog_coord;
+        f = expf(-d * d);
+    }
+    if (f < 0.0f) f = 0.0f;
+    if (f > 1.0f) f = 1.0f;
+    const Col fg = K.fog_color;
+    return {fg.r + (c.r - fg.r) * f, fg.g + (c.g - fg.g) * f, fg.b + (c.b - fg.b) * f, fg.a};
+}
+
+// pixel key of a fragment: bounds and scissor (raw scissor y against
+// top-down rows, as the reference compares them)
+__device__ __forceinline__ uint32_t prim_key(const PrimParams &P, const RasterConsts &K, int32_t px, int32_t py)
+{
+    if (px < 0 || px >= P.S.fb_w || py < 0 || py >= P.S.fb_h) return PRIM_NO_PIXEL;
+    if (K.rs->flags & MTGL_F_SCISSOR_TEST) {
+        const int32_t *sc = K.rs->scissor;
+        if (px < sc[0] || px >= sc[0] + sc[2] || py < sc[1] || py >= sc[1] + sc[3]) return PRIM_NO_PIXEL;
+    }
+    return (uint32_t)py * (uint32_t)P.S.fb_w + (uint32_t)px;
+}
+
+__device__ __forceinline__ void prim_put(const PrimParams &P, uint64_t j, uint32_t key, float depth, Col c)
+{
+    P.keys[j] = key;
+    P.vals[j] = (uint32_t)j;
+    P.dep[j] = depth;
+    P.col[j] = make_float4(c.r, c.g, c.b, c.a);
+}
+
+__device__ __forceinline__ int prim_abs(int32_t v) { return v < 0 ? (int32_t)(0u - (uint32_t)v) : v; }
+
+// reference draw_line_full (src/raster.c:107-246); returns the fragments it
+// produces: one per step and width offset.  Where the alpha test rejects a
+// pixel the reference's loop continues without stepping and never ends;
+// here the step's fragments are discarded and the line goes on.
+template <bool EMIT>
+__device__ uint64_t prim_line(const PrimParams &P, const RasterConsts &K, int32_t x0, int32_t y0, float z0, int32_t x1,
+                              int32_t y1, float z1, Col c0, Col c1, float ez0, float ez1, float u0, float v0, float u1,
+                              float v1, uint64_t base)
+{
+    const int32_t dx = isub(x1, x0), dy = isub(y1, y0);
+    const int32_t adx = prim_abs(dx), ady = prim_abs(dy);
+    int lw = (int)(K.rs->line_width + 0.5f);
+    if (lw < 1) lw = 1;
+    if (adx < 0 || ady < 0) return 0;   // a coordinate at INT_MIN: no clipped segment gets there
+    const uint64_t steps = (uint64_t)(adx > ady ? adx : ady) + 1u;
+    const uint64_t n = steps * (uint64_t)lw;
+    if (n > PRIM_MAX_FRAGS) {
+        if (!EMIT) atomicOr(&P.ctr->unsupported, 2u);
+        return 0;
+    }
+    if (!EMIT) return n;
+    const int32_t sx = dx < 0 ? -1 : 1, sy = dy < 0 ? -1 : 1;
+    int32_t err = adx - ady;
+    int32_t total = adx > ady ? adx : ady;
+    if (total == 0) total = 1;
+    const int half = lw / 2;
+    const int ex = adx > ady ? 0 : 1, ey = adx > ady ? 1 : 0;
+    int32_t cx = x0, cy = y0;
+    for (uint64_t step = 0; step < steps; step++) {
+        const float t = (float)(int32_t)step / (float)total;
+        const float z = z0 + t * (z1 - z0);
+        const float depth = prim_depth(K, z);
+        Col c = col_lerp(c0, c1, t);
+        bool keep = true;
+        if (K.textured) {
+            const float u = u0 + t * (u1 - u0), v = v0 + t * (v1 - v0);
+            int taps = 0;
+            const Col tc = col_unpack(sample_texture(K, K.L0, K.L1, u, v, 0.0f, taps));
+            if (K.alpha_on && !cmp_f(K.alpha_cmp, tc.a, K.alpha_ref)) keep = false;
+            else c = col_mul(c, tc);
+        } else if (K.alpha_on && !cmp_f(K.alpha_cmp, c.a, K.alpha_ref)) {
+            keep = false;
+        }
+        if (keep && K.fog_on) c = prim_fog(K, c, -(ez0 + t * (ez1 - ez0)));
+        const uint64_t j = base + step * (uint64_t)lw;
+        for (int w = 0; w < lw; w++) {
+            const int o = lw == 1 ? 0 : w - half;
+            const uint32_t key = keep ? prim_key(P, K, cx + o * ex, cy + o * ey) : PRIM_NO_PIXEL;
+            prim_put(P, j + (uint64_t)w, key, depth, c);
+        }
+        const int32_t e2 = err * 2;
+        if (e2 > -ady) { err -= ady; cx += sx; }
+        if (e2 < adx) { err += adx; cy += sy; }
+    }
+    return n;
+}
+
+__device__ __forceinline__ int prim_outcode(const CV &v)
+{
+    int code = 0;
+    if (v.x < -v.w) code |= 1; else if (v.x > v.w) code |= 2;
+    if (v.y < -v.w) code |= 4; else if (v.y > v.w) code |= 8;
+    if (v.z < -v.w) code |= 16; else if (v.z > v.w) code |= 32;
+    return code;
+}
+
+// reference clip_line (src/clipping.h:152-230): each pass moves one endpoint
+// onto a plane, so a finite segment needs at most 12; the bound only keeps a
+// NaN-free loop finite by construction
+__device__ bool prim_clip_line(CV &a, CV &b, bool cold)
+{
+    int c0 = prim_outcode(a), c1 = prim_outcode(b);
+    for (int it = 0; it < 64; it++) {
+        if (!(c0 | c1)) return true;
+        if (c0 & c1) return false;
+        const int out = c0 ? c0 : c1;
+        const int plane = (out & 1) ? 2 : (out & 2) ? 3 : (out & 4) ? 4 : (out & 8) ? 5 : (out & 16) ? 0 : 1;
+        const float d0 = plane_dist(a, plane), d1 = plane_dist(b, plane);
+        const float den = d0 - d1;
+        if (fabsf(den) < 1e-10f) return false;
+        CV cl;
+        cv_lerp_snap(a, b, d0 / den, plane, cold, cl);
+        if (out == c0) { a = cl; c0 = prim_outcode(a); }
+        else { b = cl; c1 = prim_outcode(b); }
+    }
+    return false;
+}
+
+__device__ __forceinline__ int32_t prim_sx(const mtgl_raster_state *rs, float x)
+{
+    return cvtt((x + 1.0f) * 0.5f * (float)rs->viewport[2] + (float)rs->viewport[0]);
+}
+__device__ __forceinline__ int32_t prim_sy(const mtgl_raster_state *rs, float y)
+{
+    return cvtt((1.0f - y) * 0.5f * (float)rs->viewport[3] + (float)rs->viewport[1]);
+}
+
+// reference draw_line_segment (src/raster.c:249-288)
+template <bool EMIT>
+__device__ uint64_t prim_segment(const PrimParams &P, const RasterConsts &K, uint32_t ia, uint32_t ib, uint64_t base)
+{
+    CV a, b;
+    prim_load(P, ia, a);
+    prim_load(P, ib, b);
+    if (!prim_clip_line(a, b, P.vcold != nullptr)) return 0;
+    float z0 = 0.0f, z1 = 0.0f;
+    if (fabsf(a.w) >= 1e-6f) { const float iw = 1.0f / a.w; a.x *= iw; a.y *= iw; z0 = a.z * iw; }
+    else { a.x = 0.0f; a.y = 0.0f; }
+    if (fabsf(b.w) >= 1e-6f) { const float iw = 1.0f / b.w; b.x *= iw; b.y *= iw; z1 = b.z * iw; }
+    else { b.x = 0.0f; b.y = 0.0f; }
+    const mtgl_raster_state *rs = K.rs;
+    return prim_line<EMIT>(P, K, prim_sx(rs, a.x), prim_sy(rs, a.y), z0, prim_sx(rs, b.x), prim_sy(rs, b.y), z1,
+                           Col{a.r, a.g, a.b, a.a}, Col{b.r, b.g, b.b, b.a}, a.ez, b.ez, a.s, a.t, b.s, b.t, base);
+}
+
+// reference flush_points, one vertex (src/raster.c:1043-1155)
+template <bool EMIT>
+__device__ uint64_t prim_point(const PrimParams &P, const RasterConsts &K, uint32_t i, uint64_t base)
+{
+    CV q;
+    prim_load(P, i, q);
+    if (q.x < -q.w || q.x > q.w || q.y < -q.w || q.y > q.w || q.z < -q.w || q.z > q.w || q.w <= 0.0f) return 0;
+    int ps = (int)(K.rs->point_size + 0.5f);
+    if (ps < 1) ps = 1;
+    const uint64_t n = (uint64_t)ps * (uint64_t)ps;
+    if (!EMIT) return n;
+    const float nx = q.x / q.w, ny = q.y / q.w, nz = q.z / q.w;
+    const int32_t cx = prim_sx(K.rs, nx), cy = prim_sy(K.rs, ny);
+    const float depth = prim_depth(K, nz);
+    Col c = {q.r, q.g, q.b, q.a};
+    bool keep = true;
+    if (K.textured) {
+        int taps = 0;
+        const Col tc = col_unpack(sample_texture(K, K.L0, K.L1, q.s, q.t, 0.0f, taps));
+        if (K.alpha_on && !cmp_f(K.alpha_cmp, tc.a, K.alpha_ref)) keep = false;
+        else c = col_mul(c, tc);
+    } else if (K.alpha_on && !cmp_f(K.alpha_cmp, c.a, K.alpha_ref)) {
+        keep = false;
+    }
+    if (keep && K.fog_on) c = prim_fog(K, c, -q.ez);
+    const int half = ps / 2;
+    uint64_t j = base;
+    for (int py = 0; py < ps; py++)
+        for (int px = 0; px < ps; px++, j++)
+            prim_put(P, j, keep ? prim_key(P, K, cx - half + px, cy - half + py) : PRIM_NO_PIXEL, depth, c);
+    return n;
+}
+
+// reference render_triangle with a LINE or POINT polygon mode
+// (src/raster.c:901-958, draw_triangle_wireframe / draw_triangle_points
+// :841-898): clip, divide, fan, cull, then per fan triangle three lines or
+// three one-pixel points at the corners (no texture for points; GL_PHONG
+// corners lit per vertex with the front material)
+template <bool EMIT>
+__device__ uint64_t prim_triangle(const PrimParams &P, const RasterConsts &K, uint32_t t, uint64_t base)
+{
+    const mtgl_raster_state *rs = K.rs;
+    const bool cold = P.vcold != nullptr;
+    uint32_t ia, ib, ic;
+    tri_vertices(P.mode, t, ia, ib, ic);
+    CV a[MAX_CLIP], b[MAX_CLIP];
+    prim_load(P, ia, b[0]);
+    prim_load(P, ib, b[1]);
+    prim_load(P, ic, b[2]);
+    int n = clip_plane(b, 3, a, 0, cold);
+    n = n ? clip_plane(a, n, b, 1, cold) : 0;
+    n = n ? clip_plane(b, n, a, 2, cold) : 0;
+    n = n ? clip_plane(a, n, b, 3, cold) : 0;
+    n = n ? clip_plane(b, n, a, 4, cold) : 0;
+    n = n ? clip_plane(a, n, b, 5, cold) : 0;
+    if (n < 3) return 0;
+    for (int k = 0; k < n; k++) perspective_divide(b[k]);
+    const bool phong = K.lighting_on && K.phong;
+    const mtgl_material &m = rs->lighting.front;
+    uint64_t total = 0;
+    for (int j = 1; j + 1 < n; j++) {
+        const CV *v[3] = {&b[0], &b[j], &b[j + 1]};
+        int32_t X[3], Y[3];
+        for (int k = 0; k < 3; k++) {
+            X[k] = prim_sx(rs, v[k]->x);
+            Y[k] = prim_sy(rs, v[k]->y);
+        }
+        const float sa = (float)isub(X[1], X[0]) * (float)isub(Y[2], Y[0]) -
+                         (float)isub(X[2], X[0]) * (float)isub(Y[1], Y[0]);
+        if (rs->flags & MTGL_F_CULL_FACE) {
+            const int front = rs->front_face == GL_CCW ? (sa < 0) : (sa > 0);
+            const int cull = rs->cull_face_mode == GL_FRONT ? front : (rs->cull_face_mode == GL_BACK ? !front : 1);
+            if (cull) continue;
+        }
+        const int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
+        const uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
+        Col c[3];
+        for (int k = 0; k < 3; k++) {
+            c[k] = phong ? compute_lighting(&rs->lighting, rs->flags, v[k]->ep, v[k]->en, col_of(m.ambient),
+                                            col_of(m.diffuse), col_of(m.specular), col_of(m.emission), m.shininess)
+                         : Col{v[k]->r, v[k]->g, v[k]->b, v[k]->a};
+        }
+        if (pm == GL_LINE) {
+            for (int k = 0; k < 3; k++) {
+                const int q = (k + 1) % 3;
+                total += prim_line<EMIT>(P, K, X[k], Y[k], v[k]->z, X[q], Y[q], v[q]->z, c[k], c[q], v[k]->ez,
+                                         v[q]->ez, v[k]->s, v[k]->t, v[q]->s, v[q]->t, base + total);
+            }
+        } else {
+            for (int k = 0; k < 3; k++) {
+                if (EMIT) {
+                    Col ck = c[k];
+                    bool keep = !(K.alpha_on && !cmp_f(K.alpha_cmp, ck.a, K.alpha_ref));
+                    if (keep && K.fog_on) ck = prim_fog(K, ck, -v[k]->ez);
+                    prim_put(P, base + total, keep ? prim_key(P, K, X[k], Y[k]) : PRIM_NO_PIXEL,
+                             prim_depth(K, v[k]->z), ck);
+                }
+                total += 1;
+            }
+        }
+    }
+    return total;
+}
+
+template <bool EMIT>
+__global__ void __launch_bounds__(64) k_prim(PrimParams P)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= P.units) return;
+    RasterConsts K;
+    raster_consts(P.S, K);
+    const uint64_t base = EMIT ? P.base[u] : 0;
+    uint64_t n;
+    if (P.tri) {
+        n = prim_triangle<EMIT>(P, K, u, base);
+    } else if (P.mode == GL_POINTS) {
+        n = prim_point<EMIT>(P, K, u, base);
+    } else {
+        uint32_t ia, ib;
+        if (P.mode == GL_LINES) { ia = 2 * u; ib = ia + 1; }
+        else { ia = u; ib = u + 1 < P.count ? u + 1 : 0; }   // strip; the loop's closing segment wraps to vertex 0
+        n = prim_segment<EMIT>(P, K, ia, ib, base);
+    }
+    if (!EMIT) P.counts[u] = n;
+}
+
+// One thread per pixel of the sorted fragments (the first of its run): the
+// reference's per-pixel steps over the pixel's fragments in order.
+__global__ void __launch_bounds__(256) k_prim_resolve(SceneRaster S, const uint32_t *keys, const uint32_t *vals,
+                                                      uint32_t n, const float *dep, const float4 *col, uint32_t npix)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (k >= npix || (i > 0 && keys[i - 1] == k)) return;
+    RasterConsts K;
+    raster_consts(S, K);
+    PixelState px = {S.color[k], S.depth[k], 0u, false, false, false};
+    for (uint32_t j = i; j < n && keys[j] == k; j++) {
+        const uint32_t f = vals[j];
+        const float d = dep[f];
+        if (K.depth_on && !cmp_f(K.depth_cmp, d, px.d)) continue;
+        if (K.depth_write) {
+            px.d = d;
+            px.dirty_d = true;
+        }
+        const float4 c = col[f];
+        color_step(K, Col{c.x, c.y, c.z, c.w}, px);
+    }
+    if (px.dirty_d) S.depth[k] = px.d;
+    if (px.dirty_c) S.color[k] = px.c;
+}
+
+// ---------------------------------------------------------------------------
+// glDrawPixels (reference src/gl_api.c:1231-1373): each source pixel lands
+// on its own framebuffer pixel, so one thread per pixel needs no ordering
+// ---------------------------------------------------------------------------
+
+struct PixDesc {
+    int32_t x, y, w, h, fb_w, fb_h;
+    uint32_t format, flags, alpha_func, depth_func, depth_mask, blend_src, blend_dst;
+    float alpha_ref;
+};
+
+// the reference's drawpixels_alpha_test / drawpixels_depth_test switch
+__device__ __forceinline__ bool dp_compare(uint32_t func, float a, float b)
+{
+    switch (func) {
+    case GL_NEVER: return false;
+    case GL_LESS: return a < b;
+    case GL_EQUAL: return a == b;
+    case GL_LEQUAL: return a <= b;
+    case GL_GREATER: return a > b;
+    case GL_NOTEQUAL: return a != b;
+    case GL_GEQUAL: return a >= b;
+    default: return true;
+    }
+}
+
+// the reference's drawpixels_blend_factor (GL_CONSTANT_* and unknown: one)
+__device__ __forceinline__ Col dp_factor(uint32_t f, Col s, Col d)
+{
+    switch (f) {
+    case GL_ZERO: return {0.0f, 0.0f, 0.0f, 0.0f};
+    case GL_SRC_COLOR: return s;
+    case GL_ONE_MINUS_SRC_COLOR: return {1.0f - s.r, 1.0f - s.g, 1.0f - s.b, 1.0f - s.a};
+    case GL_DST_COLOR: return d;
+    case GL_ONE_MINUS_DST_COLOR: return {1.0f - d.r, 1.0f - d.g, 1.0f - d.b, 1.0f - d.a};
+    case GL_SRC_ALPHA: return {s.a, s.a, s.a, s.a};
+    case GL_ONE_MINUS_SRC_ALPHA: return {1.0f - s.a, 1.0f - s.a, 1.0f - s.a, 1.0f - s.a};
+    case GL_DST_ALPHA: return {d.a, d.a, d.a, d.a};
+    case GL_ONE_MINUS_DST_ALPHA: return {1.0f - d.a, 1.0f - d.a, 1.0f - d.a, 1.0f - d.a};
+    case GL_SRC_ALPHA_SATURATE: {
+        const float f1 = (s.a < (1.0f - d.a)) ? s.a : (1.0f - d.a);
+        return {f1, f1, f1, 1.0f};
+    }
+    default: return {1.0f, 1.0f, 1.0f, 1.0f};
+    }
+}
+
+__global__ void __launch_bounds__(256) k_draw_pixels(PixDesc P, const uint8_t *src, uint32_t *color, float *depth)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)P.w * (uint32_t)P.h) return;
+    const int32_t row = (int32_t)(i / (uint32_t)P.w), col = (int32_t)(i % (uint32_t)P.w);
+    const int32_t fb_y = P.fb_h - 1 - (P.y + row);
+    const int32_t dst_x = P.x + col;
+    if (fb_y < 0 || fb_y >= P.fb_h || dst_x < 0 || dst_x >= P.fb_w) return;
+    uint32_t r, g, b, a = 255u;
+    if (P.format == GL_RGBA) {
+        const uint32_t v = reinterpret_cast<const uint32_t *>(src)[i];   // src is 256-byte aligned
+        r = v & 0xFFu; g = (v >> 8) & 0xFFu; b = (v >> 16) & 0xFFu; a = v >> 24;
+    } else if (P.format == GL_RGB) {
+        const uint8_t *q = src + 3u * (size_t)i;
+        r = q[0]; g = q[1]; b = q[2];
+    } else if (P.format == GL_LUMINANCE) {
+        r = g = b = src[i];
+    } else {
+        const uint8_t *q = src + 2u * (size_t)i;
+        r = g = b = q[0];
+        a = q[1];
+    }
+    const size_t pix = (size_t)fb_y * (size_t)P.fb_w + (size_t)dst_x;
+    const bool depth_on = (P.flags & MTGL_F_DEPTH_TEST) != 0, blend_on = (P.flags & MTGL_F_BLEND) != 0;
+    // both framebuffer reads are issued before either is needed
+    const float stored = depth_on ? depth[pix] : 0.0f;
+    const uint32_t dst = blend_on ? color[pix] : 0u;
+    if ((P.flags & MTGL_F_ALPHA_TEST) && !dp_compare(P.alpha_func, (float)a / 255.0f, P.alpha_ref)) return;
+    if (depth_on && !dp_compare(P.depth_func, 0.0f, stored)) return;
+    Col c = {(float)r / 255.0f, (float)g / 255.0f, (float)b / 255.0f, (float)a / 255.0f};
+    if (blend_on) {
+        const Col d = col_unpack(dst);
+        const Col sf = dp_factor(P.blend_src, c, d), df = dp_factor(P.blend_dst, c, d);
+        c = col_clamp(col_add(col_mul(c, sf), col_mul(d, df)));
+    }
+    if (depth_on && P.depth_mask) depth[pix] = 0.0f;
+    color[pix] = col_pack(c);
+}
+
 __global__ void __launch_bounds__(256) k_clear(uint32_t *color, float *depth, uint8_t *stencil, int32_t fb_w,
                                                int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t cv, float dv,
                                                uint8_t sv)
@@ -2857,6 +3298,9 @@ struct mtgl_dev {
     float *depth = nullptr;
     uint8_t *stencil = nullptr;
     Arena writer;                  // per pixel: last passing fragment (depth kernel)
+    Arena pixels;                  // glDrawPixels source bytes
+    Arena prim_params;             // states, staged bytes and vertex parameters of a point / line draw
+    Arena prim_dep, prim_col;      // fragments of a point / line draw: depth, colour
     DevTex tex[MTGL_MAX_OBJECTS + 1];
     DevBuf buf[MTGL_MAX_OBJECTS + 1];
     Exec ex;
@@ -3518,6 +3962,164 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
 }
 
+// ---- points, lines and glPolygonMode LINE / POINT ----
+
+// 1: a point or line mode; 2: a triangle mode whose faces are both drawn as
+// lines or points; 0: filled triangles (the tile pipeline); 3: a triangle
+// mode with one face filled and the other not (refused)
+int prim_class(const mtgl_draw *dr)
+{
+    switch (dr->mode) {
+    case GL_POINTS: case GL_LINES: case GL_LINE_STRIP: case GL_LINE_LOOP: return 1;
+    default: break;
+    }
+    const bool ff = dr->raster->polygon_mode_front == GL_FILL, bf = dr->raster->polygon_mode_back == GL_FILL;
+    if (ff && bf) return 0;
+    return (!ff && !bf) ? 2 : 3;
+}
+
+uint32_t prim_units(const mtgl_draw *dr, int cls)
+{
+    const uint32_t n = dr->count;
+    if (cls == 2) return tri_count(dr->mode, n);
+    switch (dr->mode) {
+    case GL_POINTS: return n;
+    case GL_LINES: return n / 2;
+    case GL_LINE_STRIP: return n >= 2 ? n - 1 : 0;
+    default: return n >= 2 ? n : 0;   // GL_LINE_LOOP
+    }
+}
+
+int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
+{
+    Exec &x = d->ex;
+    const uint32_t units = prim_units(dr, cls);
+    if (units == 0 || dr->count == 0) return MTGL_OK;
+    const bool cold = needs_cold(dr->raster);
+    const DevTex *tx = bound_texture(d, dr->raster);
+    // parameters: raster state, vertex states, staged bytes, vertex parameters, a zero block prefix
+    const size_t off_rs = 0;
+    size_t bytes = up256(sizeof(mtgl_raster_state));
+    const size_t off_vs = bytes;
+    bytes += up256((size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+    const size_t off_st = bytes;
+    bytes += up256((size_t)dr->staged_bytes);
+    const size_t off_vp = bytes;
+    bytes += up256(sizeof(VertexParams));
+    const size_t off_pre = bytes;
+    bytes += 256;
+    int rc;
+    if ((rc = arena_reserve(x, d->prim_params, bytes))) return rc;
+    if ((rc = pinned_reserve(x, bytes))) return rc;
+    if ((rc = arena_reserve(x, x.vhot, (size_t)dr->count * sizeof(VHot)))) return rc;
+    if (cold && (rc = arena_reserve(x, x.vcold, (size_t)dr->count * sizeof(VCold)))) return rc;
+    if ((rc = arena_reserve(x, x.tri_tiles, ((size_t)units + 1) * 8))) return rc;
+    if ((rc = arena_reserve(x, x.pair_base, ((size_t)units + 1) * 8))) return rc;
+    uint8_t *host = x.pinned;
+    const uint8_t *dev = (const uint8_t *)d->prim_params.ptr;
+    memcpy(host + off_rs, dr->raster, sizeof(mtgl_raster_state));
+    memcpy(host + off_vs, dr->vstates, (size_t)dr->n_vstates * sizeof(mtgl_vertex_state));
+    if (dr->staged_bytes) memcpy(host + off_st, dr->staged, (size_t)dr->staged_bytes);
+    const uint8_t *staged_dev = dev + off_st;
+    VertexParams v = {};
+    v.count = dr->count;
+    v.first = dr->first;
+    v.index_type = dr->index_type;
+    v.n_vstates = dr->n_vstates;
+    v.indices = resolve(d, dr->indices, staged_dev, dr->staged_bytes);
+    v.indices.enabled = dr->index_type != 0;
+    v.position = resolve(d, dr->position, staged_dev, dr->staged_bytes);
+    v.color = resolve(d, dr->color, staged_dev, dr->staged_bytes);
+    v.texcoord = resolve(d, dr->texcoord, staged_dev, dr->staged_bytes);
+    v.normal = resolve(d, dr->normal, staged_dev, dr->staged_bytes);
+    v.state_index = resolve(d, dr->state_index, staged_dev, dr->staged_bytes);
+    memcpy(v.cur_color, dr->cur_color, sizeof(v.cur_color));
+    memcpy(v.cur_texcoord, dr->cur_texcoord, sizeof(v.cur_texcoord));
+    memcpy(v.cur_normal, dr->cur_normal, sizeof(v.cur_normal));
+    v.vstates = (const mtgl_vertex_state *)(dev + off_vs);
+    v.out_hot = as<VHot>(x.vhot);
+    v.out_cold = cold ? as<VCold>(x.vcold) : nullptr;
+    memcpy(host + off_vp, &v, sizeof(v));
+    memset(host + off_pre, 0, 256);
+    HIPC(hipMemcpyAsync(d->prim_params.ptr, host, bytes, hipMemcpyHostToDevice, x.stream));
+    HIPC(hipEventRecord(x.upload_done, x.stream));
+
+    unsigned long long *counts = as<unsigned long long>(x.tri_tiles);
+    unsigned long long *base = as<unsigned long long>(x.pair_base);
+    hipLaunchKernelGGL(k_vertex, dim3(blocks_for(dr->count, 256)), dim3(256), 0, x.stream,
+                       (const VertexParams *)(dev + off_vp), (const uint32_t *)(dev + off_pre), (const uint32_t *)nullptr,
+                       1u, x.ctr, counts + units);
+
+    PrimParams P = {};
+    SceneRaster &S = P.S;
+    S.rs = (const mtgl_raster_state *)(dev + off_rs);
+    S.color = d->color;
+    S.depth = d->depth;
+    S.stencil = d->stencil;
+    S.tex = tx ? tx->px : nullptr;
+    S.tex1 = tx ? tx->m1 : nullptr;
+    S.tex_w = tx ? tx->w : 0;
+    S.tex_h = tx ? tx->h : 0;
+    S.tex1_w = tx ? tx->m1w : 0;
+    S.tex1_h = tx ? tx->m1h : 0;
+    S.fb_w = d->w;
+    S.fb_h = d->h;
+    S.tiles_x = d->tiles_x;
+    P.mode = dr->mode;
+    P.count = dr->count;
+    P.units = units;
+    P.tri = cls == 2 ? 1u : 0u;
+    P.vhot = as<VHot>(x.vhot);
+    P.vcold = cold ? as<VCold>(x.vcold) : nullptr;
+    P.counts = counts;
+    P.base = base;
+    P.ctr = x.ctr;
+    hipLaunchKernelGGL(k_prim<false>, dim3(blocks_for(units, 64)), dim3(64), 0, x.stream, P);
+    HIPC(hipGetLastError());
+    if ((rc = device_scan<unsigned long long>(x, counts, base, (uint64_t)units, false, &x.ctr->npairs))) return rc;
+    HIPC(hipMemcpyAsync(x.ctr_host, x.ctr, sizeof(Counters), hipMemcpyDeviceToHost, x.stream));
+    HIPC(hipStreamSynchronize(x.stream));
+    if (x.ctr_host->unsupported & 2u) return MTGL_ERR_OUT_OF_MEMORY;   // one primitive over PRIM_MAX_FRAGS
+    const uint64_t total = x.ctr_host->npairs;
+    if (total == 0) return MTGL_OK;
+    if (total > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    const uint32_t n = (uint32_t)total;
+    for (int k = 0; k < 2; k++) {
+        if ((rc = arena_reserve(x, x.keys[k], (size_t)n * 4))) return rc;
+        if ((rc = arena_reserve(x, x.vals[k], (size_t)n * 4))) return rc;
+    }
+    if ((rc = arena_reserve(x, d->prim_dep, (size_t)n * 4))) return rc;
+    if ((rc = arena_reserve(x, d->prim_col, (size_t)n * 16))) return rc;
+    uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
+    uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
+    P.keys = keys;
+    P.vals = vals;
+    P.dep = as<float>(d->prim_dep);
+    P.col = as<float4>(d->prim_col);
+    hipLaunchKernelGGL(k_prim<true>, dim3(blocks_for(units, 64)), dim3(64), 0, x.stream, P);
+    // stable LSD radix sort by pixel; the all-ones key of discarded fragments stays last
+    const uint32_t npix = (uint32_t)d->w * (uint32_t)d->h;
+    int key_bits = 1;
+    while (key_bits < 32 && (1u << key_bits) <= npix) key_bits++;
+    const uint32_t nblocks = (n + SORT_TILE - 1) / SORT_TILE;
+    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)RADIX * nblocks * 4))) return rc;
+    uint32_t *sc = as<uint32_t>(x.sort_tmp);
+    for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, n, shift,
+                           nblocks, sc);
+        if ((rc = device_scan<uint32_t>(x, sc, sc, (uint64_t)RADIX * nblocks, false))) return rc;
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
+                           (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)sc, keys2, vals2);
+        std::swap(keys, keys2);
+        std::swap(vals, vals2);
+    }
+    hipLaunchKernelGGL(k_prim_resolve, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, S, (const uint32_t *)keys,
+                       (const uint32_t *)vals, n, (const float *)as<float>(d->prim_dep),
+                       (const float4 *)as<float4>(d->prim_col), npix);
+    HIPC(hipGetLastError());
+    return MTGL_OK;
+}
+
 // prepare + sizing run, growing the clip overflow pool once when it was too small
 int execute_scenes(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
 {
@@ -3891,6 +4493,10 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
         if (d->buf[i].data) (void)hipFree(d->buf[i].data);
     }
     arena_free(d->writer);
+    arena_free(d->pixels);
+    arena_free(d->prim_params);
+    arena_free(d->prim_dep);
+    arena_free(d->prim_col);
     arena_free(d->one.params);
     arena_free(d->one.maps);
     exec_free(d->ex);
@@ -4096,6 +4702,18 @@ MTGL_API int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
 MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
 {
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
+    const int cls = prim_class(dr);
+    if (cls == 1 || cls == 2) {
+        // a recorded batch replays filled triangles only
+        if (g_recording) return MTGL_ERR_UNSUPPORTED;
+        d->ex.stats.draws++;
+        int rc = dev_join(d);
+        if (rc) return rc;
+        d->last.valid = false;
+        rc = run_prims(d, dr, cls);
+        d->dirty = true;
+        return rc;
+    }
     if (!draw_mode_ok(dr->mode)) return MTGL_ERR_UNSUPPORTED;
     const uint32_t ntri = tri_count(dr->mode, dr->count);
     if (g_recording) {
@@ -4144,6 +4762,63 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         l.tex_h = tx ? tx->h : 0;
     }
     return rc;
+}
+
+MTGL_API int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
+{
+    if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
+    uint32_t bpp;
+    switch (p->format) {
+    case GL_RGBA: bpp = 4; break;
+    case GL_RGB: bpp = 3; break;
+    case GL_LUMINANCE: bpp = 1; break;
+    case GL_LUMINANCE_ALPHA: bpp = 2; break;
+    default: return MTGL_OK;   // the reference reads nothing of other formats
+    }
+    const uint64_t n = (uint64_t)p->width * (uint64_t)p->height;
+    if (n == 0) return MTGL_OK;
+    if (n > 0xFFFFFFFFull) return MTGL_ERR_INVALID;
+    // a recorded batch replays clears and draws only: out of order otherwise
+    if (g_recording) return MTGL_ERR_UNSUPPORTED;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    Exec &x = d->ex;
+    const size_t bytes = (size_t)n * bpp;
+    if ((rc = arena_reserve(x, d->pixels, bytes))) return rc;
+    if ((rc = pinned_reserve(x, bytes))) return rc;
+    memcpy(x.pinned, p->pixels, bytes);
+    HIPC(hipMemcpyAsync(d->pixels.ptr, x.pinned, bytes, hipMemcpyHostToDevice, x.stream));
+    HIPC(hipEventRecord(x.upload_done, x.stream));
+    PixDesc P;
+    P.x = p->x;
+    P.y = p->y;
+    P.w = p->width;
+    P.h = p->height;
+    P.fb_w = d->w;
+    P.fb_h = d->h;
+    P.format = p->format;
+    P.flags = p->flags;
+    P.alpha_func = p->alpha_func;
+    P.depth_func = p->depth_func;
+    P.depth_mask = p->depth_mask;
+    P.blend_src = p->blend_src;
+    P.blend_dst = p->blend_dst;
+    P.alpha_ref = p->alpha_ref;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (x.timing) {
+        ev0 = take_event(x);
+        ev1 = take_event(x);
+        if (ev0) HIPC(hipEventRecord(ev0, x.stream));
+    }
+    hipLaunchKernelGGL(k_draw_pixels, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, P,
+                       (const uint8_t *)d->pixels.ptr, d->color, d->depth);
+    HIPC(hipGetLastError());
+    if (ev0 && ev1) {
+        HIPC(hipEventRecord(ev1, x.stream));
+        x.pending_events.emplace_back(ev0, ev1);
+    }
+    d->dirty = true;
+    return MTGL_OK;
 }
 
 MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int enable)

@@ -742,6 +742,206 @@ def scene_midprimitive_state(g, W, H, seed):
     g.glEnd()
 
 
+def _prim_state(g, rng, W, H, k):
+    """Per-batch fragment state shared by the line and point scenes: depth
+    function and writes, blending, colour mask, scissor, fog mode.  The alpha
+    test stays off: the reference's line loop never ends on a pixel it
+    rejects."""
+    funcs = ["GL_LESS", "GL_LEQUAL", "GL_GREATER", "GL_ALWAYS", "GL_NOTEQUAL"]
+    g.glDepthFunc(E[funcs[k % len(funcs)]])
+    g.glDepthMask(int(rng.random() < 0.7))
+    (g.glEnable if rng.random() < 0.4 else g.glDisable)(E["GL_BLEND"])
+    g.glBlendFunc(E[["GL_SRC_ALPHA", "GL_ONE", "GL_DST_COLOR", "GL_ONE_MINUS_SRC_ALPHA"][rng.integers(4)]],
+                  E[["GL_ONE_MINUS_SRC_ALPHA", "GL_ZERO", "GL_ONE", "GL_SRC_COLOR"][rng.integers(4)]])
+    if rng.random() < 0.25:
+        g.glColorMask(*(int(v) for v in rng.random(4) < 0.6))
+    else:
+        g.glColorMask(1, 1, 1, 1)
+    if rng.random() < 0.3:
+        g.glEnable(E["GL_SCISSOR_TEST"])
+        g.glScissor(int(rng.integers(0, W // 2)), int(rng.integers(0, H // 2)), int(rng.integers(W // 4, W)),
+                    int(rng.integers(H // 4, H)))
+    else:
+        g.glDisable(E["GL_SCISSOR_TEST"])
+    if rng.random() < 0.4:
+        g.glEnable(E["GL_FOG"])
+        g.glFogi(E["GL_FOG_MODE"], E[["GL_LINEAR", "GL_EXP", "GL_EXP2"][rng.integers(3)]])
+        g.glFogf(E["GL_FOG_DENSITY"], float(rng.uniform(0.05, 0.6)))
+        g.glFogf(E["GL_FOG_START"], -5.0)           # lines and points fog on the negated eye_z
+        g.glFogf(E["GL_FOG_END"], -1.0)
+    else:
+        g.glDisable(E["GL_FOG"])
+
+
+def scene_lines_points(g, W, H, seed):
+    """GL_POINTS (sizes 1 to 5, textured and alpha-tested), GL_LINES,
+    GL_LINE_STRIP and GL_LINE_LOOP (widths 1 to 4.5, textured, crossing the
+    near plane and the sides of the frustum), each batch under its own depth,
+    blend, colour-mask, scissor and fog state (reference src/raster.c:65-296,
+    :1019-1196)."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 4)
+    g.glEnd()
+    tex = _upload_texture(g, random_rgba(16, 16, seed + 90))
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"],
+                      E["GL_LINEAR"] if seed % 2 else E["GL_NEAREST"])
+    g.glBindTexture(E["GL_TEXTURE_2D"], tex)
+    modes = ["GL_POINTS", "GL_LINES", "GL_LINE_STRIP", "GL_LINE_LOOP"]
+    for k in range(16):
+        mode = modes[k % 4]
+        _prim_state(g, rng, W, H, k)
+        (g.glEnable if rng.random() < 0.4 else g.glDisable)(E["GL_TEXTURE_2D"])
+        if mode == "GL_POINTS":
+            g.glPointSize(float(rng.choice([1.0, 2.4, 3.6, 5.0])))
+            if rng.random() < 0.5:
+                g.glEnable(E["GL_ALPHA_TEST"])
+                g.glAlphaFunc(E[["GL_GREATER", "GL_LESS", "GL_GEQUAL"][rng.integers(3)]], float(rng.random()))
+        else:
+            g.glLineWidth(float(rng.choice([1.0, 2.0, 3.0, 4.5])))
+        spread = 5.0 if k % 3 == 0 else 2.5          # some primitives leave the frustum
+        g.glBegin(E[mode])
+        for _ in range(int(rng.integers(2, 9))):
+            g.glColor4f(*rng.random(4).astype(np.float32))
+            g.glTexCoord2f(*rng.uniform(-1.0, 2.0, 2).astype(np.float32))
+            v = (rng.random(3) - 0.5) * spread
+            if k % 4 == 1:
+                v[2] = rng.uniform(1.5, 3.5)             # through the near plane (eye at z = 3)
+            g.glVertex3f(*v.astype(np.float32))
+        g.glEnd()
+        g.glDisable(E["GL_ALPHA_TEST"])
+    g.glColorMask(1, 1, 1, 1)
+    g.glDepthMask(1)
+
+
+def lines_points_workload(g, W, H, seed=SEED_BASE + 9, n_lines=20_000, n_points=20_000):
+    """Many lines and points in two draws from one interleaved buffer object:
+    GL_LINES of width 2 (endpoints uniform over a box that overhangs the
+    frustum, so some segments are clipped) and GL_POINTS of size 3, under the
+    depth test with blending on the points.  Returns the fragments the draws
+    produce before the per-pixel tests (the reference's loop iterations
+    times the line width, point size squared per point in the frustum)."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    nv = 2 * n_lines + n_points
+    v = np.empty((nv, 8), np.float32)                    # x y z, r g b a, pad
+    v[:, 0:3] = ((rng.random((nv, 3)) - 0.5) * np.array([3.2, 2.0, 2.0])).astype(np.float32)
+    v[:, 3:7] = rng.random((nv, 4)).astype(np.float32)
+    v[:, 7] = 0.0
+    ids = np.zeros(1, np.uint32)
+    g.glGenBuffers(1, ids)
+    g.glBindBuffer(E["GL_ARRAY_BUFFER"], int(ids[0]))
+    g.glBufferData(E["GL_ARRAY_BUFFER"], v.nbytes, g.keep(v), E["GL_STATIC_DRAW"])
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_COLOR_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 32, 0)
+    g.glColorPointer(4, E["GL_FLOAT"], 32, 12)
+    g.glLineWidth(2.0)
+    g.glDrawArrays(E["GL_LINES"], 0, 2 * n_lines)
+    g.glEnable(E["GL_BLEND"])
+    g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE_MINUS_SRC_ALPHA"])
+    g.glPointSize(3.0)
+    g.glDrawArrays(E["GL_POINTS"], 2 * n_lines, n_points)
+    g.glDisable(E["GL_BLEND"])
+    g.glDisableClientState(E["GL_VERTEX_ARRAY"])
+    g.glDisableClientState(E["GL_COLOR_ARRAY"])
+    g.glBindBuffer(E["GL_ARRAY_BUFFER"], 0)
+    return nv
+
+
+def _polygon_mode_tris(g, rng, W, H, seed, modes):
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glEnable(E["GL_COLOR_MATERIAL"])
+    g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.keep(np.array([1.0, 1.0, 1.0, 0.0], np.float32)))
+    g.glPointSize(3.0)                                # draw_triangle_points ignores it
+    g.glLineWidth(2.0)
+    for k, (front, back) in enumerate(modes):
+        g.glPolygonMode(E["GL_FRONT"], E[front])
+        g.glPolygonMode(E["GL_BACK"], E[back])
+        g.glShadeModel(E["GL_PHONG"] if k % 2 else E["GL_SMOOTH"])
+        (g.glEnable if k % 3 == 2 else g.glDisable)(E["GL_CULL_FACE"])
+        _prim_state(g, rng, W, H, k)
+        if front == "GL_POINT" and back in ("GL_POINT", "GL_FILL") or back == "GL_POINT" and front == "GL_FILL":
+            g.glEnable(E["GL_ALPHA_TEST"])               # never on lines: see _prim_state
+            g.glAlphaFunc(E["GL_GREATER"], 0.3)
+        g.glBegin(E[["GL_TRIANGLES", "GL_QUADS", "GL_TRIANGLE_FAN"][k % 3]])
+        _random_tris_immediate(g, rng, 4, spread=4.0, normals=True)
+        g.glEnd()
+        g.glDisable(E["GL_ALPHA_TEST"])
+    g.glColorMask(1, 1, 1, 1)
+    g.glDepthMask(1)
+
+
+def scene_polygon_mode(g, W, H, seed):
+    """glPolygonMode LINE and POINT on both faces (GL_PHONG corners lit per
+    vertex with the front material, culling before the mode choice;
+    reference src/raster.c:841-898, :934-952)."""
+    modes = [("GL_LINE", "GL_LINE"), ("GL_POINT", "GL_POINT"), ("GL_LINE", "GL_POINT"), ("GL_POINT", "GL_LINE"),
+             ("GL_LINE", "GL_LINE"), ("GL_POINT", "GL_POINT")]
+    _polygon_mode_tris(g, _rng(seed), W, H, seed, modes)
+
+
+def scene_polygon_mode_mixed(g, W, H, seed):
+    """glPolygonMode with one face filled and the other drawn as lines or
+    points in the same draw: rendered by the oracle and the reference; the
+    device refuses such draws."""
+    modes = [("GL_FILL", "GL_LINE"), ("GL_POINT", "GL_FILL"), ("GL_LINE", "GL_FILL"), ("GL_FILL", "GL_POINT")]
+    _polygon_mode_tris(g, _rng(seed), W, H, seed, modes)
+
+
+def scene_draw_pixels(g, W, H, seed):
+    """glDrawPixels over rendered triangles (reference src/gl_api.c:1284-1373):
+    all four formats, rectangles hanging off every edge, alpha test, depth
+    test against the triangles (incoming depth 0) with and without depth
+    writes, every blend factor, and an invalid raster position (behind the
+    eye), which draws nothing."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 6)
+    g.glEnd()
+    g.glRasterPos3f(0.0, 0.0, 5.0)        # eye z = +2: clip w <= 0, raster position invalid
+    g.glDrawPixels(4, 4, E["GL_RGBA"], E["GL_UNSIGNED_BYTE"], g.keep(np.full(64, 255, np.uint8)))
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glOrtho(0.0, float(W), 0.0, float(H), -1.0, 1.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    fmts = [("GL_RGBA", 4), ("GL_RGB", 3), ("GL_LUMINANCE", 1), ("GL_LUMINANCE_ALPHA", 2)]
+    funcs = ["GL_NEVER", "GL_LESS", "GL_EQUAL", "GL_LEQUAL", "GL_GREATER", "GL_NOTEQUAL", "GL_GEQUAL", "GL_ALWAYS"]
+    factors = ["GL_ZERO", "GL_ONE", "GL_SRC_COLOR", "GL_ONE_MINUS_SRC_COLOR", "GL_DST_COLOR",
+               "GL_ONE_MINUS_DST_COLOR", "GL_SRC_ALPHA", "GL_ONE_MINUS_SRC_ALPHA", "GL_DST_ALPHA",
+               "GL_ONE_MINUS_DST_ALPHA", "GL_SRC_ALPHA_SATURATE", "GL_CONSTANT_COLOR"]
+    for k in range(20):
+        fmt, bpp = fmts[k % 4]
+        w, h = int(rng.integers(1, W // 2 + 1)), int(rng.integers(1, H // 2 + 1))
+        x, y = int(rng.integers(-w // 2, W - w // 2)), int(rng.integers(-h // 2, H - h // 2))
+        if k % 5 == 4:
+            g.glRasterPos2f(x + 0.37, y + 0.81)
+        else:
+            g.glRasterPos2i(x, y)
+        (g.glEnable if rng.random() < 0.5 else g.glDisable)(E["GL_ALPHA_TEST"])
+        g.glAlphaFunc(E[funcs[rng.integers(8)]], float(rng.choice([0.0, 0.5, 1.0, rng.random()])))
+        (g.glEnable if rng.random() < 0.6 else g.glDisable)(E["GL_DEPTH_TEST"])
+        g.glDepthFunc(E[funcs[rng.integers(8)]])
+        g.glDepthMask(int(rng.random() < 0.5))
+        (g.glEnable if rng.random() < 0.6 else g.glDisable)(E["GL_BLEND"])
+        g.glBlendFunc(E[factors[rng.integers(len(factors))]],
+                      E[[f for f in factors if f != "GL_SRC_ALPHA_SATURATE"][rng.integers(len(factors) - 1)]])
+        data = rng.integers(0, 256, size=w * h * bpp, dtype=np.uint8)
+        if k % 7 == 3:
+            data[::bpp] = 255                 # a luminance / red channel at full scale
+        g.glDrawPixels(w, h, E[fmt], E["GL_UNSIGNED_BYTE"], g.keep(data))
+    g.glDepthMask(1)
+
+
 def scene_vbo_oob(g, W, H, seed):
     scene_vbo(g, W, H, seed, oob_index=True)
 
@@ -919,6 +1119,13 @@ def feature_items(names, seed, size=(96, 72)):
 # scenes whose reference behaviour is undefined (reads outside a buffer); they
 # are compared device against oracle only
 UNDEFINED_IN_REFERENCE = ("vbo_oob",)
+# scenes with calls a batch does not record (glDrawPixels, points, lines and
+# polygon-mode LINE / POINT draws are refused while recording); they stay out
+# of the batched feature tests
+NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed")
+# scenes with draws the device refuses (one face filled, the other drawn as
+# lines or points); oracle against reference only
+DEVICE_UNSUPPORTED = ("polygon_mode_mixed",)
 
 FEATURE_SCENES = {
     "depth_funcs": scene_depth_funcs,
@@ -933,6 +1140,10 @@ FEATURE_SCENES = {
     "arrays": scene_arrays,
     "vbo": scene_vbo,
     "vbo_oob": scene_vbo_oob,
+    "draw_pixels": scene_draw_pixels,
+    "lines_points": scene_lines_points,
+    "polygon_mode": scene_polygon_mode,
+    "polygon_mode_mixed": scene_polygon_mode_mixed,
     "lists": scene_lists,
     "scissor_viewport": scene_scissor_viewport,
     "colormask": scene_colormask,

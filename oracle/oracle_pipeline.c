@@ -856,6 +856,239 @@ static void or_raster(mtgl_dev *dev, const mtgl_raster_state *rs, const or_rv *v
 }
 
 /* ------------------------------------------------------------------------- */
+/* points and lines (reference src/raster.c:65-296, :776-898, :1019-1196)      */
+/* ------------------------------------------------------------------------- */
+
+/* the per-pixel tail of write_line_pixel (src/raster.c:66-104), also that of
+ * flush_points (:1123-1155) and draw_point_at_screen (:777-838), whose extra
+ * alpha test and fog depend on the point only: bounds, scissor (raw scissor
+ * y against top-down rows, as there), depth test, blend, depth write, masked
+ * colour write.  No stencil. */
+static void or_prim_pixel(mtgl_dev *dev, const mtgl_raster_state *rs, int32_t px, int32_t py, float depth, or_col c)
+{
+    if (!or_in_fb(dev, px, py)) return;
+    if (rs->flags & MTGL_F_SCISSOR_TEST) {
+        const int32_t sx = rs->scissor[0], sy = rs->scissor[1], sw = rs->scissor[2], sh = rs->scissor[3];
+        if (px < sx || px >= sx + sw || py < sy || py >= sy + sh) return;
+    }
+    const size_t i = (size_t)py * dev->w + px;
+    const int depth_on = (rs->flags & MTGL_F_DEPTH_TEST) != 0;
+    if (depth_on && !or_compare(rs->depth_func, depth, dev->depth[i])) return;
+    if (rs->flags & MTGL_F_BLEND) {
+        or_col dst = or_unpack(dev->color[i]);
+        or_col sf = or_blend_factor(rs->blend_src, c, dst), df = or_blend_factor(rs->blend_dst, c, dst);
+        c = or_clamp(or_add(or_mul(c, sf), or_mul(dst, df)));
+    }
+    if (depth_on && rs->depth_mask) dev->depth[i] = depth;
+    or_write_color(dev, rs->color_mask, px, py, c);
+}
+
+/* (z + 1) * 0.5 * (far - near) + near in double, stored as float */
+static float or_prim_depth(const mtgl_raster_state *rs, float z)
+{
+    return (float)((double)((z + 1.0f) * 0.5f) * (rs->depth_far - rs->depth_near) + rs->depth_near);
+}
+
+/* fog of lines and points: the coordinate is the negated eye_z (the
+ * triangles use eye_z itself), colour rgb towards the fragment, alpha of the
+ * fog colour (color_lerp_rgb) */
+static or_col or_prim_fog(const mtgl_raster_state *rs, or_col c, float fog_coord)
+{
+    float f = 1.0f;
+    switch (rs->fog_mode) {
+    case GL_LINEAR:
+        if (rs->fog_end != rs->fog_start) f = (rs->fog_end - fog_coord) / (rs->fog_end - rs->fog_start);
+        break;
+    case GL_EXP: f = expf(-rs->fog_density * fog_coord); break;
+    case GL_EXP2: { float d = rs->fog_density * fog_coord; f = expf(-d * d); break; }
+    default: break;
+    }
+    if (f < 0.0f) f = 0.0f;
+    if (f > 1.0f) f = 1.0f;
+    or_col fg = or_from(rs->fog_color);
+    or_col r = { fg.r + (c.r - fg.r) * f, fg.g + (c.g - fg.g) * f, fg.b + (c.b - fg.b) * f, fg.a };
+    return r;
+}
+
+static const or_tex *or_prim_tex(const mtgl_dev *dev, const mtgl_raster_state *rs)
+{
+    return (rs->texture && rs->texture <= MTGL_MAX_OBJECTS && dev->tex[rs->texture].px) ? &dev->tex[rs->texture] : NULL;
+}
+
+/* reference draw_line_full (src/raster.c:107-246).  Where the alpha test
+ * rejects a pixel the reference's loop `continue`s without stepping and never
+ * ends; here the pixel is skipped and the line goes on. */
+static void or_line(mtgl_dev *dev, const mtgl_raster_state *rs, int32_t x0, int32_t y0, float z0, int32_t x1,
+                    int32_t y1, float z1, or_col c0, or_col c1, float ez0, float ez1, const float *uv0,
+                    const float *uv1)
+{
+    int32_t dx = or_isub(x1, x0), dy = or_isub(y1, y0);
+    int32_t adx = dx < 0 ? -dx : dx, ady = dy < 0 ? -dy : dy;
+    int32_t sx = dx < 0 ? -1 : 1, sy = dy < 0 ? -1 : 1;
+    int32_t err = adx - ady;
+    int32_t total = adx > ady ? adx : ady;
+    if (total == 0) total = 1;
+    int32_t step = 0;
+    int lw = (int)(rs->line_width + 0.5f);
+    if (lw < 1) lw = 1;
+    int half = lw / 2;
+    int ex = adx > ady ? 0 : 1, ey = adx > ady ? 1 : 0;
+    const or_tex *tex = (rs->flags & MTGL_F_TEXTURE_2D) ? or_prim_tex(dev, rs) : NULL;
+    int32_t cx = x0, cy = y0;
+    for (;;) {
+        float t = (float)step / (float)total;
+        float z = z0 + t * (z1 - z0);
+        float depth = or_prim_depth(rs, z);
+        or_col c = or_lerp(c0, c1, t);
+        int keep = 1;
+        if (tex) {
+            float u = uv0[0] + t * (uv1[0] - uv0[0]);
+            float v = uv0[1] + t * (uv1[1] - uv0[1]);
+            int taps = 0;
+            or_col tc = or_unpack(or_sample(tex, rs, u, v, 0.0f, &taps));
+            if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, tc.a, rs->alpha_ref)) keep = 0;
+            else c = or_mul(c, tc);
+        } else if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, c.a, rs->alpha_ref)) {
+            keep = 0;
+        }
+        if (keep) {
+            if (rs->flags & MTGL_F_FOG) c = or_prim_fog(rs, c, -(ez0 + t * (ez1 - ez0)));
+            if (lw == 1) {
+                or_prim_pixel(dev, rs, cx, cy, depth, c);
+            } else {
+                for (int w = -half; w < lw - half; w++)
+                    or_prim_pixel(dev, rs, cx + w * ex, cy + w * ey, depth, c);
+            }
+        }
+        if (cx == x1 && cy == y1) break;
+        int32_t e2 = err * 2;
+        if (e2 > -ady) { err -= ady; cx += sx; }
+        if (e2 < adx) { err += adx; cy += sy; }
+        step++;
+    }
+}
+
+static int or_outcode(const or_v4 *v)
+{
+    int code = 0;
+    if (v->x < -v->w) code |= 1; else if (v->x > v->w) code |= 2;
+    if (v->y < -v->w) code |= 4; else if (v->y > v->w) code |= 8;
+    if (v->z < -v->w) code |= 16; else if (v->z > v->w) code |= 32;
+    return code;
+}
+
+/* reference clip_line (src/clipping.h:152-230): Cohen-Sutherland on the
+ * outcodes, planes tried left, right, bottom, top, near, far */
+static int or_clip_line(or_vtx *v0, or_vtx *v1)
+{
+    int c0 = or_outcode(&v0->pos), c1 = or_outcode(&v1->pos);
+    for (;;) {
+        if (!(c0 | c1)) return 1;
+        if (c0 & c1) return 0;
+        int out = c0 ? c0 : c1;
+        int plane = out & 1 ? 2 : out & 2 ? 3 : out & 4 ? 4 : out & 8 ? 5 : out & 16 ? 0 : 1;
+        float d0 = or_plane(&v0->pos, plane), d1 = or_plane(&v1->pos, plane);
+        float denom = d0 - d1;
+        if (fabsf(denom) < 1e-10f) return 0;
+        float t = d0 / denom;
+        or_vtx cl = or_vtx_lerp(v0, v1, t);
+        or_snap(&cl, plane);
+        if (out == c0) { *v0 = cl; c0 = or_outcode(&v0->pos); }
+        else { *v1 = cl; c1 = or_outcode(&v1->pos); }
+    }
+}
+
+/* reference draw_line_segment (src/raster.c:249-288) */
+static void or_line_segment(mtgl_dev *dev, const mtgl_raster_state *rs, const or_vtx *a, const or_vtx *b)
+{
+    or_vtx v0 = *a, v1 = *b;
+    if (!or_clip_line(&v0, &v1)) return;
+    float z0 = 0.0f, z1 = 0.0f;
+    if (fabsf(v0.pos.w) >= 1e-6f) {
+        float iw = 1.0f / v0.pos.w;
+        v0.pos.x *= iw; v0.pos.y *= iw; z0 = v0.pos.z * iw;
+    } else {
+        v0.pos.x = 0.0f; v0.pos.y = 0.0f;
+    }
+    if (fabsf(v1.pos.w) >= 1e-6f) {
+        float iw = 1.0f / v1.pos.w;
+        v1.pos.x *= iw; v1.pos.y *= iw; z1 = v1.pos.z * iw;
+    } else {
+        v1.pos.x = 0.0f; v1.pos.y = 0.0f;
+    }
+    const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
+    int32_t x0 = or_cvtt((v0.pos.x + 1.0f) * 0.5f * (float)vw + (float)vx);
+    int32_t y0 = or_cvtt((1.0f - v0.pos.y) * 0.5f * (float)vh + (float)vy);
+    int32_t x1 = or_cvtt((v1.pos.x + 1.0f) * 0.5f * (float)vw + (float)vx);
+    int32_t y1 = or_cvtt((1.0f - v1.pos.y) * 0.5f * (float)vh + (float)vy);
+    or_line(dev, rs, x0, y0, z0, x1, y1, z1, v0.color, v1.color, v0.ez, v1.ez, v0.tc, v1.tc);
+}
+
+/* reference flush_points (src/raster.c:1019-1158) */
+static void or_points(mtgl_dev *dev, const mtgl_raster_state *rs, const or_vtx *vt, uint32_t n)
+{
+    int ps = (int)(rs->point_size + 0.5f);
+    if (ps < 1) ps = 1;
+    int half = ps / 2;
+    const or_tex *tex = (rs->flags & MTGL_F_TEXTURE_2D) ? or_prim_tex(dev, rs) : NULL;
+    const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
+    for (uint32_t i = 0; i < n; i++) {
+        or_v4 p = vt[i].pos;
+        if (p.x < -p.w || p.x > p.w || p.y < -p.w || p.y > p.w || p.z < -p.w || p.z > p.w || p.w <= 0.0f) continue;
+        float nx = p.x / p.w, ny = p.y / p.w, nz = p.z / p.w;
+        int32_t cx = or_cvtt((nx + 1.0f) * 0.5f * (float)vw + (float)vx);
+        int32_t cy = or_cvtt((1.0f - ny) * 0.5f * (float)vh + (float)vy);
+        float depth = or_prim_depth(rs, nz);
+        or_col c = vt[i].color;
+        if (tex) {
+            int taps = 0;
+            or_col tc = or_unpack(or_sample(tex, rs, vt[i].tc[0], vt[i].tc[1], 0.0f, &taps));
+            if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, tc.a, rs->alpha_ref)) continue;
+            c = or_mul(c, tc);
+        } else if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, c.a, rs->alpha_ref)) {
+            continue;
+        }
+        if (rs->flags & MTGL_F_FOG) c = or_prim_fog(rs, c, -vt[i].ez);
+        int32_t xs = cx - half, ys = cy - half;
+        for (int32_t py = ys; py < ys + ps; py++)
+            for (int32_t px = xs; px < xs + ps; px++) or_prim_pixel(dev, rs, px, py, depth, c);
+    }
+}
+
+/* colours of a fan triangle's corners for glPolygonMode LINE / POINT: the
+ * vertex colours, or per-vertex lighting with the front material under
+ * GL_PHONG (reference src/raster.c:852-858, :880-886) */
+static or_col or_corner_color(const mtgl_raster_state *rs, const or_vtx *v)
+{
+    if ((rs->flags & MTGL_F_LIGHTING) && rs->shade_model == GL_PHONG)
+        return or_light(&rs->lighting, rs->flags, v->ep, v->en, &rs->lighting.front);
+    return v->color;
+}
+
+/* reference draw_triangle_wireframe / draw_triangle_points (src/raster.c:841-898):
+ * corners already divided (x, y, z = NDC), screen positions as for filling */
+static void or_tri_prims(mtgl_dev *dev, const mtgl_raster_state *rs, const or_vtx *const *src, const int32_t *X,
+                         const int32_t *Y, uint32_t mode)
+{
+    or_col col[3];
+    for (int k = 0; k < 3; k++) col[k] = or_corner_color(rs, src[k]);
+    if (mode == GL_LINE) {
+        for (int k = 0; k < 3; k++) {
+            int m = (k + 1) % 3;
+            or_line(dev, rs, X[k], Y[k], src[k]->pos.z, X[m], Y[m], src[m]->pos.z, col[k], col[m], src[k]->ez,
+                    src[m]->ez, src[k]->tc, src[m]->tc);
+        }
+        return;
+    }
+    for (int k = 0; k < 3; k++) {
+        or_col c = col[k];
+        if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, c.a, rs->alpha_ref)) continue;
+        if (rs->flags & MTGL_F_FOG) c = or_prim_fog(rs, c, -src[k]->ez);
+        or_prim_pixel(dev, rs, X[k], Y[k], or_prim_depth(rs, src[k]->pos.z), c);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
 /* per-triangle: clip, divide, fan, cull (reference src/raster.c:729-774,     */
 /* :901-958)                                                                   */
 /* ------------------------------------------------------------------------- */
@@ -906,7 +1139,11 @@ static void or_render_triangle(mtgl_dev *dev, const mtgl_raster_state *rs, const
         }
         int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
         uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
-        if (pm == GL_POINT || pm == GL_LINE) { *unsupported = 1; continue; }
+        if (pm == GL_POINT || pm == GL_LINE) {
+            const int32_t X[3] = { rv[0].x, rv[1].x, rv[2].x }, Y[3] = { rv[0].y, rv[1].y, rv[2].y };
+            or_tri_prims(dev, rs, src, X, Y, pm);
+            continue;
+        }
         or_raster(dev, rs, &rv[0], &rv[1], &rv[2], back);
     }
 }
@@ -1004,6 +1241,84 @@ int mtgl_dev_write(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, int32_t w,
     return MTGL_OK;
 }
 
+/* glDrawPixels, reference src/gl_api.c:1284-1373, with the reference's own
+ * compare switch (:1231-1258) and blend factors (:1260-1282) */
+static int or_dp_compare(uint32_t func, float a, float b)
+{
+    switch (func) {
+        case GL_NEVER:    return 0;
+        case GL_LESS:     return a < b;
+        case GL_EQUAL:    return a == b;
+        case GL_LEQUAL:   return a <= b;
+        case GL_GREATER:  return a > b;
+        case GL_NOTEQUAL: return a != b;
+        case GL_GEQUAL:   return a >= b;
+        default:          return 1;
+    }
+}
+
+static or_col or_dp_factor(uint32_t f, or_col s, or_col d)
+{
+    or_col one = { 1.0f, 1.0f, 1.0f, 1.0f };
+    switch (f) {
+        case GL_ZERO: { or_col z = { 0.0f, 0.0f, 0.0f, 0.0f }; return z; }
+        case GL_SRC_COLOR: return s;
+        case GL_ONE_MINUS_SRC_COLOR: { or_col c = { 1 - s.r, 1 - s.g, 1 - s.b, 1 - s.a }; return c; }
+        case GL_DST_COLOR: return d;
+        case GL_ONE_MINUS_DST_COLOR: { or_col c = { 1 - d.r, 1 - d.g, 1 - d.b, 1 - d.a }; return c; }
+        case GL_SRC_ALPHA: { or_col c = { s.a, s.a, s.a, s.a }; return c; }
+        case GL_ONE_MINUS_SRC_ALPHA: { or_col c = { 1 - s.a, 1 - s.a, 1 - s.a, 1 - s.a }; return c; }
+        case GL_DST_ALPHA: { or_col c = { d.a, d.a, d.a, d.a }; return c; }
+        case GL_ONE_MINUS_DST_ALPHA: { or_col c = { 1 - d.a, 1 - d.a, 1 - d.a, 1 - d.a }; return c; }
+        case GL_SRC_ALPHA_SATURATE: {
+            float k = (s.a < (1 - d.a)) ? s.a : (1 - d.a);
+            or_col c = { k, k, k, 1.0f };
+            return c;
+        }
+        default: return one;
+    }
+}
+
+int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
+{
+    if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
+    if ((uint64_t)p->width * (uint64_t)p->height > 0xFFFFFFFFull) return MTGL_ERR_INVALID;
+    if (or_recording) return MTGL_ERR_UNSUPPORTED;
+    const uint8_t *src = (const uint8_t *)p->pixels;
+    int alpha_on = (p->flags & MTGL_F_ALPHA_TEST) != 0;
+    int depth_on = (p->flags & MTGL_F_DEPTH_TEST) != 0;
+    int blend_on = (p->flags & MTGL_F_BLEND) != 0;
+    for (int32_t row = 0; row < p->height; row++) {
+        int32_t fb_y = d->h - 1 - (p->y + row);
+        if (fb_y < 0 || fb_y >= d->h) continue;
+        for (int32_t col = 0; col < p->width; col++) {
+            int32_t x = p->x + col;
+            if (x < 0 || x >= d->w) continue;
+            size_t i = (size_t)row * (size_t)p->width + (size_t)col;
+            uint32_t r, g, b, a = 255;
+            if (p->format == GL_RGBA) { r = src[4 * i]; g = src[4 * i + 1]; b = src[4 * i + 2]; a = src[4 * i + 3]; }
+            else if (p->format == GL_RGB) { r = src[3 * i]; g = src[3 * i + 1]; b = src[3 * i + 2]; }
+            else if (p->format == GL_LUMINANCE) { r = g = b = src[i]; }
+            else if (p->format == GL_LUMINANCE_ALPHA) { r = g = b = src[2 * i]; a = src[2 * i + 1]; }
+            else continue;
+            if (alpha_on && !or_dp_compare(p->alpha_func, a / 255.0f, p->alpha_ref)) continue;
+            size_t pix = (size_t)fb_y * d->w + x;
+            if (depth_on && !or_dp_compare(p->depth_func, 0.0f, d->depth[pix])) continue;
+            or_col c = { r / 255.0f, g / 255.0f, b / 255.0f, a / 255.0f };
+            if (blend_on) {
+                or_col dst = or_unpack(d->color[pix]);
+                or_col sf = or_dp_factor(p->blend_src, c, dst), df = or_dp_factor(p->blend_dst, c, dst);
+                or_col m = { c.r * sf.r + dst.r * df.r, c.g * sf.g + dst.g * df.g,
+                             c.b * sf.b + dst.b * df.b, c.a * sf.a + dst.a * df.a };
+                c = or_clamp(m);
+            }
+            if (depth_on && p->depth_mask) d->depth[pix] = 0.0f;
+            d->color[pix] = or_pack(c);
+        }
+    }
+    return MTGL_OK;
+}
+
 /* level 0 plus the 2x2 box-filtered level 1: (((c00 + c10) + c01) + c11) * 0.25,
  * packed with truncation (reference texture_generate_mip1, src/textures.c:311-354) */
 int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, const uint32_t *texels)
@@ -1088,6 +1403,7 @@ int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
     const uint32_t n = dr->count;
     switch (dr->mode) {
+    case GL_POINTS: case GL_LINES: case GL_LINE_STRIP: case GL_LINE_LOOP:
     case GL_TRIANGLES: case GL_TRIANGLE_STRIP: case GL_TRIANGLE_FAN: case GL_QUADS:
     case GL_QUAD_STRIP: case GL_POLYGON:
         break;
@@ -1128,6 +1444,17 @@ int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     const mtgl_raster_state *rs = dr->raster;
     int unsupported = 0;
     switch (dr->mode) {
+    case GL_POINTS:
+        or_points(d, rs, vt, n);
+        break;
+    case GL_LINES:
+        for (uint32_t i = 0; i + 1 < n; i += 2) or_line_segment(d, rs, &vt[i], &vt[i + 1]);
+        break;
+    case GL_LINE_STRIP:
+    case GL_LINE_LOOP:
+        for (uint32_t i = 0; i + 1 < n; i++) or_line_segment(d, rs, &vt[i], &vt[i + 1]);
+        if (dr->mode == GL_LINE_LOOP && n >= 2) or_line_segment(d, rs, &vt[n - 1], &vt[0]);
+        break;
     case GL_TRIANGLES:
         for (uint32_t i = 0; i + 2 < n; i += 3) or_render_triangle(d, rs, &vt[i], &vt[i + 1], &vt[i + 2], &unsupported);
         break;

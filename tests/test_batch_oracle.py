@@ -9,7 +9,7 @@ from conftest import plane_diffs
 from mytinygl_amd import scenes as S
 
 SMALL = S.batch_specs(10, seed=S.SEED_BASE + 7, fb=(16, 80), tris=(20, 1500), tex=(16, 64))
-FEATURE = sorted(S.FEATURE_SCENES)
+FEATURE = sorted(n for n in S.FEATURE_SCENES if n not in S.NOT_BATCHABLE)
 
 
 def _immediate(g, items):

@@ -17,7 +17,7 @@ from test_gpu_parity import _check_frame
 
 pytestmark = pytest.mark.gpu
 
-FEATURE = sorted(S.FEATURE_SCENES)
+FEATURE = sorted(n for n in S.FEATURE_SCENES if n not in S.NOT_BATCHABLE)
 SMALL = S.batch_specs(48, seed=S.SEED_BASE + 8, fb=(16, 256), tris=(20, 8000), tex=(16, 128))
 
 

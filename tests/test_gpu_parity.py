@@ -14,7 +14,7 @@ from mytinygl_amd import scenes as S
 
 pytestmark = pytest.mark.gpu
 
-FEATURE = sorted(S.FEATURE_SCENES)
+FEATURE = sorted(n for n in S.FEATURE_SCENES if n not in S.DEVICE_UNSUPPORTED)
 
 
 def _size(name):
@@ -87,3 +87,14 @@ def test_repeatable(mtgl):
     fn = S.FEATURE_SCENES["overdraw_small"]
     (a, _), (b, _) = S.render(mtgl, fn, 160, 120, 7), S.render(mtgl, fn, 160, 120, 7)
     assert plane_diffs(a, b) == {"color": 0, "depth": 0, "stencil": 0}
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_lines_points_workload(mtgl, oracle, seed):
+    """20,000 width-2 lines and 20,000 blended size-3 points at 640x480 from
+    a buffer object: many fragments per pixel, resolved in submission order."""
+    fn = lambda g, W, H, s: S.lines_points_workload(g, W, H, S.SEED_BASE + 9 + s)
+    (a, ea), (b, eb) = S.render(oracle, fn, 640, 480, seed), S.render(mtgl, fn, 640, 480, seed)
+    assert ea == eb
+    assert (a[0] != a[0][0, 0]).sum() > 10_000
+    _check_frame(f"lines_points_workload/{seed}", a, b)
