@@ -23,6 +23,9 @@ scenes), with its own bounded CPU sample.
 
 "draw_pixels": glDrawPixels of a 1080p RGBA image with depth test and
 blending, per call and per kernel, with the oracle timed on five calls.
+
+"lines_points": 100,000 width-2 lines and 100,000 blended size-3 points per
+1080p frame, with the oracle timed on two frames.
 """
 from __future__ import annotations
 
@@ -348,6 +351,51 @@ def pixels_bench(reps: int, cpu_reps: int):
     return out
 
 
+def lines_bench(frames: int, cpu_frames: int, n_lines: int = 100_000, n_points: int = 100_000):
+    """Points and lines at 1080p: one frame = clear, glDrawArrays(GL_LINES) of
+    n_lines width-2 segments and glDrawArrays(GL_POINTS) of n_points blended
+    size-3 points from a buffer object, depth test on.  Fragments = the
+    pixels the reference's line and point loops visit (steps x width, size
+    squared), counted by the device."""
+    def run(g, nframes):
+        ctx = g.create_context(W, H)
+        st = S.setup_lines_points(g, W, H, S.SEED_BASE + 9, n_lines, n_points)
+
+        def frame():
+            g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"])
+            S.draw_lines_points(g, st)
+
+        frame()
+        g.glFinish()
+        g.stats_reset(ctx)
+        frame()
+        g.glFinish()
+        frags = g.stats(ctx).fragments
+        t0 = time.perf_counter()
+        for _ in range(nframes):
+            frame()
+        g.glFinish()
+        sec = time.perf_counter() - t0
+        g.destroy_context(ctx)
+        g.release()
+        return frags, sec
+
+    g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
+    frags, sec = run(g, frames)
+    out = {"workload": f"{n_lines} GL_LINES of width 2 and {n_points} blended GL_POINTS of size 3 at 1920x1080, "
+                       "depth test, one buffer object",
+           "frames": frames, "ms_per_frame": round(sec * 1e3 / frames, 4), "fragments_per_frame": int(frags),
+           "mfragments_s": round(frags * frames / sec / 1e6, 1)}
+    lib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+    if cpu_frames > 0 and os.path.exists(lib):
+        ofr, osec = run(GLLibrary(lib, "oracle"), cpu_frames)
+        out["cpu_baseline"] = {"ms_per_frame": round(osec * 1e3 / cpu_frames, 2), "cores": 1, "kind": "port",
+                               "mfragments_s": round(ofr * cpu_frames / osec / 1e6, 2),
+                               "sample": f"{cpu_frames} frames of the same workload on the oracle, one thread"}
+        out["fragments_match_oracle"] = int(ofr) == int(frags)
+    return out
+
+
 def copy_calibration(nbytes: int, reps: int = 5) -> None:
     """Plain device-to-device copies of nbytes (torch's elementwise copy
     kernel): the reference the HBM counters FETCH_SIZE / WRITE_SIZE are
@@ -379,6 +427,9 @@ def main():
     ap.add_argument("--no-pixels", action="store_true", help="skip the glDrawPixels leg")
     ap.add_argument("--only-pixels", action="store_true", help="run only the glDrawPixels leg (profiling)")
     ap.add_argument("--pixels-calls", type=int, default=50)
+    ap.add_argument("--no-lines", action="store_true", help="skip the points-and-lines leg")
+    ap.add_argument("--only-lines", action="store_true", help="run only the points-and-lines leg (profiling)")
+    ap.add_argument("--lines-frames", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -398,6 +449,9 @@ def main():
         return
     if args.only_pixels:
         print(json.dumps(pixels_bench(args.pixels_calls, 0)), flush=True)
+        return
+    if args.only_lines:
+        print(json.dumps(lines_bench(args.lines_frames, 0)), flush=True)
         return
 
     g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
@@ -450,6 +504,7 @@ def main():
                                                       0.0 if args.no_cpu else args.cpu_seconds / 2)
     differentiable = None if args.no_diff else diff_bench(args.diff_steps, 2, 0 if args.no_cpu else 2)
     pixels = None if args.no_pixels else pixels_bench(args.pixels_calls, 0 if args.no_cpu else 5)
+    lines = None if args.no_lines else lines_bench(args.lines_frames, 0 if args.no_cpu else 2)
 
     out = {
         "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
@@ -478,6 +533,7 @@ def main():
         "batched": batched,
         "differentiable": differentiable,
         "draw_pixels": pixels,
+        "lines_points": lines,
     }
     if cpu_obj:
         out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None

@@ -183,8 +183,11 @@ typedef struct {
 
 /* Work counters accumulated by the fragment stage (reset by mtgl_dev_stats_reset).
  * A fragment is one (triangle, pixel) pair passing the coverage test of
- * reference src/raster.c:539-540; frag_bytes counts the algorithmic HBM bytes of
- * the reference per-fragment path (stencil, depth, texel taps, colour). */
+ * reference src/raster.c:539-540, or one pixel visited by the point and line
+ * loops (src/raster.c:152-243, :1113-1155: steps x line width, point size
+ * squared, one per corner point); frag_bytes counts the algorithmic HBM bytes
+ * of the reference per-fragment path of triangles (stencil, depth, texel
+ * taps, colour). */
 typedef struct {
     uint64_t fragments;
     uint64_t frag_bytes;

@@ -3503,8 +3503,7 @@ int exec_stats(Exec &x, mtgl_dev_stats *out)
     HIPC(hipStreamSynchronize(x.stream));
     int rc = resolve_timing(x);
     if (rc) return rc;
-    *out = x.stats;
-    out->fragments = 0;
+    *out = x.stats;   // fragments: those of point and line draws so far, plus the triangles' below
     out->frag_bytes = 0;
     for (uint32_t i = 0; i < STAT_SLOTS; i++) {
         out->fragments += x.stat_host[2 * i];
@@ -4081,6 +4080,7 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
     HIPC(hipStreamSynchronize(x.stream));
     if (x.ctr_host->unsupported & 2u) return MTGL_ERR_OUT_OF_MEMORY;   // one primitive over PRIM_MAX_FRAGS
     const uint64_t total = x.ctr_host->npairs;
+    x.stats.fragments += total;   // pixels the reference's point and line loops visit
     if (total == 0) return MTGL_OK;
     if (total > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
     const uint32_t n = (uint32_t)total;

@@ -816,19 +816,19 @@ def scene_lines_points(g, W, H, seed):
     g.glDepthMask(1)
 
 
-def lines_points_workload(g, W, H, seed=SEED_BASE + 9, n_lines=20_000, n_points=20_000):
-    """Many lines and points in two draws from one interleaved buffer object:
-    GL_LINES of width 2 (endpoints uniform over a box that overhangs the
-    frustum, so some segments are clipped) and GL_POINTS of size 3, under the
-    depth test with blending on the points.  Returns the fragments the draws
-    produce before the per-pixel tests (the reference's loop iterations
-    times the line width, point size squared per point in the frustum)."""
+def setup_lines_points(g, W, H, seed=SEED_BASE + 9, n_lines=20_000, n_points=20_000):
+    """State and one interleaved buffer object of the many-lines-and-points
+    workload: n_lines short segments (first endpoint and n_points points
+    uniform over a box that overhangs the frustum, so some segments are
+    clipped; second endpoint N(0, 0.04) away, tens of pixels at 1080p) and
+    random colours."""
     rng = _rng(seed)
     _perspective(g, W, H)
     g.glEnable(E["GL_DEPTH_TEST"])
     nv = 2 * n_lines + n_points
     v = np.empty((nv, 8), np.float32)                    # x y z, r g b a, pad
     v[:, 0:3] = ((rng.random((nv, 3)) - 0.5) * np.array([3.2, 2.0, 2.0])).astype(np.float32)
+    v[1:2 * n_lines:2, 0:3] = v[0:2 * n_lines:2, 0:3] + rng.normal(0.0, 0.04, (n_lines, 3)).astype(np.float32)
     v[:, 3:7] = rng.random((nv, 4)).astype(np.float32)
     v[:, 7] = 0.0
     ids = np.zeros(1, np.uint32)
@@ -839,17 +839,30 @@ def lines_points_workload(g, W, H, seed=SEED_BASE + 9, n_lines=20_000, n_points=
     g.glEnableClientState(E["GL_COLOR_ARRAY"])
     g.glVertexPointer(3, E["GL_FLOAT"], 32, 0)
     g.glColorPointer(4, E["GL_FLOAT"], 32, 12)
+    g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE_MINUS_SRC_ALPHA"])
+    return n_lines, n_points
+
+
+def draw_lines_points(g, st):
+    """GL_LINES of width 2, then GL_POINTS of size 3 blended, under the depth
+    test."""
+    n_lines, n_points = st
     g.glLineWidth(2.0)
     g.glDrawArrays(E["GL_LINES"], 0, 2 * n_lines)
     g.glEnable(E["GL_BLEND"])
-    g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE_MINUS_SRC_ALPHA"])
     g.glPointSize(3.0)
     g.glDrawArrays(E["GL_POINTS"], 2 * n_lines, n_points)
     g.glDisable(E["GL_BLEND"])
+
+
+def lines_points_workload(g, W, H, seed=SEED_BASE + 9, n_lines=20_000, n_points=20_000):
+    """setup_lines_points and one draw_lines_points."""
+    st = setup_lines_points(g, W, H, seed, n_lines, n_points)
+    draw_lines_points(g, st)
     g.glDisableClientState(E["GL_VERTEX_ARRAY"])
     g.glDisableClientState(E["GL_COLOR_ARRAY"])
     g.glBindBuffer(E["GL_ARRAY_BUFFER"], 0)
-    return nv
+    return 2 * st[0] + st[1]
 
 
 def _polygon_mode_tris(g, rng, W, H, seed, modes):

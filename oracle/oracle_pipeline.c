@@ -936,6 +936,7 @@ static void or_line(mtgl_dev *dev, const mtgl_raster_state *rs, int32_t x0, int3
     const or_tex *tex = (rs->flags & MTGL_F_TEXTURE_2D) ? or_prim_tex(dev, rs) : NULL;
     int32_t cx = x0, cy = y0;
     for (;;) {
+        dev->stats.fragments += (uint64_t)lw;   /* pixels the loop visits */
         float t = (float)step / (float)total;
         float z = z0 + t * (z1 - z0);
         float depth = or_prim_depth(rs, z);
@@ -1035,6 +1036,7 @@ static void or_points(mtgl_dev *dev, const mtgl_raster_state *rs, const or_vtx *
     for (uint32_t i = 0; i < n; i++) {
         or_v4 p = vt[i].pos;
         if (p.x < -p.w || p.x > p.w || p.y < -p.w || p.y > p.w || p.z < -p.w || p.z > p.w || p.w <= 0.0f) continue;
+        dev->stats.fragments += (uint64_t)ps * (uint64_t)ps;
         float nx = p.x / p.w, ny = p.y / p.w, nz = p.z / p.w;
         int32_t cx = or_cvtt((nx + 1.0f) * 0.5f * (float)vw + (float)vx);
         int32_t cy = or_cvtt((1.0f - ny) * 0.5f * (float)vh + (float)vy);
@@ -1082,6 +1084,7 @@ static void or_tri_prims(mtgl_dev *dev, const mtgl_raster_state *rs, const or_vt
     }
     for (int k = 0; k < 3; k++) {
         or_col c = col[k];
+        dev->stats.fragments += 1;
         if ((rs->flags & MTGL_F_ALPHA_TEST) && !or_compare(rs->alpha_func, c.a, rs->alpha_ref)) continue;
         if (rs->flags & MTGL_F_FOG) c = or_prim_fog(rs, c, -src[k]->ez);
         or_prim_pixel(dev, rs, X[k], Y[k], or_prim_depth(rs, src[k]->pos.z), c);
