@@ -26,6 +26,9 @@ blending, per call and per kernel, with the oracle timed on five calls.
 
 "lines_points": 100,000 width-2 lines and 100,000 blended size-3 points per
 1080p frame, with the oracle timed on two frames.
+
+"display_lists": 64 replays of an icosphere display list per 1080p frame,
+against the same blocks in immediate mode (vertex records staged per glEnd).
 """
 from __future__ import annotations
 
@@ -396,6 +399,97 @@ def lines_bench(frames: int, cpu_frames: int, n_lines: int = 100_000, n_points: 
     return out
 
 
+def lists_bench(frames: int, cpu_frames: int, copies: int = 64):
+    """Display lists at 1080p: an icosphere (subdivision 3, 1,280 triangles,
+    3,840 vertices with per-vertex colours and normals, one light) drawn
+    `copies` times per frame under different modelviews, three ways: replays
+    of a display list holding it as one glBegin/glEnd block (the records stay
+    in a device segment after the first replay), glDrawArrays from client
+    arrays (staged and sent on every draw), and glDrawArrays from a buffer
+    object (resident)."""
+    pos, tri = S.icosphere(3)
+    rng = np.random.default_rng(S.SEED_BASE + 10)
+    vcol = rng.random((len(pos), 4)).astype(np.float32)
+    idx = tri.reshape(-1)
+    inter = np.concatenate([pos[idx].astype(np.float32), vcol[idx], pos[idx].astype(np.float32)], 1)   # x y z r g b a nx ny nz
+    inter = np.ascontiguousarray(inter, np.float32)
+    nvtx = len(idx)
+
+    def run(g, nframes, how):
+        ctx = g.create_context(W, H)
+        S._perspective(g, W, H, dist=-12.0)
+        g.glEnable(E["GL_DEPTH_TEST"])
+        g.glEnable(E["GL_LIGHTING"])
+        g.glEnable(E["GL_LIGHT0"])
+        g.glEnable(E["GL_COLOR_MATERIAL"])
+        g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.keep(np.array([1.0, 1.0, 1.0, 0.0], np.float32)))
+        lid = g.glGenLists(1)
+        ptr = g.keep(inter)
+        if how == "list":
+            g.glNewList(lid, E["GL_COMPILE"])
+            g.glBegin(E["GL_TRIANGLES"])
+            for v in inter:
+                g.glColor4f(*v[3:7])
+                g.glNormal3f(*v[7:10])
+                g.glVertex3f(*v[0:3])
+            g.glEnd()
+            g.glEndList()
+        else:
+            off = 0
+            if how == "vbo":
+                ids = np.zeros(1, np.uint32)
+                g.glGenBuffers(1, ids)
+                g.glBindBuffer(E["GL_ARRAY_BUFFER"], int(ids[0]))
+                g.glBufferData(E["GL_ARRAY_BUFFER"], inter.nbytes, ptr, E["GL_STATIC_DRAW"])
+            for arr in ("GL_VERTEX_ARRAY", "GL_COLOR_ARRAY", "GL_NORMAL_ARRAY"):
+                g.glEnableClientState(E[arr])
+            base = 0 if how == "vbo" else ptr
+            g.glVertexPointer(3, E["GL_FLOAT"], 40, base)
+            g.glColorPointer(4, E["GL_FLOAT"], 40, base + 12)
+            g.glNormalPointer(E["GL_FLOAT"], 40, base + 28)
+        side = int(np.ceil(np.sqrt(copies)))
+
+        def frame():
+            g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"])
+            for k in range(copies):
+                g.glPushMatrix()
+                g.glTranslatef(2.2 * (k % side - side / 2 + 0.5), 1.4 * (k // side - side / 2 + 0.5), 0.0)
+                g.glScalef(0.6, 0.6, 0.6)
+                if how == "list":
+                    g.glCallList(lid)
+                else:
+                    g.glDrawArrays(E["GL_TRIANGLES"], 0, nvtx)
+                g.glPopMatrix()
+
+        frame()
+        g.glFinish()
+        g.stats_reset(ctx)
+        t0 = time.perf_counter()
+        for _ in range(nframes):
+            frame()
+        g.glFinish()
+        sec = time.perf_counter() - t0
+        st = g.stats(ctx)
+        g.destroy_context(ctx)
+        g.release()
+        return sec * 1e3 / nframes, st.upload_bytes / nframes, st.fragments / nframes
+
+    lib = os.path.join(ROOT, "mytinygl_amd", "libmtgl.so")
+    res = {how: run(GLLibrary(lib, "mtgl"), frames, how) for how in ("list", "client_arrays", "vbo")}
+    out = {"workload": f"{copies} icospheres (1,280 triangles, 3,840 vertices, one light) per 1080p frame: display "
+                       "list replays vs glDrawArrays from client arrays vs from a buffer object",
+           "frames": frames, "fragments_per_frame": int(res["list"][2])}
+    for how, (ms, up, _) in res.items():
+        out[f"ms_per_frame_{how}"] = round(ms, 3)
+        out[f"upload_bytes_per_frame_{how}"] = int(up)
+    olib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+    if cpu_frames > 0 and os.path.exists(olib):
+        ms_cpu, _, _ = run(GLLibrary(olib, "oracle"), cpu_frames, "list")
+        out["cpu_baseline"] = {"ms_per_frame": round(ms_cpu, 2), "cores": 1, "kind": "port",
+                               "sample": f"{cpu_frames} frames of the list workload on the oracle, one thread"}
+    return out
+
+
 def copy_calibration(nbytes: int, reps: int = 5) -> None:
     """Plain device-to-device copies of nbytes (torch's elementwise copy
     kernel): the reference the HBM counters FETCH_SIZE / WRITE_SIZE are
@@ -430,6 +524,9 @@ def main():
     ap.add_argument("--no-lines", action="store_true", help="skip the points-and-lines leg")
     ap.add_argument("--only-lines", action="store_true", help="run only the points-and-lines leg (profiling)")
     ap.add_argument("--lines-frames", type=int, default=20)
+    ap.add_argument("--no-lists", action="store_true", help="skip the display-list leg")
+    ap.add_argument("--only-lists", action="store_true", help="run only the display-list leg (profiling)")
+    ap.add_argument("--lists-frames", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -452,6 +549,9 @@ def main():
         return
     if args.only_lines:
         print(json.dumps(lines_bench(args.lines_frames, 0)), flush=True)
+        return
+    if args.only_lists:
+        print(json.dumps(lists_bench(args.lists_frames, 0)), flush=True)
         return
 
     g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
@@ -505,6 +605,7 @@ def main():
     differentiable = None if args.no_diff else diff_bench(args.diff_steps, 2, 0 if args.no_cpu else 2)
     pixels = None if args.no_pixels else pixels_bench(args.pixels_calls, 0 if args.no_cpu else 5)
     lines = None if args.no_lines else lines_bench(args.lines_frames, 0 if args.no_cpu else 2)
+    lists = None if args.no_lists else lists_bench(args.lists_frames, 0 if args.no_cpu else 1)
 
     out = {
         "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
@@ -534,6 +635,7 @@ def main():
         "differentiable": differentiable,
         "draw_pixels": pixels,
         "lines_points": lines,
+        "display_lists": lists,
     }
     if cpu_obj:
         out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None

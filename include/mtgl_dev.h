@@ -154,7 +154,8 @@ typedef struct {
     uint32_t type;       /* GL_FLOAT / GL_UNSIGNED_BYTE (attributes);
                             GL_UNSIGNED_BYTE / _SHORT / _INT (indices, state ids) */
     int32_t  stride;     /* bytes between consecutive elements (> 0) */
-    uint32_t buffer;     /* 0 = staged block, else buffer object id */
+    uint32_t buffer;     /* 0 = staged block, MTGL_SEGMENT_BIT | id = display-list segment,
+                            else buffer object id */
     uint32_t pad_;
     int64_t  offset;     /* byte offset of element 0 (may be negative for a staged
                             window that starts at a later element) */
@@ -200,6 +201,7 @@ typedef struct {
                                    measured with events on the context stream when
                                    timing is enabled */
     uint64_t raster_launches;
+    uint64_t upload_bytes;      /* host bytes draws staged and display-list segments sent */
 } mtgl_dev_stats;
 
 typedef struct mtgl_dev mtgl_dev;
@@ -235,6 +237,18 @@ int  mtgl_dev_tex_free(mtgl_dev *dev, uint32_t id);
 int  mtgl_dev_buffer_data(mtgl_dev *dev, uint32_t id, int64_t size, const void *data);
 int  mtgl_dev_buffer_sub_data(mtgl_dev *dev, uint32_t id, int64_t offset, int64_t size, const void *data);
 int  mtgl_dev_buffer_free(mtgl_dev *dev, uint32_t id);
+
+/* Display-list segments: device-resident copies of the vertex records a
+ * display list's glBegin/glEnd block produced when it was last replayed
+ * (reference lists replay their recorded calls, src/lists.c:185-425,
+ * src/gl_api.c:2290-2455; the host layer keeps replaying them and sends the
+ * records again only when they differ).  A draw addresses segment `id`
+ * (1..MTGL_MAX_SEGMENT) through mtgl_array.buffer = MTGL_SEGMENT_BIT | id.
+ * size <= 0 frees the segment. */
+#define MTGL_SEGMENT_BIT  0x80000000u
+#define MTGL_MAX_SEGMENT  0x00FFFFFFu
+int  mtgl_dev_segment_data(mtgl_dev *dev, uint32_t id, int64_t size, const void *data);
+int  mtgl_dev_segment_free(mtgl_dev *dev, uint32_t id);
 
 /* Render one draw into the context's framebuffer: replaces glEnd's flush_*
  * dispatch (reference src/gl_api.c:648-659, flush_* at src/raster.c:961-1231)

@@ -146,7 +146,11 @@ MTGL_EXPORT void gl_destroy_context(GLState *c)
 {
     if (!c) return;
     if (g_ctx == c) g_ctx = NULL;
-    for (size_t i = 0; i < GLS_MAX_LISTS; i++) free(c->lists[i].cmds);
+    for (size_t i = 0; i < GLS_MAX_LISTS; i++) {
+        free(c->lists[i].cmds);
+        for (size_t k = 0; k < c->lists[i].nsegs; k++) free(c->lists[i].segs[k].host);
+        free(c->lists[i].segs);   /* the device segments go with the device context */
+    }
     for (size_t i = 0; i < MTGL_MAX_OBJECTS; i++) free(c->buffers[i].shadow);
     free(c->imm);
     free(c->vstates);
@@ -602,8 +606,17 @@ void gls_flush_immediate(GLState *c)
         d.n_vstates = (uint32_t)c->vstate_count;
         d.vstates = c->vstates;
         d.raster = &rs;
-        d.staged = c->imm;
-        d.staged_bytes = (uint64_t)(n * sizeof(gls_imm_vertex));
+        const size_t bytes = n * sizeof(gls_imm_vertex);
+        /* a replayed list block draws from its device segment */
+        const uint32_t seg = gls_list_segment(c, c->imm, bytes);
+        if (seg) {
+            const uint32_t b = MTGL_SEGMENT_BIT | seg;
+            d.position.buffer = d.color.buffer = d.texcoord.buffer = d.normal.buffer = b;
+            if (c->vstate_count > 1) d.state_index.buffer = b;
+        } else {
+            d.staged = c->imm;
+            d.staged_bytes = (uint64_t)bytes;
+        }
         int rc = mtgl_dev_draw(c->dev, &d);
         gls_report_device_error(c, rc, "glEnd");
     }

@@ -7,6 +7,15 @@
  * points, so list geometry goes through the device pipeline with the matrices
  * current at replay time.  List ids are 1-based, a contiguous run of freed ids
  * is reused first, at most 1024 ids exist, nesting stops at 64 levels.
+ *
+ * Device-resident list geometry: the vertex records a replayed glBegin/glEnd
+ * block produces (object-space attributes and a state index; the matrices
+ * and lighting travel separately per draw) are kept in a device segment per
+ * block.  Every replay still runs the recorded calls on the host, so every
+ * side effect on the current colour, material and matrices is the
+ * reference's; the records are compared with the segment's host copy and
+ * sent to the device only when they differ (a block that inherits the
+ * current colour or normal from outside the list can change between calls).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -54,6 +63,55 @@ static void list_reset(gls_list *l)
     l->capacity = 0;
     l->valid = GL_FALSE;
     l->allocated = GL_TRUE;
+    l->segs = NULL;
+    l->nsegs = l->segs_capacity = 0;
+}
+
+void gls_list_free_segments(GLState *c, gls_list *l)
+{
+    for (size_t i = 0; i < l->nsegs; i++) {
+        if (c->dev) mtgl_dev_segment_free(c->dev, l->segs[i].id);
+        free(l->segs[i].host);
+    }
+    free(l->segs);
+    l->segs = NULL;
+    l->nsegs = l->segs_capacity = 0;
+}
+
+uint32_t gls_list_segment(GLState *c, const void *recs, size_t bytes)
+{
+    if (!c || c->replay_list == 0 || bytes == 0) return 0;
+    gls_list *l = list_lookup(c, c->replay_list);
+    if (!l) return 0;
+    gls_lseg *s = NULL;
+    for (size_t i = 0; i < l->nsegs; i++)
+        if (l->segs[i].cmd == c->replay_cmd) { s = &l->segs[i]; break; }
+    if (s && s->bytes == bytes && memcmp(s->host, recs, bytes) == 0) return s->id;
+    if (!s) {
+        if (c->seg_next >= MTGL_MAX_SEGMENT) return 0;
+        if (l->nsegs == l->segs_capacity) {
+            size_t cap = l->segs_capacity ? l->segs_capacity * 2 : 4;
+            gls_lseg *p = (gls_lseg *)realloc(l->segs, cap * sizeof(*p));
+            if (!p) return 0;
+            l->segs = p;
+            l->segs_capacity = cap;
+        }
+        s = &l->segs[l->nsegs++];
+        s->cmd = c->replay_cmd;
+        s->id = ++c->seg_next;
+        s->host = NULL;
+        s->bytes = 0;
+    }
+    uint8_t *h = (uint8_t *)realloc(s->host, bytes);
+    if (!h) return 0;
+    s->host = h;
+    memcpy(s->host, recs, bytes);
+    s->bytes = bytes;
+    if (mtgl_dev_segment_data(c->dev, s->id, (int64_t)bytes, recs) != MTGL_OK) {
+        s->bytes = 0;   /* compared unequal next time: sent again */
+        return 0;
+    }
+    return s->id;
 }
 
 MTGL_EXPORT GLuint glGenLists(GLsizei range)
@@ -88,6 +146,7 @@ MTGL_EXPORT void glDeleteLists(GLuint list, GLsizei range)
     for (GLsizei i = 0; i < range; i++) {
         gls_list *l = list_lookup(c, list + (GLuint)i);
         if (!l) continue;
+        gls_list_free_segments(c, l);
         free(l->cmds);
         l->cmds = NULL;
         l->count = l->capacity = 0;
@@ -103,6 +162,7 @@ MTGL_EXPORT void glNewList(GLuint list, GLenum mode)
     if (mode != GL_COMPILE && mode != GL_COMPILE_AND_EXECUTE) { gls_set_error(c, GL_INVALID_ENUM); return; }
     gls_list *l = list_lookup(c, list);
     if (!l) { gls_set_error(c, GL_INVALID_VALUE); return; }
+    gls_list_free_segments(c, l);
     l->count = 0;
     l->valid = GL_FALSE;
     c->list_index = list;
@@ -129,7 +189,17 @@ static void execute_list(GLState *c, GLuint id)
         const GLfloat *f = k.f;
         switch (k.op) {
         case LOP_BEGIN:         glBegin(k.e[0]); break;
-        case LOP_END:           glEnd(); break;
+        case LOP_END: {
+            /* nested replays restore the outer block's marks */
+            const GLuint pl = c->replay_list;
+            const size_t pc = c->replay_cmd;
+            c->replay_list = id;
+            c->replay_cmd = i;
+            glEnd();
+            c->replay_list = pl;
+            c->replay_cmd = pc;
+            break;
+        }
         case LOP_VERTEX:        glVertex3f(f[0], f[1], f[2]); break;
         case LOP_COLOR:         glColor4f(f[0], f[1], f[2], f[3]); break;
         case LOP_TEXCOORD:      glTexCoord2f(f[0], f[1]); break;

@@ -64,11 +64,23 @@ typedef struct {
     GLdouble d[6];                /* glOrtho / glFrustum operands */
 } gls_list_cmd;
 
+/* Device-resident vertex records of one glBegin/glEnd block of a list: the
+ * records its last replay sent (host copy, to compare) and the device segment
+ * holding them. */
+typedef struct {
+    size_t cmd;                   /* index of the block's LOP_END in the list */
+    uint32_t id;                  /* device segment id */
+    uint8_t *host;
+    size_t bytes;
+} gls_lseg;
+
 typedef struct {
     gls_list_cmd *cmds;
     size_t count, capacity;
     GLboolean valid;
     GLboolean allocated;
+    gls_lseg *segs;
+    size_t nsegs, segs_capacity;
 } gls_list;
 
 /* One glVertex call as seen by the host: object-space attributes plus the index
@@ -156,6 +168,10 @@ struct GLState {
     GLuint list_base, list_index;
     GLenum list_mode;
     GLuint list_call_depth;
+    /* the list block whose glEnd is being replayed (replay_list 0: none) */
+    GLuint replay_list;
+    size_t replay_cmd;
+    uint32_t seg_next;            /* next unused device segment id */
 
     /* vertices buffered since the last successful glEnd */
     gls_imm_vertex *imm;
@@ -182,6 +198,11 @@ void gls_apply_color_material(GLState *c, const float *color);
 void gls_flush_immediate(GLState *c);
 void gls_report_device_error(GLState *c, int rc, const char *what);
 int  gls_list_recording(GLState *c);
+void gls_list_free_segments(GLState *c, gls_list *l);
+/* the device segment that holds `bytes` of vertex records `recs` for the list
+ * block being replayed, refreshed when they differ from its last replay; 0
+ * when none applies */
+uint32_t gls_list_segment(GLState *c, const void *recs, size_t bytes);
 int  gls_list_record(GLState *c, const gls_list_cmd *cmd);
 
 #endif /* MTGL_GL_STATE_H */

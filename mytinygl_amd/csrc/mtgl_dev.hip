@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "GL/gl.h"
@@ -3301,6 +3302,7 @@ struct mtgl_dev {
     Arena pixels;                  // glDrawPixels source bytes
     Arena prim_params;             // states, staged bytes and vertex parameters of a point / line draw
     Arena prim_dep, prim_col;      // fragments of a point / line draw: depth, colour
+    std::unordered_map<uint32_t, DevBuf> segs;   // display-list segments by id
     DevTex tex[MTGL_MAX_OBJECTS + 1];
     DevBuf buf[MTGL_MAX_OBJECTS + 1];
     Exec ex;
@@ -3409,6 +3411,12 @@ DArr resolve(const mtgl_dev *d, const mtgl_array &a, const uint8_t *staged_dev, 
     if (a.buffer == 0) {
         r.src = staged_dev;
         r.size = (int64_t)staged_bytes;
+    } else if (a.buffer & MTGL_SEGMENT_BIT) {
+        const auto it = d->segs.find(a.buffer & ~MTGL_SEGMENT_BIT);
+        if (it != d->segs.end()) {
+            r.src = it->second.data;
+            r.size = it->second.size;
+        }
     } else if (a.buffer <= MTGL_MAX_OBJECTS && d->buf[a.buffer].data) {
         r.src = d->buf[a.buffer].data;
         r.size = d->buf[a.buffer].size;
@@ -4497,6 +4505,9 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     arena_free(d->prim_params);
     arena_free(d->prim_dep);
     arena_free(d->prim_col);
+    for (auto &kv : d->segs)
+        if (kv.second.data) (void)hipFree(kv.second.data);
+    d->segs.clear();
     arena_free(d->one.params);
     arena_free(d->one.maps);
     exec_free(d->ex);
@@ -4699,10 +4710,53 @@ MTGL_API int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
     return MTGL_OK;
 }
 
+MTGL_API int mtgl_dev_segment_data(mtgl_dev *d, uint32_t id, int64_t size, const void *data)
+{
+    if (!d || id == 0 || id > MTGL_MAX_SEGMENT) return MTGL_ERR_INVALID;
+    if (size <= 0) return mtgl_dev_segment_free(d, id);
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
+    DevBuf &b = d->segs[id];
+    // a recorded batch holds the old allocation's address
+    if (b.data) invalidate_batches(d);
+    d->ex.stats.upload_bytes += (uint64_t)size;
+    if (b.data && b.size == size) {
+        if (data) HIPC(hipMemcpy(b.data, data, (size_t)size, hipMemcpyHostToDevice));
+        return MTGL_OK;
+    }
+    uint8_t *p = nullptr;
+    HIPC(hipMalloc((void **)&p, (size_t)size));
+    hipError_t e = data ? hipMemcpy(p, data, (size_t)size, hipMemcpyHostToDevice) : hipMemset(p, 0, (size_t)size);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return map_err(e);
+    }
+    if (b.data) HIPC(hipFree(b.data));
+    b.data = p;
+    b.size = size;
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_segment_free(mtgl_dev *d, uint32_t id)
+{
+    if (!d || id == 0 || id > MTGL_MAX_SEGMENT) return MTGL_ERR_INVALID;
+    const auto it = d->segs.find(id);
+    if (it == d->segs.end()) return MTGL_OK;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
+    invalidate_batches(d);
+    if (it->second.data) HIPC(hipFree(it->second.data));
+    d->segs.erase(it);
+    return MTGL_OK;
+}
+
 MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
 {
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
     const int cls = prim_class(dr);
+    if (!g_recording) d->ex.stats.upload_bytes += dr->staged_bytes;
     if (cls == 1 || cls == 2) {
         // a recorded batch replays filled triangles only
         if (g_recording) return MTGL_ERR_UNSUPPORTED;

@@ -93,7 +93,7 @@ class MtglDevStats(C.Structure):
     _fields_ = [
         ("fragments", C.c_uint64), ("frag_bytes", C.c_uint64), ("triangles_in", C.c_uint64),
         ("triangles_raster", C.c_uint64), ("bin_pairs", C.c_uint64), ("draws", C.c_uint64),
-        ("raster_ms", C.c_double), ("raster_launches", C.c_uint64),
+        ("raster_ms", C.c_double), ("raster_launches", C.c_uint64), ("upload_bytes", C.c_uint64),
     ]
 
 

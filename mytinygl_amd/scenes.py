@@ -655,6 +655,67 @@ def scene_lists(g, W, H, seed):
     g.glCallList(base)
 
 
+def scene_lists_replay(g, W, H, seed):
+    """Display lists replayed many times: one block whose colours are all
+    recorded (the same vertex records every call), one that inherits the
+    current colour and normal from outside the list (records that change with
+    them), a list recompiled between calls, a deleted list, lines in a list,
+    and one glBegin outside a list closed by a list's glEnd (reference
+    src/lists.c:185-425, src/gl_api.c:2290-2455)."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    base = g.glGenLists(5)
+    g.glNewList(base, E["GL_COMPILE"])                 # fully recorded block
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 3, spread=1.5)
+    g.glEnd()
+    g.glEndList()
+    g.glNewList(base + 1, E["GL_COMPILE"])             # inherits colour and normal
+    g.glBegin(E["GL_TRIANGLES"])
+    for _ in range(6):
+        g.glVertex3f(*((rng.random(3) - 0.5) * 1.5).astype(np.float32))
+    g.glEnd()
+    g.glEndList()
+    g.glNewList(base + 2, E["GL_COMPILE"])             # lines
+    g.glBegin(E["GL_LINE_STRIP"])
+    for _ in range(5):
+        g.glColor3f(*rng.random(3).astype(np.float32))
+        g.glVertex3f(*((rng.random(3) - 0.5) * 1.5).astype(np.float32))
+    g.glEnd()
+    g.glEndList()
+    g.glNewList(base + 3, E["GL_COMPILE"])             # vertices and the glEnd of an outer glBegin
+    for _ in range(3):
+        g.glColor3f(*rng.random(3).astype(np.float32))
+        g.glVertex3f(*((rng.random(3) - 0.5) * 1.5).astype(np.float32))
+    g.glEnd()
+    g.glEndList()
+    for k in range(6):
+        g.glPushMatrix()
+        g.glTranslatef(float(rng.uniform(-0.8, 0.8)), float(rng.uniform(-0.6, 0.6)), 0.0)
+        g.glRotatef(float(rng.uniform(0, 360)), 0.0, 0.0, 1.0)
+        g.glCallList(base)
+        g.glColor3f(*rng.random(3).astype(np.float32))
+        g.glCallList(base + 1)
+        g.glCallList(base + 1)                         # same colour again
+        g.glCallList(base + 2)
+        g.glBegin(E["GL_TRIANGLES"])
+        g.glColor3f(*rng.random(3).astype(np.float32))
+        g.glVertex3f(*((rng.random(3) - 0.5) * 1.5).astype(np.float32))
+        g.glCallList(base + 3)
+        g.glPopMatrix()
+        if k == 2:                                     # recompile: new geometry in the same list
+            g.glNewList(base, E["GL_COMPILE"])
+            g.glBegin(E["GL_QUADS"])
+            for _ in range(8):
+                g.glColor4f(*rng.random(4).astype(np.float32))
+                g.glVertex3f(*((rng.random(3) - 0.5) * 1.5).astype(np.float32))
+            g.glEnd()
+            g.glEndList()
+        if k == 4:
+            g.glDeleteLists(base + 2, 1)
+
+
 def scene_scissor_viewport(g, W, H, seed):
     rng = _rng(seed)
     _perspective(g, W, H)
@@ -1135,7 +1196,7 @@ UNDEFINED_IN_REFERENCE = ("vbo_oob",)
 # scenes with calls a batch does not record (glDrawPixels, points, lines and
 # polygon-mode LINE / POINT draws are refused while recording); they stay out
 # of the batched feature tests
-NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed")
+NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "lists_replay")
 # scenes with draws the device refuses (one face filled, the other drawn as
 # lines or points); oracle against reference only
 DEVICE_UNSUPPORTED = ("polygon_mode_mixed",)
@@ -1158,6 +1219,7 @@ FEATURE_SCENES = {
     "polygon_mode": scene_polygon_mode,
     "polygon_mode_mixed": scene_polygon_mode_mixed,
     "lists": scene_lists,
+    "lists_replay": scene_lists_replay,
     "scissor_viewport": scene_scissor_viewport,
     "colormask": scene_colormask,
     "edge_cases": scene_edge_cases,

@@ -76,6 +76,8 @@ struct mtgl_dev {
     uint8_t *stencil;
     or_tex tex[MTGL_MAX_OBJECTS + 1];
     or_buf buf[MTGL_MAX_OBJECTS + 1];
+    or_buf *segs;          /* display-list segments by id (index 0 unused) */
+    uint32_t nsegs;
     mtgl_dev_stats stats;
     /* differentiable capture of the latest draw */
     int last_valid;
@@ -270,6 +272,12 @@ static or_col or_light(const mtgl_lighting *L, uint32_t flags, const float *eye_
 static const uint8_t *or_source(mtgl_dev *dev, const mtgl_draw *d, const mtgl_array *a, int64_t *size)
 {
     if (a->buffer == 0) { *size = (int64_t)d->staged_bytes; return (const uint8_t *)d->staged; }
+    if (a->buffer & MTGL_SEGMENT_BIT) {
+        uint32_t id = a->buffer & ~MTGL_SEGMENT_BIT;
+        if (id >= dev->nsegs || !dev->segs[id].data) { *size = 0; return NULL; }
+        *size = dev->segs[id].size;
+        return dev->segs[id].data;
+    }
     if (a->buffer > MTGL_MAX_OBJECTS || !dev->buf[a->buffer].data) { *size = 0; return NULL; }
     *size = dev->buf[a->buffer].size;
     return dev->buf[a->buffer].data;
@@ -1182,6 +1190,8 @@ void mtgl_dev_destroy(mtgl_dev *d)
         free(d->tex[i].m1);
         free(d->buf[i].data);
     }
+    for (uint32_t i = 0; i < d->nsegs; i++) free(d->segs[i].data);
+    free(d->segs);
     free(d->color);
     free(d->depth);
     free(d->stencil);
@@ -1401,6 +1411,40 @@ int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
 
 /* one draw: per-vertex T&L of every emitted vertex, then primitive assembly in
  * the reference's flush order (src/raster.c:961-1017, :1199-1231) */
+int mtgl_dev_segment_data(mtgl_dev *d, uint32_t id, int64_t size, const void *data)
+{
+    if (!d || id == 0 || id > MTGL_MAX_SEGMENT) return MTGL_ERR_INVALID;
+    if (size <= 0) return mtgl_dev_segment_free(d, id);
+    if (id >= d->nsegs) {
+        uint32_t n = d->nsegs ? d->nsegs : 16;
+        while (n <= id) n *= 2;
+        or_buf *p = (or_buf *)realloc(d->segs, (size_t)n * sizeof(or_buf));
+        if (!p) return MTGL_ERR_OUT_OF_MEMORY;
+        memset(p + d->nsegs, 0, (size_t)(n - d->nsegs) * sizeof(or_buf));
+        d->segs = p;
+        d->nsegs = n;
+    }
+    uint8_t *q = (uint8_t *)malloc((size_t)size);
+    if (!q) return MTGL_ERR_OUT_OF_MEMORY;
+    if (data) memcpy(q, data, (size_t)size); else memset(q, 0, (size_t)size);
+    d->stats.upload_bytes += (uint64_t)size;
+    free(d->segs[id].data);
+    d->segs[id].data = q;
+    d->segs[id].size = size;
+    return MTGL_OK;
+}
+
+int mtgl_dev_segment_free(mtgl_dev *d, uint32_t id)
+{
+    if (!d || id == 0 || id > MTGL_MAX_SEGMENT) return MTGL_ERR_INVALID;
+    if (id < d->nsegs) {
+        free(d->segs[id].data);
+        d->segs[id].data = NULL;
+        d->segs[id].size = 0;
+    }
+    return MTGL_OK;
+}
+
 int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
 {
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
@@ -1416,6 +1460,7 @@ int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     if (or_recording) return or_record_draw(d, dr);
     or_diff_prepare(d, dr);
     d->stats.draws++;
+    d->stats.upload_bytes += dr->staged_bytes;
     if (n == 0) return MTGL_OK;
     or_vtx *vt = (or_vtx *)malloc((size_t)n * sizeof(or_vtx));
     if (!vt) return MTGL_ERR_OUT_OF_MEMORY;

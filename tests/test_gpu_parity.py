@@ -98,3 +98,33 @@ def test_lines_points_workload(mtgl, oracle, seed):
     assert ea == eb
     assert (a[0] != a[0][0, 0]).sum() > 10_000
     _check_frame(f"lines_points_workload/{seed}", a, b)
+
+
+def test_list_segments_device(mtgl, oracle):
+    """A list replayed six times sends its records once on the device too,
+    and renders what the oracle renders."""
+    out = []
+    for g in (oracle, mtgl):
+        ctx = g.create_context(96, 72)
+        try:
+            S._perspective(g, 96, 72)
+            g.glClear(S.E["GL_COLOR_BUFFER_BIT"] | S.E["GL_DEPTH_BUFFER_BIT"])
+            g.glEnable(S.E["GL_DEPTH_TEST"])
+            lid = g.glGenLists(1)
+            rng = np.random.default_rng(3)
+            g.glNewList(lid, S.E["GL_COMPILE"])
+            g.glBegin(S.E["GL_TRIANGLES"])
+            S._random_tris_immediate(g, rng, 40, spread=1.5)
+            g.glEnd()
+            g.glEndList()
+            g.stats_reset(ctx)
+            for k in range(6):
+                g.glRotatef(15.0, 0.0, 0.0, 1.0)
+                g.glCallList(lid)
+            st = g.stats(ctx)
+            out.append((st.upload_bytes, st.draws, g.read_framebuffer(ctx, 96, 72)))
+        finally:
+            g.destroy_context(ctx)
+    (ua, da, a), (ub, db, b) = out
+    assert ua == ub == 120 * 52 and da == db == 6
+    _check_frame("list_segments", a, b)
