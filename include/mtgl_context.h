@@ -35,6 +35,12 @@ int32_t   mtgl_context_width(GLState *ctx);
 int32_t   mtgl_context_height(GLState *ctx);
 mtgl_dev *mtgl_context_device(GLState *ctx);
 
+/* Texture level of detail of the context's later draws: MTGL_LOD_REFERENCE
+ * (the default) or MTGL_LOD_FULL_CHAIN (see mtgl_dev.h).  No counterpart in
+ * the reference; for callers that want full mipmapping with a per-pixel LOD.
+ * The CPU oracle renders the reference's sampling in either mode. */
+void mtgl_texture_lod_mode(GLState *ctx, uint32_t mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,18 @@ extern "C" {
 #define MTGL_F_SCISSOR_TEST    (1u << 10)
 #define MTGL_F_STENCIL_TEST    (1u << 11)
 
+/* ---- texture level of detail (mtgl_raster_state.lod_mode) ----
+ * MTGL_LOD_REFERENCE: the reference's sampling, level 0 and a level 1 with one
+ * LOD per triangle (src/textures.c:457-557, src/raster.c:505-529).
+ * MTGL_LOD_FULL_CHAIN, an extension with no counterpart in the reference: the
+ * whole chain of box-filtered levels (each halving the previous one's sides,
+ * built like level 1) and a LOD per pixel, log2 of the larger screen-space
+ * derivative length of the texture coordinates in texels; GL 1.5 level
+ * selection (nearest level at ceil(lod + 0.5) - 1, linear between floor(lod)
+ * and the next).  Points and lines sample at LOD 0 in both modes. */
+#define MTGL_LOD_REFERENCE   0u
+#define MTGL_LOD_FULL_CHAIN  1u
+
 /* ---- buffers selected by mtgl_dev_clear / mtgl_dev_read / mtgl_dev_write ---- */
 #define MTGL_BUF_COLOR    1u
 #define MTGL_BUF_DEPTH    2u
@@ -142,7 +154,8 @@ typedef struct {
     float    fog_density, fog_start, fog_end;
     mtgl_color fog_color;
     float    line_width, point_size;
-    uint32_t pad_[2];
+    uint32_t lod_mode;              /* MTGL_LOD_REFERENCE or MTGL_LOD_FULL_CHAIN (extension) */
+    uint32_t pad_;
     mtgl_lighting lighting;         /* lights/materials at glEnd (GL_PHONG, two-sided) */
 } mtgl_raster_state;
 

@@ -165,6 +165,13 @@ MTGL_EXPORT int32_t mtgl_context_width(GLState *c) { return c ? c->width : 0; }
 MTGL_EXPORT int32_t mtgl_context_height(GLState *c) { return c ? c->height : 0; }
 MTGL_EXPORT mtgl_dev *mtgl_context_device(GLState *c) { return c ? c->dev : NULL; }
 
+MTGL_EXPORT void mtgl_texture_lod_mode(GLState *c, uint32_t mode)
+{
+    if (!c) return;
+    if (mode != MTGL_LOD_REFERENCE && mode != MTGL_LOD_FULL_CHAIN) { gls_set_error(c, GL_INVALID_ENUM); return; }
+    c->lod_mode = mode;
+}
+
 MTGL_EXPORT int mtgl_read_framebuffer(GLState *c, uint32_t *color, float *depth, uint8_t *stencil)
 {
     if (!c) return MTGL_ERR_INVALID;
@@ -248,6 +255,7 @@ void gls_snapshot_raster_state(GLState *c, mtgl_raster_state *rs)
             rs->wrap_t = t->wrap_t;
         }
     }
+    rs->lod_mode = c->lod_mode;
     rs->tex_env_mode = c->tex_env_mode;
     rs->tex_env_color = c->tex_env_color;
     rs->perspective_hint = c->perspective_hint;

@@ -187,6 +187,7 @@ struct GLState {
 
     GLenum error;
     uint32_t warned;              /* one-shot diagnostics already printed */
+    uint32_t lod_mode;            /* MTGL_LOD_* (extension) */
 };
 
 /* internal helpers shared by the host translation units */

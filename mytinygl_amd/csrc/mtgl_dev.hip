@@ -208,7 +208,10 @@ struct SceneRaster {
     int32_t fb_w, fb_h, tiles_x;
     uint32_t tile_off;            // first global tile of the scene
     uint32_t kind;
-    uint32_t pad_;
+    uint32_t lod_mode;            // MTGL_LOD_*
+    const uint32_t *chain;        // levels 2.. of the full chain, back to back (null: none)
+    int32_t nlev;                 // levels of the chain, 0 and 1 included
+    int32_t pad_;
 };
 
 struct RasterParams {
@@ -1385,6 +1388,9 @@ struct RasterConsts {
     Col tex_env_color, fog_color;
     double depth_near, depth_far;
     TexLevel L0, L1;
+    bool chain_on;                // full chain with a per-pixel LOD (extension)
+    const uint32_t *chain;
+    int32_t nlev;
 };
 
 __device__ __forceinline__ int32_t wrap_coord(int32_t x, int32_t w, bool repeat, bool pow2)
@@ -1517,6 +1523,84 @@ __device__ __forceinline__ void texcoord_at(const RTri *r, bool persp, float b0,
     }
 }
 
+// ---- full mip chain with a per-pixel LOD (MTGL_LOD_FULL_CHAIN, an extension) ----
+
+// level k of the chain: 0 and 1 as in the reference's sampling, then the
+// levels stored back to back after level 1, each halving the previous sides
+__device__ __forceinline__ TexLevel chain_level(const RasterConsts &K, int k)
+{
+    if (k <= 0) return K.L0;
+    if (k == 1) return K.L1;
+    int32_t w = K.L1.w / 2, h = K.L1.h / 2;
+    size_t off = 0;
+    for (int j = 2; j < k; j++) {
+        off += (size_t)w * (size_t)h;
+        w /= 2;
+        h /= 2;
+    }
+    return {K.chain + off, w, h, (w & (w - 1)) == 0 && (h & (h - 1)) == 0};
+}
+
+// log2 of the larger of |d(s, t)/dx| and |d(s, t)/dy| in texels at a pixel
+// with weights b: the derivatives of the weights are the reference's edge
+// function coefficients over the area; with perspective the coordinates are
+// (sum b u/w) / (sum b 1/w), differentiated as a quotient
+__device__ float pixel_lod(const RasterConsts &K, const RTri *r, float b0, float b1, float b2)
+{
+    const float ia = r->inv_area;
+    const float x0 = (float)r->x0, y0 = (float)r->y0, x1 = (float)r->x1, y1 = (float)r->y1;
+    const float x2 = (float)r->x2, y2 = (float)r->y2;
+    const float d0x = (y2 - y1) * ia, d0y = -(x2 - x1) * ia;
+    const float d1x = (y0 - y2) * ia, d1y = -(x0 - x2) * ia;
+    const float d2x = (y1 - y0) * ia, d2y = -(x1 - x0) * ia;
+    float ux, uy, vx, vy;
+    if (K.persp) {
+        const float a0 = r->w0, a1 = r->w1, a2 = r->w2;
+        const float u0 = r->uv0[0] * a0, u1 = r->uv1[0] * a1, u2 = r->uv2[0] * a2;
+        const float v0 = r->uv0[1] * a0, v1 = r->uv1[1] * a1, v2 = r->uv2[1] * a2;
+        const float q = b0 * a0 + b1 * a1 + b2 * a2;
+        const float pu = b0 * u0 + b1 * u1 + b2 * u2, pv = b0 * v0 + b1 * v1 + b2 * v2;
+        const float qx = d0x * a0 + d1x * a1 + d2x * a2, qy = d0y * a0 + d1y * a1 + d2y * a2;
+        const float iq2 = 1.0f / (q * q);
+        ux = ((d0x * u0 + d1x * u1 + d2x * u2) * q - pu * qx) * iq2;
+        uy = ((d0y * u0 + d1y * u1 + d2y * u2) * q - pu * qy) * iq2;
+        vx = ((d0x * v0 + d1x * v1 + d2x * v2) * q - pv * qx) * iq2;
+        vy = ((d0y * v0 + d1y * v1 + d2y * v2) * q - pv * qy) * iq2;
+    } else {
+        ux = d0x * r->uv0[0] + d1x * r->uv1[0] + d2x * r->uv2[0];
+        uy = d0y * r->uv0[0] + d1y * r->uv1[0] + d2y * r->uv2[0];
+        vx = d0x * r->uv0[1] + d1x * r->uv1[1] + d2x * r->uv2[1];
+        vy = d0y * r->uv0[1] + d1y * r->uv1[1] + d2y * r->uv2[1];
+    }
+    const float tw = (float)K.L0.w, th = (float)K.L0.h;
+    ux *= tw; uy *= tw; vx *= th; vy *= th;
+    const float rho = fmaxf(sqrtf(ux * ux + vx * vx), sqrtf(uy * uy + vy * vy));
+    return log2f(rho);
+}
+
+// GL 1.5 level selection over the chain; each level sampled and packed to
+// RGBA8, two levels blended like the reference's level 0 / level 1 blend
+__device__ uint32_t sample_chain(const RasterConsts &K, float u, float v, float lod, int &taps)
+{
+    const bool rep_s = K.rep_s, rep_t = K.rep_t;
+    wrap_uv(rep_s, rep_t, u, v);
+    if (!(lod > 0.0f)) return sample_level(K.L0, rep_s, rep_t, u, v, K.mag_filter == GL_LINEAR, taps);
+    const uint32_t f = K.min_filter;
+    if (f == GL_NEAREST || f == GL_LINEAR) return sample_level(K.L0, rep_s, rep_t, u, v, f == GL_LINEAR, taps);
+    const bool lin = f == GL_LINEAR_MIPMAP_NEAREST || f == GL_LINEAR_MIPMAP_LINEAR;
+    const int top = K.nlev - 1;
+    if (f == GL_NEAREST_MIPMAP_NEAREST || f == GL_LINEAR_MIPMAP_NEAREST) {
+        const float d = fminf(fmaxf(ceilf(lod + 0.5f) - 1.0f, 0.0f), (float)top);
+        return sample_level(chain_level(K, (int)d), rep_s, rep_t, u, v, lin, taps);
+    }
+    if (lod >= (float)top) return sample_level(chain_level(K, top), rep_s, rep_t, u, v, lin, taps);
+    const float fl = floorf(lod);
+    const int d = (int)fl;
+    const uint32_t c0 = sample_level(chain_level(K, d), rep_s, rep_t, u, v, lin, taps);
+    const uint32_t c1 = sample_level(chain_level(K, d + 1), rep_s, rep_t, u, v, lin, taps);
+    return col_pack(col_lerp(col_unpack(c0), col_unpack(c1), lod - fl));
+}
+
 // The part of one fragment's pipeline (src/raster.c:546-721) that does not
 // read the framebuffer: window depth, colour through lighting, texturing and
 // fog, whether the alpha test keeps it, and how many texels it read.  It
@@ -1564,7 +1648,8 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
         float u, v;
         texcoord_at(r, K.persp, b0, b1, b2, u, v);
         int taps = 0;
-        Col tc = col_unpack(sample_texture(K, K.L0, K.L1, u, v, r->lod, taps));
+        Col tc = col_unpack(K.chain_on ? sample_chain(K, u, v, pixel_lod(K, r, b0, b1, b2), taps)
+                                       : sample_texture(K, K.L0, K.L1, u, v, r->lod, taps));
         if (K.alpha_on && !cmp_f(K.alpha_cmp, tc.a, K.alpha_ref)) {
             c_out = c;
             meta_out = (uint32_t)taps << 1;
@@ -1945,6 +2030,9 @@ __device__ __forceinline__ void raster_consts(const SceneRaster &P, RasterConsts
     K.L0 = {P.tex, P.tex_w, P.tex_h, P.tex_w > 0 && (P.tex_w & (P.tex_w - 1)) == 0 && (P.tex_h & (P.tex_h - 1)) == 0};
     K.L1 = {P.tex1, P.tex1_w, P.tex1_h,
             P.tex1_w > 0 && (P.tex1_w & (P.tex1_w - 1)) == 0 && (P.tex1_h & (P.tex1_h - 1)) == 0};
+    K.chain_on = P.lod_mode == MTGL_LOD_FULL_CHAIN && P.tex != nullptr;
+    K.chain = P.chain;
+    K.nlev = P.nlev > 0 ? P.nlev : 1;
 }
 
 // Per-lane counts summed over the wave (bytes as two 32-bit halves) and
@@ -3217,6 +3305,8 @@ namespace {
 struct DevTex {
     int32_t w = 0, h = 0, m1w = 0, m1h = 0;
     uint32_t *px = nullptr, *m1 = nullptr;
+    uint32_t *chain = nullptr;     // levels 2.. back to back
+    int32_t nlev = 0;              // levels, 0 and 1 included
 };
 
 struct DevBuf {
@@ -3777,6 +3867,9 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         r.tex_h = p.tx ? p.tx->h : 0;
         r.tex1_w = p.tx ? p.tx->m1w : 0;
         r.tex1_h = p.tx ? p.tx->m1h : 0;
+        r.lod_mode = dr->raster->lod_mode;
+        r.chain = p.tx ? p.tx->chain : nullptr;
+        r.nlev = p.tx ? p.tx->nlev : 0;
         r.fb_w = d->w;
         r.fb_h = d->h;
         r.tiles_x = d->tiles_x;
@@ -4498,6 +4591,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     for (int i = 0; i <= MTGL_MAX_OBJECTS; i++) {
         if (d->tex[i].px) (void)hipFree(d->tex[i].px);
         if (d->tex[i].m1) (void)hipFree(d->tex[i].m1);
+        if (d->tex[i].chain) (void)hipFree(d->tex[i].chain);
         if (d->buf[i].data) (void)hipFree(d->buf[i].data);
     }
     arena_free(d->writer);
@@ -4632,16 +4726,43 @@ MTGL_API int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, 
                            mh);
         HIPC(hipGetLastError());
     }
+    // the rest of the chain (sampled only under MTGL_LOD_FULL_CHAIN): each
+    // level the box filter of the one before, while both sides are >= 2
+    int32_t nlev = m1 ? 2 : 1;
+    size_t chain_n = 0;
+    for (int32_t lw = mw, lh = mh; lw >= 2 && lh >= 2; lw /= 2, lh /= 2) chain_n += (size_t)(lw / 2) * (lh / 2);
+    uint32_t *chain = nullptr;
+    if (chain_n) {
+        hipError_t e = hipMalloc((void **)&chain, chain_n * 4);
+        if (e != hipSuccess) {
+            (void)hipFree(px);
+            if (m1) (void)hipFree(m1);
+            return map_err(e);
+        }
+        const uint32_t *src = m1;
+        uint32_t *dst = chain;
+        for (int32_t lw = mw, lh = mh; lw >= 2 && lh >= 2; lw /= 2, lh /= 2) {
+            hipLaunchKernelGGL(k_mip1, dim3(blocks_for((uint64_t)(lw / 2) * (lh / 2), 256)), dim3(256), 0,
+                               d->ex.stream, src, lw, dst, lw / 2, lh / 2);
+            src = dst;
+            dst += (size_t)(lw / 2) * (lh / 2);
+            nlev++;
+        }
+        HIPC(hipGetLastError());
+    }
     HIPC(hipStreamSynchronize(d->ex.stream));
     DevTex &t = d->tex[id];
     if (t.px) HIPC(hipFree(t.px));
     if (t.m1) HIPC(hipFree(t.m1));
+    if (t.chain) HIPC(hipFree(t.chain));
     t.px = px;
     t.w = w;
     t.h = h;
     t.m1 = m1;
     t.m1w = mw;
     t.m1h = mh;
+    t.chain = chain;
+    t.nlev = nlev;
     return MTGL_OK;
 }
 
@@ -4655,6 +4776,7 @@ MTGL_API int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
     if (t.px || t.m1) invalidate_batches(d);
     if (t.px) HIPC(hipFree(t.px));
     if (t.m1) HIPC(hipFree(t.m1));
+    if (t.chain) HIPC(hipFree(t.chain));
     t = DevTex();
     return MTGL_OK;
 }

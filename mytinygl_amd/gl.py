@@ -206,6 +206,8 @@ class GLLibrary:
                 fn.restype = C.c_int
             self.lib.mtgl_batch_destroy.argtypes = [vp]
             self.lib.mtgl_batch_destroy.restype = None
+            self.lib.mtgl_texture_lod_mode.argtypes = [vp, C.c_uint32]
+            self.lib.mtgl_texture_lod_mode.restype = None
         self._read.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         self._read.restype = C.c_int
 
@@ -226,6 +228,11 @@ class GLLibrary:
             raise RuntimeError(f"{self.kind}: gl_create_context({width}, {height}) failed")
         self.gl_make_current(ctx)
         return ctx
+
+    def texture_lod_mode(self, ctx: int, mode: int) -> None:
+        """MTGL_LOD_REFERENCE (0) or MTGL_LOD_FULL_CHAIN (1) for the context's
+        later draws (extension; see include/mtgl_dev.h)."""
+        self.lib.mtgl_texture_lod_mode(ctx, mode)
 
     def destroy_context(self, ctx: int) -> None:
         self.gl_make_current(None)

@@ -716,6 +716,51 @@ def scene_lists_replay(g, W, H, seed):
             g.glDeleteLists(base + 2, 1)
 
 
+# screen-aligned quads (x0, row0, x1, row1, s extent, t extent) over a 256x256
+# framebuffer whose LODs with a 64x64 texture span magnification to beyond the
+# last level, with isotropic and anisotropic footprints
+LOD_QUADS = [
+    (4, 4, 67, 67, 0.5, 0.5),          # lod -0.98 (magnified)
+    (72, 4, 103, 35, 0.6, 0.6),        # 0.31
+    (108, 4, 139, 35, 1.2, 1.0),       # 1.31
+    (144, 4, 159, 19, 1.5, 0.5),       # 2.68 (anisotropic)
+    (164, 4, 171, 11, 1.3, 1.3),       # 3.57
+    (176, 4, 179, 7, 3.3, 2.0),        # 6.14: the last level
+    (184, 4, 185, 5, 40.0, 1.0),       # 11.3: beyond the chain
+    (4, 72, 131, 103, 3.0, 0.2),       # 0.60 (s minified, t magnified)
+    (136, 72, 151, 135, 0.3, 2.4),     # 1.29 (t)
+    (156, 72, 219, 199, 2.7, 6.9),     # 1.80
+]
+
+
+def lod_quads(g, W, H, tex: np.ndarray, min_filter: str, mag_filter: str = "GL_LINEAR", quads=None):
+    """Textured screen-aligned quads (GL_REPLACE, GL_REPEAT) at integer pixel
+    corners under glOrtho(0, W, 0, H): the texture coordinates are affine in
+    the pixel position, each quad has one LOD."""
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glOrtho(0.0, float(W), 0.0, float(H), -1.0, 1.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    tid = _upload_texture(g, tex)
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E[min_filter])
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"], E[mag_filter])
+    g.glEnable(E["GL_TEXTURE_2D"])
+    g.glTexEnvi(E["GL_TEXTURE_ENV"], E["GL_TEXTURE_ENV_MODE"], E["GL_REPLACE"])
+    g.glBegin(E["GL_QUADS"])
+    for q in (LOD_QUADS if quads is None else quads):
+        x0, r0, x1, r1, su, sv = q[:6]
+        s0, t0 = q[6:8] if len(q) > 6 else (0.0, 0.0)   # optional texture-coordinate origin
+        # GL y of a framebuffer row r is H - r
+        for (x, r, s, t) in ((x0, r0, s0, t0), (x1, r0, s0 + su, t0), (x1, r1, s0 + su, t0 + sv),
+                             (x0, r1, s0, t0 + sv)):
+            g.glTexCoord2f(s, t)
+            g.glVertex2f(float(x), float(H - r))
+    g.glEnd()
+    g.glDisable(E["GL_TEXTURE_2D"])
+    return tid
+
+
 def scene_scissor_viewport(g, W, H, seed):
     rng = _rng(seed)
     _perspective(g, W, H)
