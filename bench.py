@@ -28,7 +28,11 @@ blending, per call and per kernel, with the oracle timed on five calls.
 1080p frame, with the oracle timed on two frames.
 
 "display_lists": 64 replays of an icosphere display list per 1080p frame,
-against the same blocks in immediate mode (vertex records staged per glEnd).
+against display-list-free draws from client arrays and from a buffer object.
+
+"configs": the icosphere at 640x480 and the 200,000-triangle overdraw
+workload at 3840x2160 (blending, stencil, culling), each next to the
+reference on one core.
 """
 from __future__ import annotations
 
@@ -102,6 +106,75 @@ def cpu_baseline(budget_s: float):
     g.destroy_context(ctx)
     g.release()
     return {"frames": frames, "seconds": el, "kind": kind}
+
+
+def configs_bench(frames: dict, cpu_budget_s: float):
+    """The other two single-frame workloads: the icosphere at 640x480 (1,280
+    triangles, one light, depth test; about 51k fragments) and the overdraw
+    workload at 3840x2160 (200,000 random triangles, back faces culled,
+    SRC_ALPHA blending, stencil GREATER/INCR, depth test; about 108.7M
+    fragments).  One frame = clear of every plane + one glDrawArrays from
+    client arrays (staged per draw, as the reference reads them).  The
+    reference (strict build, one thread) renders the same frames for a
+    bounded time."""
+    cases = {"icosphere_640x480": (S.setup_icosphere, S.draw_icosphere, 640, 480),
+             "overdraw_3840x2160": (S.setup_overdraw, S.draw_overdraw, 3840, 2160)}
+    ref = os.path.join(ROOT, "oracle", "_ref", "libmtgl_ref.so")
+    kind = "reference"
+    if not os.path.exists(ref):
+        ref, kind = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so"), "port"
+    bits = E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"] | E["GL_STENCIL_BUFFER_BIT"]
+    out = {}
+    for name, (setup, draw, w, h) in cases.items():
+        g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
+        ctx = g.create_context(w, h)
+        g.glClearColor(0.0, 0.0, 0.0, 1.0)
+        n = setup(g, w, h)
+        for _ in range(2):
+            g.glClear(bits)
+            draw(g, n)
+        g.glFinish()
+        g.stats_reset(ctx)
+        g.set_timing(ctx, True)
+        nf = frames[name]
+        t0 = time.perf_counter()
+        for _ in range(nf):
+            g.glClear(bits)
+            draw(g, n)
+        g.glFinish()
+        sec = time.perf_counter() - t0
+        st = g.stats(ctx)
+        g.set_timing(ctx, False)
+        g.destroy_context(ctx)
+        g.release()
+        frags = st.fragments / nf
+        rec = {"frames": nf, "ms_per_frame": round(sec * 1e3 / nf, 4), "fragments_per_frame": int(frags),
+               "mfragments_s": round(frags * nf / sec / 1e6, 1),
+               "fragment_stage_ms": round(st.raster_ms / max(1, st.raster_launches), 5),
+               "algorithmic_bytes_per_frame": int(st.frag_bytes / nf)}
+        if st.raster_ms > 0:
+            rec["fragment_stage_gb_s"] = round(st.frag_bytes / (st.raster_ms * 1e-3) / 1e9, 1)
+        if cpu_budget_s > 0 and os.path.exists(ref):
+            c = GLLibrary(ref, "reference" if kind == "reference" else "oracle")
+            cctx = c.create_context(w, h)
+            c.glClearColor(0.0, 0.0, 0.0, 1.0)
+            cn = setup(c, w, h)
+            cf, t0 = 0, time.perf_counter()
+            while True:
+                c.glClear(bits)
+                draw(c, cn)
+                cf += 1
+                el = time.perf_counter() - t0
+                if el >= cpu_budget_s or cf >= 200:
+                    break
+            c.destroy_context(cctx)
+            c.release()
+            rec["cpu_baseline"] = {"ms_per_frame": round(el * 1e3 / cf, 2), "mfragments_s": round(frags * cf / el / 1e6, 2),
+                                   "cores": 1, "kind": kind,
+                                   "sample": f"{cf} frames of the same workload in {el:.1f} s, one thread"}
+            rec["speedup_vs_cpu"] = round((el / cf) / (sec / nf), 1)
+        out[name] = rec
+    return out
 
 
 def batch_bench(launches: int, per_draw_reps: int, cpu_seconds: float):
@@ -527,6 +600,8 @@ def main():
     ap.add_argument("--no-lists", action="store_true", help="skip the display-list leg")
     ap.add_argument("--only-lists", action="store_true", help="run only the display-list leg (profiling)")
     ap.add_argument("--lists-frames", type=int, default=8)
+    ap.add_argument("--no-configs", action="store_true", help="skip the icosphere 640x480 and overdraw 4K legs")
+    ap.add_argument("--only-configs", action="store_true", help="run only those legs (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -552,6 +627,9 @@ def main():
         return
     if args.only_lists:
         print(json.dumps(lists_bench(args.lists_frames, 0)), flush=True)
+        return
+    if args.only_configs:
+        print(json.dumps(configs_bench({"icosphere_640x480": 200, "overdraw_3840x2160": 3}, 0.0)), flush=True)
         return
 
     g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
@@ -606,6 +684,8 @@ def main():
     pixels = None if args.no_pixels else pixels_bench(args.pixels_calls, 0 if args.no_cpu else 5)
     lines = None if args.no_lines else lines_bench(args.lines_frames, 0 if args.no_cpu else 2)
     lists = None if args.no_lists else lists_bench(args.lists_frames, 0 if args.no_cpu else 1)
+    configs = None if args.no_configs else configs_bench({"icosphere_640x480": 200, "overdraw_3840x2160": 5},
+                                                         0.0 if args.no_cpu else 5.0)
 
     out = {
         "metric": "Mfragments/s at 1080p (textured sphere, one GPU)",
@@ -636,6 +716,7 @@ def main():
         "draw_pixels": pixels,
         "lines_points": lines,
         "display_lists": lists,
+        "configs": configs,
     }
     if cpu_obj:
         out["speedup_vs_cpu"] = round(mfrag / cpu_obj["value"], 1) if cpu_obj["value"] else None
