@@ -62,6 +62,7 @@ constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in quarters
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | quarter << 28 (0: whole tile, 1..4)
+constexpr uint32_t MED_BIN = 96;        // whole tiles with longer bins join the front of the tile list
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
@@ -122,7 +123,7 @@ struct Counters {
     unsigned long long fragments;
     unsigned long long frag_bytes;
     unsigned long long tri_raster;   // fan triangles set up by the current draw
-    unsigned int nheavy;          // non-empty tiles of the current launch with bins over the heavy-bin threshold
+    unsigned int nheavy;          // work items at the front of the tile list: heavy-tile quarters, medium tiles
     unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
     unsigned int nclip;
@@ -1192,9 +1193,10 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
 }
 
 // The work items of the launch: four quarters (two pixel rows each) of every
-// tile whose bin holds more than heavy_bin entries, from the front of
-// `list`, and every other non-empty tile whole, from its back, so that the
-// raster kernels, which walk the list from the front, start the longest work
+// tile whose bin holds more than heavy_bin entries and every tile with more
+// than MED_BIN entries whole, from the front of `list`, and every other
+// non-empty tile whole, from its back.  The raster kernels walk the list from
+// the front and their blocks start in list order, so the long items start
 // first.  One thread per tile; one atomic per block and list end.
 __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
                                                    const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
@@ -1206,12 +1208,14 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     __shared__ uint32_t s_base[2];
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ne = t < ntiles && tile_gen[t] == gen;
-    const bool hv = ne && tile_end[t] - tile_start[t] > heavy_bin;
-    const bool lt = ne && !hv;
+    const uint32_t len = ne ? tile_end[t] - tile_start[t] : 0u;
+    const bool hv = ne && len > heavy_bin;
+    const bool md = ne && !hv && len > MED_BIN;
+    const bool lt = ne && !hv && !md;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t bh = __ballot(hv), bl = __ballot(lt);
+    const uint64_t bh = __ballot(hv), bm = __ballot(md), bl = __ballot(lt);
     if (lane == 0) {
-        s_n[0][wave] = (uint32_t)__popcll(bh);
+        s_n[0][wave] = 4u * (uint32_t)__popcll(bh) + (uint32_t)__popcll(bm);   // front items
         s_n[1][wave] = (uint32_t)__popcll(bl);
     }
     __syncthreads();
@@ -1224,7 +1228,7 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
             th += a;
             tl += c;
         }
-        s_base[0] = th ? atomicAdd(&ctr->nheavy, 4u * th) : 0u;
+        s_base[0] = th ? atomicAdd(&ctr->nheavy, th) : 0u;
         s_base[1] = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
     }
     __syncthreads();
@@ -1235,11 +1239,12 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
         const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
         item = make_uint4(t - tile_pre[sc], tile_start[t], tile_end[t], sc);
     }
+    const uint32_t of = s_base[0] + s_n[0][wave] + 4u * (uint32_t)__popcll(bh & below) + (uint32_t)__popcll(bm & below);
     if (hv) {
-        const uint32_t o = s_base[0] + 4u * (s_n[0][wave] + (uint32_t)__popcll(bh & below));
         for (uint32_t q = 0; q < 4u; q++)
-            if (o + q < list_cap) list[o + q] = make_uint4(item.x | ((q + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
+            if (of + q < list_cap) list[of + q] = make_uint4(item.x | ((q + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
     }
+    if (md && of < list_cap) list[of] = item;
     if (lt) {
         const uint32_t o = s_base[1] + s_n[1][wave] + (uint32_t)__popcll(bl & below);
         if (o < list_cap) list[list_cap - 1u - o] = item;
@@ -1934,9 +1939,10 @@ __device__ __forceinline__ uint64_t depth_pass(const RasterConsts &K, const RCov
 
 // The raster kernels run a persistent grid over a list of work items: items
 // [0, nheavy) are quarters (two pixel rows) of the tiles whose bins hold more
-// than the heavy-bin threshold, four per tile, at the front of the list; the
-// other items are whole non-empty tiles, from the back of the list, so the
-// longest items start first.  Pass k of block b takes item k * gridDim.x + b
+// than the heavy-bin threshold, four per tile, and whole tiles with more than
+// MED_BIN entries, at the front of the list; the other items are whole
+// non-empty tiles, from the back of the list, so the longest items start
+// first.  Pass k of block b takes item k * gridDim.x + b
 // (even k) or k * gridDim.x + gridDim.x - 1 - b (odd k).  Grids hold up to
 // three times the resident blocks, so while items last the dispatcher hands
 // each freed slot the next block: a slot busy with a long item takes no
