@@ -63,6 +63,7 @@ constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in quarters
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | quarter << 28 (0: whole tile, 1..4)
 constexpr uint32_t MED_BIN = 96;        // whole tiles with longer bins join the front of the tile list
+constexpr int GEOM_BLOCK = 256;         // threads per block of the vertex, setup and expansion kernels
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
@@ -3767,8 +3768,8 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     const Sizes z = scene_sizes(in, n);
     uint64_t vb = 0, sb = 0;
     for (const Plan &p : plan) {
-        vb += blocks_for(p.s->dr->count, 256);
-        sb += blocks_for(p.ntri, 256);
+        vb += blocks_for(p.s->dr->count, GEOM_BLOCK);
+        sb += blocks_for(p.ntri, GEOM_BLOCK);
     }
     if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / 4u ||
         vb > 0x7FFFFFFFull)
@@ -3890,8 +3891,8 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         voff += dr->count;
         toff += p.ntri;
         tileoff += (uint32_t)(d->tiles_x * d->tiles_y);
-        vbo += blocks_for(dr->count, 256);
-        sbo += blocks_for(p.ntri, 256);
+        vbo += blocks_for(dr->count, GEOM_BLOCK);
+        sbo += blocks_for(p.ntri, GEOM_BLOCK);
     }
     vbp[ns] = vbo;
     sbp[ns] = sbo;
@@ -3930,9 +3931,9 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     if (L.ns == 0) return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
     int rc;
     const uint32_t ttot = L.tri_total;
-    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(256), 0, x.stream, L.vp, L.vblock_pre, L.vmap, L.ns, x.ctr,
+    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(GEOM_BLOCK), 0, x.stream, L.vp, L.vblock_pre, L.vmap, L.ns, x.ctr,
                        as<unsigned long long>(x.tri_tiles) + ttot);
-    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(256), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
+    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(GEOM_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
     hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
                        (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
     HIPC(hipGetLastError());
@@ -3984,7 +3985,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
     uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
     // the clip queue has been consumed: it holds the long triangles now
-    hipLaunchKernelGGL(k_expand, dim3(blocks_for(ttot, 256)), dim3(256), 0, x.stream,
+    hipLaunchKernelGGL(k_expand, dim3(blocks_for(ttot, GEOM_BLOCK)), dim3(GEOM_BLOCK), 0, x.stream,
                        (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
                        keys, vals, as<uint32_t>(x.clip_list), x.ctr);

@@ -971,6 +971,61 @@ def lines_points_workload(g, W, H, seed=SEED_BASE + 9, n_lines=20_000, n_points=
     return 2 * st[0] + st[1]
 
 
+def scene_prim_edges(g, W, H, seed):
+    """Edge cases of points and lines: zero-length segments, a GL_LINES
+    block with an odd vertex count, a one-vertex GL_LINE_LOOP and
+    GL_LINE_STRIP, widths and sizes below one (read as one) and large ones,
+    segments leaving every side of the frustum, points on the frustum's
+    edges and behind the eye, lines and points under a 1:1 orthographic
+    projection at integer and half-integer positions, and a depth range
+    other than [0, 1] (reference src/raster.c:107-296, :1019-1196)."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glDepthRange(0.2, 0.7)
+
+    def vtx(v, c=None):
+        g.glColor4f(*(rng.random(4) if c is None else c).astype(np.float32))
+        g.glVertex3f(*np.asarray(v, np.float32))
+    for width in (0.3, 1.0, 6.0):
+        g.glLineWidth(width)
+        g.glBegin(E["GL_LINES"])
+        vtx((0.1, 0.1, 0.0)); vtx((0.1, 0.1, 0.0))                 # zero length
+        vtx((-5.0, 0.2, 0.0)); vtx((5.0, -0.3, 0.5))               # left and right planes
+        vtx((0.3, -4.0, 0.2)); vtx((-0.2, 4.0, -0.4))              # bottom and top
+        vtx((0.0, 0.0, 2.95)); vtx((0.4, 0.3, -50.0))              # near and far
+        vtx((0.5, 0.5, 0.0))                                       # odd vertex: ignored
+        g.glEnd()
+    for mode in ("GL_LINE_LOOP", "GL_LINE_STRIP"):
+        g.glBegin(E[mode])
+        vtx((0.2, -0.2, 0.0))                                      # one vertex: nothing
+        g.glEnd()
+    for size in (0.2, 1.0, 9.0):
+        g.glPointSize(size)
+        g.glBegin(E["GL_POINTS"])
+        for _ in range(6):
+            vtx((rng.random(3) - 0.5) * 2.4)
+        vtx((0.0, 0.0, 3.5))                                       # behind the eye
+        vtx((1.0e-7, 0.0, 2.9))                                    # just in front of the near plane
+        g.glEnd()
+    g.glDepthRange(0.0, 1.0)
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glOrtho(0.0, float(W), 0.0, float(H), -1.0, 1.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    g.glLineWidth(1.0)
+    g.glBegin(E["GL_LINE_LOOP"])
+    for x, y in ((2.0, 2.0), (W - 3.0, 2.5), (W - 3.5, H - 3.0), (2.5, H - 2.5)):
+        vtx((x, y, 0.0))
+    g.glEnd()
+    g.glPointSize(2.0)
+    g.glBegin(E["GL_POINTS"])
+    for x, y in ((0.0, 0.0), (W - 1.0, H - 1.0), (W / 2 + 0.5, H / 2 + 0.5), (-1.0, 5.0)):
+        vtx((x, y, 0.0))
+    g.glEnd()
+
+
 def _polygon_mode_tris(g, rng, W, H, seed, modes):
     _perspective(g, W, H)
     g.glEnable(E["GL_DEPTH_TEST"])
@@ -1241,7 +1296,7 @@ UNDEFINED_IN_REFERENCE = ("vbo_oob",)
 # scenes with calls a batch does not record (glDrawPixels, points, lines and
 # polygon-mode LINE / POINT draws are refused while recording); they stay out
 # of the batched feature tests
-NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "lists_replay")
+NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "lists_replay", "prim_edges")
 # scenes with draws the device refuses (one face filled, the other drawn as
 # lines or points); oracle against reference only
 DEVICE_UNSUPPORTED = ("polygon_mode_mixed",)
@@ -1261,6 +1316,7 @@ FEATURE_SCENES = {
     "vbo_oob": scene_vbo_oob,
     "draw_pixels": scene_draw_pixels,
     "lines_points": scene_lines_points,
+    "prim_edges": scene_prim_edges,
     "polygon_mode": scene_polygon_mode,
     "polygon_mode_mixed": scene_polygon_mode_mixed,
     "lists": scene_lists,
