@@ -589,11 +589,11 @@ static void emit_vertex(GLState *c, float x, float y, float z)
     gls_apply_color_material(c, v->color);
 }
 
-/* glEnd: hand the buffered vertices to the device as one draw whose attributes
- * are the staged vertex records. */
-void gls_flush_immediate(GLState *c)
+/* The draw of n vertex records: from display-list segment seg, or (seg 0)
+ * the buffered records, themselves kept in a segment when a list block is
+ * being replayed.  Resets the vertex buffer as glEnd does. */
+void gls_draw_records(GLState *c, size_t n, uint32_t seg)
 {
-    size_t n = c->imm_count;
     if (n > 0) {
         mtgl_raster_state rs;
         gls_snapshot_raster_state(c, &rs);
@@ -616,7 +616,7 @@ void gls_flush_immediate(GLState *c)
         d.raster = &rs;
         const size_t bytes = n * sizeof(gls_imm_vertex);
         /* a replayed list block draws from its device segment */
-        const uint32_t seg = gls_list_segment(c, c->imm, bytes);
+        if (!seg) seg = gls_list_segment(c, c->imm, bytes);
         if (seg) {
             const uint32_t b = MTGL_SEGMENT_BIT | seg;
             d.position.buffer = d.color.buffer = d.texcoord.buffer = d.normal.buffer = b;
@@ -631,6 +631,26 @@ void gls_flush_immediate(GLState *c)
     c->imm_count = 0;
     c->vstate_count = 0;
     c->tl_dirty = 1;
+}
+
+/* glEnd: hand the buffered vertices to the device as one draw whose attributes
+ * are the staged vertex records. */
+void gls_flush_immediate(GLState *c) { gls_draw_records(c, c->imm_count, 0); }
+
+/* The vertex-state snapshot the first glVertex of a block takes (emit_vertex),
+ * for a list block replayed without its calls. */
+int gls_begin_snapshot(GLState *c)
+{
+    if (c->vstate_capacity == 0) {
+        mtgl_vertex_state *p = (mtgl_vertex_state *)realloc(c->vstates, 4 * sizeof(*p));
+        if (!p) return 0;
+        c->vstates = p;
+        c->vstate_capacity = 4;
+    }
+    gls_snapshot_vertex_state(c, &c->vstates[0]);
+    c->vstate_count = 1;
+    c->tl_dirty = 0;
+    return 1;
 }
 
 MTGL_EXPORT void glVertex2f(GLfloat x, GLfloat y)

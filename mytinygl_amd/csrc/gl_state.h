@@ -72,6 +72,17 @@ typedef struct {
     uint32_t id;                  /* device segment id */
     uint8_t *host;
     size_t bytes;
+    /* A block of vertex / colour / texcoord / normal calls only, last
+     * replayed through its calls from an empty vertex buffer: its records are
+     * a function of the current colour, texcoord and normal at glBegin, so a
+     * replay from the same values skips the calls and draws the segment. */
+    int fast;
+    size_t begin;                 /* index of the block's LOP_BEGIN */
+    float start[9];               /* current colour, texcoord, normal at glBegin */
+    uint32_t start_mask;          /* which of them (bits 0-2) the records take */
+    float end[9];                 /* the same after the block */
+    float last_color[4];          /* colour of the block's last vertex */
+    size_t nverts;
 } gls_lseg;
 
 typedef struct {
@@ -81,6 +92,7 @@ typedef struct {
     GLboolean allocated;
     gls_lseg *segs;
     size_t nsegs, segs_capacity;
+    size_t seg_hint;              /* segment the next block of a replay most likely uses */
 } gls_list;
 
 /* One glVertex call as seen by the host: object-space attributes plus the index
@@ -197,6 +209,8 @@ void gls_snapshot_vertex_state(GLState *c, mtgl_vertex_state *vs);
 void gls_snapshot_raster_state(GLState *c, mtgl_raster_state *rs);
 void gls_apply_color_material(GLState *c, const float *color);
 void gls_flush_immediate(GLState *c);
+void gls_draw_records(GLState *c, size_t n, uint32_t seg);
+int  gls_begin_snapshot(GLState *c);
 void gls_report_device_error(GLState *c, int rc, const char *what);
 int  gls_list_recording(GLState *c);
 void gls_list_free_segments(GLState *c, gls_list *l);

@@ -716,6 +716,92 @@ def scene_lists_replay(g, W, H, seed):
             g.glDeleteLists(base + 2, 1)
 
 
+def _lists_repeat_blocks(g, rng):
+    """The calls of one list: a triangle block that sets colour and normal
+    before every vertex (inherits the texcoord), a translation, a strip whose
+    first vertex inherits colour and normal, and textured quads that inherit
+    the colour and set a colour after their last vertex."""
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 4, spread=1.2, normals=True)
+    g.glEnd()
+    g.glTranslatef(0.2, -0.1, 0.0)
+    g.glBegin(E["GL_TRIANGLE_STRIP"])
+    g.glTexCoord2f(0.25, 0.75)
+    g.glVertex3f(-0.6, -0.5, 0.1)
+    g.glColor3f(*rng.random(3).astype(np.float32))
+    g.glVertex3f(-0.4, 0.5, -0.1)
+    g.glNormal3f(*rng.normal(size=3).astype(np.float32))
+    for _ in range(4):
+        g.glVertex3f(*((rng.random(3) - 0.5) * 1.2).astype(np.float32))
+    g.glEnd()
+    g.glBegin(E["GL_QUADS"])
+    for _ in range(8):
+        g.glTexCoord2f(*rng.random(2).astype(np.float32))
+        g.glVertex3f(*((rng.random(3) - 0.5) * 1.2).astype(np.float32))
+    g.glColor3f(*rng.random(3).astype(np.float32))
+    g.glEnd()
+
+
+def lists_repeat(g, W, H, seed, inline=False, probe=None):
+    """A list of three blocks replayed eight times under lighting and
+    GL_COLOR_MATERIAL, between changes of the current colour, normal and
+    texcoord, of the colour-material mode, and a vertex left pending outside
+    glBegin; after each call a triangle with no attribute calls draws with what
+    the list left, every other time with GL_COLOR_MATERIAL off so that it
+    shows the material the list's last vertex wrote.  inline=True issues the
+    list's calls directly (what the replays must equal); probe(g) is called
+    after each call of the list."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.fv(0.5, 1.0, 1.0, 0.0))
+    g.glEnable(E["GL_COLOR_MATERIAL"])
+    list_seed = int(rng.integers(1 << 30))
+    lid = 0
+    if not inline:
+        lid = g.glGenLists(1)
+        g.glNewList(lid, E["GL_COMPILE"])
+        _lists_repeat_blocks(g, np.random.default_rng(list_seed))
+        g.glEndList()
+    for k in range(8):
+        if k % 3 == 0:
+            g.glColor3f(*rng.random(3).astype(np.float32))
+        if k % 4 == 1:
+            g.glNormal3f(*rng.normal(size=3).astype(np.float32))
+        if k == 5:
+            g.glTexCoord2f(0.5, 0.125)
+        if k == 3:
+            g.glColorMaterial(E["GL_FRONT"], E["GL_DIFFUSE"])
+        if k == 6:
+            g.glDisable(E["GL_COLOR_MATERIAL"])
+        if k == 4:                                     # joins the list's first block
+            g.glVertex3f(0.3, 0.3, 0.0)
+        g.glPushMatrix()
+        g.glTranslatef(float(rng.uniform(-0.6, 0.6)), float(rng.uniform(-0.5, 0.5)), 0.0)
+        g.glRotatef(float(rng.uniform(0, 360)), 0.0, 0.0, 1.0)
+        if inline:
+            _lists_repeat_blocks(g, np.random.default_rng(list_seed))
+        else:
+            g.glCallList(lid)
+        if probe is not None:
+            probe(g)
+        if k % 2 == 1 and k < 6:
+            g.glDisable(E["GL_COLOR_MATERIAL"])
+        g.glBegin(E["GL_TRIANGLES"])
+        for v in ((0.0, 0.0, 0.2), (0.5, 0.1, 0.2), (0.1, 0.5, 0.2)):
+            g.glVertex3f(*v)
+        g.glEnd()
+        if k % 2 == 1 and k < 6:
+            g.glEnable(E["GL_COLOR_MATERIAL"])
+        g.glPopMatrix()
+
+
+def scene_lists_repeat(g, W, H, seed):
+    lists_repeat(g, W, H, seed)
+
+
 # screen-aligned quads (x0, row0, x1, row1, s extent, t extent) over a 256x256
 # framebuffer whose LODs with a 64x64 texture span magnification to beyond the
 # last level, with isotropic and anisotropic footprints
@@ -1294,9 +1380,10 @@ def feature_items(names, seed, size=(96, 72)):
 # are compared device against oracle only
 UNDEFINED_IN_REFERENCE = ("vbo_oob",)
 # scenes with calls a batch does not record (glDrawPixels, points, lines and
-# polygon-mode LINE / POINT draws are refused while recording); they stay out
-# of the batched feature tests
-NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "lists_replay", "prim_edges")
+# polygon-mode LINE / POINT draws are refused while recording) or whose draws
+# read display-list segments, which a batch does not track; they stay out of
+# the batched feature tests
+NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "lists_replay", "lists_repeat", "prim_edges")
 # scenes with draws the device refuses (one face filled, the other drawn as
 # lines or points); oracle against reference only
 DEVICE_UNSUPPORTED = ("polygon_mode_mixed",)
@@ -1321,6 +1408,7 @@ FEATURE_SCENES = {
     "polygon_mode_mixed": scene_polygon_mode_mixed,
     "lists": scene_lists,
     "lists_replay": scene_lists_replay,
+    "lists_repeat": scene_lists_repeat,
     "scissor_viewport": scene_scissor_viewport,
     "colormask": scene_colormask,
     "edge_cases": scene_edge_cases,

@@ -97,3 +97,38 @@ def test_recompile_and_immediate_equal(oracle):
     assert ea == eb == 0
     assert plane_diffs(a, b) == {"color": 0, "depth": 0, "stencil": 0}
     assert (a[0] != a[0][0, 0]).sum() > 100
+
+
+def _state(g):
+    out = []
+    for pname, n in (("GL_CURRENT_COLOR", 4), ("GL_CURRENT_NORMAL", 3), ("GL_CURRENT_TEXTURE_COORDS", 4)):
+        v = np.zeros(4, np.float32)
+        g.glGetFloatv(E[pname], v)
+        out.append(v[:n].copy())
+    for face in ("GL_FRONT", "GL_BACK"):
+        for pname in ("GL_AMBIENT", "GL_DIFFUSE"):
+            v = np.zeros(4, np.float32)
+            g.glGetMaterialfv(E[face], E[pname], v)
+            out.append(v.copy())
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("seed", [3, 11])
+def test_repeated_replays_equal_calls(oracle, seed):
+    """Replays whose calls the host skips (same inherited attributes as the
+    previous replay) leave the current colour, normal, texcoord and the
+    colour-material writes exactly as the calls do, and render the same
+    frame; checked after every step of the repeated-list program."""
+    states = {}
+    frames = {}
+    for inline in (False, True):
+        rec = []
+        frames[inline], err = S.render(
+            oracle, lambda g, W, H, s: S.lists_repeat(g, W, H, s, inline=inline,
+                                                      probe=lambda g: rec.append(_state(g))),
+            96, 72, seed)
+        assert err == 0
+        states[inline] = np.stack(rec)
+    assert np.array_equal(states[False].view(np.uint32), states[True].view(np.uint32))
+    assert plane_diffs(frames[False], frames[True]) == {"color": 0, "depth": 0, "stencil": 0}
+    assert (frames[False][0] != frames[False][0][0, 0]).sum() > 200
