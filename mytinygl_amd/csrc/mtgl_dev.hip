@@ -60,8 +60,12 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in quarters
-constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | quarter << 28 (0: whole tile, 1..4)
+constexpr uint64_t SCAN2_MAX_BLOCKS = 256;   // scans of at most this many tiles take two launches
+constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in parts
+constexpr uint32_t HEAVY_SPLIT = 8;     // parts of a heavy tile, TILE / HEAVY_SPLIT pixel rows each
+constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | part << 28 (0: whole tile, 1..HEAVY_SPLIT)
+constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
+static_assert(TILE % HEAVY_SPLIT == 0 && HEAVY_SPLIT < (1u << (32 - ITEM_TILE_BITS)), "heavy-tile parts");
 constexpr uint32_t MED_BIN = 96;        // whole tiles with longer bins join the front of the tile list
 constexpr int GEOM_BLOCK = 256;         // threads per block of the vertex, setup and expansion kernels
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
@@ -124,7 +128,7 @@ struct Counters {
     unsigned long long fragments;
     unsigned long long frag_bytes;
     unsigned long long tri_raster;   // fan triangles set up by the current draw
-    unsigned int nheavy;          // work items at the front of the tile list: heavy-tile quarters, medium tiles
+    unsigned int nheavy;          // work items at the front of the tile list: heavy-tile parts, medium tiles
     unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
     unsigned int nclip;
@@ -224,9 +228,9 @@ struct RasterParams {
     const RTri *rt;
     const RCold *rcold;
     const uint32_t *vals;          // sorted pair values (record slot)
-    const uint4 *list;             // work items (tile | quarter << 28, e0, e1, scene): nheavy from the front, nlight from the back
+    const uint4 *list;             // work items (tile | part << 28, e0, e1, scene): nheavy from the front, nlight from the back
     uint32_t ntiles;
-    uint32_t list_cap;             // items the list holds (4 per tile)
+    uint32_t list_cap;             // items the list holds (HEAVY_SPLIT per tile)
     uint32_t part;                 // list items of this launch: 0 all, 1 the heavy ones, 2 the others
     unsigned long long *stats;     // per block: fragments, algorithmic bytes (summed when read)
     const float4 *cov;             // first 64 record bytes per pair, in pair order
@@ -923,8 +927,9 @@ __device__ __forceinline__ T wave_inclusive_scan(T v)
     return v;
 }
 
-// block-wide exclusive scan of one value per thread (blockDim == SCAN_THREADS
-// or SORT_THREADS == 256); returns the exclusive prefix and the block total
+// block-wide exclusive scan of one value per thread (blockDim a multiple of
+// 64, at most 1024; lds_waves holds one value per wave); returns the
+// exclusive prefix and the block total
 template <typename T>
 __device__ __forceinline__ T block_exclusive_scan(T v, T *lds_waves, T &total)
 {
@@ -993,6 +998,34 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64
         if (base + k < n) out[base + k] = ex;
         ex += vals[k];
     }
+}
+
+// k_scan_apply for short scans (at most SCAN2_MAX_BLOCKS tiles), with the
+// middle launch folded in: each block adds up the sums of the blocks before
+// it; the last block writes the grand total to *total_out.
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_short(const T *in, uint64_t n, const T *block_sums,
+                                                                   T *out, T *total_out)
+{
+    __shared__ T waves[SCAN_THREADS / 64];
+    T part = 0;
+    for (uint64_t j = threadIdx.x; j < blockIdx.x; j += SCAN_THREADS) part += block_sums[j];
+    T offset;
+    block_exclusive_scan<T>(part, waves, offset);
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    T vals[SCAN_ITEMS];
+    T s = 0;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        vals[k] = base + k < n ? in[base + k] : (T)0;
+        s += vals[k];
+    }
+    T total;
+    T ex = block_exclusive_scan<T>(s, waves, total) + offset;
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = ex;
+        ex += vals[k];
+    }
+    if (total_out && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total_out = offset + total;
 }
 
 // ---------------------------------------------------------------------------
@@ -1193,8 +1226,9 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
     }
 }
 
-// The work items of the launch: four quarters (two pixel rows each) of every
-// tile whose bin holds more than heavy_bin entries and every tile with more
+// The work items of the launch: HEAVY_SPLIT parts (TILE / HEAVY_SPLIT pixel
+// rows each) of every tile whose bin holds more than heavy_bin entries and
+// every tile with more
 // than MED_BIN entries whole, from the front of `list`, and every other
 // non-empty tile whole, from its back.  The raster kernels walk the list from
 // the front and their blocks start in list order, so the long items start
@@ -1216,7 +1250,7 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t bh = __ballot(hv), bm = __ballot(md), bl = __ballot(lt);
     if (lane == 0) {
-        s_n[0][wave] = 4u * (uint32_t)__popcll(bh) + (uint32_t)__popcll(bm);   // front items
+        s_n[0][wave] = HEAVY_SPLIT * (uint32_t)__popcll(bh) + (uint32_t)__popcll(bm);   // front items
         s_n[1][wave] = (uint32_t)__popcll(bl);
     }
     __syncthreads();
@@ -1234,15 +1268,16 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
-    // an item: scene-local tile | quarter << ITEM_TILE_BITS, bin start, bin end, scene
+    // an item: scene-local tile | part << ITEM_TILE_BITS, bin start, bin end, scene
     uint4 item = make_uint4(0u, 0u, 0u, 0u);
     if (ne) {
         const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
         item = make_uint4(t - tile_pre[sc], tile_start[t], tile_end[t], sc);
     }
-    const uint32_t of = s_base[0] + s_n[0][wave] + 4u * (uint32_t)__popcll(bh & below) + (uint32_t)__popcll(bm & below);
+    const uint32_t of = s_base[0] + s_n[0][wave] + HEAVY_SPLIT * (uint32_t)__popcll(bh & below) +
+                        (uint32_t)__popcll(bm & below);
     if (hv) {
-        for (uint32_t q = 0; q < 4u; q++)
+        for (uint32_t q = 0; q < HEAVY_SPLIT; q++)
             if (of + q < list_cap) list[of + q] = make_uint4(item.x | ((q + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
     }
     if (md && of < list_cap) list[of] = item;
@@ -1900,7 +1935,7 @@ __device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, in
         if (!(live && xi >= minX && xi <= maxX && xi < fb_w)) a0[i] = -__builtin_inff();
         if (!(yi >= minY && yi <= maxY && yi < fb_h)) b0[i] = __builtin_inff();
     }
-    // only the rows [r0, r1) of the item (a quarter of a heavy tile, or all)
+    // only the rows [r0, r1) of the item (a part of a heavy tile, or all)
     int32_t mlo = 0, mhi = 0;
     if (r0 <= 0 && 0 < r1) cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi);
     if (r0 <= 1 && 1 < r1) cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi);
@@ -1939,7 +1974,7 @@ __device__ __forceinline__ uint64_t depth_pass(const RasterConsts &K, const RCov
 }
 
 // The raster kernels run a persistent grid over a list of work items: items
-// [0, nheavy) are quarters (two pixel rows) of the tiles whose bins hold more
+// [0, nheavy) are parts (TILE / HEAVY_SPLIT pixel rows) of the tiles whose bins hold more
 // than the heavy-bin threshold, four per tile, and whole tiles with more than
 // MED_BIN entries, at the front of the list; the other items are whole
 // non-empty tiles, from the back of the list, so the longest items start
@@ -1964,8 +1999,8 @@ __device__ __forceinline__ uint32_t list_item(uint32_t it0, uint32_t k, uint32_t
 }
 
 
-// list item -> scene, the scene-local tile, the quarter (0 whole tile, q = 1..4
-// pixel rows 2q-2, 2q-1) and the tile's bin [e0, e1)
+// list item -> scene, the scene-local tile, the part (0 whole tile, q = 1..HEAVY_SPLIT
+// pixel rows [(q - 1) * PART_ROWS, q * PART_ROWS)) and the tile's bin [e0, e1)
 __device__ __forceinline__ uint4 load_item(const RasterParams &P, uint32_t it, uint32_t nh)
 {
     return P.list[it < nh ? it : P.list_cap - 1u - (it - nh)];
@@ -1980,6 +2015,15 @@ __device__ __forceinline__ void decode_item(uint4 v, uint32_t &sc, uint32_t &til
     e0 = __builtin_amdgcn_readfirstlane(v.y);
     e1 = __builtin_amdgcn_readfirstlane(v.z);
     sc = __builtin_amdgcn_readfirstlane(v.w);
+}
+
+// Items at the front of the list (heavy-tile parts and medium tiles) set
+// the length of a launch: their waves take issue priority over the waves of
+// light items that share their SIMD.
+__device__ __forceinline__ void item_priority(uint32_t it, uint32_t nh)
+{
+    if (it < nh) __builtin_amdgcn_s_setprio(2);
+    else __builtin_amdgcn_s_setprio(0);
 }
 
 __device__ __forceinline__ void item_tile(const RasterParams &P, uint32_t it, uint32_t nh, uint32_t &sc, uint32_t &tile,
@@ -2105,12 +2149,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const uint4 hdr_next = it_next < nt ? load_item(P, it_next, nh) : make_uint4(0u, 0u, 0u, 0u);
     uint32_t sc, tile, e0, e1, quarter;
     decode_item(hdr, sc, tile, e0, e1, quarter);
+    item_priority(it, nh);
     hdr = hdr_next;
     it = it_next;
     const SceneRaster &S = P.scenes[sc];
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (!(kind & KIND_LAST_WRITER) || ((kind & KIND_STENCIL) != 0u) != STENCIL) continue;
-    const int32_t r0 = quarter ? 2 * (int32_t)quarter - 2 : 0, r1 = quarter ? r0 + 2 : TILE;
+    const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
     if (sc != cur) {
         raster_consts(S, K);
         K.stencil_on = STENCIL;   // the instantiation matches the scene's state
@@ -2271,7 +2316,7 @@ __global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
         raster_consts(S, K);
         cur = sc;
     }
-    const int32_t r0 = quarter ? 2 * (int32_t)quarter - 2 : 0, r1 = quarter ? r0 + 2 : TILE;
+    const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
     if ((lane >> 3) < r0 || (lane >> 3) >= r1) continue;   // not this item's pixel
     const int32_t x = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE + (lane & 7);
     const int32_t y = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE + (lane >> 3);
@@ -2321,10 +2366,11 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     if (it >= nt) break;
     uint32_t sc, tile, e0, e1, quarter;
     item_tile(P, it, nh, sc, tile, e0, e1, quarter);
+    item_priority(it, nh);
     const SceneRaster &S = P.scenes[sc];
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (kind & KIND_LAST_WRITER) continue;
-    const int32_t r0 = quarter ? 2 * (int32_t)quarter - 2 : 0, r1 = quarter ? r0 + 2 : TILE;
+    const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
     if (sc != cur) {
         raster_consts(S, K);
         need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
@@ -3488,6 +3534,12 @@ int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen, T *total_
     if (rc) return rc;
     T *sums = as<T>(x.scan_tmp);
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n, sums);
+    if (nb <= SCAN2_MAX_BLOCKS) {
+        hipLaunchKernelGGL(k_scan_apply_short<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n,
+                           (const T *)sums, out, total_out);
+        HIPC(hipGetLastError());
+        return MTGL_OK;
+    }
     hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, x.stream, sums, nb, total_out);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n,
                        (const T *)sums, out);
@@ -3771,7 +3823,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         vb += blocks_for(p.s->dr->count, GEOM_BLOCK);
         sb += blocks_for(p.ntri, GEOM_BLOCK);
     }
-    if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / 4u ||
+    if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / HEAVY_SPLIT ||
         vb > 0x7FFFFFFFull)
         return MTGL_ERR_OUT_OF_MEMORY;
     L.tri_total = (uint32_t)z.tri;
@@ -3968,7 +4020,8 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     const uint32_t ntiles = L.tiles_total;
     if ((rc = arena_reserve(x, x.tile_start, (size_t)ntiles * 4, frozen))) return rc;
     if ((rc = arena_reserve(x, x.tile_end, (size_t)ntiles * 4, frozen))) return rc;
-    if ((rc = arena_reserve(x, x.tile_list, (size_t)ntiles * 4 * sizeof(uint4), frozen))) return rc;   // up to 4 items per tile
+    const uint32_t list_cap = HEAVY_SPLIT * ntiles;   // up to HEAVY_SPLIT items per tile
+    if ((rc = arena_reserve(x, x.tile_list, (size_t)list_cap * sizeof(uint4), frozen))) return rc;
     if (!frozen) {
         if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
             if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
@@ -4015,7 +4068,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
                        (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
-                       L.ns, as<uint4>(x.tile_list), 4u * ntiles, x.ctr);
+                       L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
     hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)vals, (const RTri *)as<RTri>(x.rt), n, as<float4>(x.cov));
     HIPC(hipGetLastError());
@@ -4031,7 +4084,7 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     rp.vals = vals;
     rp.list = as<uint4>(x.tile_list);
     rp.ntiles = ntiles;
-    rp.list_cap = 4u * ntiles;
+    rp.list_cap = list_cap;
     rp.part = 0;
     rp.cov = as<float4>(x.cov);
     rp.tile_start = as<uint32_t>(x.tile_start);
