@@ -61,7 +61,7 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 constexpr uint64_t SCAN2_MAX_BLOCKS = 256;   // scans of at most this many tiles take two launches
-constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in parts
+constexpr uint32_t HEAVY_BIN = 256;     // default: tiles with longer bins are rasterised first, in parts
 constexpr uint32_t HEAVY_SPLIT = 8;     // parts of a heavy tile, TILE / HEAVY_SPLIT pixel rows each
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | part << 28 (0: whole tile, 1..HEAVY_SPLIT)
 constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
@@ -233,7 +233,6 @@ struct RasterParams {
     uint32_t list_cap;             // items the list holds (HEAVY_SPLIT per tile)
     uint32_t part;                 // list items of this launch: 0 all, 1 the heavy ones, 2 the others
     unsigned long long *stats;     // per block: fragments, algorithmic bytes (summed when read)
-    const float4 *cov;             // first 64 record bytes per pair, in pair order
     const uint32_t *tile_start;
     const uint32_t *tile_end;      // 0 for an empty tile
     Counters *ctr;
@@ -1287,17 +1286,6 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     }
 }
 
-// First 64 bytes of each pair's record in sorted (bin) order, so the depth
-// kernel reads every chunk of a bin as one contiguous 4 KiB block; one
-// thread per 16 bytes.
-__global__ void __launch_bounds__(256) k_gather_cov(const uint32_t *vals, const RTri *rt, uint32_t n, float4 *out)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (uint64_t)n * RT_COV_VEC4) return;
-    const uint32_t e = (uint32_t)(i / RT_COV_VEC4), q = (uint32_t)(i % RT_COV_VEC4);
-    out[i] = reinterpret_cast<const float4 *>(rt + vals[e])[q];
-}
-
 // per-launch counters (k_vertex's first thread, before any other kernel of the launch)
 __device__ __forceinline__ void reset_counters(Counters *ctr, unsigned long long *scan_tail)
 {
@@ -2172,8 +2160,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         if (K.stencil_on) px.s = S.stencil[pix];
     }
     uint32_t writer = 0;
-    // the next chunk's entries are loaded (contiguous, in bin order) while the
-    // current one is processed
+    // the next chunk's entries (record index, then the record's first 64
+    // bytes) are loaded while the current one is processed
     float4 qn[RT_COV_VEC4];
     uint32_t ridn = 0;
     {
@@ -2181,7 +2169,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         const uint32_t e = v ? e0 + lane : e0;
         ridn = P.vals[e];
 #pragma unroll
-        for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = P.cov[(size_t)e * RT_COV_VEC4 + i];
+        for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = reinterpret_cast<const float4 *>(P.rt + ridn)[i];
     }
     for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
         const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
@@ -2195,7 +2183,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             const uint32_t e = v ? c0 + CHUNK + lane : c0 + CHUNK;
             ridn = P.vals[e];
 #pragma unroll
-            for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = P.cov[(size_t)e * RT_COV_VEC4 + i];
+            for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = reinterpret_cast<const float4 *>(P.rt + ridn)[i];
         }
         const RCov cv = make_cov(q);
         wave_sync();   // the previous chunk's entries are no longer read
@@ -3386,7 +3374,7 @@ struct Exec {
     hipEvent_t upload_done = nullptr;
     Arena vhot, vcold, rt, rcold, box, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
-    Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list, cov;
+    Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list;
     uint32_t gen = 0;                              // launch generation stamped into tile_gen
     uint32_t heavy_bin = HEAVY_BIN;                // bins longer than this go to the front of the tile list
     uint32_t grid_div = 1;                         // raster grids: resident blocks / grid_div
@@ -3629,7 +3617,7 @@ void exec_free(Exec &x)
     if (x.stream) (void)hipStreamSynchronize(x.stream);
     Arena *arenas[] = {&x.vhot, &x.vcold, &x.rt, &x.rcold, &x.box, &x.n_rt, &x.rec_base, &x.tri_tiles, &x.pair_base,
                        &x.scan_tmp, &x.clip_list, &x.keys[0], &x.keys[1], &x.vals[0], &x.vals[1], &x.sort_tmp,
-                       &x.tile_gen, &x.tile_start, &x.tile_end, &x.tile_list, &x.cov};
+                       &x.tile_gen, &x.tile_start, &x.tile_end, &x.tile_list};
     for (Arena *a : arenas) arena_free(*a);
     for (auto &pr : x.pending_events) {
         (void)hipEventDestroy(pr.first);
@@ -4062,15 +4050,12 @@ int run_launch(Exec &x, Launch &L, bool frozen)
         std::swap(keys, keys2);
         std::swap(vals, vals2);
     }
-    if ((rc = arena_reserve(x, x.cov, (size_t)n * RT_COV_VEC4 * sizeof(float4), frozen))) return rc;
     hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, n, ntiles,
                        as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen), x.gen);
     hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
                        (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
                        L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
-    hipLaunchKernelGGL(k_gather_cov, dim3(blocks_for((uint64_t)n * RT_COV_VEC4, 256)), dim3(256), 0, x.stream,
-                       (const uint32_t *)vals, (const RTri *)as<RTri>(x.rt), n, as<float4>(x.cov));
     HIPC(hipGetLastError());
 
     // ---- K4
@@ -4086,7 +4071,6 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     rp.ntiles = ntiles;
     rp.list_cap = list_cap;
     rp.part = 0;
-    rp.cov = as<float4>(x.cov);
     rp.tile_start = as<uint32_t>(x.tile_start);
     rp.tile_end = as<uint32_t>(x.tile_end);
     rp.ctr = x.ctr;
