@@ -151,7 +151,9 @@ struct DArr {
 };
 
 struct VertexParams {
-    uint32_t count;
+    uint32_t count;               // vertex work items: indices, or vertex slots when `unique`
+    uint32_t unique;              // slot i < count - 1 is vertex ubase + i, the last slot any other index
+    int32_t ubase;
     int32_t first;
     uint32_t index_type;
     uint32_t n_vstates;
@@ -170,6 +172,11 @@ struct VertexParams {
 struct SetupParams {
     uint32_t mode;
     uint32_t count;
+    uint32_t unique;              // vertex records per slot (see VertexParams), looked up through the indices
+    int32_t ubase;
+    uint32_t nslots;
+    uint32_t index_type;
+    DArr indices;
     uint32_t ntri;
     uint32_t tri_off;             // first global triangle of the scene
     uint32_t scene;
@@ -461,11 +468,10 @@ __global__ void __launch_bounds__(256) k_scene_maps(const uint32_t *vblock_pre, 
     for (uint32_t i = tile_pre[s] + threadIdx.x; i < tile_pre[s + 1]; i += blockDim.x) tmap[i] = s;
 }
 
-__device__ int32_t fetch_index(const VertexParams &P, uint32_t i)
+__device__ int32_t fetch_index_of(const DArr &a, uint32_t index_type, int32_t first, uint32_t i)
 {
-    if (P.index_type == 0) return (int32_t)((uint32_t)P.first + i);
-    const DArr &a = P.indices;
-    int64_t isz = P.index_type == GL_UNSIGNED_INT ? 4 : (P.index_type == GL_UNSIGNED_SHORT ? 2 : 1);
+    if (index_type == 0) return (int32_t)((uint32_t)first + i);
+    int64_t isz = index_type == GL_UNSIGNED_INT ? 4 : (index_type == GL_UNSIGNED_SHORT ? 2 : 1);
     int64_t at = a.offset + (int64_t)i * isz;
     if (!a.src || at < 0 || at + isz > a.size) return -1;
     const uint8_t *p = a.src + at;
@@ -480,6 +486,11 @@ __device__ int32_t fetch_index(const VertexParams &P, uint32_t i)
         return (int32_t)v;
     }
     return (int32_t)p[0];
+}
+
+__device__ __forceinline__ int32_t fetch_index(const VertexParams &P, uint32_t i)
+{
+    return fetch_index_of(P.indices, P.index_type, P.first, i);
 }
 
 __device__ __forceinline__ float sanitize(float v, float nan_value)
@@ -501,7 +512,9 @@ __global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const ui
     const VertexParams &P = Ps[sc];
     const uint32_t i = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
     if (i >= P.count) return;
-    int32_t idx = fetch_index(P, i);
+    // a slot past the addressed span stands for every index no array reaches:
+    // -1 reads the defaults such an index reads
+    const int32_t idx = P.unique ? (i + 1u < P.count ? P.ubase + (int32_t)i : -1) : fetch_index(P, i);
     float pos[4], col[4], tc[2], nrm[3];
     fetch(P.position, idx, pos, 4);
     if (P.position.comps == 2) pos[2] = 0.0f;
@@ -685,6 +698,15 @@ __device__ int clip_plane(const CV *in, int n, CV *out, int plane, bool cold)
     return m;
 }
 
+// the vertex record of stream position i: i itself, or the slot of its index
+__device__ __forceinline__ uint32_t vertex_slot(const SetupParams &P, uint32_t i)
+{
+    if (!P.unique) return i;
+    const int32_t idx = fetch_index_of(P.indices, P.index_type, 0, i);
+    const int64_t k = (int64_t)idx - P.ubase;
+    return (idx >= 0 && k >= 0 && k < (int64_t)P.nslots - 1) ? (uint32_t)k : P.nslots - 1u;
+}
+
 __device__ __forceinline__ void load_cv(const SetupParams &P, uint32_t v, bool cold, CV &o)
 {
     const float4 *h = reinterpret_cast<const float4 *>(&P.vhot[v]);
@@ -822,9 +844,9 @@ __global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint
     uint32_t ia, ib, ic;
     tri_vertices(P.mode, t, ia, ib, ic);
     CV v[3];
-    load_cv(P, ia, cold, v[0]);
-    load_cv(P, ib, cold, v[1]);
-    load_cv(P, ic, cold, v[2]);
+    load_cv(P, vertex_slot(P, ia), cold, v[0]);
+    load_cv(P, vertex_slot(P, ib), cold, v[1]);
+    load_cv(P, vertex_slot(P, ic), cold, v[2]);
     bool all_inside = true;
 #pragma unroll
     for (int k = 0; k < 3; k++)
@@ -872,9 +894,9 @@ __global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const 
     uint32_t ia, ib, ic;
     tri_vertices(P.mode, t, ia, ib, ic);
     CV a[MAX_CLIP], b[MAX_CLIP];
-    load_cv(P, ia, cold, b[0]);
-    load_cv(P, ib, cold, b[1]);
-    load_cv(P, ic, cold, b[2]);
+    load_cv(P, vertex_slot(P, ia), cold, b[0]);
+    load_cv(P, vertex_slot(P, ib), cold, b[1]);
+    load_cv(P, vertex_slot(P, ic), cold, b[2]);
     int n = clip_plane(b, 3, a, 0, cold);
     n = n ? clip_plane(a, n, b, 1, cold) : 0;
     n = n ? clip_plane(b, n, a, 2, cold) : 0;
@@ -3707,6 +3729,47 @@ struct SceneIn {
     const mtgl_draw *dr;
 };
 
+// Vertex work of a draw.  An indexed draw whose arrays address fewer vertices
+// than it has indices transforms each addressed vertex once: slot k is vertex
+// base + k for k < n - 1, and slot n - 1 stands for every index no array
+// addresses (each array fetch of such an index reads the same defaults).
+// Setup then finds a stream position's record through its index.  Other
+// draws transform one vertex per stream position.
+struct VSlots {
+    uint32_t n;
+    int32_t base;
+    bool unique;
+};
+
+VSlots vertex_slots(const mtgl_dev *d, const mtgl_draw *dr)
+{
+    VSlots r = {dr->count, 0, false};
+    if (dr->index_type == 0 || (dr->n_vstates > 1 && dr->state_index.enabled)) return r;
+    // the indices each array reads in range (the bounds test of fetch) and their hull
+    const uint8_t marker = 0;
+    const mtgl_array *arrs[4] = {&dr->position, &dr->color, &dr->texcoord, &dr->normal};
+    int64_t lo = INT64_MAX, hi = -1;
+    for (int k = 0; k < 4; k++) {
+        if (k > 0 && !arrs[k]->enabled) continue;
+        const DArr a = resolve(d, *arrs[k], &marker, dr->staged_bytes);
+        if (!a.src || a.stride <= 0) continue;
+        const int64_t elem = (int64_t)a.comps * (a.type == GL_FLOAT ? 4 : 1);
+        const int64_t room = a.size - elem - a.offset;   // at + elem <= size
+        if (room < 0) continue;
+        const int64_t ilo = a.offset >= 0 ? 0 : (-a.offset + a.stride - 1) / a.stride;
+        const int64_t ihi = std::min<int64_t>(room / a.stride, 0x7FFFFFFF);
+        if (ilo > ihi) continue;
+        lo = std::min(lo, ilo);
+        hi = std::max(hi, ihi);
+    }
+    const int64_t n = hi < lo ? 1 : hi - lo + 2;
+    if (n > (int64_t)dr->count) return r;
+    r.n = (uint32_t)n;
+    r.base = hi < lo ? 0 : (int32_t)lo;
+    r.unique = true;
+    return r;
+}
+
 // Vertex, triangle and tile totals of a set of scenes and the buffers they need.
 struct Sizes {
     uint64_t vtx = 0, tri = 0, tiles = 0;
@@ -3721,7 +3784,7 @@ Sizes scene_sizes(const SceneIn *in, uint32_t n)
         if (!draw_mode_ok(dr->mode)) continue;
         const uint32_t ntri = tri_count(dr->mode, dr->count);
         if (ntri == 0) continue;
-        z.vtx += dr->count;
+        z.vtx += vertex_slots(in[i].dev, dr).n;
         z.tri += ntri;
         z.tiles += (uint64_t)in[i].dev->tiles_x * (uint64_t)in[i].dev->tiles_y;
         z.cold |= needs_cold(dr->raster);
@@ -3808,7 +3871,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     const Sizes z = scene_sizes(in, n);
     uint64_t vb = 0, sb = 0;
     for (const Plan &p : plan) {
-        vb += blocks_for(p.s->dr->count, GEOM_BLOCK);
+        vb += blocks_for(vertex_slots(p.s->dev, p.s->dr).n, GEOM_BLOCK);
         sb += blocks_for(p.ntri, GEOM_BLOCK);
     }
     if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / HEAVY_SPLIT ||
@@ -3851,8 +3914,11 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         const uint8_t *staged_dev = dev + p.off_st;
         const uint64_t stb = dr->staged_bytes;
 
+        const VSlots vs = vertex_slots(d, dr);
         VertexParams v = {};
-        v.count = dr->count;
+        v.count = vs.n;
+        v.unique = vs.unique;
+        v.ubase = vs.base;
         v.first = dr->first;
         v.index_type = dr->index_type;
         v.n_vstates = dr->n_vstates;
@@ -3874,6 +3940,11 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         SetupParams q = {};
         q.mode = dr->mode;
         q.count = dr->count;
+        q.unique = vs.unique;
+        q.ubase = vs.base;
+        q.nslots = vs.n;
+        q.index_type = dr->index_type;
+        q.indices = v.indices;
         q.ntri = p.ntri;
         q.tri_off = toff;
         q.scene = s;
@@ -3928,10 +3999,10 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         sbp[s] = sbo;
         tp[s] = toff;
         tlp[s] = tileoff;
-        voff += dr->count;
+        voff += vs.n;
         toff += p.ntri;
         tileoff += (uint32_t)(d->tiles_x * d->tiles_y);
-        vbo += blocks_for(dr->count, GEOM_BLOCK);
+        vbo += blocks_for(vs.n, GEOM_BLOCK);
         sbo += blocks_for(p.ntri, GEOM_BLOCK);
     }
     vbp[ns] = vbo;
