@@ -753,7 +753,7 @@ def main():
                    "fog": "GL_LINEAR 1..3"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profile(),
-                     "kernel": "fragment stage (k_raster_depth + k_raster_color)", "kernel_ms": round(k_ms, 5),
+                     "kernel": "fragment stage (k_raster_depth: coverage, in-order depth pass, shading of the last passing fragment)", "kernel_ms": round(k_ms, 5),
                      "launches": int(st.raster_launches),
                      "bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu_obj,

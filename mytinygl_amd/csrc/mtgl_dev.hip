@@ -2129,14 +2129,17 @@ constexpr int DCAP = 512;          // fragment depths held in LDS per batch (dep
 // a partial colour mask: when nothing blends, only the last fragment that
 // passes a pixel can leave its colour there, so this kernel runs coverage and
 // the in-order depth/stencil pass, writes depth and stencil, and records per
-// pixel 1 + the record slot of that last fragment (0: none) for K4b.  One 8x8
-// tile per one-wave block; it reads only the first 64 bytes of each record.
+// pixel 1 + the record slot of that last fragment (0: none), then shades that
+// one fragment per pixel (barycentrics, colour, lighting, texture, fog) from
+// its record.  One 8x8 tile per one-wave block; the depth pass reads only the
+// first 64 bytes of each record.  Three waves per SIMD: the shading needs
+// more registers than four would leave without spilling.
 // Per chunk: coverage (cover_chunk); then batches of at most DCAP fragments
 // in pixel-major order: the fragments' window depths one per lane, and per
 // pixel lane the depth/stencil steps in submission order over its own
 // fragments, which only load, compare and select.
 template <bool STENCIL>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_raster_depth(RasterParams P)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster_depth(RasterParams P)
 {
     __shared__ RCov s_cov[CHUNK];
     __shared__ uint32_t s_rid[CHUNK];
@@ -2299,51 +2302,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         if (px.dirty_d) S.depth[pix] = px.d;
         if (px.dirty_s) S.stencil[pix] = (uint8_t)px.s;
         S.writer[pix] = writer;
+        // the colour of the pixel's last passing fragment, from its record
+        if ((kind & KIND_COLOR) && writer) {
+            const RTri *r = P.rt + (writer - 1u);
+            float w0e, w1e, w2e;
+            edge_values(r, x, y, w0e, w1e, w2e);
+            const float inv = r->inv_area;
+            Col c;
+            float dep;
+            uint32_t meta;
+            fragment_source(P, K, r, writer - 1u, w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
+            S.color[pix] = col_pack(col_clamp(c));
+        }
     }
     }
     add_stats(P, frag_count, frag_bytes);
-}
-
-// K4b: the colour of each pixel's last passing fragment (K4a's record),
-// clamped and packed; one tile per one-wave block, one pixel per lane, the
-// record read from HBM / L2.
-__global__ void __launch_bounds__(64) k_raster_color(RasterParams P)
-{
-    RasterConsts K;
-    uint32_t cur = 0xFFFFFFFFu;
-    const int lane = threadIdx.x;
-    uint32_t nh, it0, nt;
-    list_counts(P, nh, it0, nt);
-    for (uint32_t pass = 0;; pass++) {
-    const uint32_t it = list_item(it0, pass, blockIdx.x, gridDim.x);
-    if (it >= nt) break;
-    uint32_t sc, tile, e0, e1, quarter;
-    item_tile(P, it, nh, sc, tile, e0, e1, quarter);
-    const SceneRaster &S = P.scenes[sc];
-    const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
-    if ((kind & (KIND_LAST_WRITER | KIND_COLOR)) != (KIND_LAST_WRITER | KIND_COLOR)) continue;
-    if (sc != cur) {
-        raster_consts(S, K);
-        cur = sc;
-    }
-    const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
-    if ((lane >> 3) < r0 || (lane >> 3) >= r1) continue;   // not this item's pixel
-    const int32_t x = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE + (lane & 7);
-    const int32_t y = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE + (lane >> 3);
-    if (x >= S.fb_w || y >= S.fb_h) continue;
-    const size_t pix = (size_t)y * S.fb_w + x;
-    const uint32_t writer = S.writer[pix];
-    if (writer == 0u) continue;
-    const RTri *r = P.rt + (writer - 1u);
-    float w0e, w1e, w2e;
-    edge_values(r, x, y, w0e, w1e, w2e);
-    const float inv = r->inv_area;
-    Col c;
-    float dep;
-    uint32_t meta;
-    fragment_source(P, K, r, writer - 1u, w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
-    S.color[pix] = col_pack(col_clamp(c));
-    }
 }
 
 // K4, general: blending, a partial colour mask, or an alpha test on a
@@ -3040,7 +3013,7 @@ struct DiffView {
 // Per pixel of a context's latest draw (a single-scene launch of a
 // last-writer kind): the triangle of the fragment that left its colour, that
 // fragment's colour weights and its bilinear texture footprint, computed as
-// k_raster_color and sample_texture compute them.  Tiles the launch did not
+// the depth kernel's shading and sample_texture compute them.  Tiles the launch did not
 // list (tile_gen != gen) and pixels no fragment reached get tri = -1 and
 // their colour; fragments of clipped triangles (overflow slots) get -2.
 __global__ void __launch_bounds__(256) k_gbuffer(const SceneRaster *scene, const RTri *rt, const uint32_t *writer,
@@ -3388,7 +3361,7 @@ struct Arena {
 struct Exec {
     hipStream_t stream = nullptr;
     int cu_count = 256;
-    uint32_t resident[4] = {256, 256, 256, 256};   // blocks at once: depth<stencil>, depth<no stencil>, colour, general
+    uint32_t resident[3] = {256, 256, 256};   // blocks at once: depth<stencil>, depth<no stencil>, general
     Counters *ctr = nullptr;                       // device
     Counters *ctr_host = nullptr;                  // pinned mirror for readback
     uint8_t *pinned = nullptr;                     // staging for uploads
@@ -3616,9 +3589,9 @@ int exec_init(Exec &x, int device)
         x.cu_count <= 0)
         x.cu_count = 256;
     // resident one-wave blocks of each raster kernel (registers and LDS bound)
-    const void *kern[4] = {(const void *)k_raster_depth<true>, (const void *)k_raster_depth<false>,
-                           (const void *)k_raster_color, (const void *)k_raster};
-    for (int k = 0; k < 4; k++) {
+    const void *kern[3] = {(const void *)k_raster_depth<true>, (const void *)k_raster_depth<false>,
+                           (const void *)k_raster};
+    for (int k = 0; k < 3; k++) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
         x.resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)x.cu_count, STAT_SLOTS);
@@ -4163,12 +4136,10 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     const uint32_t LW = KIND_LAST_WRITER, ST = KIND_STENCIL, CO = KIND_COLOR;
     const uint32_t lw_stencil = (1u << (LW | ST)) | (1u << (LW | ST | CO));
     const uint32_t lw_plain = (1u << LW) | (1u << (LW | CO));
-    const uint32_t lw_color = (1u << (LW | CO)) | (1u << (LW | ST | CO));
     const uint32_t general = (1u << 0) | (1u << ST) | (1u << CO) | (1u << (ST | CO));
     if (L.kinds & lw_stencil) hipLaunchKernelGGL(k_raster_depth<true>, grid(0), dim3(64), 0, x.stream, rp);
     if (L.kinds & lw_plain) hipLaunchKernelGGL(k_raster_depth<false>, grid(1), dim3(64), 0, x.stream, rp);
-    if (L.kinds & lw_color) hipLaunchKernelGGL(k_raster_color, grid(2), dim3(64), 0, x.stream, rp);
-    if (L.kinds & general) hipLaunchKernelGGL(k_raster, grid(3), dim3(64), 0, x.stream, rp);
+    if (L.kinds & general) hipLaunchKernelGGL(k_raster, grid(2), dim3(64), 0, x.stream, rp);
     HIPC(hipGetLastError());
     if (ev0 && ev1) {
         HIPC(hipEventRecord(ev1, x.stream));
