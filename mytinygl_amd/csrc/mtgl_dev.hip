@@ -68,6 +68,7 @@ constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
 static_assert(TILE % HEAVY_SPLIT == 0 && HEAVY_SPLIT < (1u << (32 - ITEM_TILE_BITS)), "heavy-tile parts");
 constexpr uint32_t MED_BIN = 96;        // whole tiles with longer bins join the front of the tile list
 constexpr int GEOM_BLOCK = 256;         // threads per block of the vertex, setup and expansion kernels
+constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
@@ -136,6 +137,7 @@ struct Counters {
     unsigned int overflow_full;   // a request did not fit: the host grows the pool and reruns
     unsigned int sticky;          // overflow_full of every launch since the host last cleared it
     unsigned int nbig;            // triangles queued for the one-wave pair expansion
+    unsigned int nbig_setup;      // the same triangles as setup counts them (read back before the expansion)
     unsigned long long npairs;    // (tile, record) pairs of the launch, written by the pair-offset scan
 };
 
@@ -869,9 +871,11 @@ __global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint
         P.clip_list[slot] = P.tri_off + t;
     }
     // one counter update per wave
-    const uint64_t kept = __ballot(nout != 0);
-    if (kept && (threadIdx.x & 63) == __builtin_ctzll(__ballot(1)))
-        atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
+    const uint64_t kept = __ballot(nout != 0), big = __ballot(tiles > EXPAND_SMALL);
+    if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) {
+        if (kept) atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
+        if (big) atomicAdd(&P.ctr->nbig_setup, (uint32_t)__popcll(big));
+    }
     P.n_rt[t] = nout;
     P.rec_base[t] = P.tri_off + t;
     P.tri_tiles[t] = tiles;
@@ -930,6 +934,7 @@ __global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const 
     P.rec_base[t] = P.over_base + base;
     P.tri_tiles[t] = tiles;
     if (nout) atomicAdd(&P.ctr->tri_raster, (unsigned long long)nout);
+    if (tiles > EXPAND_SMALL) atomicAdd(&P.ctr->nbig_setup, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,7 +1058,6 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_short(const T *in, 
 // K3a: expand triangles into (tile, fan triangle) pairs, in submission order
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
 
 // (tile, record) pairs of fan record j of a triangle: tiles k0 <= k < k1 of
 // its tile box (row-major), pair p0 + k onwards; pairs at or beyond cap are
@@ -1319,6 +1323,7 @@ __device__ __forceinline__ void reset_counters(Counters *ctr, unsigned long long
     ctr->nclip = 0;
     ctr->noverflow = 0;
     ctr->nbig = 0;
+    ctr->nbig_setup = 0;
     ctr->sticky |= ctr->overflow_full;
     ctr->overflow_full = 0;
 }
@@ -4074,7 +4079,10 @@ int run_launch(Exec &x, Launch &L, bool frozen)
                        (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
                        keys, vals, as<uint32_t>(x.clip_list), x.ctr);
-    hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
+    // a sized run whose setup queued no long triangle has nothing for it (a
+    // frozen run may also have a tail to fill)
+    if (frozen || x.ctr_host->nbig_setup != 0)
+        hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
                        x.stream, (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
                        keys, vals, (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
