@@ -1156,11 +1156,17 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys
     __shared__ uint32_t hist[RADIX];
     hist[threadIdx.x] = 0;
     __syncthreads();
-    uint32_t base = blockIdx.x * SORT_TILE;
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    // every key of the thread is loaded before the first count
+    uint32_t kk[SORT_ITEMS];
+#pragma unroll
     for (int r = 0; r < SORT_ITEMS; r++) {
-        uint32_t i = base + r * SORT_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&hist[(keys[i] >> shift) & (RADIX - 1)], 1u);
+        const uint32_t i = base + r * SORT_THREADS + threadIdx.x;
+        kk[r] = i < n ? keys[i] : 0u;
     }
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++)
+        if (base + r * SORT_THREADS + threadIdx.x < n) atomicAdd(&hist[(kk[r] >> shift) & (RADIX - 1)], 1u);
     __syncthreads();
     counts[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
@@ -1177,12 +1183,22 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *k
     for (int w = 0; w < SORT_THREADS / 64; w++) wave_hist[w][threadIdx.x] = 0;
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t base = blockIdx.x * SORT_TILE;
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    // every key and value of the thread is loaded before the first round
+    // (the rounds are separated by barriers, which loads do not cross)
+    uint32_t kk[SORT_ITEMS], vv[SORT_ITEMS];
+#pragma unroll
+    for (int r = 0; r < SORT_ITEMS; r++) {
+        const uint32_t i = base + r * SORT_THREADS + threadIdx.x;
+        kk[r] = i < n ? keys_in[i] : 0u;
+        vv[r] = i < n ? vals_in[i] : 0u;
+    }
+#pragma unroll
     for (int r = 0; r < SORT_ITEMS; r++) {
         uint32_t i = base + r * SORT_THREADS + threadIdx.x;
         bool valid = i < n;
-        uint32_t key = valid ? keys_in[i] : 0u;
-        uint32_t val = valid ? vals_in[i] : 0u;
+        uint32_t key = kk[r];
+        uint32_t val = vv[r];
         uint32_t d = (key >> shift) & (RADIX - 1);
         // lanes of this wave holding the same digit
         uint64_t peers = __ballot(valid);
