@@ -2395,6 +2395,9 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
         if (K.stencil_on) px.s = S.stencil[pix];
     }
 
+    // the next chunk's record indices are loaded while the current chunk is
+    // processed (its records then take one memory wait, not two)
+    uint32_t ridn = (uint32_t)lane < e1 - e0 ? P.vals[e0 + lane] : 0u;
     for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
         const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
         const bool valid = (uint32_t)lane < cn;
@@ -2403,8 +2406,10 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
 #pragma unroll
         for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t rid = 0;
+        const uint32_t rid_here = ridn;
+        if (c0 + CHUNK < e1) ridn = (uint32_t)lane < e1 - (c0 + CHUNK) ? P.vals[c0 + CHUNK + lane] : 0u;
         if (valid) {
-            rid = P.vals[c0 + lane];
+            rid = rid_here;
             const float4 *src = reinterpret_cast<const float4 *>(&P.rt[rid]);
 #pragma unroll
             for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = src[i];
