@@ -127,6 +127,8 @@ def configs_bench(frames: dict, cpu_budget_s: float):
     bits = E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"] | E["GL_STENCIL_BUFFER_BIT"]
     out = {}
     for name, (setup, draw, w, h) in cases.items():
+        if name not in frames:
+            continue
         g = GLLibrary(os.path.join(ROOT, "mytinygl_amd", "libmtgl.so"), "mtgl")
         ctx = g.create_context(w, h)
         g.glClearColor(0.0, 0.0, 0.0, 1.0)
@@ -648,6 +650,8 @@ def main():
     ap.add_argument("--lists-frames", type=int, default=8)
     ap.add_argument("--no-configs", action="store_true", help="skip the icosphere 640x480 and overdraw 4K legs")
     ap.add_argument("--only-configs", action="store_true", help="run only those legs (profiling)")
+    ap.add_argument("--only-config", choices=["icosphere_640x480", "overdraw_3840x2160"],
+                    help="run only this one of those legs, 3 frames (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain device copies of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -673,6 +677,9 @@ def main():
         return
     if args.only_lists:
         print(json.dumps(lists_bench(args.lists_frames, 0)), flush=True)
+        return
+    if args.only_config:
+        print(json.dumps(configs_bench({args.only_config: 3}, 0.0)), flush=True)
         return
     if args.only_configs:
         print(json.dumps(configs_bench({"icosphere_640x480": 200, "overdraw_3840x2160": 3}, 0.0)), flush=True)

@@ -30,6 +30,14 @@ GLState *gl_get_current_context(void);
  * Any of the pointers may be NULL.  Returns MTGL_OK or an MTGL_ERR_* code. */
 int mtgl_read_framebuffer(GLState *ctx, uint32_t *color, float *depth, uint8_t *stencil);
 
+/* Present: the colour plane, top row first, as packed RGBA8 rows
+ * (SDL_PIXELFORMAT_ABGR8888) of width * 4 bytes at pixels + row * pitch, after
+ * all queued work finished; pitch >= width * 4.  The reference's SDL glue
+ * uploads GLState.framebuffer.color with pitch width * 4
+ * (include/mytinygl/sdl.h:74-84); include/mytinygl/sdl.h here calls this on
+ * the streaming texture's locked memory.  Returns MTGL_OK or MTGL_ERR_*. */
+int mtgl_present(GLState *ctx, void *pixels, int32_t pitch);
+
 /* Framebuffer size and the device context behind a GL context (for counters). */
 int32_t   mtgl_context_width(GLState *ctx);
 int32_t   mtgl_context_height(GLState *ctx);

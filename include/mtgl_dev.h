@@ -239,6 +239,26 @@ int  mtgl_dev_clear(mtgl_dev *dev, uint32_t buffers, int32_t x0, int32_t y0, int
 int  mtgl_dev_read(mtgl_dev *dev, uint32_t buffer, int32_t x, int32_t y, int32_t w, int32_t h, void *dst);
 int  mtgl_dev_write(mtgl_dev *dev, uint32_t buffer, int32_t x, int32_t y, int32_t w, int32_t h, const void *src);
 
+/* glReadPixels (reference src/gl_api.c:1180-1229): a width x height window whose
+ * bottom-left pixel is window pixel (x, y) in GL convention (y counts up from
+ * the bottom row), written to host memory `dst` as tightly packed
+ * GL_UNSIGNED_BYTE pixels, rows bottom-up.  format GL_RGBA writes r, g, b, a
+ * (4 bytes), GL_RGB writes r, g, b (3 bytes).  Window rows outside the
+ * framebuffer read as zero bytes; pixels left or right of it, in rows that
+ * exist, read as (0, 0, 0, 255).  The window may extend past every edge.
+ * width or height <= 0, or another format, writes nothing and returns
+ * MTGL_OK (the reference's loops write nothing for them).  The pixels are
+ * packed and flipped on the device and copied to `dst` in one transfer. */
+int  mtgl_dev_read_pixels(mtgl_dev *dev, int32_t x, int32_t y, int32_t width, int32_t height, uint32_t format,
+                          void *dst);
+
+/* Present: the whole colour plane, top row first, each row width * 4 bytes of
+ * packed RGBA8 (SDL_PIXELFORMAT_ABGR8888 on little-endian hosts) at dst +
+ * row * pitch, pitch >= width * 4.  Replaces the reference's SDL glue, which
+ * uploads GLState.framebuffer.color with framebuffer_get_pitch() as pitch
+ * (include/mytinygl/sdl.h:74-84, src/framebuffer.h:136-143). */
+int  mtgl_dev_present(mtgl_dev *dev, void *dst, int64_t pitch);
+
 /* Texture level 0 of texture `id` (1..256) as packed RGBA8 texels; the device also
  * builds the 2x2 box-filtered level 1 (reference texture_upload_*,
  * src/textures.c:141-269, texture_generate_mip1 :311-354). */

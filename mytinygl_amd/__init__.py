@@ -13,8 +13,29 @@ fallback: without the library, or without a GPU, context creation fails.
 from __future__ import annotations
 
 import os
+import sys
 
-from .gl import ENUMS, GLLibrary
+
+def _hip_runtime_order() -> None:
+    """Import torch (when it is installed) before anything loads libmtgl.so.
+
+    torch ships its own HIP runtime under the same soname as the system one
+    (libamdhip64.so.7).  The first one loaded serves the whole process: with
+    torch's loaded first both torch and libmtgl.so find the GPU, while with
+    libmtgl.so's loaded first torch's kernels find none.  Importing the
+    package therefore imports torch first; MTGL_NO_TORCH=1 skips it (for
+    processes that never use torch)."""
+    if "torch" in sys.modules or os.environ.get("MTGL_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
+_hip_runtime_order()
+
+from .gl import ENUMS, GLLibrary  # noqa: E402
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmtgl.so")
 

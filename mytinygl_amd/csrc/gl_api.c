@@ -32,6 +32,11 @@ void gls_set_error(GLState *c, GLenum err)
     if (c && c->error == GL_NO_ERROR) c->error = err;
 }
 
+/* A device failure becomes a sticky GL error: GL_OUT_OF_MEMORY for an
+ * allocation failure, GL_INVALID_OPERATION for everything else, including a
+ * call the device refuses (a point, line or glDrawPixels call while a batch is
+ * recording), so that a C caller sees with glGetError that the call drew
+ * nothing.  The refusal is also reported once per context on stderr. */
 void gls_report_device_error(GLState *c, int rc, const char *what)
 {
     if (rc == MTGL_OK) return;
@@ -41,9 +46,9 @@ void gls_report_device_error(GLState *c, int rc, const char *what)
                     what);
             c->warned |= 1u;
         }
-        return;
+    } else {
+        fprintf(stderr, "libmtgl: device error %d in %s\n", rc, what);
     }
-    fprintf(stderr, "libmtgl: device error %d in %s\n", rc, what);
     gls_set_error(c, rc == MTGL_ERR_OUT_OF_MEMORY ? GL_OUT_OF_MEMORY : GL_INVALID_OPERATION);
 }
 
@@ -180,6 +185,12 @@ MTGL_EXPORT int mtgl_read_framebuffer(GLState *c, uint32_t *color, float *depth,
     if (depth && rc == MTGL_OK) rc = mtgl_dev_read(c->dev, MTGL_BUF_DEPTH, 0, 0, c->width, c->height, depth);
     if (stencil && rc == MTGL_OK) rc = mtgl_dev_read(c->dev, MTGL_BUF_STENCIL, 0, 0, c->width, c->height, stencil);
     return rc;
+}
+
+MTGL_EXPORT int mtgl_present(GLState *c, void *pixels, int32_t pitch)
+{
+    if (!c || !pixels) return MTGL_ERR_INVALID;
+    return mtgl_dev_present(c->dev, pixels, pitch);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1108,47 +1119,18 @@ MTGL_EXPORT void glHint(GLenum target, GLenum mode)
 /* pixel transfer (reference src/gl_api.c:1180-1424)                          */
 /* ------------------------------------------------------------------------- */
 
-/* Rows are returned bottom-up (GL window convention); pixels outside the
- * framebuffer read as 0 (alpha 255 inside a row that exists). */
+/* Rows are returned bottom-up (GL window convention); rows outside the
+ * framebuffer read as zero bytes, pixels beside it in rows that exist as
+ * (0, 0, 0, 255).  The device packs and flips the window and copies it to
+ * `pixels` in one transfer (mtgl_dev_read_pixels).  Negative sizes write
+ * nothing (the reference's memset would take them as huge sizes). */
 MTGL_EXPORT void glReadPixels(GLint x, GLint y, GLsizei width, GLsizei height, GLenum format, GLenum type, GLvoid *pixels)
 {
     CTX_OR_RETURN();
     if (type != GL_UNSIGNED_BYTE || !pixels) return;
-    if (width <= 0 || height <= 0) return;
-    int bpp = format == GL_RGBA ? 4 : (format == GL_RGB ? 3 : 0);
-    uint8_t *dst = (uint8_t *)pixels;
-    /* window of framebuffer rows/columns that exist */
-    int32_t cx0 = x < 0 ? 0 : x;
-    int32_t cx1 = x + width > c->width ? c->width : x + width;
-    uint32_t *row = NULL;
-    if (cx1 > cx0) {
-        row = (uint32_t *)malloc((size_t)(cx1 - cx0) * sizeof(uint32_t));
-        if (!row) { gls_set_error(c, GL_OUT_OF_MEMORY); return; }
-    }
-    for (GLsizei r = 0; r < height; r++) {
-        GLint fb_y = c->height - 1 - (y + r);
-        uint8_t *out = dst + (size_t)r * (size_t)width * (size_t)bpp;
-        if (fb_y < 0 || fb_y >= c->height) {
-            if (bpp) memset(out, 0, (size_t)width * (size_t)bpp);
-            continue;
-        }
-        if (row) {
-            int rc = mtgl_dev_read(c->dev, MTGL_BUF_COLOR, cx0, fb_y, cx1 - cx0, 1, row);
-            if (rc != MTGL_OK) { gls_report_device_error(c, rc, "glReadPixels"); break; }
-        }
-        for (GLsizei col = 0; col < width; col++) {
-            GLint sx = x + col;
-            uint32_t p = 0xFF000000u;
-            if (sx >= 0 && sx < c->width) p = row[sx - cx0];
-            if (bpp) {
-                out[col * bpp + 0] = (uint8_t)(p & 0xFF);
-                out[col * bpp + 1] = (uint8_t)((p >> 8) & 0xFF);
-                out[col * bpp + 2] = (uint8_t)((p >> 16) & 0xFF);
-                if (bpp == 4) out[col * bpp + 3] = (uint8_t)(p >> 24);
-            }
-        }
-    }
-    free(row);
+    if (width <= 0 || height <= 0 || (format != GL_RGBA && format != GL_RGB)) return;
+    int rc = mtgl_dev_read_pixels(c->dev, x, y, width, height, format, pixels);
+    if (rc != MTGL_OK) gls_report_device_error(c, rc, "glReadPixels");
 }
 
 /* Pixels at the raster position with alpha test, depth test and blending

@@ -733,26 +733,16 @@ __device__ __forceinline__ void perspective_divide(CV &v)
 
 // one fan triangle (c[0], c[j], c[j+1]) after the divide; returns 1 when a
 // record was written
-__device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, const CV &A, const CV &B, const CV &C,
-                         bool cold, RTri *outp, RCold *coldp, RBox *boxp, int *tiles_out)
+// The rasteriser set-up of one filled fan triangle (src/raster.c:458-529):
+// the box clamped to viewport and scissor, the area, the drop below half a
+// pixel and the per-triangle LOD, into record r; box gets the box clamped to
+// the framebuffer as well (pixels outside it have no observable effect).
+// Returns 0 when the triangle produces no fragment.
+__device__ int fan_record(const mtgl_raster_state *rs, int32_t fb_w, int32_t fb_h, int32_t tex_w, int32_t tex_h,
+                          const CV &A, const CV &B, const CV &C, const int32_t *X, const int32_t *Y, int back, RTri &r,
+                          int32_t *box)
 {
     const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
-    const CV *v[3] = {&A, &B, &C};
-    int32_t X[3], Y[3];
-    for (int k = 0; k < 3; k++) {
-        X[k] = cvtt((v[k]->x + 1.0f) * 0.5f * (float)vw + (float)vx);
-        Y[k] = cvtt((1.0f - v[k]->y) * 0.5f * (float)vh + (float)vy);
-    }
-    float sa = (float)isub(X[1], X[0]) * (float)isub(Y[2], Y[0]) - (float)isub(X[2], X[0]) * (float)isub(Y[1], Y[0]);
-    if (rs->flags & MTGL_F_CULL_FACE) {
-        int front = rs->front_face == GL_CCW ? (sa < 0) : (sa > 0);
-        int cull = rs->cull_face_mode == GL_FRONT ? front : (rs->cull_face_mode == GL_BACK ? !front : 1);
-        if (cull) return 0;
-    }
-    int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
-    uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
-    if (pm == GL_POINT || pm == GL_LINE) { atomicOr(&P.ctr->unsupported, 1u); return 0; }
-
     int32_t minX = X[0] < X[1] ? (X[0] < X[2] ? X[0] : X[2]) : (X[1] < X[2] ? X[1] : X[2]);
     int32_t minY = Y[0] < Y[1] ? (Y[0] < Y[2] ? Y[0] : Y[2]) : (Y[1] < Y[2] ? Y[1] : Y[2]);
     int32_t maxX = X[0] > X[1] ? (X[0] > X[2] ? X[0] : X[2]) : (X[1] > X[2] ? X[1] : X[2]);
@@ -773,19 +763,19 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     const float fx2 = (float)X[2], fy2 = (float)Y[2];
     float area = (fx2 - fx0) * (fy1 - fy0) - (fy2 - fy0) * (fx1 - fx0);
     if (fabsf(area) < 0.5f) return 0;
-    // pixels outside the framebuffer have no observable effect: bin only the
-    // framebuffer part of the box
-    int32_t bx0 = minX < 0 ? 0 : minX, by0 = minY < 0 ? 0 : minY;
-    int32_t bx1 = maxX >= P.fb_w ? P.fb_w - 1 : maxX, by1 = maxY >= P.fb_h ? P.fb_h - 1 : maxY;
-    if (bx0 > bx1 || by0 > by1) return 0;
+    box[0] = minX < 0 ? 0 : minX;
+    box[1] = minY < 0 ? 0 : minY;
+    box[2] = maxX >= fb_w ? fb_w - 1 : maxX;
+    box[3] = maxY >= fb_h ? fb_h - 1 : maxY;
+    if (box[0] > box[2] || box[1] > box[3]) return 0;
 
     float lod = 0.0f;
-    if (P.tex_w > 0) {
+    if (tex_w > 0) {
         float screen_area = fabsf(area) * 0.5f;
-        float du1 = (B.s - A.s) * (float)P.tex_w;
-        float dv1 = (B.t - A.t) * (float)P.tex_h;
-        float du2 = (C.s - A.s) * (float)P.tex_w;
-        float dv2 = (C.t - A.t) * (float)P.tex_h;
+        float du1 = (B.s - A.s) * (float)tex_w;
+        float dv1 = (B.t - A.t) * (float)tex_h;
+        float du2 = (C.s - A.s) * (float)tex_w;
+        float dv2 = (C.t - A.t) * (float)tex_h;
         float texel_area = fabsf(du1 * dv2 - du2 * dv1) * 0.5f;
         if (screen_area > 0.0f) {
             float tpp = texel_area / screen_area;
@@ -795,7 +785,6 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
             }
         }
     }
-    RTri r;
     r.x0 = X[0]; r.y0 = Y[0]; r.x1 = X[1]; r.y1 = Y[1]; r.x2 = X[2]; r.y2 = Y[2];
     r.minX = minX; r.minY = minY; r.maxX = maxX; r.maxY = maxY;
     r.area = area;
@@ -810,8 +799,43 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     r.lod = lod;
     r.back = back;
     r.pad0[0] = r.pad0[1] = r.pad0[2] = 0;
+    return 1;
+}
+
+__device__ __forceinline__ void cold_record(const CV &A, const CV &B, const CV &C, RCold &q)
+{
+    for (int k = 0; k < 3; k++) {
+        q.ep[k] = A.ep[k]; q.ep[3 + k] = B.ep[k]; q.ep[6 + k] = C.ep[k];
+        q.en[k] = A.en[k]; q.en[3 + k] = B.en[k]; q.en[6 + k] = C.en[k];
+    }
+    q.pad[0] = q.pad[1] = 0.0f;
+}
+
+__device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, const CV &A, const CV &B, const CV &C,
+                         bool cold, RTri *outp, RCold *coldp, RBox *boxp, int *tiles_out)
+{
+    const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
+    const CV *v[3] = {&A, &B, &C};
+    int32_t X[3], Y[3];
+    for (int k = 0; k < 3; k++) {
+        X[k] = cvtt((v[k]->x + 1.0f) * 0.5f * (float)vw + (float)vx);
+        Y[k] = cvtt((1.0f - v[k]->y) * 0.5f * (float)vh + (float)vy);
+    }
+    float sa = (float)isub(X[1], X[0]) * (float)isub(Y[2], Y[0]) - (float)isub(X[2], X[0]) * (float)isub(Y[1], Y[0]);
+    if (rs->flags & MTGL_F_CULL_FACE) {
+        int front = rs->front_face == GL_CCW ? (sa < 0) : (sa > 0);
+        int cull = rs->cull_face_mode == GL_FRONT ? front : (rs->cull_face_mode == GL_BACK ? !front : 1);
+        if (cull) return 0;
+    }
+    int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
+    uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
+    if (pm == GL_POINT || pm == GL_LINE) { atomicOr(&P.ctr->unsupported, 1u); return 0; }
+
+    RTri r;
+    int32_t bb[4];
+    if (!fan_record(rs, P.fb_w, P.fb_h, P.tex_w, P.tex_h, A, B, C, X, Y, back, r, bb)) return 0;
     *outp = r;
-    const int32_t tx0 = bx0 / TILE, ty0 = by0 / TILE, tx1 = bx1 / TILE, ty1 = by1 / TILE;
+    const int32_t tx0 = bb[0] / TILE, ty0 = bb[1] / TILE, tx1 = bb[2] / TILE, ty1 = bb[3] / TILE;
     RBox bx;
     bx.key0 = P.tile_off + (uint32_t)(ty0 * P.tiles_x + tx0);
     bx.tw = (uint32_t)(tx1 - tx0 + 1);
@@ -820,11 +844,7 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     *boxp = bx;
     if (cold) {
         RCold q;
-        for (int k = 0; k < 3; k++) {
-            q.ep[k] = A.ep[k]; q.ep[3 + k] = B.ep[k]; q.ep[6 + k] = C.ep[k];
-            q.en[k] = A.en[k]; q.en[3 + k] = B.en[k]; q.en[6 + k] = C.en[k];
-        }
-        q.pad[0] = q.pad[1] = 0.0f;
+        cold_record(A, B, C, q);
         *coldp = q;
     }
     *tiles_out = (int)bx.ntiles;
@@ -1680,8 +1700,8 @@ __device__ uint32_t sample_chain(const RasterConsts &K, float u, float v, float 
 // computed in parallel and in any order; resolve_fragment then applies the
 // stencil, depth, blend and mask steps in submission order.  meta = kept
 // (bit 0) | texel taps << 1.
-__device__ void fragment_source(const RasterParams &P, const RasterConsts &K, const RTri *r, uint32_t rid, float b0,
-                                float b1, float b2, Col &c_out, float &depth_out, uint32_t &meta_out)
+__device__ void fragment_source(const RasterConsts &K, const RTri *r, const RCold *q, float b0, float b1, float b2,
+                                Col &c_out, float &depth_out, uint32_t &meta_out)
 {
     const mtgl_raster_state *rs = K.rs;
     const float z = b0 * r->z0 + b1 * r->z1 + b2 * r->z2;
@@ -1701,7 +1721,6 @@ __device__ void fragment_source(const RasterParams &P, const RasterConsts &K, co
     if (K.lighting_on) {
         const bool two = r->back && K.two_side;
         if (K.phong || two) {
-            const RCold *q = &P.rcold[rid];
             float ep[3], en[3];
             for (int k = 0; k < 3; k++) {
                 ep[k] = q->ep[k] * b0 + q->ep[3 + k] * b1 + q->ep[6 + k] * b2;
@@ -2332,7 +2351,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             Col c;
             float dep;
             uint32_t meta;
-            fragment_source(P, K, r, writer - 1u, w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
+            fragment_source(K, r, P.rcold ? P.rcold + (writer - 1u) : nullptr, w0e * inv, w1e * inv, w2e * inv, c, dep,
+                            meta);
             S.color[pix] = col_pack(col_clamp(c));
         }
     }
@@ -2463,7 +2483,8 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
                     Col c;
                     float dep;
                     uint32_t meta;
-                    fragment_source(P, K, r, s_rid[k], w0e * inv, w1e * inv, w2e * inv, c, dep, meta);
+                    fragment_source(K, r, P.rcold ? P.rcold + s_rid[k] : nullptr, w0e * inv, w1e * inv, w2e * inv, c,
+                                    dep, meta);
                     s_col[f] = make_float4(c.r, c.g, c.b, c.a);
                     s_dep[f] = dep;
                     s_meta[f] = (uint8_t)meta;
@@ -2728,6 +2749,56 @@ __device__ uint64_t prim_point(const PrimParams &P, const RasterConsts &K, uint3
     return n;
 }
 
+// Fragment values of a filled face in a draw that also draws faces as lines
+// or points: bit 31 marks a triangle fragment (the full stencil / depth /
+// alpha-test pipeline of src/raster.c:547-721 applies to it in the resolve),
+// bit 30 a fragment the alpha test of its texel discards (after the stencil
+// update); the low 30 bits index its depth and colour.
+constexpr uint32_t PRIM_TRI_BIT = 0x80000000u;
+constexpr uint32_t PRIM_KILL_BIT = 0x40000000u;
+constexpr uint32_t PRIM_INDEX_MASK = 0x3FFFFFFFu;
+
+// One filled fan triangle of such a draw, one thread: the reference's
+// set-up and row-major pixel loop (src/raster.c:458-545) over the box
+// clamped to the framebuffer, each covered pixel's depth and colour as the
+// tile pipeline computes them (fragment_source), in the reference's pixel
+// order.  Returns the covered pixels.
+template <bool EMIT>
+__device__ uint64_t prim_fill(const PrimParams &P, const RasterConsts &K, const CV &A, const CV &B, const CV &C,
+                              const int32_t *X, const int32_t *Y, int back, uint64_t base)
+{
+    RTri r;
+    int32_t bb[4];
+    if (!fan_record(K.rs, P.S.fb_w, P.S.fb_h, P.S.tex_w, P.S.tex_h, A, B, C, X, Y, back, r, bb)) return 0;
+    RCold q;
+    const bool cold = P.vcold != nullptr;
+    if (cold) cold_record(A, B, C, q);
+    uint64_t n = 0;
+    for (int32_t y = bb[1]; y <= bb[3]; y++) {
+        for (int32_t x = bb[0]; x <= bb[2]; x++) {
+            float e0, e1, e2;
+            edge_values(&r, x, y, e0, e1, e2);
+            const bool inside = (r.area > 0.0f && e0 >= 0.0f && e1 >= 0.0f && e2 >= 0.0f) ||
+                                (r.area < 0.0f && e0 <= 0.0f && e1 <= 0.0f && e2 <= 0.0f);
+            if (!inside) continue;
+            if (EMIT) {
+                Col c;
+                float dep;
+                uint32_t meta;
+                fragment_source(K, &r, cold ? &q : nullptr, e0 * r.inv_area, e1 * r.inv_area, e2 * r.inv_area, c,
+                                dep, meta);
+                const uint64_t j = base + n;
+                P.keys[j] = (uint32_t)y * (uint32_t)P.S.fb_w + (uint32_t)x;
+                P.vals[j] = (uint32_t)j | PRIM_TRI_BIT | ((meta & 1u) ? 0u : PRIM_KILL_BIT);
+                P.dep[j] = dep;
+                P.col[j] = make_float4(c.r, c.g, c.b, c.a);
+            }
+            n++;
+        }
+    }
+    return n;
+}
+
 // reference render_triangle with a LINE or POINT polygon mode
 // (src/raster.c:901-958, draw_triangle_wireframe / draw_triangle_points
 // :841-898): clip, divide, fan, cull, then per fan triangle three lines or
@@ -2771,6 +2842,10 @@ __device__ uint64_t prim_triangle(const PrimParams &P, const RasterConsts &K, ui
         }
         const int back = rs->front_face == GL_CCW ? (sa >= 0) : (sa < 0);
         const uint32_t pm = back ? rs->polygon_mode_back : rs->polygon_mode_front;
+        if (pm != GL_LINE && pm != GL_POINT) {
+            total += prim_fill<EMIT>(P, K, *v[0], *v[1], *v[2], X, Y, back, base + total);
+            continue;
+        }
         Col c[3];
         for (int k = 0; k < 3; k++) {
             c[k] = phong ? compute_lighting(&rs->lighting, rs->flags, v[k]->ep, v[k]->en, col_of(m.ambient),
@@ -2832,10 +2907,19 @@ __global__ void __launch_bounds__(256) k_prim_resolve(SceneRaster S, const uint3
     if (k >= npix || (i > 0 && keys[i - 1] == k)) return;
     RasterConsts K;
     raster_consts(S, K);
-    PixelState px = {S.color[k], S.depth[k], 0u, false, false, false};
+    PixelState px = {S.color[k], S.depth[k], K.stencil_on ? (uint32_t)S.stencil[k] : 0u, false, false, false};
     for (uint32_t j = i; j < n && keys[j] == k; j++) {
-        const uint32_t f = vals[j];
+        const uint32_t v = vals[j], f = v & PRIM_INDEX_MASK;
         const float d = dep[f];
+        if (v & PRIM_TRI_BIT) {
+            // a filled face's fragment: stencil and depth tests, stencil
+            // update, alpha-test discard, depth write, colour
+            unsigned long long bytes = 0;
+            if (!depth_stencil_step(K, d, 0u, (v & PRIM_KILL_BIT) == 0u, px, bytes)) continue;
+            const float4 c = col[f];
+            color_step(K, Col{c.x, c.y, c.z, c.w}, px);
+            continue;
+        }
         if (K.depth_on && !cmp_f(K.depth_cmp, d, px.d)) continue;
         if (K.depth_write) {
             px.d = d;
@@ -2846,6 +2930,7 @@ __global__ void __launch_bounds__(256) k_prim_resolve(SceneRaster S, const uint3
     }
     if (px.dirty_d) S.depth[k] = px.d;
     if (px.dirty_c) S.color[k] = px.c;
+    if (px.dirty_s) S.stencil[k] = (uint8_t)px.s;
 }
 
 // ---------------------------------------------------------------------------
@@ -2945,6 +3030,31 @@ __global__ void __launch_bounds__(256) k_clear(uint32_t *color, float *depth, ui
     if (color) color[p] = cv;
     if (depth) depth[p] = dv;
     if (stencil) stencil[p] = sv;
+}
+
+// glReadPixels packing (reference src/gl_api.c:1180-1229), one thread per
+// output pixel of the band of window rows [r0, r0 + nrows): window row r is
+// framebuffer row fb_h - 1 - (y + r); rows outside the framebuffer are zero
+// bytes, columns outside it in rows that exist read as (0, 0, 0, 255).  bpp 4
+// stores the packed pixel (its bytes are r, g, b, a in memory), bpp 3 the
+// first three bytes.
+__global__ void __launch_bounds__(256) k_read_pixels(const uint32_t *color, int32_t fb_w, int32_t fb_h, int32_t x,
+                                                     int32_t y, int32_t w, int32_t r0, int32_t nrows, int32_t bpp,
+                                                     uint8_t *out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)w * (uint64_t)nrows) return;
+    const int64_t r = r0 + (int64_t)(i / (uint64_t)w), col = (int64_t)(i % (uint64_t)w);
+    const int64_t fy = (int64_t)fb_h - 1 - ((int64_t)y + r), sx = (int64_t)x + col;
+    uint32_t p = 0u;
+    if (fy >= 0 && fy < fb_h) p = (sx >= 0 && sx < fb_w) ? color[fy * fb_w + sx] : 0xFF000000u;
+    if (bpp == 4) {
+        reinterpret_cast<uint32_t *>(out)[i] = p;
+    } else {
+        out[3 * i + 0] = (uint8_t)(p & 0xFFu);
+        out[3 * i + 1] = (uint8_t)((p >> 8) & 0xFFu);
+        out[3 * i + 2] = (uint8_t)((p >> 16) & 0xFFu);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_mip1(const uint32_t *src, int32_t w, uint32_t *dst, int32_t mw, int32_t mh)
@@ -3452,6 +3562,7 @@ struct mtgl_dev {
     uint8_t *stencil = nullptr;
     Arena writer;                  // per pixel: last passing fragment (depth kernel)
     Arena pixels;                  // glDrawPixels source bytes
+    Arena readback;                // glReadPixels: packed rows of one band
     Arena prim_params;             // states, staged bytes and vertex parameters of a point / line draw
     Arena prim_dep, prim_col;      // fragments of a point / line draw: depth, colour
     std::unordered_map<uint32_t, DevBuf> segs;   // display-list segments by id
@@ -4181,7 +4292,9 @@ int run_launch(Exec &x, Launch &L, bool frozen)
 
 // 1: a point or line mode; 2: a triangle mode whose faces are both drawn as
 // lines or points; 0: filled triangles (the tile pipeline); 3: a triangle
-// mode with one face filled and the other not (refused)
+// mode with one face filled and the other drawn as lines or points (the
+// point/line path with filled faces rasterised per triangle, so that every
+// fragment of the draw is resolved per pixel in submission order)
 int prim_class(const mtgl_draw *dr)
 {
     switch (dr->mode) {
@@ -4196,7 +4309,7 @@ int prim_class(const mtgl_draw *dr)
 uint32_t prim_units(const mtgl_draw *dr, int cls)
 {
     const uint32_t n = dr->count;
-    if (cls == 2) return tri_count(dr->mode, n);
+    if (cls >= 2) return tri_count(dr->mode, n);
     switch (dr->mode) {
     case GL_POINTS: return n;
     case GL_LINES: return n / 2;
@@ -4277,13 +4390,16 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
     S.tex_h = tx ? tx->h : 0;
     S.tex1_w = tx ? tx->m1w : 0;
     S.tex1_h = tx ? tx->m1h : 0;
+    S.lod_mode = dr->raster->lod_mode;
+    S.chain = tx ? tx->chain : nullptr;
+    S.nlev = tx ? tx->nlev : 0;
     S.fb_w = d->w;
     S.fb_h = d->h;
     S.tiles_x = d->tiles_x;
     P.mode = dr->mode;
     P.count = dr->count;
     P.units = units;
-    P.tri = cls == 2 ? 1u : 0u;
+    P.tri = cls >= 2 ? 1u : 0u;
     P.vhot = as<VHot>(x.vhot);
     P.vcold = cold ? as<VCold>(x.vcold) : nullptr;
     P.counts = counts;
@@ -4298,7 +4414,8 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
     const uint64_t total = x.ctr_host->npairs;
     x.stats.fragments += total;   // pixels the reference's point and line loops visit
     if (total == 0) return MTGL_OK;
-    if (total > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    // fragment indices travel through the sort in the value's low bits
+    if (total > (cls == 3 ? (uint64_t)PRIM_INDEX_MASK + 1u : 0x7FFFFFFFull)) return MTGL_ERR_OUT_OF_MEMORY;
     const uint32_t n = (uint32_t)total;
     for (int k = 0; k < 2; k++) {
         if ((rc = arena_reserve(x, x.keys[k], (size_t)n * 4))) return rc;
@@ -4711,6 +4828,7 @@ MTGL_API void mtgl_dev_destroy(mtgl_dev *d)
     }
     arena_free(d->writer);
     arena_free(d->pixels);
+    arena_free(d->readback);
     arena_free(d->prim_params);
     arena_free(d->prim_dep);
     arena_free(d->prim_col);
@@ -4796,6 +4914,46 @@ MTGL_API int mtgl_dev_read(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, in
     HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy2D(dst, (size_t)w * esz, base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, (size_t)w * esz,
                      (size_t)h, hipMemcpyDeviceToHost));
+    return MTGL_OK;
+}
+
+// Packed on the device in bands of rows that fit READBACK_BAND bytes (one row
+// at least), each copied to the caller's memory as soon as it is packed.
+constexpr size_t READBACK_BAND = 64u << 20;
+
+MTGL_API int mtgl_dev_read_pixels(mtgl_dev *d, int32_t x, int32_t y, int32_t width, int32_t height, uint32_t format,
+                                  void *dst)
+{
+    if (!d || !dst) return MTGL_ERR_INVALID;
+    const int32_t bpp = format == GL_RGBA ? 4 : (format == GL_RGB ? 3 : 0);
+    if (width <= 0 || height <= 0 || bpp == 0) return MTGL_OK;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    const size_t row_bytes = (size_t)width * (size_t)bpp;
+    const int32_t band = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)height, READBACK_BAND / row_bytes));
+    if ((rc = arena_reserve(d->ex, d->readback, row_bytes * (size_t)band))) return rc;
+    uint8_t *out = (uint8_t *)dst;
+    for (int32_t r0 = 0; r0 < height; r0 += band) {
+        const int32_t nr = std::min(band, height - r0);
+        const uint64_t n = (uint64_t)width * (uint64_t)nr;
+        hipLaunchKernelGGL(k_read_pixels, dim3((uint32_t)blocks_for(n, 256)), dim3(256), 0, d->ex.stream, d->color,
+                           d->w, d->h, x, y, width, r0, nr, bpp, as<uint8_t>(d->readback));
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(out + (size_t)r0 * row_bytes, d->readback.ptr, row_bytes * (size_t)nr,
+                            hipMemcpyDeviceToHost, d->ex.stream));
+    }
+    HIPC(hipStreamSynchronize(d->ex.stream));
+    return MTGL_OK;
+}
+
+MTGL_API int mtgl_dev_present(mtgl_dev *d, void *dst, int64_t pitch)
+{
+    if (!d || !dst || pitch < (int64_t)d->w * 4) return MTGL_ERR_INVALID;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
+    HIPC(hipMemcpy2D(dst, (size_t)pitch, d->color, (size_t)d->w * 4, (size_t)d->w * 4, (size_t)d->h,
+                     hipMemcpyDeviceToHost));
     return MTGL_OK;
 }
 
@@ -4994,7 +5152,7 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
     const int cls = prim_class(dr);
     if (!g_recording) d->ex.stats.upload_bytes += dr->staged_bytes;
-    if (cls == 1 || cls == 2) {
+    if (cls != 0) {
         // a recorded batch replays filled triangles only
         if (g_recording) return MTGL_ERR_UNSUPPORTED;
         d->ex.stats.draws++;

@@ -209,6 +209,23 @@ class DiffViews:
         g.lib.mtgl_diff_info(self.h, C.byref(info))
         self.info = info
 
+    # ---- argument checks: the kernels read and write these exact layouts
+    def shapes(self):
+        """Expected shapes of vcol, light_rgb, tex and of an image / its
+        gradient: [vertices][4], [MAX_LIGHTS][3], [tex_h][tex_w][4] and
+        [views][H][W][4], all float32."""
+        i = self.info
+        return {"vcol": (int(i.vertices), 4), "light_rgb": (MAX_LIGHTS, 3), "tex": (int(i.tex_h), int(i.tex_w), 4),
+                "grad": (self.nviews, self.H, self.W, 4)}
+
+    def check(self, **arrays):
+        """Raise ValueError unless every named array has its expected shape
+        (see shapes()); dtypes are checked by the callers."""
+        want = self.shapes()
+        for name, a in arrays.items():
+            if tuple(a.shape) != want[name]:
+                raise ValueError(f"{name} has shape {tuple(a.shape)}, expected {want[name]}")
+
     def gbuffer(self, view: int) -> np.ndarray:
         out = np.empty(self.W * self.H, GPX_DTYPE)
         rc = self.g.lib.mtgl_diff_gbuffer(self.h, view, out.ctypes.data)
@@ -218,8 +235,9 @@ class DiffViews:
 
     # ---- host arrays (the CPU oracle)
     def forward_np(self, vcol, light_rgb, tex):
-        img = np.empty((self.nviews, self.H, self.W, 4), np.float32)
         a = [np.ascontiguousarray(x, np.float32) for x in (vcol, light_rgb, tex)]
+        self.check(vcol=a[0], light_rgb=a[1], tex=a[2])
+        img = np.empty((self.nviews, self.H, self.W, 4), np.float32)
         rc = self.g.lib.mtgl_diff_forward(self.h, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data,
                                           img.ctypes.data, None)
         if rc != 0:
@@ -228,6 +246,7 @@ class DiffViews:
 
     def backward_np(self, vcol, light_rgb, tex, grad):
         a = [np.ascontiguousarray(x, np.float32) for x in (vcol, light_rgb, tex, grad)]
+        self.check(vcol=a[0], light_rgb=a[1], tex=a[2], grad=a[3])
         gv = np.empty_like(a[0])
         gl = np.empty((MAX_LIGHTS, 3), np.float32)
         gt = np.empty_like(a[2])
@@ -257,6 +276,9 @@ def _torch_fn():
             for t in (vcol, light_rgb, tex):
                 if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
                     raise ValueError("vcol, light_rgb and tex must be contiguous float32 device tensors")
+                if t.device != vcol.device:
+                    raise ValueError("vcol, light_rgb and tex must be on one device")
+            views.check(vcol=vcol, light_rgb=light_rgb, tex=tex)
             img = torch.empty((views.nviews, views.H, views.W, 4), device=vcol.device, dtype=torch.float32)
             st = torch.cuda.current_stream(vcol.device).cuda_stream
             rc = views.g.lib.mtgl_diff_forward(views.h, vcol.data_ptr(), light_rgb.data_ptr(), tex.data_ptr(),
@@ -271,7 +293,8 @@ def _torch_fn():
         def backward(ctx, grad):
             vcol, light_rgb, tex = ctx.saved_tensors
             views = ctx.views
-            grad = grad.contiguous()
+            grad = grad.to(device=vcol.device, dtype=torch.float32).contiguous()
+            views.check(grad=grad)
             gv = torch.empty_like(vcol)
             gl = torch.empty_like(light_rgb)
             gt = torch.empty_like(tex)

@@ -264,6 +264,65 @@ def scene_textured_sphere(g, W, H, seed=SEED_BASE + 2):
     draw_textured_sphere(g, setup_textured_sphere(g, W, H, seed))
 
 
+def setup_libm_spheres(g, W=1920, H=1080, seed=SEED_BASE + 7, grid=96):
+    """Two grid spheres at 1920x1080 whose every pixel goes through the
+    libm-dependent steps of the fragment path: GL_PHONG lighting per pixel
+    (Blinn-Phong specular, powf, src/lighting.h:126-132) from a positional spot
+    light (spot exponent, powf, :103-110) and a directional light, one sphere
+    under GL_EXP fog and one under GL_EXP2 fog (expf, src/raster.c:686-691).
+    Returns the index count of one sphere."""
+    verts, idx = sphere_grid(grid, seed)
+    g.glViewport(0, 0, W, H)
+    aspect = W / H
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glFrustum(-0.1 * aspect, 0.1 * aspect, -0.1, 0.1, 0.1, 100.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glEnable(E["GL_LIGHTING"])
+    g.glShadeModel(E["GL_PHONG"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glLightfv(E["GL_LIGHT0"], E["GL_POSITION"], g.fv(0.3, 0.5, 1.0, 0.0))
+    g.glLightfv(E["GL_LIGHT0"], E["GL_DIFFUSE"], g.fv(0.5, 0.5, 0.45, 1.0))
+    g.glLightfv(E["GL_LIGHT0"], E["GL_SPECULAR"], g.fv(0.6, 0.6, 0.6, 1.0))
+    g.glEnable(E["GL_LIGHT1"])
+    g.glLightfv(E["GL_LIGHT1"], E["GL_POSITION"], g.fv(0.0, 1.5, 0.5, 1.0))
+    g.glLightfv(E["GL_LIGHT1"], E["GL_SPOT_DIRECTION"], g.fv(0.0, -1.0, -1.2))
+    g.glLightf(E["GL_LIGHT1"], E["GL_SPOT_CUTOFF"], 35.0)
+    g.glLightf(E["GL_LIGHT1"], E["GL_SPOT_EXPONENT"], 7.5)
+    g.glLightf(E["GL_LIGHT1"], E["GL_LINEAR_ATTENUATION"], 0.2)
+    g.glLightfv(E["GL_LIGHT1"], E["GL_DIFFUSE"], g.fv(0.9, 0.7, 0.4, 1.0))
+    g.glLightfv(E["GL_LIGHT1"], E["GL_SPECULAR"], g.fv(1.0, 1.0, 1.0, 1.0))
+    g.glLightModeli(E["GL_LIGHT_MODEL_LOCAL_VIEWER"], 1)
+    g.glMaterialfv(E["GL_FRONT_AND_BACK"], E["GL_SPECULAR"], g.fv(0.8, 0.8, 0.8, 1.0))
+    g.glMaterialf(E["GL_FRONT_AND_BACK"], E["GL_SHININESS"], 37.3)
+    g.glMaterialfv(E["GL_FRONT_AND_BACK"], E["GL_DIFFUSE"], g.fv(0.7, 0.55, 0.4, 1.0))
+    g.glEnable(E["GL_FOG"])
+    g.glFogfv(E["GL_FOG_COLOR"], g.fv(0.3, 0.35, 0.4, 1.0))
+    bufs = np.zeros(2, np.uint32)
+    g.glGenBuffers(2, bufs)
+    g.glBindBuffer(E["GL_ARRAY_BUFFER"], int(bufs[0]))
+    g.glBufferData(E["GL_ARRAY_BUFFER"], verts.nbytes, g.keep(verts), E["GL_STATIC_DRAW"])
+    g.glBindBuffer(E["GL_ELEMENT_ARRAY_BUFFER"], int(bufs[1]))
+    g.glBufferData(E["GL_ELEMENT_ARRAY_BUFFER"], idx.nbytes, g.keep(idx), E["GL_STATIC_DRAW"])
+    g.glEnableClientState(E["GL_VERTEX_ARRAY"])
+    g.glEnableClientState(E["GL_NORMAL_ARRAY"])
+    g.glVertexPointer(3, E["GL_FLOAT"], 32, 0)
+    g.glNormalPointer(E["GL_FLOAT"], 32, 12)
+    return len(idx)
+
+
+def draw_libm_spheres(g, n):
+    for k, (mode, dens) in enumerate((("GL_EXP", 0.45), ("GL_EXP2", 0.38))):
+        g.glFogi(E["GL_FOG_MODE"], E[mode])
+        g.glFogf(E["GL_FOG_DENSITY"], dens)
+        g.glLoadIdentity()
+        g.glTranslatef(-0.75 + 1.5 * k, 0.0, -2.6 - 0.4 * k)
+        g.glRotatef(30.0 + 50.0 * k, 0.3, 1.0, 0.1)
+        g.glDrawElements(E["GL_TRIANGLES"], n, E["GL_UNSIGNED_INT"], 0)
+
+
 def overdraw_geometry(W: int, H: int, ntri: int, seed: int, max_log_area: float = 4.0):
     """Triangles with area 10^U(0, max_log_area) px, centroid ~ U(screen), random
     rotation and aspect in [1, 4], constant eye depth U(0.1, 100), RGB U[0,1],
@@ -951,8 +1010,8 @@ def _prim_state(g, rng, W, H, k):
         g.glColorMask(1, 1, 1, 1)
     if rng.random() < 0.3:
         g.glEnable(E["GL_SCISSOR_TEST"])
-        g.glScissor(int(rng.integers(0, W // 2)), int(rng.integers(0, H // 2)), int(rng.integers(W // 4, W)),
-                    int(rng.integers(H // 4, H)))
+        g.glScissor(int(rng.integers(0, max(1, W // 2))), int(rng.integers(0, max(1, H // 2))),
+                    int(rng.integers(W // 4, max(W // 4 + 1, W))), int(rng.integers(H // 4, max(H // 4 + 1, H))))
     else:
         g.glDisable(E["GL_SCISSOR_TEST"])
     if rng.random() < 0.4:
@@ -1149,10 +1208,87 @@ def scene_polygon_mode(g, W, H, seed):
 
 def scene_polygon_mode_mixed(g, W, H, seed):
     """glPolygonMode with one face filled and the other drawn as lines or
-    points in the same draw: rendered by the oracle and the reference; the
-    device refuses such draws."""
+    points in the same draw (reference src/raster.c:936-957): the mode is
+    chosen per fan triangle, so filled, wireframe and point fragments of one
+    draw meet on the same pixels in submission order."""
     modes = [("GL_FILL", "GL_LINE"), ("GL_POINT", "GL_FILL"), ("GL_LINE", "GL_FILL"), ("GL_FILL", "GL_POINT")]
     _polygon_mode_tris(g, _rng(seed), W, H, seed, modes)
+
+
+def scene_polygon_mode_mixed_dense(g, W, H, seed):
+    """Many overlapping triangles per mixed-mode draw, so that filled faces
+    and the other faces' lines and points cross on most pixels: a stencil
+    test passing up to three layers (GL_GEQUAL 2, GL_INCR) under blending, a textured draw whose filled faces
+    take the alpha test on the texel (points too; lines never, as the
+    reference's line loop does not end on a rejected pixel), GL_PHONG with
+    fog, and culling of one face."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    g.glClearStencil(0)
+    g.glClear(E["GL_STENCIL_BUFFER_BIT"])
+    tex = random_rgba(16, 16, seed + 5)
+    tex[..., 3] = rng.integers(0, 256, (16, 16), dtype=np.uint8)
+    _upload_texture(g, tex)
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MIN_FILTER"], E["GL_LINEAR_MIPMAP_LINEAR"])
+    g.glTexParameteri(E["GL_TEXTURE_2D"], E["GL_TEXTURE_MAG_FILTER"], E["GL_LINEAR"])
+    g.glLineWidth(2.0)
+    g.glPointSize(2.0)
+    # 1: front filled, back wireframe, stencil GL_GEQUAL 2 / GL_INCR (up to
+    # three layers), blending
+    g.glEnable(E["GL_STENCIL_TEST"])
+    g.glStencilFunc(E["GL_GEQUAL"], 2, 0xFF)
+    g.glStencilOp(E["GL_KEEP"], E["GL_INCR"], E["GL_INCR"])
+    g.glEnable(E["GL_BLEND"])
+    g.glBlendFunc(E["GL_SRC_ALPHA"], E["GL_ONE_MINUS_SRC_ALPHA"])
+    g.glPolygonMode(E["GL_FRONT"], E["GL_FILL"])
+    g.glPolygonMode(E["GL_BACK"], E["GL_LINE"])
+    g.glBegin(E["GL_TRIANGLES"])
+    _random_tris_immediate(g, rng, 60, spread=2.0, alpha=(0.3, 1.0))
+    g.glEnd()
+    g.glDisable(E["GL_BLEND"])
+    g.glDisable(E["GL_STENCIL_TEST"])
+    # 2: textured; front points, back filled; the alpha test on the texel
+    g.glEnable(E["GL_TEXTURE_2D"])
+    g.glEnable(E["GL_ALPHA_TEST"])
+    g.glAlphaFunc(E["GL_GREATER"], 0.4)
+    g.glPolygonMode(E["GL_FRONT"], E["GL_POINT"])
+    g.glPolygonMode(E["GL_BACK"], E["GL_FILL"])
+    g.glBegin(E["GL_QUADS"])
+    _random_tris_immediate(g, rng, 40, spread=2.0, tex=True)
+    g.glEnd()
+    g.glDisable(E["GL_ALPHA_TEST"])
+    # 3: textured, front filled, back wireframe, GL_LEQUAL, linear fog
+    g.glDepthFunc(E["GL_LEQUAL"])
+    g.glEnable(E["GL_FOG"])
+    g.glFogi(E["GL_FOG_MODE"], E["GL_LINEAR"])
+    g.glFogf(E["GL_FOG_START"], 1.0)
+    g.glFogf(E["GL_FOG_END"], 6.0)
+    g.glPolygonMode(E["GL_FRONT"], E["GL_FILL"])
+    g.glPolygonMode(E["GL_BACK"], E["GL_LINE"])
+    g.glBegin(E["GL_TRIANGLE_STRIP"])
+    _random_tris_immediate(g, rng, 30, spread=2.5, tex=True)
+    g.glEnd()
+    g.glDisable(E["GL_FOG"])
+    g.glDisable(E["GL_TEXTURE_2D"])
+    # 4: GL_PHONG, front wireframe, back filled; back faces culled, then front
+    g.glEnable(E["GL_LIGHTING"])
+    g.glEnable(E["GL_LIGHT0"])
+    g.glEnable(E["GL_COLOR_MATERIAL"])
+    g.glShadeModel(E["GL_PHONG"])
+    g.glPolygonMode(E["GL_FRONT"], E["GL_LINE"])
+    g.glPolygonMode(E["GL_BACK"], E["GL_FILL"])
+    for face in ("GL_BACK", "GL_FRONT"):
+        g.glEnable(E["GL_CULL_FACE"])
+        g.glCullFace(E[face])
+        g.glBegin(E["GL_TRIANGLE_FAN"])
+        _random_tris_immediate(g, rng, 12, spread=2.5, normals=True)
+        g.glEnd()
+    g.glDisable(E["GL_CULL_FACE"])
+    g.glDisable(E["GL_LIGHTING"])
+    g.glShadeModel(E["GL_SMOOTH"])
+    g.glPolygonMode(E["GL_FRONT_AND_BACK"], E["GL_FILL"])
+    g.glDepthFunc(E["GL_LESS"])
 
 
 def scene_draw_pixels(g, W, H, seed):
@@ -1181,7 +1317,7 @@ def scene_draw_pixels(g, W, H, seed):
                "GL_ONE_MINUS_DST_ALPHA", "GL_SRC_ALPHA_SATURATE", "GL_CONSTANT_COLOR"]
     for k in range(20):
         fmt, bpp = fmts[k % 4]
-        w, h = int(rng.integers(1, W // 2 + 1)), int(rng.integers(1, H // 2 + 1))
+        w, h = int(rng.integers(1, max(2, W // 2 + 1))), int(rng.integers(1, max(2, H // 2 + 1)))
         x, y = int(rng.integers(-w // 2, W - w // 2)), int(rng.integers(-h // 2, H - h // 2))
         if k % 5 == 4:
             g.glRasterPos2f(x + 0.37, y + 0.81)
@@ -1383,10 +1519,10 @@ UNDEFINED_IN_REFERENCE = ("vbo_oob",)
 # polygon-mode LINE / POINT draws are refused while recording) or whose draws
 # read display-list segments, which a batch does not track; they stay out of
 # the batched feature tests
-NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "lists_replay", "lists_repeat", "prim_edges")
-# scenes with draws the device refuses (one face filled, the other drawn as
-# lines or points); oracle against reference only
-DEVICE_UNSUPPORTED = ("polygon_mode_mixed",)
+NOT_BATCHABLE = ("draw_pixels", "lines_points", "polygon_mode", "polygon_mode_mixed", "polygon_mode_mixed_dense",
+                 "lists_replay", "lists_repeat", "prim_edges")
+# scenes with draws the device refuses; oracle against reference only
+DEVICE_UNSUPPORTED = ()
 
 FEATURE_SCENES = {
     "depth_funcs": scene_depth_funcs,
@@ -1406,6 +1542,7 @@ FEATURE_SCENES = {
     "prim_edges": scene_prim_edges,
     "polygon_mode": scene_polygon_mode,
     "polygon_mode_mixed": scene_polygon_mode_mixed,
+    "polygon_mode_mixed_dense": scene_polygon_mode_mixed_dense,
     "lists": scene_lists,
     "lists_replay": scene_lists_replay,
     "lists_repeat": scene_lists_repeat,

@@ -1198,7 +1198,30 @@ void mtgl_dev_destroy(mtgl_dev *d)
     free(d);
 }
 
-int mtgl_dev_finish(mtgl_dev *d) { return d ? MTGL_OK : MTGL_ERR_INVALID; }
+/* Failure injection for the host layer's error tests (not part of the seam's
+ * header): the next `count` calls of the seam that can fail on a device
+ * (clear, draw, glDrawPixels, glReadPixels, texture and buffer uploads,
+ * finish) return `rc` without doing anything. */
+static int or_fail_rc, or_fail_count;
+
+__attribute__((visibility("default"))) void oracle_inject_failure(int rc, int count)
+{
+    or_fail_rc = rc;
+    or_fail_count = count;
+}
+
+static int or_injected(void)
+{
+    if (or_fail_count <= 0) return MTGL_OK;
+    or_fail_count--;
+    return or_fail_rc;
+}
+
+int mtgl_dev_finish(mtgl_dev *d)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    return or_injected();
+}
 
 static int or_record_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_t x1, int32_t y1,
                            uint32_t color, float depth, uint8_t stencil);
@@ -1210,6 +1233,10 @@ int mtgl_dev_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0, int32_
                    uint32_t color, float depth, uint8_t stencil)
 {
     if (!d) return MTGL_ERR_INVALID;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
     if (or_recording) return or_record_clear(d, buffers, x0, y0, x1, y1, color, depth, stencil);
     if (x0 < 0) x0 = 0;
     if (y0 < 0) y0 = 0;
@@ -1241,6 +1268,57 @@ int mtgl_dev_read(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, int32_t w, 
     if (x < 0 || y < 0 || w < 0 || h < 0 || x + w > d->w || y + h > d->h) return MTGL_ERR_INVALID;
     for (int32_t r = 0; r < h; r++)
         memcpy((uint8_t *)dst + (size_t)r * w * esz, base + ((size_t)(y + r) * d->w + x) * esz, (size_t)w * esz);
+    return MTGL_OK;
+}
+
+/* glReadPixels, reference src/gl_api.c:1180-1229: window row r is framebuffer
+ * row h - 1 - (y + r); rows outside are zero bytes, pixels beside the
+ * framebuffer in rows that exist are (0, 0, 0, 255); RGBA or RGB bytes. */
+int mtgl_dev_read_pixels(mtgl_dev *d, int32_t x, int32_t y, int32_t width, int32_t height, uint32_t format,
+                         void *dst)
+{
+    if (!d || !dst) return MTGL_ERR_INVALID;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
+    const int bpp = format == GL_RGBA ? 4 : (format == GL_RGB ? 3 : 0);
+    if (width <= 0 || height <= 0 || bpp == 0) return MTGL_OK;
+    uint8_t *out = (uint8_t *)dst;
+    for (int32_t row = 0; row < height; row++) {
+        const int64_t fb_y = (int64_t)d->h - 1 - ((int64_t)y + row);
+        uint8_t *o = out + (size_t)row * (size_t)width * (size_t)bpp;
+        if (fb_y < 0 || fb_y >= d->h) {
+            memset(o, 0, (size_t)width * (size_t)bpp);
+            continue;
+        }
+        for (int32_t col = 0; col < width; col++) {
+            const int64_t sx = (int64_t)x + col;
+            uint8_t r = 0, g = 0, b = 0, a = 255;
+            if (sx >= 0 && sx < d->w) {
+                const uint32_t p = d->color[fb_y * d->w + sx];
+                r = p & 0xFF;
+                g = (p >> 8) & 0xFF;
+                b = (p >> 16) & 0xFF;
+                a = (p >> 24) & 0xFF;
+            }
+            o[(size_t)col * bpp + 0] = r;
+            o[(size_t)col * bpp + 1] = g;
+            o[(size_t)col * bpp + 2] = b;
+            if (bpp == 4) o[(size_t)col * bpp + 3] = a;
+        }
+    }
+    return MTGL_OK;
+}
+
+/* the colour plane, top row first, rows `pitch` bytes apart (the reference's
+ * SDL glue uploads framebuffer.color with pitch width * 4,
+ * include/mytinygl/sdl.h:74-84) */
+int mtgl_dev_present(mtgl_dev *d, void *dst, int64_t pitch)
+{
+    if (!d || !dst || pitch < (int64_t)d->w * 4) return MTGL_ERR_INVALID;
+    for (int32_t r = 0; r < d->h; r++)
+        memcpy((uint8_t *)dst + (size_t)r * (size_t)pitch, d->color + (size_t)r * d->w, (size_t)d->w * 4);
     return MTGL_OK;
 }
 
@@ -1297,6 +1375,10 @@ int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
     if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
     if ((uint64_t)p->width * (uint64_t)p->height > 0xFFFFFFFFull) return MTGL_ERR_INVALID;
     if (or_recording) return MTGL_ERR_UNSUPPORTED;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
     const uint8_t *src = (const uint8_t *)p->pixels;
     int alpha_on = (p->flags & MTGL_F_ALPHA_TEST) != 0;
     int depth_on = (p->flags & MTGL_F_DEPTH_TEST) != 0;
@@ -1339,6 +1421,10 @@ int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, const uin
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS || w <= 0 || h <= 0 || w > MTGL_MAX_TEX_SIZE ||
         h > MTGL_MAX_TEX_SIZE || !texels)
         return MTGL_ERR_INVALID;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
     or_tex *t = &d->tex[id];
     size_t n = (size_t)w * (size_t)h;
     uint32_t *px = (uint32_t *)malloc(n * sizeof(uint32_t));
@@ -1380,6 +1466,10 @@ int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
 int mtgl_dev_buffer_data(mtgl_dev *d, uint32_t id, int64_t size, const void *data)
 {
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
     if (size <= 0) { free(d->buf[id].data); d->buf[id].data = NULL; d->buf[id].size = 0; return MTGL_OK; }
     uint8_t *p = (uint8_t *)malloc((size_t)size);
     if (!p) return MTGL_ERR_OUT_OF_MEMORY;
@@ -1396,6 +1486,10 @@ int mtgl_dev_buffer_sub_data(mtgl_dev *d, uint32_t id, int64_t offset, int64_t s
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS || !d->buf[id].data || !data || offset < 0 || size < 0 ||
         offset + size > d->buf[id].size)
         return MTGL_ERR_INVALID;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
     memcpy(d->buf[id].data + offset, data, (size_t)size);
     return MTGL_OK;
 }
@@ -1448,6 +1542,10 @@ int mtgl_dev_segment_free(mtgl_dev *d, uint32_t id)
 int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
 {
     if (!d || !dr || !dr->raster || !dr->vstates || dr->n_vstates == 0) return MTGL_ERR_INVALID;
+    {
+        const int frc = or_injected();
+        if (frc) return frc;
+    }
     const uint32_t n = dr->count;
     switch (dr->mode) {
     case GL_POINTS: case GL_LINES: case GL_LINE_STRIP: case GL_LINE_LOOP:
@@ -1638,6 +1736,14 @@ static int or_record_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y0
 
 static int or_record_draw(mtgl_dev *d, const mtgl_draw *dr)
 {
+    /* a batch replays filled triangles only: points, lines and polygon modes
+     * other than GL_FILL are refused while recording */
+    switch (dr->mode) {
+    case GL_POINTS: case GL_LINES: case GL_LINE_STRIP: case GL_LINE_LOOP: return MTGL_ERR_UNSUPPORTED;
+    default: break;
+    }
+    if (dr->raster->polygon_mode_front != GL_FILL || dr->raster->polygon_mode_back != GL_FILL)
+        return MTGL_ERR_UNSUPPORTED;
     or_cmd *c = or_batch_push(or_recording, d);
     if (!c) return MTGL_ERR_OUT_OF_MEMORY;
     c->draw = *dr;

@@ -258,7 +258,58 @@ def script_no_context(g):
     return [int(g.glGetError()), int(g.glIsEnabled(E["GL_DEPTH_TEST"]))]
 
 
+def _read(g, x, y, w, h, fmt, typ="GL_UNSIGNED_BYTE", room=None):
+    """glReadPixels into a buffer pre-filled with 0xCD; returns the SHA-256 of
+    the whole buffer (bytes past the window show what was written) and the
+    count of bytes changed."""
+    import hashlib
+    n = room if room is not None else max(0, w) * max(0, h) * 4 + 16
+    buf = np.full(max(n, 16), 0xCD, np.uint8)
+    g.glReadPixels(x, y, w, h, E[fmt], E[typ], buf)
+    return [hashlib.sha256(buf.tobytes()).hexdigest()[:24], int((buf != 0xCD).sum()), _err(g)]
+
+
+def script_read_pixels(g, W=32, H=24):
+    """glReadPixels (src/gl_api.c:1180-1229) on a frame of overlapping
+    triangles with per-vertex alpha: RGBA and RGB, windows inside, past every
+    edge, wholly outside, larger than the framebuffer, one pixel at each
+    corner, zero and negative sizes, and formats and types that read nothing."""
+    rng = np.random.default_rng(11)
+    g.glClearColor(0.25, 0.5, 0.75, 0.125)
+    g.glClear(E["GL_COLOR_BUFFER_BIT"] | E["GL_DEPTH_BUFFER_BIT"])
+    g.glMatrixMode(E["GL_PROJECTION"])
+    g.glLoadIdentity()
+    g.glOrtho(0.0, float(W), 0.0, float(H), -1.0, 1.0)
+    g.glMatrixMode(E["GL_MODELVIEW"])
+    g.glLoadIdentity()
+    g.glBegin(E["GL_TRIANGLES"])
+    for _ in range(12):
+        for _ in range(3):
+            r, gg, b, a = (float(v) for v in rng.random(4))
+            g.glColor4f(r, gg, b, a)
+            g.glVertex2f(float(rng.uniform(-4, W + 4)), float(rng.uniform(-4, H + 4)))
+    g.glEnd()
+    out = [_err(g)]
+    windows = [
+        (0, 0, W, H), (3, 2, 10, 7), (-5, -3, 12, 9), (W - 6, H - 4, 11, 9), (-7, H - 3, W + 14, 6),
+        (W - 2, -6, 9, H + 12), (-3, -3, W + 6, H + 6), (W, 0, 5, 4), (-5, 0, 5, 4), (0, H, 4, 3),
+        (0, -3, 4, 3), (W + 10, H + 10, 3, 3), (-100, -100, 3, 2), (0, 0, 1, 1), (W - 1, 0, 1, 1),
+        (0, H - 1, 1, 1), (W - 1, H - 1, 1, 1), (5, 5, W, 1), (5, 5, 1, H),
+    ]
+    for fmt in ("GL_RGBA", "GL_RGB"):
+        for (x, y, w, h) in windows:
+            out.append(_read(g, x, y, w, h, fmt))
+    # nothing written: zero sizes, a negative height, other formats and types
+    out.append(_read(g, 0, 0, 0, 4, "GL_RGBA", room=64))
+    out.append(_read(g, 0, 0, 4, 0, "GL_RGB", room=64))
+    out.append(_read(g, 0, 0, 4, -2, "GL_RGBA", room=64))
+    out.append(_read(g, 0, 0, 4, 4, "GL_LUMINANCE", room=64))
+    out.append(_read(g, 0, 0, 4, 4, "GL_RGBA", "GL_FLOAT", room=64))
+    return out
+
+
 SCRIPTS = {
+    "read_pixels": script_read_pixels,
     "errors": script_errors,
     "matrices": script_matrices,
     "state_queries": script_state_queries,

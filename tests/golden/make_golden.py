@@ -47,6 +47,7 @@ CONFIGS = {
     "icosphere_640x480": (S.setup_icosphere, S.draw_icosphere, 640, 480),
     "textured_sphere_1920x1080": (S.setup_textured_sphere, S.draw_textured_sphere, 1920, 1080),
     "overdraw_3840x2160": (S.setup_overdraw, S.draw_overdraw, 3840, 2160),
+    "libm_spheres_1920x1080": (S.setup_libm_spheres, S.draw_libm_spheres, 1920, 1080),
 }
 
 

@@ -18,6 +18,8 @@ REF_HEADER = "/root/reference/include/GL/gl.h"
 def declared_functions():
     names = set()
     for dirpath, _, files in os.walk(INCLUDE):
+        if os.path.basename(dirpath) == "mytinygl":
+            continue   # header-only SDL glue: static inline helpers, nothing exported
         for fn in files:
             if not fn.endswith(".h"):
                 continue

@@ -69,3 +69,27 @@ def test_fp64_model_and_finite_differences(small):
     assert len(rows) == 64 + 64 + 24
     assert worst < 2e-3, [r for r in rows if abs(r[3] - r[4]) > 2e-3 * max(1e-3, abs(r[3]))][:5]
     assert np.abs(gl).sum() > 0 and np.abs(gt).sum() > 0 and np.abs(gv).sum() > 0
+
+
+def test_wrong_shapes_raise_before_any_call(small):
+    """Arguments whose shapes differ from the captured views' are refused
+    with ValueError (the kernels would read or write past them)."""
+    P, V = small
+    img = V.forward_np(P["vcol"], P["light_rgb"], P["tex"])
+    bad = {
+        "vcol": P["vcol"][:-1],
+        "light_rgb": P["light_rgb"][:4],
+        "tex": P["tex"][:, :16],
+    }
+    for name, arr in bad.items():
+        args = dict(vcol=P["vcol"], light_rgb=P["light_rgb"], tex=P["tex"])
+        args[name] = arr
+        with pytest.raises(ValueError, match=name):
+            V.forward_np(**args)
+        with pytest.raises(ValueError, match=name):
+            V.backward_np(grad=img, **args)
+    with pytest.raises(ValueError, match="grad"):
+        V.backward_np(P["vcol"], P["light_rgb"], P["tex"], img[:1])
+    # float64 inputs of the right shapes are converted, not misread
+    a = V.forward_np(P["vcol"].astype(np.float64), P["light_rgb"], P["tex"])
+    assert np.array_equal(a, img)

@@ -45,6 +45,21 @@ def test_feature_scene(mtgl, oracle, name, seed):
     _check_frame(f"{name}/{seed}", a, b)
 
 
+# framebuffer sides that are not multiples of the 8x8 tiles: partial tiles on
+# the right and bottom edges, single rows and columns, one pixel
+ODD_SIZES = [(97, 71), (333, 17), (1, 1000), (1, 1)]
+
+
+@pytest.mark.parametrize("name", FEATURE)
+@pytest.mark.parametrize("size", ODD_SIZES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_feature_scene_odd_size(mtgl, oracle, name, size):
+    W, H = size
+    fn = S.FEATURE_SCENES[name]
+    (a, ea), (b, eb) = S.render(oracle, fn, W, H, 1), S.render(mtgl, fn, W, H, 1)
+    assert ea == eb
+    _check_frame(f"{name}/{W}x{H}", a, b)
+
+
 def _render_config(g, setup, draw, W, H):
     ctx = g.create_context(W, H)
     try:
@@ -65,6 +80,9 @@ CONFIGS = {
     "icosphere_640x480": (S.setup_icosphere, S.draw_icosphere, 640, 480),
     "textured_sphere_1920x1080": (S.setup_textured_sphere, S.draw_textured_sphere, 1920, 1080),
     "overdraw_3840x2160": (S.setup_overdraw, S.draw_overdraw, 3840, 2160),
+    # every pixel through per-pixel powf (Phong specular, spot) and expf (EXP,
+    # EXP2 fog): the fp64-rounded libm of the device against the host's at scale
+    "libm_spheres_1920x1080": (S.setup_libm_spheres, S.draw_libm_spheres, 1920, 1080),
 }
 
 

@@ -18,6 +18,7 @@ CONFIGS = {
     "icosphere_640x480": (S.setup_icosphere, S.draw_icosphere),
     "textured_sphere_1920x1080": (S.setup_textured_sphere, S.draw_textured_sphere),
     "overdraw_3840x2160": (S.setup_overdraw, S.draw_overdraw),
+    "libm_spheres_1920x1080": (S.setup_libm_spheres, S.draw_libm_spheres),
 }
 
 

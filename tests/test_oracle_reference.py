@@ -22,6 +22,20 @@ def test_scene_vs_reference(oracle, reference, name, seed):
     assert plane_diffs(a, b) == {"color": 0, "depth": 0, "stencil": 0}
 
 
+# framebuffer sides that are not multiples of the device's 8x8 tiles
+ODD_SIZES = [(97, 71), (333, 17), (1, 1000), (1, 1)]
+
+
+@pytest.mark.parametrize("name", FEATURE)
+@pytest.mark.parametrize("size", ODD_SIZES, ids=lambda s: f"{s[0]}x{s[1]}")
+def test_scene_vs_reference_odd_size(oracle, reference, name, size):
+    W, H = size
+    fn = S.FEATURE_SCENES[name]
+    (a, ea), (b, eb) = S.render(oracle, fn, W, H, 6), S.render(reference, fn, W, H, 6)
+    assert ea == eb
+    assert plane_diffs(a, b) == {"color": 0, "depth": 0, "stencil": 0}
+
+
 @pytest.mark.parametrize("name", sorted(api_scripts.SCRIPTS))
 def test_api_vs_reference(oracle, reference, name):
     a = json.loads(json.dumps(api_scripts.run_script(oracle, name)))
