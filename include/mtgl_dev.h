@@ -312,6 +312,7 @@ typedef struct {
 
 int  mtgl_dev_draw_pixels(mtgl_dev *dev, const mtgl_pixels *pixels);
 
+
 /* Counters and timing; no counterpart in the reference (fragments passing
  * coverage, algorithmic bytes of the per-fragment pipeline, fragment-stage time). */
 int  mtgl_dev_set_timing(mtgl_dev *dev, int enable);

@@ -125,20 +125,28 @@ struct __align__(16) RCold {
     float pad[2];
 };
 
+// Counters of an Exec.  The first group belongs to one launch and is reset
+// by the launch's first kernel; the second persists across launches.
 struct Counters {
-    unsigned long long fragments;
-    unsigned long long frag_bytes;
-    unsigned long long tri_raster;   // fan triangles set up by the current draw
+    unsigned long long tri_raster;   // fan triangles set up by the launch
+    unsigned long long npairs;    // (tile, record) pairs of the launch, written by the bin-count scan
     unsigned int nheavy;          // work items at the front of the tile list: heavy-tile parts, medium tiles
     unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
     unsigned int nclip;
     unsigned int noverflow;       // overflow slots requested by clipped triangles
-    unsigned int overflow_full;   // a request did not fit: the host grows the pool and reruns
-    unsigned int sticky;          // overflow_full of every launch since the host last cleared it
+    unsigned int overflow_full;   // bit 0: the clip pool was too small; bit 1: the pair arrays were
     unsigned int nbig;            // triangles queued for the one-wave pair expansion
-    unsigned int nbig_setup;      // the same triangles as setup counts them (read back before the expansion)
-    unsigned long long npairs;    // (tile, record) pairs of the launch, written by the pair-offset scan
+    unsigned int pad0;
+    // persistent
+    unsigned int sticky;          // overflow_full of every launch since the host last cleared it
+    unsigned int poison;          // 0, or the sequence number of the first launch that overflowed: every
+                                  // later framebuffer write of this Exec is skipped until the host clears it
+    unsigned long long poison_pairs;   // pairs that launch needed
+    unsigned int poison_flags;    // its overflow_full
+    unsigned int pad1;
+    unsigned long long acc_tri_raster; // fan triangles and pairs of every launch that did not overflow
+    unsigned long long acc_bin_pairs;
 };
 
 // attribute stream resolved to a device pointer
@@ -174,11 +182,8 @@ struct VertexParams {
 struct SetupParams {
     uint32_t mode;
     uint32_t count;
-    uint32_t unique;              // vertex records per slot (see VertexParams), looked up through the indices
-    int32_t ubase;
-    uint32_t nslots;
-    uint32_t index_type;
-    DArr indices;
+    const VertexParams *vp;       // the scene's vertex streams and states (set-up transforms its own vertices)
+    uint32_t cold;                // eye-space position and normal are kept (per-fragment lighting)
     uint32_t ntri;
     uint32_t tri_off;             // first global triangle of the scene
     uint32_t scene;
@@ -187,8 +192,6 @@ struct SetupParams {
     int32_t tiles_x;
     int32_t tex_w, tex_h;         // 0 when the draw is untextured
     const mtgl_raster_state *rs;
-    const VHot *vhot;
-    const VCold *vcold;
     RTri *rt;                     // the scene's own slots
     RCold *rcold;
     RBox *box;
@@ -198,7 +201,7 @@ struct SetupParams {
     uint32_t over_base;           // global slot of rt_over[0]
     uint32_t *n_rt;               // fan triangles kept per input triangle
     uint32_t *rec_base;           // first global record slot per input triangle
-    unsigned long long *tri_tiles;
+    unsigned long long *tri_tiles;  // (tile, record) pairs per input triangle
     uint32_t *clip_list;          // global triangles that cross a frustum plane
     uint32_t overflow_cap;        // slots of the overflow pool
     Counters *ctr;
@@ -236,14 +239,12 @@ struct RasterParams {
     uint32_t nscenes;
     const RTri *rt;
     const RCold *rcold;
-    const uint32_t *vals;          // sorted pair values (record slot)
+    const uint32_t *vals;          // per bin, in submission order: record slot
     const uint4 *list;             // work items (tile | part << 28, e0, e1, scene): nheavy from the front, nlight from the back
     uint32_t ntiles;
     uint32_t list_cap;             // items the list holds (HEAVY_SPLIT per tile)
     uint32_t part;                 // list items of this launch: 0 all, 1 the heavy ones, 2 the others
     unsigned long long *stats;     // per block: fragments, algorithmic bytes (summed when read)
-    const uint32_t *tile_start;
-    const uint32_t *tile_end;      // 0 for an empty tile
     Counters *ctr;
 };
 
@@ -501,22 +502,22 @@ __device__ __forceinline__ float sanitize(float v, float nan_value)
     return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
 }
 
-// blocks are dealt to scenes in order: block_pre holds each scene's first block
-__device__ __forceinline__ void reset_counters(Counters *ctr, unsigned long long *scan_tail);
+// clip-space vertex carried through clipping (the object-space normal the
+// reference also interpolates is never read after clipping and is dropped)
+struct CV {
+    float x, y, z, w;
+    float r, g, b, a;
+    float s, t, ez;
+    float ep[3], en[3];
+};
 
-// (block 0 also resets the launch's counters and the pair-offset scan's tail)
-__global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const uint32_t *block_pre,
-                                                const uint32_t *block_scene, uint32_t nscenes, Counters *ctr,
-                                                unsigned long long *scan_tail)
+// emit_vertex (reference src/gl_api.c:263-348) of stream position i, whose
+// array element is idx (-1: an element no array holds, which reads each
+// array's defaults): attribute fetch with get_array_element's rules
+// (src/gl_api.c:1758-1797), glColor4f sanitising, transform, Gouraud
+// lighting, texture matrix.  ep / en are written when cold is set.
+__device__ void vertex_at(const VertexParams &P, uint32_t i, int32_t idx, bool cold, CV &o)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) reset_counters(ctr, scan_tail);
-    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
-    const VertexParams &P = Ps[sc];
-    const uint32_t i = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
-    if (i >= P.count) return;
-    // a slot past the addressed span stands for every index no array reaches:
-    // -1 reads the defaults such an index reads
-    const int32_t idx = P.unique ? (i + 1u < P.count ? P.ubase + (int32_t)i : -1) : fetch_index(P, i);
     float pos[4], col[4], tc[2], nrm[3];
     fetch(P.position, idx, pos, 4);
     if (P.position.comps == 2) pos[2] = 0.0f;
@@ -540,7 +541,6 @@ __global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const ui
     }
     const mtgl_vertex_state *vs = &P.vstates[si];
 
-    // emit_vertex (reference src/gl_api.c:263-348)
     float eye[4];
     xform(vs->modelview, pos[0], pos[1], pos[2], 1.0f, eye);
     float en4[4];
@@ -573,15 +573,38 @@ __global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const ui
     if (t4[3] != 0.0f && t4[3] != 1.0f) { s_out = t4[0] / t4[3]; t_out = t4[1] / t4[3]; }
     else { s_out = t4[0]; t_out = t4[1]; }
 
+    o.x = clip[0]; o.y = clip[1]; o.z = clip[2]; o.w = clip[3];
+    o.r = vc.r; o.g = vc.g; o.b = vc.b; o.a = vc.a;
+    o.s = s_out; o.t = t_out; o.ez = -eye[2];
+    for (int k = 0; k < 3; k++) {
+        o.ep[k] = cold ? ep[k] : 0.0f;
+        o.en[k] = cold ? en[k] : 0.0f;
+    }
+}
+
+// One thread per vertex record of a point / line draw (the tile pipeline
+// transforms its vertices inside set-up): records for prim_load.
+__global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const uint32_t *block_pre,
+                                                const uint32_t *block_scene, uint32_t nscenes)
+{
+    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
+    const VertexParams &P = Ps[sc];
+    const uint32_t i = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
+    if (i >= P.count) return;
+    // a slot past the addressed span stands for every index no array reaches:
+    // -1 reads the defaults such an index reads
+    const int32_t idx = P.unique ? (i + 1u < P.count ? P.ubase + (int32_t)i : -1) : fetch_index(P, i);
+    CV o;
+    vertex_at(P, i, idx, P.out_cold != nullptr, o);
     VHot h;
-    h.x = clip[0]; h.y = clip[1]; h.z = clip[2]; h.w = clip[3];
-    h.r = vc.r; h.g = vc.g; h.b = vc.b; h.a = vc.a;
-    h.s = s_out; h.t = t_out; h.ez = -eye[2]; h.pad = 0.0f;
+    h.x = o.x; h.y = o.y; h.z = o.z; h.w = o.w;
+    h.r = o.r; h.g = o.g; h.b = o.b; h.a = o.a;
+    h.s = o.s; h.t = o.t; h.ez = o.ez; h.pad = 0.0f;
     P.out_hot[i] = h;
     if (P.out_cold) {
         VCold c;
-        c.ep[0] = ep[0]; c.ep[1] = ep[1]; c.ep[2] = ep[2];
-        c.en[0] = en[0]; c.en[1] = en[1]; c.en[2] = en[2];
+        c.ep[0] = o.ep[0]; c.ep[1] = o.ep[1]; c.ep[2] = o.ep[2];
+        c.en[0] = o.en[0]; c.en[1] = o.en[1]; c.en[2] = o.en[2];
         c.pad[0] = c.pad[1] = 0.0f;
         P.out_cold[i] = c;
     }
@@ -624,15 +647,6 @@ __device__ __forceinline__ void tri_vertices(uint32_t mode, uint32_t t, uint32_t
     default: a = 0; b = t + 1; c = t + 2; break;   // fan, polygon
     }
 }
-
-// clip-space vertex carried through clipping (the object-space normal the
-// reference also interpolates is never read after clipping and is dropped)
-struct CV {
-    float x, y, z, w;
-    float r, g, b, a;
-    float s, t, ez;
-    float ep[3], en[3];
-};
 
 __device__ __forceinline__ float plane_dist(const CV &v, int plane)
 {
@@ -700,28 +714,13 @@ __device__ int clip_plane(const CV *in, int n, CV *out, int plane, bool cold)
     return m;
 }
 
-// the vertex record of stream position i: i itself, or the slot of its index
-__device__ __forceinline__ uint32_t vertex_slot(const SetupParams &P, uint32_t i)
+// the transformed vertex of stream position i (set-up recomputes the
+// vertices a triangle shares with its neighbours: the transform is a few
+// hundred operations, cheaper than a launch and a round trip through HBM)
+__device__ __forceinline__ void load_cv(const SetupParams &P, uint32_t i, bool cold, CV &o)
 {
-    if (!P.unique) return i;
-    const int32_t idx = fetch_index_of(P.indices, P.index_type, 0, i);
-    const int64_t k = (int64_t)idx - P.ubase;
-    return (idx >= 0 && k >= 0 && k < (int64_t)P.nslots - 1) ? (uint32_t)k : P.nslots - 1u;
-}
-
-__device__ __forceinline__ void load_cv(const SetupParams &P, uint32_t v, bool cold, CV &o)
-{
-    const float4 *h = reinterpret_cast<const float4 *>(&P.vhot[v]);
-    float4 a = h[0], b = h[1], c = h[2];
-    o.x = a.x; o.y = a.y; o.z = a.z; o.w = a.w;
-    o.r = b.x; o.g = b.y; o.b = b.z; o.a = b.w;
-    o.s = c.x; o.t = c.y; o.ez = c.z;
-    if (cold) {
-        const VCold &q = P.vcold[v];
-        for (int k = 0; k < 3; k++) { o.ep[k] = q.ep[k]; o.en[k] = q.en[k]; }
-    } else {
-        for (int k = 0; k < 3; k++) { o.ep[k] = 0.0f; o.en[k] = 0.0f; }
-    }
+    const VertexParams &V = *P.vp;
+    vertex_at(V, i, fetch_index(V, i), cold, o);
 }
 
 __device__ __forceinline__ void perspective_divide(CV &v)
@@ -853,52 +852,52 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
 
 // fast path: triangles inside every frustum plane pass through the clipper
 // unchanged (each plane re-emits the vertices in order), so they are set up
-// directly; the others are queued for k_setup_clip
+// directly; the others are queued for k_setup_clip.  Every kept record's
+// tiles are counted into the bins.
 __global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint32_t *block_pre,
                                                const uint32_t *block_scene, uint32_t nscenes)
 {
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     const SetupParams &P = Ss[sc];
     const uint32_t t = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
-    if (t >= P.ntri) return;
+    const bool act = t < P.ntri;
     const mtgl_raster_state *rs = P.rs;
-    const bool cold = P.vcold != nullptr;
-    uint32_t ia, ib, ic;
-    tri_vertices(P.mode, t, ia, ib, ic);
-    CV v[3];
-    load_cv(P, vertex_slot(P, ia), cold, v[0]);
-    load_cv(P, vertex_slot(P, ib), cold, v[1]);
-    load_cv(P, vertex_slot(P, ic), cold, v[2]);
-    bool all_inside = true;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int pl = 0; pl < 6; pl++)
-            all_inside = all_inside && (plane_dist(v[k], pl) >= 0);
+    const bool cold = P.cold != 0u;
     uint32_t nout = 0;
-    unsigned long long tiles = 0;
-    if (all_inside) {
-        for (int k = 0; k < 3; k++) perspective_divide(v[k]);
-        int nt = 0;
-        RTri *out = P.rt + t;
-        RCold *cout = cold ? P.rcold + t : nullptr;
-        if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, P.box + t, &nt)) {
-            nout = 1;
-            tiles = (unsigned long long)nt;
+    RBox bx = {0u, 1u, 0u, 0u};
+    if (act) {
+        uint32_t ia, ib, ic;
+        tri_vertices(P.mode, t, ia, ib, ic);
+        CV v[3];
+        load_cv(P, ia, cold, v[0]);
+        load_cv(P, ib, cold, v[1]);
+        load_cv(P, ic, cold, v[2]);
+        bool all_inside = true;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int pl = 0; pl < 6; pl++)
+                all_inside = all_inside && (plane_dist(v[k], pl) >= 0);
+        if (all_inside) {
+            for (int k = 0; k < 3; k++) perspective_divide(v[k]);
+            int nt = 0;
+            RTri *out = P.rt + t;
+            RCold *cout = cold ? P.rcold + t : nullptr;
+            if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, &bx, &nt)) {
+                nout = 1;
+                P.box[t] = bx;
+            }
+        } else {
+            uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
+            P.clip_list[slot] = P.tri_off + t;
         }
-    } else {
-        uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
-        P.clip_list[slot] = P.tri_off + t;
+        P.n_rt[t] = nout;
+        P.rec_base[t] = P.tri_off + t;
     }
+    if (act) P.tri_tiles[t] = nout ? (unsigned long long)bx.ntiles : 0ull;
     // one counter update per wave
-    const uint64_t kept = __ballot(nout != 0), big = __ballot(tiles > EXPAND_SMALL);
-    if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) {
-        if (kept) atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
-        if (big) atomicAdd(&P.ctr->nbig_setup, (uint32_t)__popcll(big));
-    }
-    P.n_rt[t] = nout;
-    P.rec_base[t] = P.tri_off + t;
-    P.tri_tiles[t] = tiles;
+    const uint64_t kept = __ballot(nout != 0);
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
 }
 
 // slow path: Sutherland-Hodgman in the reference's plane order (near, far,
@@ -907,54 +906,58 @@ __global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint
 __global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const uint32_t *tri_pre, uint32_t nscenes,
                                                    const uint32_t *clip_list, const Counters *ctr)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *(volatile const uint32_t *)&ctr->nclip) return;
-    const uint32_t g = clip_list[i];
-    const uint32_t sc = find_scene(tri_pre, nscenes, g);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nq = __builtin_amdgcn_readfirstlane(*(volatile const uint32_t *)&ctr->nclip);
+    if (blockIdx.x * blockDim.x >= nq) return;   // the whole wave: nothing queued for it
+    const bool act = i < nq;
+    const uint32_t g = act ? clip_list[i] : 0u;
+    const uint32_t sc = act ? find_scene(tri_pre, nscenes, g) : 0u;
     const SetupParams &P = Ss[sc];
     const uint32_t t = g - P.tri_off;
     const mtgl_raster_state *rs = P.rs;
-    const bool cold = P.vcold != nullptr;
-    uint32_t ia, ib, ic;
-    tri_vertices(P.mode, t, ia, ib, ic);
-    CV a[MAX_CLIP], b[MAX_CLIP];
-    load_cv(P, vertex_slot(P, ia), cold, b[0]);
-    load_cv(P, vertex_slot(P, ib), cold, b[1]);
-    load_cv(P, vertex_slot(P, ic), cold, b[2]);
-    int n = clip_plane(b, 3, a, 0, cold);
-    n = n ? clip_plane(a, n, b, 1, cold) : 0;
-    n = n ? clip_plane(b, n, a, 2, cold) : 0;
-    n = n ? clip_plane(a, n, b, 3, cold) : 0;
-    n = n ? clip_plane(b, n, a, 4, cold) : 0;
-    n = n ? clip_plane(a, n, b, 5, cold) : 0;
+    const bool cold = P.cold != 0u;
     uint32_t nout = 0;
-    unsigned long long tiles = 0;
     uint32_t base = 0;
-    if (n >= 3) {
-        base = atomicAdd(&P.ctr->noverflow, (uint32_t)(n - 2));
-        if (base + (uint32_t)(n - 2) > P.overflow_cap) {
-            atomicOr(&P.ctr->overflow_full, 1u);
-            n = 0;
-        }
-    }
-    if (n >= 3) {
-        RTri *out = P.rt_over + base;
-        RCold *cout = cold ? P.rcold_over + base : nullptr;
-        for (int k = 0; k < n; k++) perspective_divide(b[k]);
-        for (int j = 1; j + 1 < n; j++) {
-            int nt = 0;
-            if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr,
-                          P.box_over + base + nout, &nt)) {
-                nout++;
-                tiles += (unsigned long long)nt;
+    if (act) {
+        uint32_t ia, ib, ic;
+        tri_vertices(P.mode, t, ia, ib, ic);
+        CV a[MAX_CLIP], b[MAX_CLIP];
+        load_cv(P, ia, cold, b[0]);
+        load_cv(P, ib, cold, b[1]);
+        load_cv(P, ic, cold, b[2]);
+        int n = clip_plane(b, 3, a, 0, cold);
+        n = n ? clip_plane(a, n, b, 1, cold) : 0;
+        n = n ? clip_plane(b, n, a, 2, cold) : 0;
+        n = n ? clip_plane(a, n, b, 3, cold) : 0;
+        n = n ? clip_plane(b, n, a, 4, cold) : 0;
+        n = n ? clip_plane(a, n, b, 5, cold) : 0;
+        if (n >= 3) {
+            base = atomicAdd(&P.ctr->noverflow, (uint32_t)(n - 2));
+            if (base + (uint32_t)(n - 2) > P.overflow_cap) {
+                atomicOr(&P.ctr->overflow_full, 1u);
+                n = 0;
             }
         }
+        unsigned long long tiles = 0;
+        if (n >= 3) {
+            RTri *out = P.rt_over + base;
+            RCold *cout = cold ? P.rcold_over + base : nullptr;
+            for (int k = 0; k < n; k++) perspective_divide(b[k]);
+            for (int j = 1; j + 1 < n; j++) {
+                int nt = 0;
+                RBox q;
+                if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr, &q, &nt)) {
+                    P.box_over[base + nout] = q;
+                    nout++;
+                    tiles += (unsigned long long)nt;
+                }
+            }
+        }
+        P.n_rt[t] = nout;
+        P.rec_base[t] = P.over_base + base;
+        P.tri_tiles[t] = tiles;
+        if (nout) atomicAdd(&P.ctr->tri_raster, (unsigned long long)nout);
     }
-    P.n_rt[t] = nout;
-    P.rec_base[t] = P.over_base + base;
-    P.tri_tiles[t] = tiles;
-    if (nout) atomicAdd(&P.ctr->tri_raster, (unsigned long long)nout);
-    if (tiles > EXPAND_SMALL) atomicAdd(&P.ctr->nbig_setup, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1028,7 +1031,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_sums(T *sums, uint64_t nb
 }
 
 template <typename T>
-__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64_t n, const T *block_offsets, T *out)
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64_t n, const T *block_offsets, T *out,
+                                                             T *clear)
 {
     __shared__ T waves[SCAN_THREADS / 64];
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
@@ -1038,6 +1042,10 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64
         vals[k] = base + k < n ? in[base + k] : (T)0;
         s += vals[k];
     }
+    // the input read for the last time: cleared for its next use when asked
+    if (clear)
+        for (int k = 0; k < SCAN_ITEMS; k++)
+            if (base + k < n) clear[base + k] = (T)0;
     T total;
     T ex = block_exclusive_scan<T>(s, waves, total) + block_offsets[blockIdx.x];
     for (int k = 0; k < SCAN_ITEMS; k++) {
@@ -1051,7 +1059,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, uint64
 // it; the last block writes the grand total to *total_out.
 template <typename T>
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_short(const T *in, uint64_t n, const T *block_sums,
-                                                                   T *out, T *total_out)
+                                                                   T *out, T *total_out, T *clear)
 {
     __shared__ T waves[SCAN_THREADS / 64];
     T part = 0;
@@ -1065,6 +1073,9 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_short(const T *in, 
         vals[k] = base + k < n ? in[base + k] : (T)0;
         s += vals[k];
     }
+    if (clear)
+        for (int k = 0; k < SCAN_ITEMS; k++)
+            if (base + k < n) clear[base + k] = (T)0;
     T total;
     T ex = block_exclusive_scan<T>(s, waves, total) + offset;
     for (int k = 0; k < SCAN_ITEMS; k++) {
@@ -1074,7 +1085,6 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_short(const T *in, 
     if (total_out && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total_out = offset + total;
 }
 
-// ---------------------------------------------------------------------------
 // K3a: expand triangles into (tile, fan triangle) pairs, in submission order
 // ---------------------------------------------------------------------------
 
@@ -1109,10 +1119,28 @@ __device__ __forceinline__ void expand_record(const RBox &q, uint32_t rec, uint6
 __global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
                                                 const uint32_t *rec_base, const RBox *box, uint64_t cap,
                                                 uint32_t *keys, uint32_t *vals,
-                                                uint32_t *big, Counters *ctr)
+                                                uint32_t *big, Counters *ctr, uint32_t seq)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0 && pair_base[ntri] > cap) atomicOr(&ctr->overflow_full, 2u);
+    if (t == 0) {
+        // the launch's pair total against the capacity of the pair arrays: a
+        // launch that overflowed (pairs, or the clip pool) poisons the Exec
+        // with its sequence number, so that its raster stage and every later
+        // framebuffer write are skipped until the host has grown the arrays
+        // and run it again; a launch that fits adds to the running counts
+        const unsigned long long total = pair_base[ntri];
+        const uint32_t over = ctr->overflow_full | (total > cap ? 2u : 0u);
+        if (over) {
+            ctr->overflow_full = over;
+            if (atomicCAS(&ctr->poison, 0u, seq) == 0u) {
+                ctr->poison_pairs = total;
+                ctr->poison_flags = over;
+            }
+        } else if (ctr->poison == 0u) {   // (a launch after a poisoned one runs again: counted then)
+            ctr->acc_tri_raster += ctr->tri_raster;
+            ctr->acc_bin_pairs += total;
+        }
+    }
     uint64_t p0 = 0, np = 0;
     if (t < ntri) {
         p0 = pair_base[t];
@@ -1139,9 +1167,8 @@ __global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_b
 }
 
 // The queued triangles, one wave each (lanes stride over each record's tile
-// box), then the tail: capacity the pair total does not fill (a replay sized
-// by an earlier run of the same launch) gets a key beyond every tile, which
-// sorts last.
+// box).  The sort and the tile ranges read only the pair total, so the
+// capacity past it is left as it is.
 __global__ void __launch_bounds__(64) k_expand_big(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
                                                    const uint32_t *rec_base, const RBox *box, uint64_t cap,
                                                    uint32_t *keys, uint32_t *vals,
@@ -1159,20 +1186,24 @@ __global__ void __launch_bounds__(64) k_expand_big(const unsigned long long *pai
             p0 += q.ntiles;
         }
     }
-    const uint64_t np = pair_base[ntri];
-    for (uint64_t p = np + (uint64_t)blockIdx.x * 64u + lane; p < cap; p += (uint64_t)gridDim.x * 64u) {
-        keys[p] = 0xFFFFFFFFu;
-        vals[p] = 0u;
-    }
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (key, value) pairs by key
+__device__ __forceinline__ bool exec_poisoned(const Counters *ctr)
+{
+    return __builtin_amdgcn_readfirstlane(*(volatile const uint32_t *)&ctr->poison) != 0u;
 }
 
 // ---------------------------------------------------------------------------
 // stable LSD radix sort of (key, value) pairs by key
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys, uint32_t n, int shift,
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys, uint32_t n_max,
+                                                            const unsigned long long *n_dev, int shift,
                                                             uint32_t nblocks, uint32_t *counts)
 {
+    const uint32_t n = n_dev ? (uint32_t)min((unsigned long long)n_max, *n_dev) : n_max;
     __shared__ uint32_t hist[RADIX];
     hist[threadIdx.x] = 0;
     __syncthreads();
@@ -1192,10 +1223,13 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(const uint32_t *keys
 }
 
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *keys_in, const uint32_t *vals_in,
-                                                               uint32_t n, int shift, uint32_t nblocks,
+                                                               uint32_t n_max, const unsigned long long *n_dev,
+                                                               int shift, uint32_t nblocks,
                                                                const uint32_t *offsets, uint32_t *keys_out,
                                                                uint32_t *vals_out)
 {
+    const uint32_t n = n_dev ? (uint32_t)min((unsigned long long)n_max, *n_dev) : n_max;
+    if ((uint64_t)blockIdx.x * SORT_TILE >= n) return;   // beyond the keys: its counts are zero
     __shared__ uint32_t wave_hist[SORT_THREADS / 64][RADIX];
     __shared__ uint32_t running[RADIX];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1246,7 +1280,6 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *k
     }
 }
 
-// ---------------------------------------------------------------------------
 // K3c: tile segments of the sorted pairs
 // ---------------------------------------------------------------------------
 
@@ -1264,6 +1297,28 @@ __device__ __forceinline__ uint32_t dpp_incl_scan(uint32_t v)
     return v;
 }
 
+// Inclusive prefix minimum (MAX false) or maximum over the 64 lanes, the
+// same DPP pattern as dpp_incl_scan; lanes without a source keep the
+// identity, and NaN operands are passed over (v_min / v_max return the other).
+template <bool MAX>
+__device__ __forceinline__ float dpp_incl_minmax(float v)
+{
+    const int id = __float_as_int(MAX ? -__builtin_inff() : __builtin_inff());
+#define MINMAX_STEP(ctrl, rm, bm)                                                                          \
+    {                                                                                                      \
+        const float o = __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), ctrl, rm, bm, false)); \
+        v = MAX ? fmaxf(v, o) : fminf(v, o);                                                              \
+    }
+    MINMAX_STEP(0x111, 0xf, 0xf)   // row_shr:1
+    MINMAX_STEP(0x112, 0xf, 0xf)   // row_shr:2
+    MINMAX_STEP(0x114, 0xf, 0xe)   // row_shr:4, lanes 4..15
+    MINMAX_STEP(0x118, 0xf, 0xc)   // row_shr:8, lanes 8..15
+    MINMAX_STEP(0x142, 0xa, 0xf)   // row_bcast:15 into rows 1, 3
+    MINMAX_STEP(0x143, 0xc, 0xf)   // row_bcast:31 into rows 2, 3
+#undef MINMAX_STEP
+    return v;
+}
+
 __device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(v), 63);
@@ -1272,10 +1327,12 @@ __device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v)
 // Tile ranges of the sorted pair keys, one thread per key: the first and
 // last key of each run write its range; the run's tile is stamped with the
 // draw's generation so no per-draw clear of the tile arrays is needed.
-__global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t n, uint32_t ntiles,
+__global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t n_max,
+                                                  const unsigned long long *n_dev, uint32_t ntiles,
                                                   uint32_t *tile_start, uint32_t *tile_end, uint32_t *tile_gen,
                                                   uint32_t gen)
 {
+    const uint32_t n = (uint32_t)min((unsigned long long)n_max, *n_dev);
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t k = keys[p];
@@ -1302,6 +1359,7 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
 {
     __shared__ uint32_t s_n[2][4];
     __shared__ uint32_t s_base[2];
+    if (exec_poisoned(ctr)) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool ne = t < ntiles && tile_gen[t] == gen;
     const uint32_t len = ne ? tile_end[t] - tile_start[t] : 0u;
@@ -1348,20 +1406,33 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     }
 }
 
-// per-launch counters (k_vertex's first thread, before any other kernel of the launch)
-__device__ __forceinline__ void reset_counters(Counters *ctr, unsigned long long *scan_tail)
+
+// ---------------------------------------------------------------------------
+// K4: fine raster and per-fragment pipeline
+// ---------------------------------------------------------------------------
+
+
+// per-launch counters (k_launch_begin, before any other kernel of the launch)
+__device__ __forceinline__ void reset_counters(Counters *ctr)
 {
-    *scan_tail = 0;   // the pair-offset scan's last element becomes the total
     ctr->tri_raster = 0;
+    ctr->npairs = 0;
     ctr->nheavy = 0;
     ctr->nlight = 0;
     ctr->unsupported = 0;
     ctr->nclip = 0;
     ctr->noverflow = 0;
     ctr->nbig = 0;
-    ctr->nbig_setup = 0;
     ctr->sticky |= ctr->overflow_full;
     ctr->overflow_full = 0;
+}
+
+__global__ void __launch_bounds__(64) k_launch_begin(Counters *ctr, unsigned long long *scan_tail)
+{
+    if (threadIdx.x == 0) {
+        reset_counters(ctr);
+        if (scan_tail) *scan_tail = 0;   // the pair-offset scan's last element becomes the total
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1483,7 +1554,16 @@ struct RasterConsts {
     bool chain_on;                // full chain with a per-pixel LOD (extension)
     const uint32_t *chain;
     int32_t nlev;
+    // How a pixel's depth test over fragments in submission order reduces
+    // (no stencil): DEPTH_IND, each fragment's test reads only the depth
+    // before the first (mask off, EQUAL, NEVER, ALWAYS, no depth test);
+    // DEPTH_MIN / DEPTH_MAX, the depth a fragment is tested against is the
+    // smallest / largest of the first depth and every earlier fragment's
+    // (LESS, LEQUAL / GREATER, GEQUAL with writes); DEPTH_SERIAL otherwise.
+    uint32_t depth_mode;
 };
+
+enum : uint32_t { DEPTH_IND = 0u, DEPTH_MIN = 1u, DEPTH_MAX = 2u, DEPTH_SERIAL = 3u };
 
 __device__ __forceinline__ int32_t wrap_coord(int32_t x, int32_t w, bool repeat, bool pow2)
 {
@@ -2043,9 +2123,27 @@ __device__ __forceinline__ void list_counts(const RasterParams &P, uint32_t &nh,
     if (P.part == 1u) nt = nh;
 }
 
+// The dispatcher deals consecutive blocks of a grid to the 8 XCDs in turn,
+// each with its own L2.  Blocks are renumbered so that runs of XCD_RUN
+// consecutive list items go to blocks of one XCD, the runs to the XCDs in
+// turn: neighbouring tiles, whose pixel rows share cache lines and whose
+// bins share triangle records, then meet in one L2, and the long items at the
+// front of the list spread over every XCD.  Grids that are not a multiple of
+// NXCD * XCD_RUN blocks keep their numbering.
+constexpr uint32_t NXCD = 8;
+constexpr uint32_t XCD_RUN = 8;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G)
+{
+    if (G % (NXCD * XCD_RUN) != 0u) return b;
+    const uint32_t x = b % NXCD, i = b / NXCD;
+    return ((i / XCD_RUN) * NXCD + x) * XCD_RUN + i % XCD_RUN;
+}
+
 __device__ __forceinline__ uint32_t list_item(uint32_t it0, uint32_t k, uint32_t b, uint32_t G)
 {
-    return it0 + k * G + ((k & 1u) ? G - 1u - b : b);
+    const uint32_t v = xcd_block(b, G);
+    return it0 + k * G + ((k & 1u) ? G - 1u - v : v);
 }
 
 
@@ -2134,6 +2232,12 @@ __device__ __forceinline__ void raster_consts(const SceneRaster &P, RasterConsts
     K.chain_on = P.lod_mode == MTGL_LOD_FULL_CHAIN && P.tex != nullptr;
     K.chain = P.chain;
     K.nlev = P.nlev > 0 ? P.nlev : 1;
+    if (K.stencil_on) K.depth_mode = DEPTH_SERIAL;
+    else if (!K.depth_write) K.depth_mode = DEPTH_IND;
+    else if (rs->depth_func == GL_LESS || rs->depth_func == GL_LEQUAL) K.depth_mode = DEPTH_MIN;
+    else if (rs->depth_func == GL_GREATER || rs->depth_func == GL_GEQUAL) K.depth_mode = DEPTH_MAX;
+    else if (rs->depth_func == GL_NOTEQUAL) K.depth_mode = DEPTH_SERIAL;
+    else K.depth_mode = DEPTH_IND;
 }
 
 // Per-lane counts summed over the wave (bytes as two 32-bit halves) and
@@ -2187,6 +2291,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     __shared__ float s_dep[DCAP];
     __shared__ uint16_t s_slot[DCAP];   // taps << 12 | entry << 6 | pixel
 
+    if (exec_poisoned(P.ctr)) return;   // the launch runs again once its arrays have grown
     RasterConsts K;
     uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
@@ -2236,6 +2341,91 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 #pragma unroll
         for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = reinterpret_cast<const float4 *>(P.rt + ridn)[i];
     }
+    if (quarter != 0u && K.depth_mode != DEPTH_SERIAL) {
+        // One pixel row of a heavy tile, the depth pass with one bin entry
+        // per lane.  Per chunk of 64 entries in submission order and per
+        // pixel of the row: each lane tests its entry's coverage and window
+        // depth, and the fragment's test against the depth before it is
+        // resolved across lanes: in DEPTH_MIN / DEPTH_MAX mode that depth is
+        // the running minimum / maximum of the pixel's depth and every earlier
+        // covered fragment's (a fragment that fails changes nothing; NaN
+        // depths are passed over and fail), so an exclusive prefix min / max
+        // gives every fragment's test exactly as the in-order loop does; in
+        // DEPTH_IND mode every test reads the depth before the chunk.  The
+        // pixel then holds the depth of its last passing fragment.
+        const int32_t yrow = Y0 + r0;
+        const float yf = (float)yrow;
+        const int own0 = TILE * r0;   // lane of the row's first pixel
+        for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
+            const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
+            const bool valid = (uint32_t)lane < cn;
+            float4 q[RT_COV_VEC4];
+#pragma unroll
+            for (int i = 0; i < RT_COV_VEC4; i++) q[i] = qn[i];
+            const uint32_t rid = ridn;
+            if (c0 + CHUNK < e1) {
+                const bool v = (uint32_t)lane < e1 - (c0 + CHUNK);
+                const uint32_t e = v ? c0 + CHUNK + lane : c0 + CHUNK;
+                ridn = P.vals[e];
+#pragma unroll
+                for (int i = 0; i < RT_COV_VEC4; i++) qn[i] = reinterpret_cast<const float4 *>(P.rt + ridn)[i];
+            }
+            const RCov cv = make_cov(q);
+            const float area = REC_F(q, area);
+            const int32_t minX = REC_I(q, minX), maxX = REC_I(q, maxX);
+            const bool live = valid && (area > 0.0f || area < 0.0f) && yrow >= REC_I(q, minY) &&
+                              yrow <= REC_I(q, maxY) && yrow < fb_h;
+            const float sg = area > 0.0f ? 1.0f : -1.0f;
+            const uint32_t taps = texel_taps(K, REC_F(q, lod));
+            const uint32_t pass_add = 4u * taps + K.pass_bytes;
+            for (int j = 0; j < TILE; j++) {
+                const int32_t xj = X0 + j;
+                if (xj >= fb_w) break;
+                const float xf = (float)xj;
+                const float w0e = (xf - cv.fx1) * cv.d0y - (yf - cv.fy1) * cv.d0x;
+                const float w1e = (xf - cv.fx2) * cv.d1y - (yf - cv.fy2) * cv.d1x;
+                const float w2e = (xf - cv.fx0) * cv.d2y - (yf - cv.fy0) * cv.d2x;
+                const bool inside = live && xj >= minX && xj <= maxX && sg * w0e >= 0.0f && sg * w1e >= 0.0f &&
+                                    sg * w2e >= 0.0f;
+                const float bb0 = w0e * cv.inv_area, bb1 = w1e * cv.inv_area, bb2 = w2e * cv.inv_area;
+                const float z = bb0 * cv.z0 + bb1 * cv.z1 + bb2 * cv.z2;
+                float depth;
+                if (K.unit_range) depth = (z + 1.0f) * 0.5f;
+                else depth = (float)((double)((z + 1.0f) * 0.5f) * (K.depth_far - K.depth_near) + K.depth_near);
+                // the pixel's depth before this chunk, from the lane that holds it
+                const float dcur = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(px.d), own0 + j));
+                bool pass;
+                if (K.depth_mode == DEPTH_IND) {
+                    pass = inside && (!K.depth_on || cmp_f(K.depth_cmp, depth, dcur));
+                } else {
+                    const bool mx = K.depth_mode == DEPTH_MAX;
+                    const float idf = mx ? -__builtin_inff() : __builtin_inff();
+                    const float incl = mx ? dpp_incl_minmax<true>(inside ? depth : idf)
+                                          : dpp_incl_minmax<false>(inside ? depth : idf);
+                    float excl = __shfl_up(incl, 1);
+                    if (lane == 0) excl = idf;
+                    const float thr = mx ? fmaxf(dcur, excl) : fminf(dcur, excl);
+                    // a pixel holding NaN passes nothing: every test against it fails
+                    pass = inside && !__builtin_isnan(dcur) && cmp_f(K.depth_cmp, depth, thr);
+                }
+                frag_count += inside ? 1u : 0u;
+                frag_bytes += (inside && K.depth_on ? 4u : 0u) + (pass ? pass_add : 0u);
+                const uint64_t pm = __ballot(pass);
+                if (pm) {
+                    const int lp = 63 - __builtin_clzll(pm);
+                    const float dl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(depth), lp));
+                    const uint32_t wr = 1u + (uint32_t)__builtin_amdgcn_readlane((int)rid, lp);
+                    if (lane == own0 + j) {
+                        if (K.depth_write) {
+                            px.d = dl;
+                            px.dirty_d = true;
+                        }
+                        writer = wr;
+                    }
+                }
+            }
+        }
+    } else
     for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
         const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
         const bool valid = (uint32_t)lane < cn;
@@ -2378,6 +2568,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     __shared__ uint16_t s_slot[FCAP];
     __shared__ uint8_t s_meta[FCAP];
 
+    if (exec_poisoned(P.ctr)) return;   // the launch runs again once its arrays have grown
     RasterConsts K;
     uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
@@ -2980,8 +3171,10 @@ __device__ __forceinline__ Col dp_factor(uint32_t f, Col s, Col d)
     }
 }
 
-__global__ void __launch_bounds__(256) k_draw_pixels(PixDesc P, const uint8_t *src, uint32_t *color, float *depth)
+__global__ void __launch_bounds__(256) k_draw_pixels(PixDesc P, const uint8_t *src, uint32_t *color, float *depth,
+                                                     const Counters *ctr)
 {
+    if (exec_poisoned(ctr)) return;   // an earlier launch of the stream runs again first
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)P.w * (uint32_t)P.h) return;
     const int32_t row = (int32_t)(i / (uint32_t)P.w), col = (int32_t)(i % (uint32_t)P.w);
@@ -3021,8 +3214,9 @@ __global__ void __launch_bounds__(256) k_draw_pixels(PixDesc P, const uint8_t *s
 
 __global__ void __launch_bounds__(256) k_clear(uint32_t *color, float *depth, uint8_t *stencil, int32_t fb_w,
                                                int32_t x0, int32_t y0, int32_t w, int32_t h, uint32_t cv, float dv,
-                                               uint8_t sv)
+                                               uint8_t sv, const Counters *ctr)
 {
+    if (exec_poisoned(ctr)) return;   // an earlier launch of the stream runs again first
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)w * (uint64_t)h) return;
     int32_t y = y0 + (int32_t)(i / (uint64_t)w), x = x0 + (int32_t)(i % (uint64_t)w);
@@ -3500,13 +3694,21 @@ struct Exec {
     uint32_t resident[3] = {256, 256, 256};   // blocks at once: depth<stencil>, depth<no stencil>, general
     Counters *ctr = nullptr;                       // device
     Counters *ctr_host = nullptr;                  // pinned mirror for readback
-    uint8_t *pinned = nullptr;                     // staging for uploads
+    uint8_t *pinned = nullptr;                     // staging for uploads that wait for their copy
     size_t pinned_cap = 0;
+    // staging of launch tables and staged arrays: blocks taken in order and
+    // released together once the stream has drained, so a launch never waits
+    // for the previous one's copy, and a launch that must run again finds its
+    // inputs where they were
+    std::vector<std::pair<uint8_t *, size_t>> ring;
+    size_t ring_blk = 0, ring_off = 0, ring_used = 0;
     hipEvent_t upload_done = nullptr;
     Arena vhot, vcold, rt, rcold, box, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
     uint32_t overflow_cap = 0;
-    Arena keys[2], vals[2], sort_tmp, tile_start, tile_end, tile_gen, tile_list;
+    Arena keys[2], vals[2], sort_tmp, tile_list;   // (tile, record) pairs and their sort
+    Arena tile_start, tile_end, tile_gen;          // per launch-wide tile: bin range, generation of the range
     uint32_t gen = 0;                              // launch generation stamped into tile_gen
+    float pair_ratio = 4.0f;                       // pairs per triangle a launch is sized for (grows)
     uint32_t heavy_bin = HEAVY_BIN;                // bins longer than this go to the front of the tile list
     uint32_t grid_div = 1;                         // raster grids: resident blocks / grid_div
     unsigned long long *stat_slots = nullptr;      // 2 x STAT_SLOTS
@@ -3521,6 +3723,8 @@ struct Exec {
 // batch keeps one per round so that a captured graph replays the same launch.
 struct Launch {
     Arena params;                  // raster/vertex states, staged bytes, scene tables
+    uint8_t *host = nullptr;       // their pinned staging (ring): scene 0's raster state, vertex states and
+    uint8_t *host_rs = nullptr, *host_vs = nullptr, *host_st = nullptr;   // staged bytes
     uint32_t ns = 0;               // scenes with triangles
     uint32_t tri_total = 0, tiles_total = 0;
     uint32_t vblocks = 0, sblocks = 0;
@@ -3554,6 +3758,24 @@ struct LastDraw {
     int32_t tex_w = 0, tex_h = 0;
 };
 
+// A clear, triangle draw or glDrawPixels queued on a context's stream that
+// the host has not yet seen finish.  Until then its inputs stay where its
+// launch staged them (the Exec's ring), so that it can run again when an
+// earlier launch of the stream overflowed (see confirm()).
+struct PendingOp {
+    enum Kind : uint32_t { CLEAR, DRAW, PIXELS };
+    uint32_t seq = 0;
+    Kind kind = CLEAR;
+    uint32_t buffers = 0;          // clear
+    int32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    uint32_t color = 0;
+    float depth = 0.0f;
+    uint8_t stencil = 0;
+    mtgl_draw draw = {};           // draw: raster state, vertex states and staged bytes in the ring
+    uint32_t ntri = 0;
+    mtgl_pixels px = {};           // glDrawPixels: source bytes in the ring
+};
+
 struct mtgl_dev {
     int device = 0;
     int32_t w = 0, h = 0, tiles_x = 0, tiles_y = 0;
@@ -3573,11 +3795,15 @@ struct mtgl_dev {
     LastDraw last;
     mtgl_batch *pending = nullptr; // batch launched into this framebuffer, not yet joined
     bool dirty = false;            // work queued on ex.stream since the last batch launch joined it
+    std::vector<PendingOp> ops;    // queued clears, draws and glDrawPixels not yet confirmed
+    uint32_t seq = 0;              // sequence number of the latest of them (never 0)
+    bool replaying = false;
 };
+
+static int confirm(mtgl_dev *d);
 
 namespace {
 
-constexpr int RC_RETRY_CLIP = 1000;   // internal: the clip pool was too small
 
 int map_err(hipError_t e)
 {
@@ -3641,12 +3867,48 @@ int pinned_reserve(Exec &x, size_t bytes)
 
 unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threads - 1) / threads); }
 
+constexpr size_t RING_BLOCK = 8u << 20;
+constexpr size_t RING_LIMIT = 256u << 20;   // staging a context holds before it waits for its queued work
+
+// bytes of pinned staging from the ring (nullptr when the host is out of pinned memory)
+uint8_t *ring_take(Exec &x, size_t bytes)
+{
+    bytes = (bytes + 255) & ~(size_t)255;
+    while (x.ring_blk < x.ring.size()) {
+        auto &b = x.ring[x.ring_blk];
+        if (x.ring_off + bytes <= b.second) {
+            uint8_t *p = b.first + x.ring_off;
+            x.ring_off += bytes;
+            x.ring_used += bytes;
+            return p;
+        }
+        x.ring_blk++;
+        x.ring_off = 0;
+    }
+    const size_t sz = std::max(bytes, RING_BLOCK);
+    uint8_t *p = nullptr;
+    if (hipHostMalloc((void **)&p, sz, hipHostMallocDefault) != hipSuccess || !p) return nullptr;
+    x.ring.emplace_back(p, sz);
+    x.ring_blk = x.ring.size() - 1;
+    x.ring_off = bytes;
+    x.ring_used += bytes;
+    return p;
+}
+
+// every block free again: only once the stream has drained
+void ring_release(Exec &x)
+{
+    x.ring_blk = 0;
+    x.ring_off = 0;
+    x.ring_used = 0;
+}
+
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // exclusive scan of n values of T; out may alias in; *total_out (when
 // given) receives the sum
 template <typename T>
-int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen, T *total_out = nullptr)
+int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen, T *total_out = nullptr, T *clear = nullptr)
 {
     if (n == 0) return MTGL_OK;
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -3656,13 +3918,13 @@ int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen, T *total_
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n, sums);
     if (nb <= SCAN2_MAX_BLOCKS) {
         hipLaunchKernelGGL(k_scan_apply_short<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n,
-                           (const T *)sums, out, total_out);
+                           (const T *)sums, out, total_out, clear);
         HIPC(hipGetLastError());
         return MTGL_OK;
     }
     hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_THREADS), 0, x.stream, sums, nb, total_out);
     hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, x.stream, in, n,
-                       (const T *)sums, out);
+                       (const T *)sums, out, clear);
     HIPC(hipGetLastError());
     return MTGL_OK;
 }
@@ -3749,7 +4011,7 @@ void exec_free(Exec &x)
     if (x.stream) (void)hipStreamSynchronize(x.stream);
     Arena *arenas[] = {&x.vhot, &x.vcold, &x.rt, &x.rcold, &x.box, &x.n_rt, &x.rec_base, &x.tri_tiles, &x.pair_base,
                        &x.scan_tmp, &x.clip_list, &x.keys[0], &x.keys[1], &x.vals[0], &x.vals[1], &x.sort_tmp,
-                       &x.tile_gen, &x.tile_start, &x.tile_end, &x.tile_list};
+                       &x.tile_list, &x.tile_start, &x.tile_end, &x.tile_gen};
     for (Arena *a : arenas) arena_free(*a);
     for (auto &pr : x.pending_events) {
         (void)hipEventDestroy(pr.first);
@@ -3762,6 +4024,9 @@ void exec_free(Exec &x)
     if (x.stat_slots) (void)hipFree(x.stat_slots);
     if (x.ctr_host) (void)hipHostFree(x.ctr_host);
     if (x.pinned) (void)hipHostFree(x.pinned);
+    for (auto &b : x.ring) (void)hipHostFree(b.first);
+    x.ring.clear();
+    ring_release(x);
     if (x.upload_done) (void)hipEventDestroy(x.upload_done);
     if (x.stream) (void)hipStreamDestroy(x.stream);
     x.ctr = nullptr;
@@ -3780,7 +4045,10 @@ int exec_stats(Exec &x, mtgl_dev_stats *out)
     HIPC(hipStreamSynchronize(x.stream));
     int rc = resolve_timing(x);
     if (rc) return rc;
+    HIPC(hipMemcpy(x.ctr_host, x.ctr, sizeof(Counters), hipMemcpyDeviceToHost));
     *out = x.stats;   // fragments: those of point and line draws so far, plus the triangles' below
+    out->triangles_raster += x.ctr_host->acc_tri_raster;
+    out->bin_pairs += x.ctr_host->acc_bin_pairs;
     out->frag_bytes = 0;
     for (uint32_t i = 0; i < STAT_SLOTS; i++) {
         out->fragments += x.stat_host[2 * i];
@@ -3880,9 +4148,9 @@ VSlots vertex_slots(const mtgl_dev *d, const mtgl_draw *dr)
     return r;
 }
 
-// Vertex, triangle and tile totals of a set of scenes and the buffers they need.
+// Triangle and tile totals of a set of scenes and the buffers they need.
 struct Sizes {
-    uint64_t vtx = 0, tri = 0, tiles = 0;
+    uint64_t tri = 0, tiles = 0;
     bool cold = false;
 };
 
@@ -3894,7 +4162,6 @@ Sizes scene_sizes(const SceneIn *in, uint32_t n)
         if (!draw_mode_ok(dr->mode)) continue;
         const uint32_t ntri = tri_count(dr->mode, dr->count);
         if (ntri == 0) continue;
-        z.vtx += vertex_slots(in[i].dev, dr).n;
         z.tri += ntri;
         z.tiles += (uint64_t)in[i].dev->tiles_x * (uint64_t)in[i].dev->tiles_y;
         z.cold |= needs_cold(dr->raster);
@@ -3906,8 +4173,6 @@ int reserve_scene_buffers(Exec &x, const Sizes &z)
 {
     int rc;
     const size_t t = (size_t)z.tri;
-    if ((rc = arena_reserve(x, x.vhot, (size_t)z.vtx * sizeof(VHot)))) return rc;
-    if (z.cold && (rc = arena_reserve(x, x.vcold, (size_t)z.vtx * sizeof(VCold)))) return rc;
     // record pool: one slot per triangle plus an overflow pool for clipped ones
     if (x.overflow_cap < 4096u) x.overflow_cap = 4096u;
     x.overflow_cap = std::max<uint32_t>(x.overflow_cap, std::min<uint32_t>((uint32_t)(z.tri / 8u), 1u << 24));
@@ -3916,9 +4181,9 @@ int reserve_scene_buffers(Exec &x, const Sizes &z)
     if ((rc = arena_reserve(x, x.box, (t + x.overflow_cap) * sizeof(RBox)))) return rc;
     if ((rc = arena_reserve(x, x.n_rt, t * sizeof(uint32_t)))) return rc;
     if ((rc = arena_reserve(x, x.rec_base, t * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(x, x.clip_list, t * sizeof(uint32_t)))) return rc;
     if ((rc = arena_reserve(x, x.tri_tiles, (t + 1) * sizeof(unsigned long long)))) return rc;
     if ((rc = arena_reserve(x, x.pair_base, (t + 1) * sizeof(unsigned long long)))) return rc;
-    if ((rc = arena_reserve(x, x.clip_list, t * sizeof(uint32_t)))) return rc;
     return MTGL_OK;
 }
 
@@ -3979,17 +4244,12 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     bytes += up256(4 * (size_t)(ns + 1) * sizeof(uint32_t));
 
     const Sizes z = scene_sizes(in, n);
-    uint64_t vb = 0, sb = 0;
-    for (const Plan &p : plan) {
-        vb += blocks_for(vertex_slots(p.s->dev, p.s->dr).n, GEOM_BLOCK);
-        sb += blocks_for(p.ntri, GEOM_BLOCK);
-    }
-    if (z.vtx > 0x7FFFFFFFull || z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / HEAVY_SPLIT ||
-        vb > 0x7FFFFFFFull)
-        return MTGL_ERR_OUT_OF_MEMORY;
+    uint64_t sb = 0;
+    for (const Plan &p : plan) sb += blocks_for(p.ntri, GEOM_BLOCK);
+    if (z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / HEAVY_SPLIT) return MTGL_ERR_OUT_OF_MEMORY;
     L.tri_total = (uint32_t)z.tri;
     L.tiles_total = (uint32_t)z.tiles;
-    L.vblocks = (uint32_t)vb;
+    L.vblocks = 0;
     L.sblocks = (uint32_t)sb;
     L.any_cold = z.cold;
 
@@ -4002,17 +4262,21 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         }
     }
     if ((rc = arena_reserve(x, L.params, bytes))) return rc;
-    if ((rc = pinned_reserve(x, bytes))) return rc;
+    uint8_t *host = ring_take(x, bytes);
+    if (!host) return MTGL_ERR_OUT_OF_MEMORY;
+    L.host = host;
+    L.host_rs = host + plan[0].off_rs;
+    L.host_vs = host + plan[0].off_vs;
+    L.host_st = host + plan[0].off_st;
 
     const uint32_t ttot = L.tri_total;
-    uint8_t *host = x.pinned;
     const uint8_t *dev = (const uint8_t *)L.params.ptr;
     VertexParams *vp = reinterpret_cast<VertexParams *>(host + off_vp);
     SetupParams *sp = reinterpret_cast<SetupParams *>(host + off_sp);
     SceneRaster *sr = reinterpret_cast<SceneRaster *>(host + off_sr);
     uint32_t *pre = reinterpret_cast<uint32_t *>(host + off_pre);
     uint32_t *vbp = pre, *sbp = pre + (ns + 1), *tp = pre + 2 * (ns + 1), *tlp = pre + 3 * (ns + 1);
-    uint32_t voff = 0, toff = 0, tileoff = 0, vbo = 0, sbo = 0;
+    uint32_t toff = 0, tileoff = 0, sbo = 0;
     for (uint32_t s = 0; s < ns; s++) {
         const Plan &p = plan[s];
         mtgl_dev *d = p.s->dev;
@@ -4024,11 +4288,10 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         const uint8_t *staged_dev = dev + p.off_st;
         const uint64_t stb = dr->staged_bytes;
 
-        const VSlots vs = vertex_slots(d, dr);
         VertexParams v = {};
-        v.count = vs.n;
-        v.unique = vs.unique;
-        v.ubase = vs.base;
+        v.count = dr->count;
+        v.unique = 0;
+        v.ubase = 0;
         v.first = dr->first;
         v.index_type = dr->index_type;
         v.n_vstates = dr->n_vstates;
@@ -4043,18 +4306,15 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         memcpy(v.cur_texcoord, dr->cur_texcoord, sizeof(v.cur_texcoord));
         memcpy(v.cur_normal, dr->cur_normal, sizeof(v.cur_normal));
         v.vstates = (const mtgl_vertex_state *)(dev + p.off_vs);
-        v.out_hot = as<VHot>(x.vhot) + voff;
-        v.out_cold = p.cold ? as<VCold>(x.vcold) + voff : nullptr;
+        v.out_hot = nullptr;
+        v.out_cold = nullptr;
         vp[s] = v;
 
         SetupParams q = {};
         q.mode = dr->mode;
         q.count = dr->count;
-        q.unique = vs.unique;
-        q.ubase = vs.base;
-        q.nslots = vs.n;
-        q.index_type = dr->index_type;
-        q.indices = v.indices;
+        q.vp = (const VertexParams *)(dev + off_vp) + s;
+        q.cold = p.cold ? 1u : 0u;
         q.ntri = p.ntri;
         q.tri_off = toff;
         q.scene = s;
@@ -4065,8 +4325,6 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         q.tex_w = p.tx ? p.tx->w : 0;
         q.tex_h = p.tx ? p.tx->h : 0;
         q.rs = rs_dev;
-        q.vhot = as<VHot>(x.vhot) + voff;
-        q.vcold = p.cold ? as<VCold>(x.vcold) + voff : nullptr;
         q.rt = as<RTri>(x.rt) + toff;
         q.rcold = L.any_cold ? as<RCold>(x.rcold) + toff : nullptr;
         q.box = as<RBox>(x.box) + toff;
@@ -4105,22 +4363,19 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         sr[s] = r;
         L.kinds |= 1u << r.kind;
 
-        vbp[s] = vbo;
+        vbp[s] = 0;
         sbp[s] = sbo;
         tp[s] = toff;
         tlp[s] = tileoff;
-        voff += vs.n;
         toff += p.ntri;
         tileoff += (uint32_t)(d->tiles_x * d->tiles_y);
-        vbo += blocks_for(vs.n, GEOM_BLOCK);
         sbo += blocks_for(p.ntri, GEOM_BLOCK);
     }
-    vbp[ns] = vbo;
+    vbp[ns] = 0;
     sbp[ns] = sbo;
     tp[ns] = toff;
     tlp[ns] = tileoff;
     HIPC(hipMemcpyAsync(L.params.ptr, host, bytes, hipMemcpyHostToDevice, x.stream));
-    HIPC(hipEventRecord(x.upload_done, x.stream));
     L.vp = (const VertexParams *)(dev + off_vp);
     L.sp = (const SetupParams *)(dev + off_sp);
     L.sr = (const SceneRaster *)(dev + off_sr);
@@ -4143,54 +4398,35 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     return MTGL_OK;
 }
 
-// The pipeline over the scenes of L.  A sizing run (frozen false) reads the
-// pair total back and sizes the binning buffers; a frozen run (a replay, or
-// one being captured into a graph) allocates, synchronises and reads back
-// nothing and takes the pair total of the sizing run.
-int run_launch(Exec &x, Launch &L, bool frozen)
+// The pipeline over the scenes of L, queued on x.stream without waiting for
+// the GPU.  The pair arrays are sized from x.pair_ratio (a frozen run, a
+// replay or one being captured into a graph, allocates nothing and uses the
+// capacity of its sizing run); a launch that needs more pairs, or more clip
+// slots than the pool holds, poisons the Exec with seq (k_bin_scatter) and
+// writes nothing: the host learns of it when it next waits for the stream and
+// runs the launch again with larger arrays.
+int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
 {
     if (L.ns == 0) return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
     int rc;
-    const uint32_t ttot = L.tri_total;
-    hipLaunchKernelGGL(k_vertex, dim3(L.vblocks), dim3(GEOM_BLOCK), 0, x.stream, L.vp, L.vblock_pre, L.vmap, L.ns, x.ctr,
-                       as<unsigned long long>(x.tri_tiles) + ttot);
-    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(GEOM_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
-    hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
-                       (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
-    HIPC(hipGetLastError());
-
-    // ---- K3: pair offsets and total, then expand / sort / segment
-    unsigned long long *pair_base = as<unsigned long long>(x.pair_base);
-    if ((rc = device_scan<unsigned long long>(x, as<unsigned long long>(x.tri_tiles), pair_base, (uint64_t)ttot + 1,
-                                              frozen, &x.ctr->npairs)))
-        return rc;
-    uint64_t npairs;
+    const uint32_t ttot = L.tri_total, ntiles = L.tiles_total;
     if (!frozen) {
-        HIPC(hipMemcpyAsync(x.ctr_host, x.ctr, sizeof(Counters), hipMemcpyDeviceToHost, x.stream));
-        HIPC(hipStreamSynchronize(x.stream));
-        // the clipped triangles asked for more slots than the pool held:
-        // nothing downstream ran; the caller grows the pool and runs again
-        if (x.ctr_host->overflow_full & 1u) return RC_RETRY_CLIP;
-        npairs = x.ctr_host->npairs;
-        L.unsupported |= x.ctr_host->unsupported != 0;
-        x.stats.triangles_raster += x.ctr_host->tri_raster;
-        if (npairs > 0x7FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
-        L.pairs_cap = npairs;
-        x.stats.bin_pairs += npairs;
-    } else {
-        npairs = L.pairs_cap;
+        const double want = std::min(2147483647.0, (double)x.pair_ratio * (double)ttot + 4096.0);
+        for (int k = 0; k < 2; k++) {
+            if ((rc = arena_reserve(x, x.keys[k], (size_t)want * 4))) return rc;
+            if ((rc = arena_reserve(x, x.vals[k], (size_t)want * 4))) return rc;
+        }
+        size_t c = x.keys[0].cap;
+        for (int k = 0; k < 2; k++) c = std::min(c, std::min(x.keys[k].cap, x.vals[k].cap));
+        L.pairs_cap = std::min<uint64_t>(c / 4, 0x7FFFFFFFull);
     }
-    if (npairs == 0) return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
-    const uint32_t n = (uint32_t)npairs;
-    for (int k = 0; k < 2; k++) {
-        if ((rc = arena_reserve(x, x.keys[k], (size_t)n * 4, frozen))) return rc;
-        if ((rc = arena_reserve(x, x.vals[k], (size_t)n * 4, frozen))) return rc;
-    }
-    const uint32_t ntiles = L.tiles_total;
+    const uint32_t cap = (uint32_t)L.pairs_cap;
     if ((rc = arena_reserve(x, x.tile_start, (size_t)ntiles * 4, frozen))) return rc;
     if ((rc = arena_reserve(x, x.tile_end, (size_t)ntiles * 4, frozen))) return rc;
     const uint32_t list_cap = HEAVY_SPLIT * ntiles;   // up to HEAVY_SPLIT items per tile
     if ((rc = arena_reserve(x, x.tile_list, (size_t)list_cap * sizeof(uint4), frozen))) return rc;
+    const uint32_t nblocks = (cap + SORT_TILE - 1) / SORT_TILE;
+    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)RADIX * nblocks * 4, frozen))) return rc;
     if (!frozen) {
         if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
             if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
@@ -4204,43 +4440,55 @@ int run_launch(Exec &x, Launch &L, bool frozen)
         hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream, as<uint32_t>(x.tile_gen),
                            ntiles, 0u);
     }
+
+    // ---- K2: set-up, with the vertex transform
+    unsigned long long *tri_tiles = as<unsigned long long>(x.tri_tiles);
+    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr, tri_tiles + ttot);
+    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(GEOM_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
+    hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
+                       (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
+    HIPC(hipGetLastError());
+
+    // ---- K3: pair offsets and total (read on the device: the host never
+    // waits), expansion in submission order, stable sort by tile, tile
+    // ranges and the work list
+    unsigned long long *pair_base = as<unsigned long long>(x.pair_base);
+    if ((rc = device_scan<unsigned long long>(x, tri_tiles, pair_base, (uint64_t)ttot + 1, frozen, &x.ctr->npairs)))
+        return rc;
+    const unsigned long long *np = &x.ctr->npairs;
     uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
     uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
     // the clip queue has been consumed: it holds the long triangles now
-    hipLaunchKernelGGL(k_expand, dim3(blocks_for(ttot, GEOM_BLOCK)), dim3(GEOM_BLOCK), 0, x.stream,
+    hipLaunchKernelGGL(k_expand, dim3(std::max(1u, blocks_for(ttot, GEOM_BLOCK))), dim3(GEOM_BLOCK), 0, x.stream,
                        (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
-                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
-                       keys, vals, as<uint32_t>(x.clip_list), x.ctr);
-    // a sized run whose setup queued no long triangle has nothing for it (a
-    // frozen run may also have a tail to fill)
-    if (frozen || x.ctr_host->nbig_setup != 0)
-        hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
+                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
+                       keys, vals, as<uint32_t>(x.clip_list), x.ctr, seq);
+    hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
                        x.stream, (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
-                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)n,
+                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
                        keys, vals, (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
-    // stable LSD radix sort by tile: pairs keep submission order within a tile;
-    // the all-ones key of unused capacity stays above every tile
+    // stable LSD radix sort by tile: pairs keep submission order within a tile
     int key_bits = 1;
     while ((1u << key_bits) <= ntiles) key_bits++;
-    const uint32_t nblocks = (n + SORT_TILE - 1) / SORT_TILE;
-    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)RADIX * nblocks * 4, frozen))) return rc;
     uint32_t *counts = as<uint32_t>(x.sort_tmp);
     for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, n, shift,
-                           nblocks, counts);
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, cap, np,
+                           shift, nblocks, counts);
         if ((rc = device_scan<uint32_t>(x, counts, counts, (uint64_t)RADIX * nblocks, frozen))) return rc;
         hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
-                           (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
+                           (const uint32_t *)vals, cap, np, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
         std::swap(keys, keys2);
         std::swap(vals, vals2);
     }
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, n, ntiles,
-                       as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen), x.gen);
+    hipLaunchKernelGGL(k_segments, dim3(blocks_for(cap, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, cap, np,
+                       ntiles, as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen), x.gen);
     hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
                        (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
                        (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
                        L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
     HIPC(hipGetLastError());
+    // raster grids: no more blocks than items, which the estimated pairs bound
+    const uint64_t est_items = std::min<uint64_t>(cap, (uint64_t)((double)x.pair_ratio * (double)ttot) + 64u);
 
     // ---- K4
     RasterParams rp = {};
@@ -4255,8 +4503,6 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     rp.ntiles = ntiles;
     rp.list_cap = list_cap;
     rp.part = 0;
-    rp.tile_start = as<uint32_t>(x.tile_start);
-    rp.tile_end = as<uint32_t>(x.tile_end);
     rp.ctr = x.ctr;
     rp.stats = x.stat_slots;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -4267,11 +4513,14 @@ int run_launch(Exec &x, Launch &L, bool frozen)
     }
     // persistent grids over the work list (longest items first): at most one
     // block per item, at most as many as are resident at once
-    const uint32_t nlist = (uint32_t)std::min<uint64_t>((uint64_t)n, 4ull * ntiles);
+    const uint32_t nlist = (uint32_t)std::min<uint64_t>(est_items, (uint64_t)HEAVY_SPLIT * ntiles);
     auto grid = [&](int k) {
-        // statistics take one slot per block: grids stay within STAT_SLOTS
+        // statistics take one slot per block: grids stay within STAT_SLOTS;
+        // larger grids are whole runs of the XCD interleave (xcd_block)
         const uint32_t g = std::min<uint32_t>(3u * x.resident[k] / x.grid_div, STAT_SLOTS);
-        return dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nlist, g)));
+        uint32_t n = std::max<uint32_t>(1u, std::min<uint32_t>(nlist, g));
+        if (n >= 64u) n &= ~63u;
+        return dim3(n);
     };
     const uint32_t LW = KIND_LAST_WRITER, ST = KIND_STENCIL, CO = KIND_COLOR;
     const uint32_t lw_stencil = (1u << (LW | ST)) | (1u << (LW | ST | CO));
@@ -4374,9 +4623,10 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
 
     unsigned long long *counts = as<unsigned long long>(x.tri_tiles);
     unsigned long long *base = as<unsigned long long>(x.pair_base);
+    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr, (unsigned long long *)nullptr);
     hipLaunchKernelGGL(k_vertex, dim3(blocks_for(dr->count, 256)), dim3(256), 0, x.stream,
                        (const VertexParams *)(dev + off_vp), (const uint32_t *)(dev + off_pre), (const uint32_t *)nullptr,
-                       1u, x.ctr, counts + units);
+                       1u);
 
     PrimParams P = {};
     SceneRaster &S = P.S;
@@ -4438,11 +4688,12 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
     if ((rc = arena_reserve(x, x.sort_tmp, (size_t)RADIX * nblocks * 4))) return rc;
     uint32_t *sc = as<uint32_t>(x.sort_tmp);
     for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, n, shift,
-                           nblocks, sc);
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, n,
+                           (const unsigned long long *)nullptr, shift, nblocks, sc);
         if ((rc = device_scan<uint32_t>(x, sc, sc, (uint64_t)RADIX * nblocks, false))) return rc;
         hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
-                           (const uint32_t *)vals, n, shift, nblocks, (const uint32_t *)sc, keys2, vals2);
+                           (const uint32_t *)vals, n, (const unsigned long long *)nullptr, shift, nblocks,
+                           (const uint32_t *)sc, keys2, vals2);
         std::swap(keys, keys2);
         std::swap(vals, vals2);
     }
@@ -4454,16 +4705,35 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
 }
 
 // prepare + sizing run, growing the clip overflow pool once when it was too small
-int execute_scenes(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
+int execute_scenes(Exec &x, Launch &L, const SceneIn *in, uint32_t n, uint32_t seq)
 {
-    for (int attempt = 0; attempt < 3; attempt++) {
-        int rc = prepare_launch(x, L, in, n);
-        if (rc) return rc;
-        rc = run_launch(x, L, false);
-        if (rc != RC_RETRY_CLIP) return rc;
-        x.overflow_cap = std::max<uint32_t>(x.ctr_host->noverflow, x.overflow_cap * 2u);
+    int rc = prepare_launch(x, L, in, n);
+    if (rc) return rc;
+    return run_launch(x, L, false, seq);
+}
+
+// After a launch poisoned x (it needed more pairs or clip slots than x
+// held): larger arrays for its next run.  ntri: the launch's triangles.
+void exec_grow(Exec &x, const Counters &c, uint32_t ntri)
+{
+    if (c.poison_flags & 1u) x.overflow_cap = std::min<uint32_t>(std::max<uint32_t>(x.overflow_cap * 2u, 4096u), 1u << 26);
+    if (c.poison_flags & 2u) {
+        const double r = 1.25 * (double)c.poison_pairs / (double)std::max<uint32_t>(ntri, 1u) + 1.0;
+        x.pair_ratio = (float)std::max<double>(x.pair_ratio, std::min(r, 2.0e9));
     }
-    return MTGL_ERR_DEVICE;
+}
+
+// Reads x's counters back (the stream must have drained) and clears the
+// poison it carries, returning it (0: none).
+int exec_take_poison(Exec &x, uint32_t &poison)
+{
+    HIPC(hipMemcpy(x.ctr_host, x.ctr, sizeof(Counters), hipMemcpyDeviceToHost));
+    poison = x.ctr_host->poison;
+    if (poison) {
+        HIPC(hipMemsetAsync((uint8_t *)x.ctr + offsetof(Counters, poison), 0, sizeof(uint32_t), x.stream));
+        HIPC(hipStreamSynchronize(x.stream));
+    }
+    return MTGL_OK;
 }
 
 }  // namespace
@@ -4623,8 +4893,10 @@ int batch_run(mtgl_batch *b, bool frozen)
             HIPC(hipGetLastError());
         }
         if (R.scenes.empty()) continue;
-        rc = frozen ? run_launch(x, R.launch, true)
-                    : execute_scenes(x, R.launch, R.scenes.data(), (uint32_t)R.scenes.size());
+        // sequence number of a round's launch: its index + 1
+        const uint32_t seq = (uint32_t)(&R - b->rounds.data()) + 1u;
+        rc = frozen ? run_launch(x, R.launch, true, seq)
+                    : execute_scenes(x, R.launch, R.scenes.data(), (uint32_t)R.scenes.size(), seq);
         if (rc && rc != MTGL_ERR_UNSUPPORTED) return rc;
     }
     return MTGL_OK;
@@ -4658,11 +4930,11 @@ int batch_reserve(mtgl_batch *b)
     Sizes m;
     for (BatchRound &R : b->rounds) {
         const Sizes z = scene_sizes(R.scenes.data(), (uint32_t)R.scenes.size());
-        m.vtx = std::max(m.vtx, z.vtx);
         m.tri = std::max(m.tri, z.tri);
+        m.tiles = std::max(m.tiles, z.tiles);
         m.cold |= z.cold;
     }
-    if (m.vtx > 0x7FFFFFFFull || m.tri > 0x0FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
+    if (m.tri > 0x0FFFFFFFull) return MTGL_ERR_OUT_OF_MEMORY;
     return m.tri ? reserve_scene_buffers(b->ex, m) : MTGL_OK;
 }
 
@@ -4671,16 +4943,30 @@ int batch_build_inner(mtgl_batch *b, int autotune)
     Exec &x = b->ex;
     int rc;
     if ((rc = batch_reserve(b))) return rc;
-    // sizing run: every round in order, reading its pair total back
-    if ((rc = batch_run(b, false))) return rc;
-    HIPC(hipStreamSynchronize(x.stream));
+    // sizing runs: every round in order, until none needs larger arrays than
+    // the earlier rounds grew them to
+    for (int attempt = 0;; attempt++) {
+        if ((rc = batch_run(b, false))) return rc;
+        HIPC(hipStreamSynchronize(x.stream));
+        uint32_t poison = 0;
+        if ((rc = exec_take_poison(x, poison))) return rc;
+        if (!poison) break;
+        if (attempt >= 16) return MTGL_ERR_OUT_OF_MEMORY;
+        exec_grow(x, *x.ctr_host, b->rounds[poison - 1].launch.tri_total);
+        ring_release(x);
+    }
     // a later round may have grown a buffer that an earlier round's tables
     // point into: prepare every round's tables again, at the final sizes
+    size_t c = x.keys[0].cap;
+    for (int k = 0; k < 2; k++) c = std::min(c, std::min(x.keys[k].cap, x.vals[k].cap));
+    const uint64_t cap = std::min<uint64_t>(c / 4, 0x7FFFFFFFull);
     for (BatchRound &R : b->rounds) {
         if (R.scenes.empty()) continue;
         if ((rc = prepare_launch(x, R.launch, R.scenes.data(), (uint32_t)R.scenes.size()))) return rc;
+        R.launch.pairs_cap = cap;
     }
     HIPC(hipStreamSynchronize(x.stream));
+    ring_release(x);
     b->info.heavy_bin = x.heavy_bin;
     b->info.grid_div = x.grid_div;
     if (autotune) {
@@ -4741,6 +5027,7 @@ int batch_build(mtgl_batch *b, int autotune)
     for (mtgl_dev *d : b->devs) {
         int rc = dev_join(d);
         if (rc) return rc;
+        if ((rc = confirm(d))) return rc;
         HIPC(hipStreamSynchronize(d->ex.stream));
         d->dirty = false;
     }
@@ -4769,6 +5056,29 @@ MTGL_API const char *mtgl_dev_backend(void)
 {
     return "hip:gfx950 (K1 vertex, K2 setup, K3 binner, K4 tile raster)";
 }
+
+// ---- the log of queued clears, draws and glDrawPixels (see PendingOp) ----
+
+static uint32_t next_seq(mtgl_dev *d)
+{
+    if (++d->seq == 0u) d->seq = 1u;
+    return d->seq;
+}
+
+static int launch_clear(mtgl_dev *d, const PendingOp &op)
+{
+    const int32_t w = op.x1 - op.x0, h = op.y1 - op.y0;
+    const uint64_t n = (uint64_t)w * (uint64_t)h;
+    uint32_t *c = (op.buffers & MTGL_BUF_COLOR) ? d->color : nullptr;
+    float *z = (op.buffers & MTGL_BUF_DEPTH) ? d->depth : nullptr;
+    uint8_t *st = (op.buffers & MTGL_BUF_STENCIL) ? d->stencil : nullptr;
+    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->ex.stream, c, z, st, d->w, op.x0, op.y0, w,
+                       h, op.color, op.depth, op.stencil, (const Counters *)d->ex.ctr);
+    HIPC(hipGetLastError());
+    d->dirty = true;
+    return MTGL_OK;
+}
+
 
 MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
 {
@@ -4849,6 +5159,7 @@ MTGL_API int mtgl_dev_finish(mtgl_dev *d)
     if (!d) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     return MTGL_OK;
 }
@@ -4886,11 +5197,19 @@ MTGL_API int mtgl_dev_clear(mtgl_dev *d, uint32_t buffers, int32_t x0, int32_t y
     }
     int rc = dev_join(d);
     if (rc) return rc;
-    const uint64_t n = (uint64_t)c.w * (uint64_t)c.h;
-    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->ex.stream, c.color, c.depth, c.stencil, d->w,
-                       x0, y0, c.w, c.h, color, depth, stencil);
-    HIPC(hipGetLastError());
-    d->dirty = true;
+    PendingOp op;
+    op.seq = next_seq(d);
+    op.kind = PendingOp::CLEAR;
+    op.buffers = buffers;
+    op.x0 = x0;
+    op.y0 = y0;
+    op.x1 = x1;
+    op.y1 = y1;
+    op.color = color;
+    op.depth = depth;
+    op.stencil = stencil;
+    if ((rc = launch_clear(d, op))) return rc;
+    d->ops.push_back(op);
     return MTGL_OK;
 }
 
@@ -4911,6 +5230,7 @@ MTGL_API int mtgl_dev_read(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, in
     if (w == 0 || h == 0) return MTGL_OK;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy2D(dst, (size_t)w * esz, base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, (size_t)w * esz,
                      (size_t)h, hipMemcpyDeviceToHost));
@@ -4929,6 +5249,7 @@ MTGL_API int mtgl_dev_read_pixels(mtgl_dev *d, int32_t x, int32_t y, int32_t wid
     if (width <= 0 || height <= 0 || bpp == 0) return MTGL_OK;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     const size_t row_bytes = (size_t)width * (size_t)bpp;
     const int32_t band = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)height, READBACK_BAND / row_bytes));
     if ((rc = arena_reserve(d->ex, d->readback, row_bytes * (size_t)band))) return rc;
@@ -4951,6 +5272,7 @@ MTGL_API int mtgl_dev_present(mtgl_dev *d, void *dst, int64_t pitch)
     if (!d || !dst || pitch < (int64_t)d->w * 4) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy2D(dst, (size_t)pitch, d->color, (size_t)d->w * 4, (size_t)d->w * 4, (size_t)d->h,
                      hipMemcpyDeviceToHost));
@@ -4966,6 +5288,7 @@ MTGL_API int mtgl_dev_write(mtgl_dev *d, uint32_t which, int32_t x, int32_t y, i
     if (w == 0 || h == 0) return MTGL_OK;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy2D(base + ((size_t)y * d->w + x) * esz, (size_t)d->w * esz, src, (size_t)w * esz, (size_t)w * esz,
                      (size_t)h, hipMemcpyHostToDevice));
@@ -4979,6 +5302,7 @@ MTGL_API int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, 
         return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     invalidate_batches(d);
     const size_t n = (size_t)w * (size_t)h;
     uint32_t *px = nullptr, *m1 = nullptr;
@@ -5044,6 +5368,7 @@ MTGL_API int mtgl_dev_tex_free(mtgl_dev *d, uint32_t id)
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     DevTex &t = d->tex[id];
     if (t.px || t.m1) invalidate_batches(d);
@@ -5059,6 +5384,7 @@ MTGL_API int mtgl_dev_buffer_data(mtgl_dev *d, uint32_t id, int64_t size, const 
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     DevBuf &b = d->buf[id];
     if (b.data) invalidate_batches(d);
@@ -5087,6 +5413,7 @@ MTGL_API int mtgl_dev_buffer_sub_data(mtgl_dev *d, uint32_t id, int64_t offset, 
     if (!b.data || offset + size > b.size) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     HIPC(hipMemcpy(b.data + offset, data, (size_t)size, hipMemcpyHostToDevice));
     return MTGL_OK;
@@ -5097,6 +5424,7 @@ MTGL_API int mtgl_dev_buffer_free(mtgl_dev *d, uint32_t id)
     if (!d || id == 0 || id > MTGL_MAX_OBJECTS) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     DevBuf &b = d->buf[id];
     if (b.data) invalidate_batches(d);
@@ -5111,6 +5439,7 @@ MTGL_API int mtgl_dev_segment_data(mtgl_dev *d, uint32_t id, int64_t size, const
     if (size <= 0) return mtgl_dev_segment_free(d, id);
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     DevBuf &b = d->segs[id];
     // a recorded batch holds the old allocation's address
@@ -5140,6 +5469,7 @@ MTGL_API int mtgl_dev_segment_free(mtgl_dev *d, uint32_t id)
     if (it == d->segs.end()) return MTGL_OK;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     HIPC(hipStreamSynchronize(d->ex.stream));
     invalidate_batches(d);
     if (it->second.data) HIPC(hipFree(it->second.data));
@@ -5158,6 +5488,8 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         d->ex.stats.draws++;
         int rc = dev_join(d);
         if (rc) return rc;
+        // the point and line path waits for its own counts: the queued work first
+        if ((rc = confirm(d))) return rc;
         d->last.valid = false;
         rc = run_prims(d, dr, cls);
         d->dirty = true;
@@ -5189,11 +5521,27 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     int rc = dev_join(d);
     if (rc) return rc;
     if (d->ex.pending_events.size() > 4096) (void)resolve_timing(d->ex);
+    const size_t staging = sizeof(mtgl_raster_state) + (size_t)dr->n_vstates * sizeof(mtgl_vertex_state) +
+                           (size_t)dr->staged_bytes + 4096;
+    if (d->ex.ring_used + staging > RING_LIMIT && (rc = confirm(d))) return rc;
     SceneIn in = {d, dr};
     d->one.pairs_cap = 0;
     d->last.valid = false;
-    rc = execute_scenes(d->ex, d->one, &in, 1);
+    const uint32_t seq = next_seq(d);
+    rc = execute_scenes(d->ex, d->one, &in, 1, seq);
     d->dirty = true;
+    if ((rc == MTGL_OK || rc == MTGL_ERR_UNSUPPORTED) && d->one.ns == 1) {
+        // logged with its inputs where the launch staged them
+        PendingOp op;
+        op.seq = seq;
+        op.kind = PendingOp::DRAW;
+        op.draw = *dr;
+        op.draw.raster = (const mtgl_raster_state *)d->one.host_rs;
+        op.draw.vstates = (const mtgl_vertex_state *)d->one.host_vs;
+        op.draw.staged = dr->staged_bytes ? d->one.host_st : nullptr;
+        op.ntri = ntri;
+        d->ops.push_back(op);
+    }
     if (rc == MTGL_OK && d->one.ns == 1) {
         LastDraw &l = d->last;
         const DevTex *tx = bound_texture(d, dr->raster);
@@ -5213,31 +5561,13 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     return rc;
 }
 
-MTGL_API int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
+
+static int launch_pixels(mtgl_dev *d, const mtgl_pixels &px, uint64_t bytes);
+
+// the launch both glDrawPixels entry points share: src holds the image in HBM
+static int draw_pixels_kernel(mtgl_dev *d, const mtgl_pixels *p, const uint8_t *src, uint64_t n, bool timed)
 {
-    if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
-    uint32_t bpp;
-    switch (p->format) {
-    case GL_RGBA: bpp = 4; break;
-    case GL_RGB: bpp = 3; break;
-    case GL_LUMINANCE: bpp = 1; break;
-    case GL_LUMINANCE_ALPHA: bpp = 2; break;
-    default: return MTGL_OK;   // the reference reads nothing of other formats
-    }
-    const uint64_t n = (uint64_t)p->width * (uint64_t)p->height;
-    if (n == 0) return MTGL_OK;
-    if (n > 0xFFFFFFFFull) return MTGL_ERR_INVALID;
-    // a recorded batch replays clears and draws only: out of order otherwise
-    if (g_recording) return MTGL_ERR_UNSUPPORTED;
-    int rc = dev_join(d);
-    if (rc) return rc;
     Exec &x = d->ex;
-    const size_t bytes = (size_t)n * bpp;
-    if ((rc = arena_reserve(x, d->pixels, bytes))) return rc;
-    if ((rc = pinned_reserve(x, bytes))) return rc;
-    memcpy(x.pinned, p->pixels, bytes);
-    HIPC(hipMemcpyAsync(d->pixels.ptr, x.pinned, bytes, hipMemcpyHostToDevice, x.stream));
-    HIPC(hipEventRecord(x.upload_done, x.stream));
     PixDesc P;
     P.x = p->x;
     P.y = p->y;
@@ -5254,19 +5584,115 @@ MTGL_API int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
     P.blend_dst = p->blend_dst;
     P.alpha_ref = p->alpha_ref;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (x.timing) {
+    if (timed && x.timing) {
         ev0 = take_event(x);
         ev1 = take_event(x);
         if (ev0) HIPC(hipEventRecord(ev0, x.stream));
     }
-    hipLaunchKernelGGL(k_draw_pixels, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, P,
-                       (const uint8_t *)d->pixels.ptr, d->color, d->depth);
+    hipLaunchKernelGGL(k_draw_pixels, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, P, src, d->color, d->depth,
+                       (const Counters *)x.ctr);
     HIPC(hipGetLastError());
     if (ev0 && ev1) {
         HIPC(hipEventRecord(ev1, x.stream));
         x.pending_events.emplace_back(ev0, ev1);
     }
     d->dirty = true;
+    return MTGL_OK;
+}
+
+// source bytes per pixel of a glDrawPixels format, 0 for formats the
+// reference reads nothing of
+static uint32_t draw_pixels_bpp(uint32_t format)
+{
+    switch (format) {
+    case GL_RGBA: return 4;
+    case GL_RGB: return 3;
+    case GL_LUMINANCE: return 1;
+    case GL_LUMINANCE_ALPHA: return 2;
+    default: return 0;
+    }
+}
+
+MTGL_API int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
+{
+    if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
+    const uint32_t bpp = draw_pixels_bpp(p->format);
+    if (bpp == 0) return MTGL_OK;   // the reference reads nothing of other formats
+    const uint64_t n = (uint64_t)p->width * (uint64_t)p->height;
+    if (n == 0) return MTGL_OK;
+    if (n > 0xFFFFFFFFull) return MTGL_ERR_INVALID;
+    // a recorded batch replays clears and draws only: out of order otherwise
+    if (g_recording) return MTGL_ERR_UNSUPPORTED;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    Exec &x = d->ex;
+    const size_t bytes = (size_t)n * bpp;
+    if (x.ring_used + bytes > RING_LIMIT && (rc = confirm(d))) return rc;
+    uint8_t *host = ring_take(x, bytes);
+    if (!host) return MTGL_ERR_OUT_OF_MEMORY;
+    memcpy(host, p->pixels, bytes);
+    PendingOp op;
+    op.seq = next_seq(d);
+    op.kind = PendingOp::PIXELS;
+    op.px = *p;
+    op.px.pixels = host;
+    if ((rc = launch_pixels(d, op.px, n * bpp))) return rc;
+    d->ops.push_back(op);
+    return MTGL_OK;
+}
+
+// (a staged glDrawPixels source, from its first run or a replay)
+static int launch_pixels(mtgl_dev *d, const mtgl_pixels &px, uint64_t bytes)
+{
+    Exec &x = d->ex;
+    int rc;
+    if ((rc = arena_reserve(x, d->pixels, (size_t)bytes))) return rc;
+    HIPC(hipMemcpyAsync(d->pixels.ptr, px.pixels, (size_t)bytes, hipMemcpyHostToDevice, x.stream));
+    return draw_pixels_kernel(d, &px, (const uint8_t *)d->pixels.ptr, (uint64_t)px.width * (uint64_t)px.height,
+                              !d->replaying);
+}
+
+// Waits for the context's queued work and makes sure every logged clear,
+// draw and glDrawPixels took effect.  A launch that overflowed its arrays
+// poisoned the stream (every later framebuffer write was skipped): the
+// arrays grow, and that launch and every later logged op run again, in
+// order, until none overflows.  Then the log and its staging are released.
+static int confirm(mtgl_dev *d)
+{
+    if (d->ops.empty()) return MTGL_OK;
+    Exec &x = d->ex;
+    for (int attempt = 0;; attempt++) {
+        HIPC(hipStreamSynchronize(x.stream));
+        uint32_t poison = 0;
+        int rc = exec_take_poison(x, poison);
+        if (rc) return rc;
+        if (!poison) break;
+        size_t first = d->ops.size();
+        for (size_t i = 0; i < d->ops.size(); i++)
+            if (d->ops[i].seq == poison) first = i;
+        if (first == d->ops.size() || attempt >= 16) return MTGL_ERR_OUT_OF_MEMORY;
+        exec_grow(x, *x.ctr_host, d->ops[first].ntri);
+        d->replaying = true;
+        const bool timing = x.timing;
+        x.timing = false;
+        for (size_t i = first; i < d->ops.size() && !rc; i++) {
+            const PendingOp &op = d->ops[i];
+            if (op.kind == PendingOp::CLEAR) {
+                rc = launch_clear(d, op);
+            } else if (op.kind == PendingOp::DRAW) {
+                SceneIn in = {d, &op.draw};
+                rc = execute_scenes(x, d->one, &in, 1, op.seq);
+                if (rc == MTGL_ERR_UNSUPPORTED) rc = MTGL_OK;
+            } else {
+                rc = launch_pixels(d, op.px, (uint64_t)op.px.width * op.px.height * draw_pixels_bpp(op.px.format));
+            }
+        }
+        x.timing = timing;
+        d->replaying = false;
+        if (rc) return rc;
+    }
+    d->ops.clear();
+    ring_release(x);
     return MTGL_OK;
 }
 
@@ -5280,12 +5706,16 @@ MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int enable)
 MTGL_API int mtgl_dev_stats_get(mtgl_dev *d, mtgl_dev_stats *out)
 {
     if (!d || !out) return MTGL_ERR_INVALID;
+    int rc = confirm(d);
+    if (rc) return rc;
     return exec_stats(d->ex, out);
 }
 
 MTGL_API int mtgl_dev_stats_reset(mtgl_dev *d)
 {
     if (!d) return MTGL_ERR_INVALID;
+    int rc = confirm(d);
+    if (rc) return rc;
     return exec_stats_reset(d->ex);
 }
 
@@ -5371,8 +5801,10 @@ MTGL_API int mtgl_batch_launch(mtgl_batch *b)
 {
     if (!b || !b->ready) return MTGL_ERR_INVALID;
     Exec &x = b->ex;
-    // after the contexts' queued work ...
+    // after the contexts' queued work (which has taken effect) ...
     for (mtgl_dev *d : b->devs) {
+        int rc = confirm(d);
+        if (rc) return rc;
         if (d->pending && d->pending != b) {
             HIPC(hipStreamWaitEvent(x.stream, d->pending->done, 0));
         }
@@ -5396,7 +5828,7 @@ MTGL_API int mtgl_batch_sync(mtgl_batch *b)
     HIPC(hipStreamSynchronize(b->ex.stream));
     Counters *c = b->ex.ctr_host;
     HIPC(hipMemcpy(c, b->ex.ctr, sizeof(Counters), hipMemcpyDeviceToHost));
-    const unsigned int over = c->sticky | c->overflow_full;
+    const unsigned int over = c->sticky | c->overflow_full | c->poison;
     if (over) {
         // a replay produced more clipped triangles or pairs than the sizing
         // run (buffer objects changed since): its output is incomplete
@@ -5561,6 +5993,7 @@ MTGL_API int mtgl_diff_add_view(mtgl_diff *df, mtgl_dev *d)
     if (!df->views.empty() && (d->w != df->w || d->h != df->h)) return MTGL_ERR_INVALID;
     int rc = dev_join(d);
     if (rc) return rc;
+    if ((rc = confirm(d))) return rc;
     Exec &x = d->ex;
     const LastDraw &l = d->last;
     DiffCapture c;

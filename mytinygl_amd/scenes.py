@@ -1291,6 +1291,44 @@ def scene_polygon_mode_mixed_dense(g, W, H, seed):
     g.glDepthFunc(E["GL_LESS"])
 
 
+def scene_heavy_bins(g, W, H, seed):
+    """Fans of 300 thin triangles around a few points, so that the tiles
+    there hold bins of several hundred entries (the device rasterises such
+    tiles one pixel row at a time), under every depth function with and
+    without depth writes, with the stencil test, and textured."""
+    rng = _rng(seed)
+    _perspective(g, W, H)
+    g.glEnable(E["GL_DEPTH_TEST"])
+    funcs = ["GL_LESS", "GL_LEQUAL", "GL_GREATER", "GL_GEQUAL", "GL_EQUAL", "GL_NOTEQUAL", "GL_ALWAYS", "GL_NEVER"]
+    for k, fn in enumerate(funcs + ["GL_LESS", "GL_GREATER"]):
+        g.glDepthFunc(E[fn])
+        g.glDepthMask(int(k % 3 != 1))
+        if k == 8:          # stencil on: the rows keep the in-order loop
+            g.glEnable(E["GL_STENCIL_TEST"])
+            g.glStencilFunc(E["GL_GEQUAL"], 3, 0xFF)
+            g.glStencilOp(E["GL_INCR"], E["GL_KEEP"], E["GL_INCR"])
+        if k == 9:          # textured, stencil off again
+            g.glDisable(E["GL_STENCIL_TEST"])
+            g.glEnable(E["GL_TEXTURE_2D"])
+            _upload_texture(g, random_rgba(8, 8, seed + 11))
+        cx, cy = rng.uniform(-0.6, 0.6, 2)
+        g.glBegin(E["GL_TRIANGLES"])
+        for t in range(300):
+            a = rng.uniform(0.0, 2.0 * np.pi)
+            da = rng.uniform(0.02, 0.3)
+            r = rng.uniform(0.2, 1.6)
+            z = float(rng.uniform(-1.0, 1.0)) if k % 4 else float(rng.choice([-0.5, 0.0, 0.5]))
+            for (px, py, pz) in ((cx, cy, z), (cx + r * np.cos(a), cy + r * np.sin(a), z + 0.3),
+                                 (cx + r * np.cos(a + da), cy + r * np.sin(a + da), z - 0.3)):
+                g.glColor4f(*rng.random(4).astype(np.float32))
+                g.glTexCoord2f(*rng.uniform(0.0, 1.0, 2).astype(np.float32))
+                g.glVertex3f(float(px), float(py), float(pz))
+        g.glEnd()
+    g.glDisable(E["GL_TEXTURE_2D"])
+    g.glDepthMask(1)
+    g.glDepthFunc(E["GL_LESS"])
+
+
 def scene_draw_pixels(g, W, H, seed):
     """glDrawPixels over rendered triangles (reference src/gl_api.c:1284-1373):
     all four formats, rectangles hanging off every edge, alpha test, depth
@@ -1543,6 +1581,7 @@ FEATURE_SCENES = {
     "polygon_mode": scene_polygon_mode,
     "polygon_mode_mixed": scene_polygon_mode_mixed,
     "polygon_mode_mixed_dense": scene_polygon_mode_mixed_dense,
+    "heavy_bins": scene_heavy_bins,
     "lists": scene_lists,
     "lists_replay": scene_lists_replay,
     "lists_repeat": scene_lists_repeat,
