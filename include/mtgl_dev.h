@@ -312,6 +312,18 @@ typedef struct {
 
 int  mtgl_dev_draw_pixels(mtgl_dev *dev, const mtgl_pixels *pixels);
 
+/* glDrawPixels that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device or pinned memory (code under stream
+ * capture, serving loops with a fixed memory footprint).  Same arguments and
+ * result as mtgl_dev_draw_pixels, but the source image is staged in a fixed
+ * device buffer of MTGL_STATIC_PIXELS_MAX bytes that the library holds on
+ * each GPU it runs on until the process ends (its address is looked up when
+ * the first context on that GPU is created).  A source image larger than the
+ * buffer (width * height * bytes per pixel) is refused with MTGL_ERR_INVALID
+ * and draws nothing.  Calls on one GPU share its buffer: the library
+ * serialises them and each returns after its kernel finished. */
+#define MTGL_STATIC_PIXELS_MAX (32u << 20)
+int  mtgl_dev_draw_pixels_static(mtgl_dev *dev, const mtgl_pixels *pixels);
 
 /* Counters and timing; no counterpart in the reference (fragments passing
  * coverage, algorithmic bytes of the per-fragment pipeline, fragment-stage time). */

@@ -5079,6 +5079,40 @@ static int launch_clear(mtgl_dev *d, const PendingOp &op)
     return MTGL_OK;
 }
 
+// ---- glDrawPixels without device allocation (mtgl_dev_draw_pixels_static) ----
+//
+// The source image is staged in `scratch`, a fixed device array that each
+// GPU the library runs on holds until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for
+// the symbol): the device address of each GPU's copy is looked up once per
+// device, with that device current, when the first context on it is created
+// (scratch_setup), and kept in g_scratch.  One mutex per device serialises
+// the calls, each of which waits for its kernel before returning, so two
+// calls never use a GPU's copy at once.
+static __device__ __align__(256) uint8_t scratch[MTGL_STATIC_PIXELS_MAX];
+
+namespace {
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_setup_mu;
+std::mutex g_scratch_mu[SCRATCH_DEVICES];
+void *g_scratch[SCRATCH_DEVICES];
+
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= SCRATCH_DEVICES) return MTGL_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_setup_mu);
+    if (g_scratch[device]) return MTGL_OK;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        fprintf(stderr, "libmtgl: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+                hipGetErrorString(e));
+        return MTGL_ERR_DEVICE;
+    }
+    g_scratch[device] = ptr;
+    return MTGL_OK;
+}
+}  // namespace
 
 MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
 {
@@ -5094,6 +5128,7 @@ MTGL_API int mtgl_dev_create(int32_t width, int32_t height, mtgl_dev **out)
     int rc = MTGL_OK;
     do {
         if (hipGetDevice(&d->device) != hipSuccess) { rc = MTGL_ERR_DEVICE; break; }
+        if ((rc = scratch_setup(d->device)) != MTGL_OK) break;
         if ((rc = exec_init(d->ex, d->device)) != MTGL_OK) break;
         d->w = width;
         d->h = height;
@@ -5611,6 +5646,43 @@ static uint32_t draw_pixels_bpp(uint32_t format)
     case GL_LUMINANCE_ALPHA: return 2;
     default: return 0;
     }
+}
+
+MTGL_API int mtgl_dev_draw_pixels_static(mtgl_dev *d, const mtgl_pixels *p)
+{
+    if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
+    const uint32_t bpp = draw_pixels_bpp(p->format);
+    if (bpp == 0) return MTGL_OK;
+    const uint64_t n = (uint64_t)p->width * (uint64_t)p->height;
+    if (n == 0) return MTGL_OK;
+    if (n > 0xFFFFFFFFull || n * bpp > (uint64_t)MTGL_STATIC_PIXELS_MAX) {
+        fprintf(stderr, "libmtgl: mtgl_dev_draw_pixels_static: %llu source bytes exceed the %llu-byte buffer\n",
+                (unsigned long long)(n * bpp), (unsigned long long)MTGL_STATIC_PIXELS_MAX);
+        return MTGL_ERR_INVALID;
+    }
+    if (g_recording) return MTGL_ERR_UNSUPPORTED;
+    if (d->device < 0 || d->device >= SCRATCH_DEVICES || !g_scratch[d->device]) return MTGL_ERR_DEVICE;
+    int rc = dev_join(d);
+    if (rc) return rc;
+    if (!d->ops.empty()) {
+        // the queued work must have taken effect; running an overflowed
+        // launch again would allocate, which this entry point may not do
+        HIPC(hipStreamSynchronize(d->ex.stream));
+        HIPC(hipMemcpy(d->ex.ctr_host, d->ex.ctr, sizeof(Counters), hipMemcpyDeviceToHost));
+        if (d->ex.ctr_host->poison) {
+            fprintf(stderr, "libmtgl: mtgl_dev_draw_pixels_static: earlier draws need larger buffers; "
+                            "call mtgl_dev_finish first\n");
+            return MTGL_ERR_INVALID;
+        }
+        d->ops.clear();
+        ring_release(d->ex);
+    }
+    uint8_t *buf = (uint8_t *)g_scratch[d->device];
+    std::lock_guard<std::mutex> lock(g_scratch_mu[d->device]);
+    HIPC(hipMemcpyAsync(buf, p->pixels, (size_t)(n * bpp), hipMemcpyHostToDevice, d->ex.stream));
+    if ((rc = draw_pixels_kernel(d, p, buf, n, false))) return rc;
+    HIPC(hipStreamSynchronize(d->ex.stream));
+    return MTGL_OK;
 }
 
 MTGL_API int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)

@@ -1414,6 +1414,24 @@ int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
     return MTGL_OK;
 }
 
+/* the no-allocation variant: the same pixels, refused when the source image
+ * exceeds the fixed staging buffer of the device library */
+int mtgl_dev_draw_pixels_static(mtgl_dev *d, const mtgl_pixels *p)
+{
+    if (!d || !p || !p->pixels || p->width < 0 || p->height < 0) return MTGL_ERR_INVALID;
+    uint64_t bpp = 0;
+    switch (p->format) {
+    case GL_RGBA: bpp = 4; break;
+    case GL_RGB: bpp = 3; break;
+    case GL_LUMINANCE: bpp = 1; break;
+    case GL_LUMINANCE_ALPHA: bpp = 2; break;
+    default: return MTGL_OK;
+    }
+    const uint64_t n = (uint64_t)p->width * (uint64_t)p->height;
+    if (n > 0xFFFFFFFFull || n * bpp > (uint64_t)MTGL_STATIC_PIXELS_MAX) return MTGL_ERR_INVALID;
+    return mtgl_dev_draw_pixels(d, p);
+}
+
 /* level 0 plus the 2x2 box-filtered level 1: (((c00 + c10) + c01) + c11) * 0.25,
  * packed with truncation (reference texture_generate_mip1, src/textures.c:311-354) */
 int mtgl_dev_tex_image(mtgl_dev *d, uint32_t id, int32_t w, int32_t h, const uint32_t *texels)
