@@ -62,6 +62,7 @@ from mytinygl_amd import scenes as S  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 W, H = 1920, 1080
 WORKLOAD = "textured_sphere_1920x1080"
+TIMING_EVERY = 8   # fragment-stage launches per one timed with HIP events
 
 
 def frame(g, n):
@@ -138,8 +139,8 @@ def configs_bench(frames: dict, cpu_budget_s: float):
             draw(g, n)
         g.glFinish()
         g.stats_reset(ctx)
-        g.set_timing(ctx, True)
         nf = frames[name]
+        g.set_timing(ctx, True, every=TIMING_EVERY if nf >= 10 * TIMING_EVERY else 1)
         t0 = time.perf_counter()
         for _ in range(nf):
             g.glClear(bits)
@@ -156,7 +157,7 @@ def configs_bench(frames: dict, cpu_budget_s: float):
                "fragment_stage_ms": round(st.raster_ms / max(1, st.raster_launches), 5),
                "algorithmic_bytes_per_frame": int(st.frag_bytes / nf)}
         if st.raster_ms > 0:
-            rec["fragment_stage_gb_s"] = round(st.frag_bytes / (st.raster_ms * 1e-3) / 1e9, 1)
+            rec["fragment_stage_gb_s"] = round(st.frag_bytes / nf / (rec["fragment_stage_ms"] * 1e-3) / 1e9, 1)
         if cpu_budget_s > 0 and os.path.exists(ref):
             c = GLLibrary(ref, "reference" if kind == "reference" else "oracle")
             cctx = c.create_context(w, h)
@@ -701,7 +702,9 @@ def main():
     bytes_per_frame = st.frag_bytes
 
     g.stats_reset(ctx)
-    g.set_timing(ctx, True)
+    # HIP events bracket one fragment-stage launch in TIMING_EVERY of the
+    # timed frames (a pair of events on every launch would add queue time)
+    g.set_timing(ctx, True, every=TIMING_EVERY)
     g.glFinish()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -718,7 +721,7 @@ def main():
     fps = args.steps / sec
     mfrag = frags_per_frame * fps / 1e6
     k_ms = st.raster_ms / max(1, st.raster_launches)
-    bytes_per_launch = st.frag_bytes / max(1, st.raster_launches)
+    bytes_per_launch = st.frag_bytes / args.steps   # one fragment-stage launch per frame
     achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)

@@ -326,8 +326,11 @@ int  mtgl_dev_draw_pixels(mtgl_dev *dev, const mtgl_pixels *pixels);
 int  mtgl_dev_draw_pixels_static(mtgl_dev *dev, const mtgl_pixels *pixels);
 
 /* Counters and timing; no counterpart in the reference (fragments passing
- * coverage, algorithmic bytes of the per-fragment pipeline, fragment-stage time). */
-int  mtgl_dev_set_timing(mtgl_dev *dev, int enable);
+ * coverage, algorithmic bytes of the per-fragment pipeline, fragment-stage time).
+ * mtgl_dev_set_timing(dev, every): 0 stops timing; every >= 1 brackets one
+ * fragment-stage launch in `every` with events (each pair of events costs a
+ * few microseconds of queue time, so a sample keeps the timed frames honest). */
+int  mtgl_dev_set_timing(mtgl_dev *dev, int every);
 int  mtgl_dev_stats_get(mtgl_dev *dev, mtgl_dev_stats *out);
 int  mtgl_dev_stats_reset(mtgl_dev *dev);
 

@@ -72,7 +72,12 @@ typedef struct {
 
 int  mtgl_diff_create(uint32_t vertices, int32_t tex_w, int32_t tex_h, mtgl_diff **out);
 void mtgl_diff_destroy(mtgl_diff *diff);
-/* Capture the latest draw of dev as the next view (synchronises dev). */
+/* Have the draws that follow on dev record each pixel's last passing fragment
+ * (4 bytes per pixel of every tile a draw touches), which mtgl_diff_add_view
+ * reads; draws on a device that was never armed skip that plane. */
+int  mtgl_diff_arm(mtgl_dev *dev);
+/* Capture the latest draw of dev as the next view (synchronises dev); the draw
+ * must have been made after mtgl_diff_arm(dev), else MTGL_ERR_UNSUPPORTED. */
 int  mtgl_diff_add_view(mtgl_diff *diff, mtgl_dev *dev);
 int  mtgl_diff_info(mtgl_diff *diff, mtgl_diff_info_t *out);
 /* Copy view's G-buffer (width * height pixels) to host memory. */

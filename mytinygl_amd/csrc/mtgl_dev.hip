@@ -60,14 +60,19 @@ constexpr int RADIX = 1 << RADIX_BITS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+constexpr int SCAN1_THREADS = 1024;     // the one-workgroup scan
+// 64-element rows per wave it holds in registers (128 bytes per lane), and
+// the elements it takes
+template <typename T> constexpr uint32_t SCAN1_ROWS = sizeof(T) == 4 ? 32u : 16u;
+template <typename T> constexpr uint64_t SCAN1_MAX = (uint64_t)SCAN1_ROWS<T> * SCAN1_THREADS;
 constexpr uint64_t SCAN2_MAX_BLOCKS = 256;   // scans of at most this many tiles take two launches
 constexpr uint32_t HEAVY_BIN = 256;     // default: tiles with longer bins are rasterised first, in parts
 constexpr uint32_t HEAVY_SPLIT = 8;     // parts of a heavy tile, TILE / HEAVY_SPLIT pixel rows each
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | part << 28 (0: whole tile, 1..HEAVY_SPLIT)
 constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
 static_assert(TILE % HEAVY_SPLIT == 0 && HEAVY_SPLIT < (1u << (32 - ITEM_TILE_BITS)), "heavy-tile parts");
-constexpr uint32_t MED_BIN = 96;        // whole tiles with longer bins join the front of the tile list
-constexpr int GEOM_BLOCK = 256;         // threads per block of the vertex, setup and expansion kernels
+constexpr uint32_t MED_BIN = 40;        // whole tiles with longer bins join the front of the tile list
+constexpr int SETUP_BLOCK = 64;         // threads per block of the set-up kernel (one wave)
 constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
@@ -128,12 +133,10 @@ struct __align__(16) RCold {
 // Counters of an Exec.  The first group belongs to one launch and is reset
 // by the launch's first kernel; the second persists across launches.
 struct Counters {
-    unsigned long long tri_raster;   // fan triangles set up by the launch
     unsigned long long npairs;    // (tile, record) pairs of the launch, written by the bin-count scan
     unsigned int nheavy;          // work items at the front of the tile list: heavy-tile parts, medium tiles
     unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
-    unsigned int nclip;
     unsigned int noverflow;       // overflow slots requested by clipped triangles
     unsigned int overflow_full;   // bit 0: the clip pool was too small; bit 1: the pair arrays were
     unsigned int nbig;            // triangles queued for the one-wave pair expansion
@@ -202,7 +205,6 @@ struct SetupParams {
     uint32_t *n_rt;               // fan triangles kept per input triangle
     uint32_t *rec_base;           // first global record slot per input triangle
     unsigned long long *tri_tiles;  // (tile, record) pairs per input triangle
-    uint32_t *clip_list;          // global triangles that cross a frustum plane
     uint32_t overflow_cap;        // slots of the overflow pool
     Counters *ctr;
 };
@@ -342,6 +344,7 @@ __device__ Col lighting_sum(const mtgl_lighting *L, uint32_t flags, const float 
     result.a = m_dif.a;
     float N[3] = {eye_normal[0], eye_normal[1], eye_normal[2]};
     if (flags & MTGL_F_NORMALIZE) normalize3(N);
+#pragma unroll 1
     for (int i = 0; i < MTGL_MAX_LIGHTS; i++) {
         const mtgl_light *lt = &L->lights[i];
         if (OVERRIDE) ndl_out[i] = 0.0f;
@@ -502,6 +505,60 @@ __device__ __forceinline__ float sanitize(float v, float nan_value)
     return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP row shifts and
+// row broadcasts (VALU only): Hillis-Steele within each row of 16 lanes,
+// then row 0's and row 1's totals carried into the rows above.
+__device__ __forceinline__ uint32_t dpp_incl_scan(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xe, false);  // row_shr:4, lanes 4..15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xc, false);  // row_shr:8, lanes 8..15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+// Inclusive prefix minimum (MAX false) or maximum over the 64 lanes, the
+// same DPP pattern as dpp_incl_scan; lanes without a source keep the
+// identity, and NaN operands are passed over (v_min / v_max return the other).
+template <bool MAX>
+__device__ __forceinline__ float dpp_incl_minmax(float v)
+{
+    const int id = __float_as_int(MAX ? -__builtin_inff() : __builtin_inff());
+#define MINMAX_STEP(ctrl, rm, bm)                                                                          \
+    {                                                                                                      \
+        const float o = __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), ctrl, rm, bm, false)); \
+        v = MAX ? fmaxf(v, o) : fminf(v, o);                                                              \
+    }
+    MINMAX_STEP(0x111, 0xf, 0xf)   // row_shr:1
+    MINMAX_STEP(0x112, 0xf, 0xf)   // row_shr:2
+    MINMAX_STEP(0x114, 0xf, 0xe)   // row_shr:4, lanes 4..15
+    MINMAX_STEP(0x118, 0xf, 0xc)   // row_shr:8, lanes 8..15
+    MINMAX_STEP(0x142, 0xa, 0xf)   // row_bcast:15 into rows 1, 3
+    MINMAX_STEP(0x143, 0xc, 0xf)   // row_bcast:31 into rows 2, 3
+#undef MINMAX_STEP
+    return v;
+}
+
+__device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(v), 63);
+}
+
+// inclusive prefix sum over the 64 lanes of a wave, any arithmetic type
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
 // clip-space vertex carried through clipping (the object-space normal the
 // reference also interpolates is never read after clipping and is dropped)
 struct CV {
@@ -516,7 +573,11 @@ struct CV {
 // array's defaults): attribute fetch with get_array_element's rules
 // (src/gl_api.c:1758-1797), glColor4f sanitising, transform, Gouraud
 // lighting, texture matrix.  ep / en are written when cold is set.
-__device__ void vertex_at(const VertexParams &P, uint32_t i, int32_t idx, bool cold, CV &o)
+// vs0 is the draw's first vertex state; MULTI draws may switch states per
+// vertex through state_index.
+template <bool MULTI>
+__device__ void vertex_at(const VertexParams &P, uint32_t i, int32_t idx, bool cold, CV &o,
+                          const mtgl_vertex_state *vs0)
 {
     float pos[4], col[4], tc[2], nrm[3];
     fetch(P.position, idx, pos, 4);
@@ -532,14 +593,15 @@ __device__ void vertex_at(const VertexParams &P, uint32_t i, int32_t idx, bool c
     if (P.normal.enabled) fetch(P.normal, idx, nrm, 3);
     else { nrm[0] = P.cur_normal[0]; nrm[1] = P.cur_normal[1]; nrm[2] = P.cur_normal[2]; }
 
-    uint32_t si = 0;
-    if (P.n_vstates > 1 && P.state_index.enabled) {
+    const mtgl_vertex_state *vs = vs0;
+    if (MULTI && P.n_vstates > 1 && P.state_index.enabled) {
+        uint32_t si = 0;
         const DArr &a = P.state_index;
         int64_t at = a.offset + (int64_t)i * a.stride;
         if (a.src && at >= 0 && at + 4 <= a.size) __builtin_memcpy(&si, a.src + at, 4);
         if (si >= P.n_vstates) si = P.n_vstates - 1;
+        vs = &P.vstates[si];
     }
-    const mtgl_vertex_state *vs = &P.vstates[si];
 
     float eye[4];
     xform(vs->modelview, pos[0], pos[1], pos[2], 1.0f, eye);
@@ -595,7 +657,7 @@ __global__ void __launch_bounds__(256) k_vertex(const VertexParams *Ps, const ui
     // -1 reads the defaults such an index reads
     const int32_t idx = P.unique ? (i + 1u < P.count ? P.ubase + (int32_t)i : -1) : fetch_index(P, i);
     CV o;
-    vertex_at(P, i, idx, P.out_cold != nullptr, o);
+    vertex_at<true>(P, i, idx, P.out_cold != nullptr, o, P.vstates);
     VHot h;
     h.x = o.x; h.y = o.y; h.z = o.z; h.w = o.w;
     h.r = o.r; h.g = o.g; h.b = o.b; h.a = o.a;
@@ -720,7 +782,7 @@ __device__ int clip_plane(const CV *in, int n, CV *out, int plane, bool cold)
 __device__ __forceinline__ void load_cv(const SetupParams &P, uint32_t i, bool cold, CV &o)
 {
     const VertexParams &V = *P.vp;
-    vertex_at(V, i, fetch_index(V, i), cold, o);
+    vertex_at<true>(V, i, fetch_index(V, i), cold, o, V.vstates);
 }
 
 __device__ __forceinline__ void perspective_divide(CV &v)
@@ -850,113 +912,138 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     return 1;
 }
 
-// fast path: triangles inside every frustum plane pass through the clipper
-// unchanged (each plane re-emits the vertices in order), so they are set up
-// directly; the others are queued for k_setup_clip.  Every kept record's
-// tiles are counted into the bins.
-__global__ void __launch_bounds__(256) k_setup(const SetupParams *Ss, const uint32_t *block_pre,
-                                               const uint32_t *block_scene, uint32_t nscenes)
+// slow path: Sutherland-Hodgman in the reference's plane order (near, far,
+// left, right, bottom, top; src/clipping.h:106-126), divide, fan split of
+// input triangle t into slots of the shared overflow pool.  Out of line: a
+// few triangles of a draw take it, and its vertex arrays live in scratch.
+// Returns the fan triangles kept; base gets the first slot, tiles their
+// (tile, record) pairs.
+__device__ __noinline__ uint32_t setup_clipped(const SetupParams &P, const mtgl_raster_state *rs, const CV &A,
+                                               const CV &B, const CV &C, bool cold, uint32_t &base,
+                                               unsigned long long &tiles)
 {
+    CV a[MAX_CLIP], b[MAX_CLIP];
+    b[0] = A;
+    b[1] = B;
+    b[2] = C;
+    int n = clip_plane(b, 3, a, 0, cold);
+    n = n ? clip_plane(a, n, b, 1, cold) : 0;
+    n = n ? clip_plane(b, n, a, 2, cold) : 0;
+    n = n ? clip_plane(a, n, b, 3, cold) : 0;
+    n = n ? clip_plane(b, n, a, 4, cold) : 0;
+    n = n ? clip_plane(a, n, b, 5, cold) : 0;
+    base = 0;
+    tiles = 0;
+    if (n < 3) return 0;
+    base = atomicAdd(&P.ctr->noverflow, (uint32_t)(n - 2));
+    if (base + (uint32_t)(n - 2) > P.overflow_cap) {
+        atomicOr(&P.ctr->overflow_full, 1u);
+        return 0;
+    }
+    uint32_t nout = 0;
+    RTri *out = P.rt_over + base;
+    RCold *cout = cold ? P.rcold_over + base : nullptr;
+    for (int k = 0; k < n; k++) perspective_divide(b[k]);
+    for (int j = 1; j + 1 < n; j++) {
+        int nt = 0;
+        RBox q;
+        if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr, &q, &nt)) {
+            P.box_over[base + nout] = q;
+            nout++;
+            tiles += (unsigned long long)nt;
+        }
+    }
+    return nout;
+}
+
+// The uniform inputs of one set-up block, copied to LDS once: every thread
+// reads the matrices, lights and viewport many times over, and from LDS that
+// costs no global round trip (the raster state's lighting block is not read
+// by set-up and is left out).
+struct SetupStage {
+    SetupParams sp;
+    VertexParams vp;
+    mtgl_vertex_state vs;
+    mtgl_raster_state rs;
+};
+
+__device__ __forceinline__ void stage_words(void *dst, const void *src, uint32_t bytes)
+{
+    uint32_t *d = static_cast<uint32_t *>(dst);
+    const uint32_t *q = static_cast<const uint32_t *>(src);
+    for (uint32_t k = threadIdx.x; k < bytes / 4u; k += blockDim.x) d[k] = q[k];
+}
+
+// Set-up of every input triangle, one thread each: triangles inside every
+// frustum plane pass through the clipper unchanged (each plane re-emits the
+// vertices in order), so they are set up directly into slot t; the others go
+// through setup_clipped.  Every kept record's tiles are counted into the
+// bins.  One wave per block (blocks spread over every CU: the kernel is a
+// chain of dependent loads, not arithmetic); MULTI draws switch vertex states
+// per vertex and read them from HBM.
+template <bool MULTI>
+__global__ void __launch_bounds__(SETUP_BLOCK) k_setup(const SetupParams *Ss, const uint32_t *block_pre,
+                                                       const uint32_t *block_scene, uint32_t nscenes,
+                                                       uint32_t *blk_kept, unsigned long long *blk_pairs)
+{
+    __shared__ SetupStage st;
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
-    const SetupParams &P = Ss[sc];
+    stage_words(&st.sp, Ss + sc, sizeof(SetupParams));
+    __syncthreads();
+    stage_words(&st.vp, st.sp.vp, sizeof(VertexParams));
+    stage_words(&st.rs, st.sp.rs, offsetof(mtgl_raster_state, lighting));
+    __syncthreads();
+    if (!MULTI) stage_words(&st.vs, st.vp.vstates, sizeof(mtgl_vertex_state));
+    __syncthreads();
+    const SetupParams &P = st.sp;
+    const VertexParams &V = st.vp;
+    const mtgl_vertex_state *vs0 = MULTI ? V.vstates : &st.vs;
     const uint32_t t = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
     const bool act = t < P.ntri;
-    const mtgl_raster_state *rs = P.rs;
+    const mtgl_raster_state *rs = &st.rs;
     const bool cold = P.cold != 0u;
     uint32_t nout = 0;
-    RBox bx = {0u, 1u, 0u, 0u};
+    unsigned long long tiles = 0;
     if (act) {
         uint32_t ia, ib, ic;
         tri_vertices(P.mode, t, ia, ib, ic);
+        const int32_t xa = fetch_index(V, ia), xb = fetch_index(V, ib), xc = fetch_index(V, ic);
         CV v[3];
-        load_cv(P, ia, cold, v[0]);
-        load_cv(P, ib, cold, v[1]);
-        load_cv(P, ic, cold, v[2]);
+        vertex_at<MULTI>(V, ia, xa, cold, v[0], vs0);
+        vertex_at<MULTI>(V, ib, xb, cold, v[1], vs0);
+        vertex_at<MULTI>(V, ic, xc, cold, v[2], vs0);
         bool all_inside = true;
 #pragma unroll
         for (int k = 0; k < 3; k++)
 #pragma unroll
             for (int pl = 0; pl < 6; pl++)
                 all_inside = all_inside && (plane_dist(v[k], pl) >= 0);
+        uint32_t rb = P.tri_off + t;
         if (all_inside) {
             for (int k = 0; k < 3; k++) perspective_divide(v[k]);
             int nt = 0;
-            RTri *out = P.rt + t;
-            RCold *cout = cold ? P.rcold + t : nullptr;
-            if (setup_fan(P, rs, v[0], v[1], v[2], cold, out, cout, &bx, &nt)) {
+            RBox bx;
+            if (setup_fan(P, rs, v[0], v[1], v[2], cold, P.rt + t, cold ? P.rcold + t : nullptr, &bx, &nt)) {
                 nout = 1;
                 P.box[t] = bx;
+                tiles = (unsigned long long)nt;
             }
         } else {
-            uint32_t slot = atomicAdd(&P.ctr->nclip, 1u);
-            P.clip_list[slot] = P.tri_off + t;
+            uint32_t base;
+            nout = setup_clipped(P, rs, v[0], v[1], v[2], cold, base, tiles);
+            rb = P.over_base + base;
         }
         P.n_rt[t] = nout;
-        P.rec_base[t] = P.tri_off + t;
-    }
-    if (act) P.tri_tiles[t] = nout ? (unsigned long long)bx.ntiles : 0ull;
-    // one counter update per wave
-    const uint64_t kept = __ballot(nout != 0);
-    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&P.ctr->tri_raster, (unsigned long long)__popcll(kept));
-}
-
-// slow path: Sutherland-Hodgman in the reference's plane order (near, far,
-// left, right, bottom, top; src/clipping.h:106-126), divide, fan split
-// one thread per queued global triangle; tri_pre holds each scene's first triangle
-__global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const uint32_t *tri_pre, uint32_t nscenes,
-                                                   const uint32_t *clip_list, const Counters *ctr)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nq = __builtin_amdgcn_readfirstlane(*(volatile const uint32_t *)&ctr->nclip);
-    if (blockIdx.x * blockDim.x >= nq) return;   // the whole wave: nothing queued for it
-    const bool act = i < nq;
-    const uint32_t g = act ? clip_list[i] : 0u;
-    const uint32_t sc = act ? find_scene(tri_pre, nscenes, g) : 0u;
-    const SetupParams &P = Ss[sc];
-    const uint32_t t = g - P.tri_off;
-    const mtgl_raster_state *rs = P.rs;
-    const bool cold = P.cold != 0u;
-    uint32_t nout = 0;
-    uint32_t base = 0;
-    if (act) {
-        uint32_t ia, ib, ic;
-        tri_vertices(P.mode, t, ia, ib, ic);
-        CV a[MAX_CLIP], b[MAX_CLIP];
-        load_cv(P, ia, cold, b[0]);
-        load_cv(P, ib, cold, b[1]);
-        load_cv(P, ic, cold, b[2]);
-        int n = clip_plane(b, 3, a, 0, cold);
-        n = n ? clip_plane(a, n, b, 1, cold) : 0;
-        n = n ? clip_plane(b, n, a, 2, cold) : 0;
-        n = n ? clip_plane(a, n, b, 3, cold) : 0;
-        n = n ? clip_plane(b, n, a, 4, cold) : 0;
-        n = n ? clip_plane(a, n, b, 5, cold) : 0;
-        if (n >= 3) {
-            base = atomicAdd(&P.ctr->noverflow, (uint32_t)(n - 2));
-            if (base + (uint32_t)(n - 2) > P.overflow_cap) {
-                atomicOr(&P.ctr->overflow_full, 1u);
-                n = 0;
-            }
-        }
-        unsigned long long tiles = 0;
-        if (n >= 3) {
-            RTri *out = P.rt_over + base;
-            RCold *cout = cold ? P.rcold_over + base : nullptr;
-            for (int k = 0; k < n; k++) perspective_divide(b[k]);
-            for (int j = 1; j + 1 < n; j++) {
-                int nt = 0;
-                RBox q;
-                if (setup_fan(P, rs, b[0], b[j], b[j + 1], cold, out + nout, cout ? cout + nout : nullptr, &q, &nt)) {
-                    P.box_over[base + nout] = q;
-                    nout++;
-                    tiles += (unsigned long long)nt;
-                }
-            }
-        }
-        P.n_rt[t] = nout;
-        P.rec_base[t] = P.over_base + base;
+        P.rec_base[t] = rb;
         P.tri_tiles[t] = tiles;
-        if (nout) atomicAdd(&P.ctr->tri_raster, (unsigned long long)nout);
+    }
+    // the block's kept fan triangles and pairs into its own slots (the pair
+    // offsets of the launch are a scan of the block sums)
+    const uint32_t kept = (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(nout), 63);
+    const unsigned long long pairs = wave_inclusive_scan(tiles);
+    if (threadIdx.x == 63) {
+        blk_kept[blockIdx.x] = kept;
+        blk_pairs[blockIdx.x] = pairs;
     }
 }
 
@@ -964,17 +1051,6 @@ __global__ void __launch_bounds__(64) k_setup_clip(const SetupParams *Ss, const 
 // scans (exclusive), one value per element; out may alias in
 // ---------------------------------------------------------------------------
 
-template <typename T>
-__device__ __forceinline__ T wave_inclusive_scan(T v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T u = __shfl_up(v, d, 64);
-        if (lane >= d) v += u;
-    }
-    return v;
-}
 
 // block-wide exclusive scan of one value per thread (blockDim a multiple of
 // 64, at most 1024; lds_waves holds one value per wave); returns the
@@ -1085,6 +1161,47 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply_short(const T *in, 
     if (total_out && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total_out = offset + total;
 }
 
+// Scans of at most SCAN1_MAX elements in one launch of one workgroup: wave w
+// owns the w-th run of `per` (at most SCAN1_ROWS) 64-element rows, loads them
+// all at once (coalesced, one round trip), adds them up; the 16 wave totals
+// are scanned in LDS and each wave scans its rows carrying its offset.  The
+// grand total goes to *total_out.
+template <typename T>
+__global__ void __launch_bounds__(SCAN1_THREADS) k_scan_single(const T *in, uint64_t n, T *out, T *total_out,
+                                                               T *clear)
+{
+    __shared__ T waves[SCAN1_THREADS / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t rows = (uint32_t)((n + 63) / 64);
+    const uint32_t per = (rows + SCAN1_THREADS / 64 - 1) / (SCAN1_THREADS / 64);
+    const uint64_t base = (uint64_t)wave * per * 64;
+    T v[SCAN1_ROWS<T>];
+    T s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN1_ROWS<T>; k++) {
+        const uint64_t i = base + (uint64_t)k * 64 + lane;
+        v[k] = k < per && i < n ? in[i] : (T)0;
+        s += v[k];
+    }
+    T total;
+    T carry = block_exclusive_scan<T>(s, waves, total);
+    // the exclusive scan of the lane totals gave lane 0 its wave's offset
+    carry = __shfl(carry, 0);
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN1_ROWS<T>; k++) {
+        if (k < per) {   // (uniform)
+            const uint64_t i = base + (uint64_t)k * 64 + lane;
+            const T inc = wave_inclusive_scan(v[k]);
+            if (i < n) {
+                out[i] = carry + inc - v[k];
+                if (clear) clear[i] = (T)0;
+            }
+            carry += __shfl(inc, 63);
+        }
+    }
+    if (total_out && threadIdx.x == 0) *total_out = total;
+}
+
 // K3a: expand triangles into (tile, fan triangle) pairs, in submission order
 // ---------------------------------------------------------------------------
 
@@ -1112,50 +1229,66 @@ __device__ __forceinline__ void expand_record(const RBox &q, uint32_t rec, uint6
     }
 }
 
-// Pairs of every triangle, in (triangle, fan record, tile) order from the
-// triangle's offset pair_base[t]: one thread per triangle writes them when
-// there are at most EXPAND_SMALL, longer triangles are queued for
-// k_expand_big (one wave each).
-__global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
-                                                const uint32_t *rec_base, const RBox *box, uint64_t cap,
-                                                uint32_t *keys, uint32_t *vals,
-                                                uint32_t *big, Counters *ctr, uint32_t seq)
+// Pairs of every triangle, in (triangle, fan record, tile) order: one wave
+// per set-up block, whose pairs start at blk_off (the scan of the set-up
+// blocks' pair sums) and follow in triangle order.  A thread writes its
+// triangle's pairs when there are at most EXPAND_SMALL, longer triangles are
+// queued for k_expand_big (one wave each) with their first pair in pair_base.
+__global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, const uint32_t *block_pre,
+                                                        const uint32_t *block_scene, uint32_t nscenes,
+                                                        const unsigned long long *blk_off,
+                                                        const unsigned long long *tri_tiles,
+                                                        unsigned long long *pair_base, const uint32_t *n_rt,
+                                                        const uint32_t *rec_base, const RBox *box, uint64_t cap,
+                                                        uint32_t *keys, uint32_t *vals, uint32_t *big,
+                                                        Counters *ctr, uint32_t seq, const uint32_t *blk_kept,
+                                                        uint32_t nsblk)
 {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) {
-        // the launch's pair total against the capacity of the pair arrays: a
-        // launch that overflowed (pairs, or the clip pool) poisons the Exec
-        // with its sequence number, so that its raster stage and every later
-        // framebuffer write are skipped until the host has grown the arrays
-        // and run it again; a launch that fits adds to the running counts
-        const unsigned long long total = pair_base[ntri];
-        const uint32_t over = ctr->overflow_full | (total > cap ? 2u : 0u);
-        if (over) {
-            ctr->overflow_full = over;
-            if (atomicCAS(&ctr->poison, 0u, seq) == 0u) {
-                ctr->poison_pairs = total;
-                ctr->poison_flags = over;
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0) {
+        // fan triangles kept by k_setup, one count per set-up block
+        uint32_t kept = 0;
+        for (uint32_t b = lane; b < nsblk; b += 64) kept += blk_kept[b];
+        kept = (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(kept), 63);
+        if (lane == 0) {
+            // the launch's pair total against the capacity of the pair arrays:
+            // a launch that overflowed (pairs, or the clip pool) poisons the
+            // Exec with its sequence number, so that its raster stage and
+            // every later framebuffer write are skipped until the host has
+            // grown the arrays and run it again; a launch that fits adds to
+            // the running counts
+            const unsigned long long total = ctr->npairs;
+            const uint32_t over = ctr->overflow_full | (total > cap ? 2u : 0u);
+            if (over) {
+                ctr->overflow_full = over;
+                if (atomicCAS(&ctr->poison, 0u, seq) == 0u) {
+                    ctr->poison_pairs = total;
+                    ctr->poison_flags = over;
+                }
+            } else if (ctr->poison == 0u) {   // (a launch after a poisoned one runs again: counted then)
+                ctr->acc_tri_raster += kept;
+                ctr->acc_bin_pairs += total;
             }
-        } else if (ctr->poison == 0u) {   // (a launch after a poisoned one runs again: counted then)
-            ctr->acc_tri_raster += ctr->tri_raster;
-            ctr->acc_bin_pairs += total;
         }
     }
-    uint64_t p0 = 0, np = 0;
-    if (t < ntri) {
-        p0 = pair_base[t];
-        np = pair_base[t + 1] - p0;
-    }
+    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
+    const uint32_t ntri = Ss[sc].ntri, tri_off = Ss[sc].tri_off;
+    const uint32_t tl = (blockIdx.x - block_pre[sc]) * SETUP_BLOCK + (uint32_t)lane;
+    const uint32_t t = tri_off + tl;
+    const uint64_t np = tl < ntri ? tri_tiles[t] : 0ull;
+    uint64_t p0 = blk_off[blockIdx.x] + wave_inclusive_scan(np) - np;
     const bool is_big = np > EXPAND_SMALL;
     // one queue slot reservation per wave
     const uint64_t bal = __ballot(is_big);
-    uint32_t base = 0;
-    const int lane = threadIdx.x & 63;
-    if (bal && lane == __builtin_ctzll(bal)) base = atomicAdd(&ctr->nbig, (uint32_t)__popcll(bal));
-    base = __shfl(base, __builtin_ctzll(bal ? bal : 1ull));
-    if (is_big) {
-        big[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = t;
-        return;
+    if (bal) {
+        uint32_t base = 0;
+        if (lane == __builtin_ctzll(bal)) base = atomicAdd(&ctr->nbig, (uint32_t)__popcll(bal));
+        base = __shfl(base, __builtin_ctzll(bal));
+        if (is_big) {
+            big[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = t;
+            pair_base[t] = p0;
+            return;
+        }
     }
     if (np == 0) return;
     const uint32_t nr = n_rt[t], rb = rec_base[t];
@@ -1169,7 +1302,7 @@ __global__ void __launch_bounds__(256) k_expand(const unsigned long long *pair_b
 // The queued triangles, one wave each (lanes stride over each record's tile
 // box).  The sort and the tile ranges read only the pair total, so the
 // capacity past it is left as it is.
-__global__ void __launch_bounds__(64) k_expand_big(const unsigned long long *pair_base, uint32_t ntri, const uint32_t *n_rt,
+__global__ void __launch_bounds__(64) k_expand_big(const unsigned long long *pair_base, const uint32_t *n_rt,
                                                    const uint32_t *rec_base, const RBox *box, uint64_t cap,
                                                    uint32_t *keys, uint32_t *vals,
                                                    const uint32_t *big, const Counters *ctr)
@@ -1283,47 +1416,6 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(const uint32_t *k
 // K3c: tile segments of the sorted pairs
 // ---------------------------------------------------------------------------
 
-// Inclusive prefix sum over the 64 lanes of a wave with DPP row shifts and
-// row broadcasts (VALU only): Hillis-Steele within each row of 16 lanes,
-// then row 0's and row 1's totals carried into the rows above.
-__device__ __forceinline__ uint32_t dpp_incl_scan(uint32_t v)
-{
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xe, false);  // row_shr:4, lanes 4..15
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xc, false);  // row_shr:8, lanes 8..15
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
-    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
-    return v;
-}
-
-// Inclusive prefix minimum (MAX false) or maximum over the 64 lanes, the
-// same DPP pattern as dpp_incl_scan; lanes without a source keep the
-// identity, and NaN operands are passed over (v_min / v_max return the other).
-template <bool MAX>
-__device__ __forceinline__ float dpp_incl_minmax(float v)
-{
-    const int id = __float_as_int(MAX ? -__builtin_inff() : __builtin_inff());
-#define MINMAX_STEP(ctrl, rm, bm)                                                                          \
-    {                                                                                                      \
-        const float o = __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), ctrl, rm, bm, false)); \
-        v = MAX ? fmaxf(v, o) : fminf(v, o);                                                              \
-    }
-    MINMAX_STEP(0x111, 0xf, 0xf)   // row_shr:1
-    MINMAX_STEP(0x112, 0xf, 0xf)   // row_shr:2
-    MINMAX_STEP(0x114, 0xf, 0xe)   // row_shr:4, lanes 4..15
-    MINMAX_STEP(0x118, 0xf, 0xc)   // row_shr:8, lanes 8..15
-    MINMAX_STEP(0x142, 0xa, 0xf)   // row_bcast:15 into rows 1, 3
-    MINMAX_STEP(0x143, 0xc, 0xf)   // row_bcast:31 into rows 2, 3
-#undef MINMAX_STEP
-    return v;
-}
-
-__device__ __forceinline__ uint32_t dpp_wave_sum(uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(v), 63);
-}
-
 // Tile ranges of the sorted pair keys, one thread per key: the first and
 // last key of each run write its range; the run's tile is stamped with the
 // draw's generation so no per-draw clear of the tile arrays is needed.
@@ -1415,24 +1507,19 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
 // per-launch counters (k_launch_begin, before any other kernel of the launch)
 __device__ __forceinline__ void reset_counters(Counters *ctr)
 {
-    ctr->tri_raster = 0;
     ctr->npairs = 0;
     ctr->nheavy = 0;
     ctr->nlight = 0;
     ctr->unsupported = 0;
-    ctr->nclip = 0;
     ctr->noverflow = 0;
     ctr->nbig = 0;
     ctr->sticky |= ctr->overflow_full;
     ctr->overflow_full = 0;
 }
 
-__global__ void __launch_bounds__(64) k_launch_begin(Counters *ctr, unsigned long long *scan_tail)
+__global__ void __launch_bounds__(64) k_launch_begin(Counters *ctr)
 {
-    if (threadIdx.x == 0) {
-        reset_counters(ctr);
-        if (scan_tail) *scan_tail = 0;   // the pair-offset scan's last element becomes the total
-    }
+    if (threadIdx.x == 0) reset_counters(ctr);
 }
 
 // ---------------------------------------------------------------------------
@@ -2528,12 +2615,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         }
         if (last >= 0) writer = 1u + s_rid[last];
     }
-    if (in_fb) {
-        if (px.dirty_d) S.depth[pix] = px.d;
-        if (px.dirty_s) S.stencil[pix] = (uint8_t)px.s;
-        S.writer[pix] = writer;
+    {
         // the colour of the pixel's last passing fragment, from its record
-        if ((kind & KIND_COLOR) && writer) {
+        // (its loads are issued before this item's stores, which they would
+        // otherwise wait behind)
+        const bool shade = in_fb && (kind & KIND_COLOR) && writer;
+        uint32_t packed = 0;
+        if (shade) {
             const RTri *r = P.rt + (writer - 1u);
             float w0e, w1e, w2e;
             edge_values(r, x, y, w0e, w1e, w2e);
@@ -2543,7 +2631,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             uint32_t meta;
             fragment_source(K, r, P.rcold ? P.rcold + (writer - 1u) : nullptr, w0e * inv, w1e * inv, w2e * inv, c, dep,
                             meta);
-            S.color[pix] = col_pack(col_clamp(c));
+            packed = col_pack(col_clamp(c));
+        }
+        if (in_fb) {
+            if (px.dirty_d) S.depth[pix] = px.d;
+            if (px.dirty_s) S.stencil[pix] = (uint8_t)px.s;
+            if (S.writer) S.writer[pix] = writer;
+            if (shade) S.color[pix] = packed;
         }
     }
     }
@@ -3226,6 +3320,30 @@ __global__ void __launch_bounds__(256) k_clear(uint32_t *color, float *depth, ui
     if (stencil) stencil[p] = sv;
 }
 
+// A clear of whole framebuffer rows: pixels [p0, p0 + n) of the planes,
+// four per thread (16-byte colour and depth stores, one 4-byte stencil store
+// where the four are inside the span; single stores at its ends).
+__global__ void __launch_bounds__(256) k_clear_span(uint32_t *color, float *depth, uint8_t *stencil, uint64_t p0,
+                                                    uint64_t n, uint32_t cv, float dv, uint8_t sv, const Counters *ctr)
+{
+    if (exec_poisoned(ctr)) return;   // an earlier launch of the stream runs again first
+    const uint64_t q = (p0 & ~3ull) + 4ull * ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const uint64_t end = p0 + n;
+    if (q >= end) return;
+    if (q >= p0 && q + 4 <= end) {
+        if (color) *reinterpret_cast<uint4 *>(color + q) = make_uint4(cv, cv, cv, cv);
+        if (depth) *reinterpret_cast<float4 *>(depth + q) = make_float4(dv, dv, dv, dv);
+        if (stencil) *reinterpret_cast<uint32_t *>(stencil + q) = 0x01010101u * sv;
+        return;
+    }
+    for (uint64_t p = q; p < q + 4; p++) {
+        if (p < p0 || p >= end) continue;
+        if (color) color[p] = cv;
+        if (depth) depth[p] = dv;
+        if (stencil) stencil[p] = sv;
+    }
+}
+
 // glReadPixels packing (reference src/gl_api.c:1180-1229), one thread per
 // output pixel of the band of window rows [r0, r0 + nrows): window row r is
 // framebuffer row fb_h - 1 - (y + r); rows outside the framebuffer are zero
@@ -3703,7 +3821,8 @@ struct Exec {
     std::vector<std::pair<uint8_t *, size_t>> ring;
     size_t ring_blk = 0, ring_off = 0, ring_used = 0;
     hipEvent_t upload_done = nullptr;
-    Arena vhot, vcold, rt, rcold, box, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, clip_list;
+    Arena vhot, vcold, rt, rcold, box, n_rt, rec_base, tri_tiles, pair_base, scan_tmp, big_list;
+    Arena blk_kept, blk_pairs, blk_off;   // per set-up block: kept fan triangles, pairs, first pair
     uint32_t overflow_cap = 0;
     Arena keys[2], vals[2], sort_tmp, tile_list;   // (tile, record) pairs and their sort
     Arena tile_start, tile_end, tile_gen;          // per launch-wide tile: bin range, generation of the range
@@ -3715,6 +3834,8 @@ struct Exec {
     std::vector<unsigned long long> stat_host;
     mtgl_dev_stats stats = {};
     bool timing = false;
+    uint32_t timing_every = 1;     // time one fragment-stage launch in this many
+    uint32_t timing_count = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
     std::vector<hipEvent_t> free_events;
 };
@@ -3729,6 +3850,7 @@ struct Launch {
     uint32_t tri_total = 0, tiles_total = 0;
     uint32_t vblocks = 0, sblocks = 0;
     bool any_cold = false;
+    bool multi_vs = false;         // some scene switches vertex states per vertex
     bool unsupported = false;
     uint32_t kinds = 0;            // bit (1 << kind) for each raster kind present
     uint64_t pairs_cap = 0;        // pairs of the sizing run (frozen replays use it)
@@ -3756,6 +3878,7 @@ struct LastDraw {
     uint32_t gen = 0;              // tile generation of its launch
     uint32_t over_base = 0;        // first overflow record slot of its launch
     int32_t tex_w = 0, tex_h = 0;
+    bool captured = false;         // its last-writer plane was recorded
 };
 
 // A clear, triangle draw or glDrawPixels queued on a context's stream that
@@ -3798,6 +3921,7 @@ struct mtgl_dev {
     std::vector<PendingOp> ops;    // queued clears, draws and glDrawPixels not yet confirmed
     uint32_t seq = 0;              // sequence number of the latest of them (never 0)
     bool replaying = false;
+    bool capture = false;          // draws record each pixel's last writer (mtgl_diff_arm)
 };
 
 static int confirm(mtgl_dev *d);
@@ -3911,6 +4035,11 @@ template <typename T>
 int device_scan(Exec &x, const T *in, T *out, uint64_t n, bool frozen, T *total_out = nullptr, T *clear = nullptr)
 {
     if (n == 0) return MTGL_OK;
+    if (n <= SCAN1_MAX<T>) {
+        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(SCAN1_THREADS), 0, x.stream, in, n, out, total_out, clear);
+        HIPC(hipGetLastError());
+        return MTGL_OK;
+    }
     uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     int rc = arena_reserve(x, x.scan_tmp, nb * sizeof(T), frozen);
     if (rc) return rc;
@@ -4010,7 +4139,7 @@ void exec_free(Exec &x)
 {
     if (x.stream) (void)hipStreamSynchronize(x.stream);
     Arena *arenas[] = {&x.vhot, &x.vcold, &x.rt, &x.rcold, &x.box, &x.n_rt, &x.rec_base, &x.tri_tiles, &x.pair_base,
-                       &x.scan_tmp, &x.clip_list, &x.keys[0], &x.keys[1], &x.vals[0], &x.vals[1], &x.sort_tmp,
+                       &x.scan_tmp, &x.big_list, &x.blk_kept, &x.blk_pairs, &x.blk_off, &x.keys[0], &x.keys[1], &x.vals[0], &x.vals[1], &x.sort_tmp,
                        &x.tile_list, &x.tile_start, &x.tile_end, &x.tile_gen};
     for (Arena *a : arenas) arena_free(*a);
     for (auto &pr : x.pending_events) {
@@ -4151,6 +4280,7 @@ VSlots vertex_slots(const mtgl_dev *d, const mtgl_draw *dr)
 // Triangle and tile totals of a set of scenes and the buffers they need.
 struct Sizes {
     uint64_t tri = 0, tiles = 0;
+    uint64_t sblocks = 0;          // set-up blocks (each scene starts a block of its own)
     bool cold = false;
 };
 
@@ -4163,6 +4293,7 @@ Sizes scene_sizes(const SceneIn *in, uint32_t n)
         const uint32_t ntri = tri_count(dr->mode, dr->count);
         if (ntri == 0) continue;
         z.tri += ntri;
+        z.sblocks += blocks_for(ntri, SETUP_BLOCK);
         z.tiles += (uint64_t)in[i].dev->tiles_x * (uint64_t)in[i].dev->tiles_y;
         z.cold |= needs_cold(dr->raster);
     }
@@ -4181,7 +4312,11 @@ int reserve_scene_buffers(Exec &x, const Sizes &z)
     if ((rc = arena_reserve(x, x.box, (t + x.overflow_cap) * sizeof(RBox)))) return rc;
     if ((rc = arena_reserve(x, x.n_rt, t * sizeof(uint32_t)))) return rc;
     if ((rc = arena_reserve(x, x.rec_base, t * sizeof(uint32_t)))) return rc;
-    if ((rc = arena_reserve(x, x.clip_list, t * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(x, x.big_list, t * sizeof(uint32_t)))) return rc;
+    const size_t sb = (size_t)std::max<uint64_t>(z.sblocks, 1u);
+    if ((rc = arena_reserve(x, x.blk_kept, sb * sizeof(uint32_t)))) return rc;
+    if ((rc = arena_reserve(x, x.blk_pairs, sb * sizeof(unsigned long long)))) return rc;
+    if ((rc = arena_reserve(x, x.blk_off, sb * sizeof(unsigned long long)))) return rc;
     if ((rc = arena_reserve(x, x.tri_tiles, (t + 1) * sizeof(unsigned long long)))) return rc;
     if ((rc = arena_reserve(x, x.pair_base, (t + 1) * sizeof(unsigned long long)))) return rc;
     return MTGL_OK;
@@ -4245,7 +4380,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
 
     const Sizes z = scene_sizes(in, n);
     uint64_t sb = 0;
-    for (const Plan &p : plan) sb += blocks_for(p.ntri, GEOM_BLOCK);
+    for (const Plan &p : plan) sb += blocks_for(p.ntri, SETUP_BLOCK);
     if (z.tri > 0x0FFFFFFFull || z.tiles >= (1ull << ITEM_TILE_BITS) / HEAVY_SPLIT) return MTGL_ERR_OUT_OF_MEMORY;
     L.tri_total = (uint32_t)z.tri;
     L.tiles_total = (uint32_t)z.tiles;
@@ -4256,7 +4391,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     int rc;
     if ((rc = reserve_scene_buffers(x, z))) return rc;
     for (const Plan &p : plan) {
-        if (raster_kind(p.s->dr->raster, p.tx) & KIND_LAST_WRITER) {
+        if ((raster_kind(p.s->dr->raster, p.tx) & KIND_LAST_WRITER) && p.s->dev->capture) {
             mtgl_dev *d = p.s->dev;
             if ((rc = arena_reserve(x, d->writer, (size_t)d->w * (size_t)d->h * 4))) return rc;
         }
@@ -4295,6 +4430,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         v.first = dr->first;
         v.index_type = dr->index_type;
         v.n_vstates = dr->n_vstates;
+        if (dr->n_vstates > 1 && dr->state_index.enabled) L.multi_vs = true;
         v.indices = resolve(d, dr->indices, staged_dev, stb);
         v.indices.enabled = dr->index_type != 0;
         v.position = resolve(d, dr->position, staged_dev, stb);
@@ -4335,7 +4471,6 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         q.n_rt = as<uint32_t>(x.n_rt) + toff;
         q.rec_base = as<uint32_t>(x.rec_base) + toff;
         q.tri_tiles = as<unsigned long long>(x.tri_tiles) + toff;
-        q.clip_list = as<uint32_t>(x.clip_list);
         q.overflow_cap = x.overflow_cap;
         q.ctr = x.ctr;
         sp[s] = q;
@@ -4346,7 +4481,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         r.depth = d->depth;
         r.stencil = d->stencil;
         r.kind = raster_kind(dr->raster, p.tx);
-        r.writer = (r.kind & KIND_LAST_WRITER) ? as<uint32_t>(d->writer) : nullptr;
+        r.writer = (r.kind & KIND_LAST_WRITER) && d->capture ? as<uint32_t>(d->writer) : nullptr;
         r.tex = p.tx ? p.tx->px : nullptr;
         r.tex1 = p.tx ? p.tx->m1 : nullptr;
         r.tex_w = p.tx ? p.tx->w : 0;
@@ -4369,7 +4504,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         tlp[s] = tileoff;
         toff += p.ntri;
         tileoff += (uint32_t)(d->tiles_x * d->tiles_y);
-        sbo += blocks_for(p.ntri, GEOM_BLOCK);
+        sbo += blocks_for(p.ntri, SETUP_BLOCK);
     }
     vbp[ns] = 0;
     sbp[ns] = sbo;
@@ -4418,7 +4553,9 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
         }
         size_t c = x.keys[0].cap;
         for (int k = 0; k < 2; k++) c = std::min(c, std::min(x.keys[k].cap, x.vals[k].cap));
-        L.pairs_cap = std::min<uint64_t>(c / 4, 0x7FFFFFFFull);
+        // the estimate, not the arenas' power-of-two size, bounds the launch
+        // (the sort's grid and histograms scale with it)
+        L.pairs_cap = std::min<uint64_t>(std::min<uint64_t>(c / 4, (uint64_t)want), 0x7FFFFFFFull);
     }
     const uint32_t cap = (uint32_t)L.pairs_cap;
     if ((rc = arena_reserve(x, x.tile_start, (size_t)ntiles * 4, frozen))) return rc;
@@ -4443,30 +4580,34 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
 
     // ---- K2: set-up, with the vertex transform
     unsigned long long *tri_tiles = as<unsigned long long>(x.tri_tiles);
-    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr, tri_tiles + ttot);
-    hipLaunchKernelGGL(k_setup, dim3(L.sblocks), dim3(GEOM_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns);
-    hipLaunchKernelGGL(k_setup_clip, dim3(blocks_for(ttot, 64)), dim3(64), 0, x.stream, L.sp, L.tri_pre, L.ns,
-                       (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
+    unsigned long long *blk_pairs = as<unsigned long long>(x.blk_pairs);
+    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
+    if (L.multi_vs)
+        hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap,
+                           L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+    else
+        hipLaunchKernelGGL(k_setup<false>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap,
+                           L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
     HIPC(hipGetLastError());
 
     // ---- K3: pair offsets and total (read on the device: the host never
     // waits), expansion in submission order, stable sort by tile, tile
     // ranges and the work list
     unsigned long long *pair_base = as<unsigned long long>(x.pair_base);
-    if ((rc = device_scan<unsigned long long>(x, tri_tiles, pair_base, (uint64_t)ttot + 1, frozen, &x.ctr->npairs)))
-        return rc;
+    unsigned long long *blk_off = as<unsigned long long>(x.blk_off);
+    if ((rc = device_scan<unsigned long long>(x, blk_pairs, blk_off, L.sblocks, frozen, &x.ctr->npairs))) return rc;
     const unsigned long long *np = &x.ctr->npairs;
     uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
     uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
-    // the clip queue has been consumed: it holds the long triangles now
-    hipLaunchKernelGGL(k_expand, dim3(std::max(1u, blocks_for(ttot, GEOM_BLOCK))), dim3(GEOM_BLOCK), 0, x.stream,
-                       (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
-                       (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
-                       keys, vals, as<uint32_t>(x.clip_list), x.ctr, seq);
+    hipLaunchKernelGGL(k_expand, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns,
+                       (const unsigned long long *)blk_off, (const unsigned long long *)tri_tiles, pair_base,
+                       (const uint32_t *)as<uint32_t>(x.n_rt), (const uint32_t *)as<uint32_t>(x.rec_base),
+                       (const RBox *)as<RBox>(x.box), (uint64_t)cap, keys, vals, as<uint32_t>(x.big_list), x.ctr, seq,
+                       (const uint32_t *)as<uint32_t>(x.blk_kept), L.sblocks);
     hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
-                       x.stream, (const unsigned long long *)pair_base, ttot, (const uint32_t *)as<uint32_t>(x.n_rt),
+                       x.stream, (const unsigned long long *)pair_base, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
-                       keys, vals, (const uint32_t *)as<uint32_t>(x.clip_list), (const Counters *)x.ctr);
+                       keys, vals, (const uint32_t *)as<uint32_t>(x.big_list), (const Counters *)x.ctr);
     // stable LSD radix sort by tile: pairs keep submission order within a tile
     int key_bits = 1;
     while ((1u << key_bits) <= ntiles) key_bits++;
@@ -4506,7 +4647,7 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     rp.ctr = x.ctr;
     rp.stats = x.stat_slots;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (x.timing && !frozen) {
+    if (x.timing && !frozen && x.timing_count++ % x.timing_every == 0u) {
         ev0 = take_event(x);
         ev1 = take_event(x);
         if (ev0) HIPC(hipEventRecord(ev0, x.stream));
@@ -4623,7 +4764,7 @@ int run_prims(mtgl_dev *d, const mtgl_draw *dr, int cls)
 
     unsigned long long *counts = as<unsigned long long>(x.tri_tiles);
     unsigned long long *base = as<unsigned long long>(x.pair_base);
-    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr, (unsigned long long *)nullptr);
+    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
     hipLaunchKernelGGL(k_vertex, dim3(blocks_for(dr->count, 256)), dim3(256), 0, x.stream,
                        (const VertexParams *)(dev + off_vp), (const uint32_t *)(dev + off_pre), (const uint32_t *)nullptr,
                        1u);
@@ -5072,8 +5213,16 @@ static int launch_clear(mtgl_dev *d, const PendingOp &op)
     uint32_t *c = (op.buffers & MTGL_BUF_COLOR) ? d->color : nullptr;
     float *z = (op.buffers & MTGL_BUF_DEPTH) ? d->depth : nullptr;
     uint8_t *st = (op.buffers & MTGL_BUF_STENCIL) ? d->stencil : nullptr;
-    hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->ex.stream, c, z, st, d->w, op.x0, op.y0, w,
-                       h, op.color, op.depth, op.stencil, (const Counters *)d->ex.ctr);
+    if (op.x0 == 0 && w == d->w) {
+        // whole rows: one contiguous span of each plane
+        const uint64_t p0 = (uint64_t)op.y0 * (uint64_t)d->w;
+        const uint64_t quads = (p0 + n - (p0 & ~3ull) + 3) / 4;
+        hipLaunchKernelGGL(k_clear_span, dim3(blocks_for(quads, 256)), dim3(256), 0, d->ex.stream, c, z, st, p0, n,
+                           op.color, op.depth, op.stencil, (const Counters *)d->ex.ctr);
+    } else {
+        hipLaunchKernelGGL(k_clear, dim3(blocks_for(n, 256)), dim3(256), 0, d->ex.stream, c, z, st, d->w, op.x0, op.y0,
+                           w, h, op.color, op.depth, op.stencil, (const Counters *)d->ex.ctr);
+    }
     HIPC(hipGetLastError());
     d->dirty = true;
     return MTGL_OK;
@@ -5592,6 +5741,7 @@ MTGL_API int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
         l.over_base = d->one.tri_total;
         l.tex_w = tx ? tx->w : 0;
         l.tex_h = tx ? tx->h : 0;
+        l.captured = d->capture;
     }
     return rc;
 }
@@ -5619,7 +5769,7 @@ static int draw_pixels_kernel(mtgl_dev *d, const mtgl_pixels *p, const uint8_t *
     P.blend_dst = p->blend_dst;
     P.alpha_ref = p->alpha_ref;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (timed && x.timing) {
+    if (timed && x.timing && x.timing_count++ % x.timing_every == 0u) {
         ev0 = take_event(x);
         ev1 = take_event(x);
         if (ev0) HIPC(hipEventRecord(ev0, x.stream));
@@ -5768,10 +5918,12 @@ static int confirm(mtgl_dev *d)
     return MTGL_OK;
 }
 
-MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int enable)
+MTGL_API int mtgl_dev_set_timing(mtgl_dev *d, int every)
 {
-    if (!d) return MTGL_ERR_INVALID;
-    d->ex.timing = enable != 0;
+    if (!d || every < 0) return MTGL_ERR_INVALID;
+    d->ex.timing = every != 0;
+    d->ex.timing_every = every ? (uint32_t)every : 1u;
+    d->ex.timing_count = 0;
     return MTGL_OK;
 }
 
@@ -5944,7 +6096,7 @@ bool diff_capturable(const LastDraw &l, int32_t tex_w, int32_t tex_h)
 {
     const mtgl_raster_state &rs = l.rs;
     const mtgl_vertex_state &vs = l.vs;
-    if (!l.valid || l.mode != GL_TRIANGLES || l.n_vstates != 1 || !l.color_array) return false;
+    if (!l.valid || !l.captured || l.mode != GL_TRIANGLES || l.n_vstates != 1 || !l.color_array) return false;
     if ((l.kind & (KIND_LAST_WRITER | KIND_COLOR)) != (KIND_LAST_WRITER | KIND_COLOR) || (rs.color_mask & 0xFu) != 0xFu)
         return false;
     if (rs.flags & (MTGL_F_BLEND | MTGL_F_ALPHA_TEST | MTGL_F_FOG)) return false;
@@ -6056,6 +6208,13 @@ MTGL_API void mtgl_diff_destroy(mtgl_diff *df)
     arena_free(df->lit);
     arena_free(df->counter);
     delete df;
+}
+
+MTGL_API int mtgl_diff_arm(mtgl_dev *d)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    d->capture = true;
+    return MTGL_OK;
 }
 
 MTGL_API int mtgl_diff_add_view(mtgl_diff *df, mtgl_dev *d)

@@ -42,7 +42,8 @@ GPX_DTYPE = np.dtype([("tri", "<i4"), ("b", "<f4", (3,)), ("texel", "<u4", (4,))
 def _bind(lib):
     vp = C.c_void_p
     for name, args in (("mtgl_diff_create", [C.c_uint32, C.c_int32, C.c_int32, C.POINTER(vp)]),
-                       ("mtgl_diff_add_view", [vp, vp]), ("mtgl_diff_info", [vp, C.POINTER(MtglDiffInfo)]),
+                       ("mtgl_diff_add_view", [vp, vp]), ("mtgl_diff_arm", [vp]),
+                       ("mtgl_diff_info", [vp, C.POINTER(MtglDiffInfo)]),
                        ("mtgl_diff_gbuffer", [vp, C.c_uint32, vp]),
                        ("mtgl_diff_forward", [vp, vp, vp, vp, vp, vp]),
                        ("mtgl_diff_backward", [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp])):
@@ -193,6 +194,9 @@ class DiffViews:
             for v in range(self.nviews):
                 ctx = g.create_context(W, H)
                 try:
+                    rc = g.lib.mtgl_diff_arm(g.device(ctx))
+                    if rc != 0:
+                        raise RuntimeError(f"{g.kind}: mtgl_diff_arm failed ({rc})")
                     draw_view(g, params, v, W, H, tex8)
                     if keep_frames:
                         self.frames.append(g.read_framebuffer(ctx, W, H)[0])

@@ -258,8 +258,9 @@ class GLLibrary:
     def stats_reset(self, ctx: int) -> None:
         self.lib.mtgl_dev_stats_reset(self.device(ctx))
 
-    def set_timing(self, ctx: int, on: bool) -> None:
-        self.lib.mtgl_dev_set_timing(self.device(ctx), 1 if on else 0)
+    def set_timing(self, ctx: int, on: bool, every: int = 1) -> None:
+        """Time one fragment-stage launch in `every` with HIP events (on), or none."""
+        self.lib.mtgl_dev_set_timing(self.device(ctx), max(1, int(every)) if on else 0)
 
     def finish(self, ctx: int) -> None:
         self.lib.mtgl_dev_finish(self.device(ctx))

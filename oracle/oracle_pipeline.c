@@ -80,7 +80,9 @@ struct mtgl_dev {
     uint32_t nsegs;
     mtgl_dev_stats stats;
     /* differentiable capture of the latest draw */
+    int armed;                       /* mtgl_diff_arm was called */
     int last_valid;
+    int last_armed;
     mtgl_draw last_dr;
     mtgl_raster_state last_rs;
     mtgl_vertex_state last_vs;
@@ -1651,7 +1653,7 @@ int mtgl_dev_draw(mtgl_dev *d, const mtgl_draw *dr)
     return unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
 }
 
-int mtgl_dev_set_timing(mtgl_dev *d, int enable) { (void)enable; return d ? MTGL_OK : MTGL_ERR_INVALID; }
+int mtgl_dev_set_timing(mtgl_dev *d, int every) { return d && every >= 0 ? MTGL_OK : MTGL_ERR_INVALID; }
 
 int mtgl_dev_stats_get(mtgl_dev *d, mtgl_dev_stats *out)
 {
@@ -1935,6 +1937,7 @@ static void or_diff_prepare(mtgl_dev *d, const mtgl_draw *dr)
     }
     d->gb_on = 1;
     d->last_valid = 1;
+    d->last_armed = d->armed;
 }
 
 typedef struct {
@@ -1978,10 +1981,20 @@ void mtgl_diff_destroy(mtgl_diff *df)
     free(df);
 }
 
+/* mtgl_diff_arm: the draws that follow are captured (the oracle records the
+ * writer of every pixel regardless; the flag mirrors the device's refusal of
+ * a draw made before arming). */
+int mtgl_diff_arm(mtgl_dev *d)
+{
+    if (!d) return MTGL_ERR_INVALID;
+    d->armed = 1;
+    return MTGL_OK;
+}
+
 int mtgl_diff_add_view(mtgl_diff *df, mtgl_dev *d)
 {
     if (!df || !d) return MTGL_ERR_INVALID;
-    if (!d->last_valid || !d->gbr) return MTGL_ERR_UNSUPPORTED;
+    if (!d->last_valid || !d->last_armed || !d->gbr) return MTGL_ERR_UNSUPPORTED;
     const or_tex *t = &d->tex[d->last_rs.texture];
     if (!t->px || t->w != df->tex_w || t->h != df->tex_h) return MTGL_ERR_UNSUPPORTED;
     if (df->nviews && (d->w != df->w || d->h != df->h)) return MTGL_ERR_INVALID;

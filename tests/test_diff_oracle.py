@@ -93,3 +93,31 @@ def test_wrong_shapes_raise_before_any_call(small):
     # float64 inputs of the right shapes are converted, not misread
     a = V.forward_np(P["vcol"].astype(np.float64), P["light_rgb"], P["tex"])
     assert np.array_equal(a, img)
+
+
+def check_unarmed_draw_is_refused(g):
+    """A draw made before mtgl_diff_arm left no last-writer plane: capturing it
+    is refused with MTGL_ERR_UNSUPPORTED (4); after arming it is taken."""
+    import ctypes as C
+    P = D.diff_params(grid=8, tex_size=16, views=1)
+    tex8, _ = D.quantise(P["tex"])
+    D._bind(g.lib)
+    h = C.c_void_p()
+    assert g.lib.mtgl_diff_create(P["verts"].shape[0], 16, 16, C.byref(h)) == 0
+    try:
+        ctx = g.create_context(W, H)
+        try:
+            D.draw_view(g, P, 0, W, H, tex8)
+            assert g.lib.mtgl_diff_add_view(h, g.device(ctx)) == 4
+            assert g.lib.mtgl_diff_arm(g.device(ctx)) == 0
+            D.draw_view(g, P, 0, W, H, tex8)
+            assert g.lib.mtgl_diff_add_view(h, g.device(ctx)) == 0
+        finally:
+            g.destroy_context(ctx)
+            g.release()
+    finally:
+        g.lib.mtgl_diff_destroy(h)
+
+
+def test_unarmed_draw_is_refused(oracle):
+    check_unarmed_draw_is_refused(oracle)

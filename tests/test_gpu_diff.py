@@ -103,3 +103,8 @@ def test_full_workload(mtgl):
         assert worst < 2e-3
     finally:
         Vd.close()
+
+
+def test_unarmed_draw_is_refused_on_device(mtgl):
+    from test_diff_oracle import check_unarmed_draw_is_refused
+    check_unarmed_draw_is_refused(mtgl)
