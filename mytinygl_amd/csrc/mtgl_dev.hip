@@ -314,9 +314,14 @@ __device__ __forceinline__ void normalize3(float *v)
     }
 }
 
-__device__ __forceinline__ float pow_cr(float b, float e) { return (float)pow((double)b, (double)e); }
-__device__ __forceinline__ float exp_cr(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float log2_cr(float x) { return (float)log2((double)x); }
+// Out of line: inlined, their fp64 polynomial constants are hoisted into the
+// raster kernels' loop preheaders, held across every item and spilled to
+// scratch (per-lane stores to memory on every wave); as calls they cost a
+// call only on the paths that evaluate them (fog EXP/EXP2, spot and specular
+// powers, per-pixel LOD).
+__device__ __attribute__((noinline)) float pow_cr(float b, float e) { return (float)pow((double)b, (double)e); }
+__device__ __attribute__((noinline)) float exp_cr(float x) { return (float)exp((double)x); }
+__device__ __attribute__((noinline)) float log2_cr(float x) { return (float)log2((double)x); }
 
 // m * (x, y, z, w) with left-to-right row sums
 __device__ __forceinline__ void xform(const float *m, float x, float y, float z, float w, float *o)
