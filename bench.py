@@ -70,15 +70,36 @@ def frame(g, n):
     S.draw_textured_sphere(g, n)
 
 
-def traffic_from_profile():
-    """HBM bytes per fragment-kernel launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
-    or None when no summary is present."""
+LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12   # 128 B/clk per CU, 256 CUs, 2.4 GHz
+
+
+def profile_figures():
+    """Figures of the committed rocprofv3 summary of the fragment kernel on
+    this workload (PMC counters: HBM bytes per launch from FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration, LDS bytes and the traced
+    kernel time, VALU busy and lane use, occupancy), {} when absent."""
     path = os.path.join(ROOT, "profiles", "fragment_stage_textured_sphere_1080p.md")
     if not os.path.exists(path):
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+        return {}
+    txt = open(path).read()
+    out = {}
+    for key in ("traffic_bytes_per_launch", "lds_bytes_per_launch", "lds_kernel_us", "valu_busy",
+                "valu_lane_utilization", "occupancy"):
+        m = re.search(rf"^{key}:\s*([0-9.eE+]+)", txt, re.M)
+        if m:
+            out[key] = float(m.group(1))
+    return out
+
+
+def cpu_library():
+    """(library, loader kind, baseline kind) of the CPU baselines: the
+    reference renderer built strict from its own sources under oracle/_ref
+    when present, else the oracle restatement."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "libmtgl_ref.so")
+    if os.path.exists(ref):
+        return ref, "reference", "reference"
+    port = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+    return (port, "oracle", "port") if os.path.exists(port) else (None, None, None)
 
 
 def cpu_baseline(budget_s: float):
@@ -401,6 +422,7 @@ def diff_cpu_baseline(P, T, nviews: int):
     V.close()
     g.release()
     return {"step_ms": round(sec * 1e3 * 64 / nviews, 1), "unit": "ms per 64-view step", "cores": 1, "kind": "port",
+            "cpu": cpu_model(), "extrapolated": True, "views_timed": nviews,
             "sample": f"{nviews} of the 64 views, forward + backward in {sec:.2f} s, one thread, scaled to 64"}
 
 
@@ -459,9 +481,9 @@ def pixels_bench(reps: int, cpu_reps: int):
                         "kernel": "k_draw_pixels"},
            "note": "ms_per_call includes the host-to-device copy of the image (pageable host memory -> pinned "
                    "staging -> HBM); the kernel reads an image already in HBM"}
-    lib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
-    if cpu_reps > 0 and os.path.exists(lib):
-        o = GLLibrary(lib, "oracle")
+    lib, lk, kind = cpu_library()
+    if cpu_reps > 0 and lib:
+        o = GLLibrary(lib, lk)
         octx = o.create_context(W, H)
         setup(o)
         optr = o.keep(img)
@@ -471,8 +493,9 @@ def pixels_bench(reps: int, cpu_reps: int):
         csec = time.perf_counter() - t0
         o.destroy_context(octx)
         o.release()
-        out["cpu_baseline"] = {"ms_per_call": round(csec * 1e3 / cpu_reps, 3), "cores": 1, "kind": "port",
-                               "sample": f"{cpu_reps} calls of the same workload on the oracle, one thread"}
+        out["cpu_baseline"] = {"ms_per_call": round(csec * 1e3 / cpu_reps, 3), "cores": 1, "kind": kind,
+                               "cpu": cpu_model(),
+                               "sample": f"{cpu_reps} calls of the same workload, one thread"}
     return out
 
 
@@ -492,10 +515,12 @@ def lines_bench(frames: int, cpu_frames: int, n_lines: int = 100_000, n_points: 
 
         frame()
         g.glFinish()
-        g.stats_reset(ctx)
-        frame()
-        g.glFinish()
-        frags = g.stats(ctx).fragments
+        frags = 0
+        if g.kind != "reference":   # (the reference keeps no counters)
+            g.stats_reset(ctx)
+            frame()
+            g.glFinish()
+            frags = g.stats(ctx).fragments
         t0 = time.perf_counter()
         for _ in range(nframes):
             frame()
@@ -511,13 +536,12 @@ def lines_bench(frames: int, cpu_frames: int, n_lines: int = 100_000, n_points: 
                        "depth test, one buffer object",
            "frames": frames, "ms_per_frame": round(sec * 1e3 / frames, 4), "fragments_per_frame": int(frags),
            "mfragments_s": round(frags * frames / sec / 1e6, 1)}
-    lib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
-    if cpu_frames > 0 and os.path.exists(lib):
-        ofr, osec = run(GLLibrary(lib, "oracle"), cpu_frames)
-        out["cpu_baseline"] = {"ms_per_frame": round(osec * 1e3 / cpu_frames, 2), "cores": 1, "kind": "port",
-                               "mfragments_s": round(ofr * cpu_frames / osec / 1e6, 2),
-                               "sample": f"{cpu_frames} frames of the same workload on the oracle, one thread"}
-        out["fragments_match_oracle"] = int(ofr) == int(frags)
+    lib, lk, kind = cpu_library()
+    if cpu_frames > 0 and lib:
+        _, osec = run(GLLibrary(lib, lk), cpu_frames)
+        out["cpu_baseline"] = {"ms_per_frame": round(osec * 1e3 / cpu_frames, 2), "cores": 1, "kind": kind,
+                               "cpu": cpu_model(), "mfragments_s": round(frags * cpu_frames / osec / 1e6, 2),
+                               "sample": f"{cpu_frames} frames of the same workload, one thread"}
     return out
 
 
@@ -565,7 +589,9 @@ def lists_bench(frames: int, cpu_frames: int, copies: int = 64):
                 g.glBufferData(E["GL_ARRAY_BUFFER"], inter.nbytes, ptr, E["GL_STATIC_DRAW"])
             for arr in ("GL_VERTEX_ARRAY", "GL_COLOR_ARRAY", "GL_NORMAL_ARRAY"):
                 g.glEnableClientState(E[arr])
-            base = 0 if how == "vbo" else ptr
+            # byte addresses: the buffer offset 0, or the client array's address
+            # (kept alive by keep); the attributes sit at +0, +12 and +28
+            base = 0 if how == "vbo" else int(ptr.ctypes.data)
             g.glVertexPointer(3, E["GL_FLOAT"], 40, base)
             g.glColorPointer(4, E["GL_FLOAT"], 40, base + 12)
             g.glNormalPointer(E["GL_FLOAT"], 40, base + 28)
@@ -585,16 +611,19 @@ def lists_bench(frames: int, cpu_frames: int, copies: int = 64):
 
         frame()
         g.glFinish()
-        g.stats_reset(ctx)
+        counted = g.kind != "reference"   # (the reference keeps no counters)
+        if counted:
+            g.stats_reset(ctx)
         t0 = time.perf_counter()
         for _ in range(nframes):
             frame()
         g.glFinish()
         sec = time.perf_counter() - t0
-        st = g.stats(ctx)
+        st = g.stats(ctx) if counted else None
         g.destroy_context(ctx)
         g.release()
-        return sec * 1e3 / nframes, st.upload_bytes / nframes, st.fragments / nframes
+        return (sec * 1e3 / nframes, st.upload_bytes / nframes if st else 0.0,
+                st.fragments / nframes if st else 0.0)
 
     lib = os.path.join(ROOT, "mytinygl_amd", "libmtgl.so")
     res = {how: run(GLLibrary(lib, "mtgl"), frames, how) for how in ("list", "client_arrays", "vbo")}
@@ -604,11 +633,11 @@ def lists_bench(frames: int, cpu_frames: int, copies: int = 64):
     for how, (ms, up, _) in res.items():
         out[f"ms_per_frame_{how}"] = round(ms, 3)
         out[f"upload_bytes_per_frame_{how}"] = int(up)
-    olib = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
-    if cpu_frames > 0 and os.path.exists(olib):
-        ms_cpu, _, _ = run(GLLibrary(olib, "oracle"), cpu_frames, "list")
-        out["cpu_baseline"] = {"ms_per_frame": round(ms_cpu, 2), "cores": 1, "kind": "port",
-                               "sample": f"{cpu_frames} frames of the list workload on the oracle, one thread"}
+    olib, lk, kind = cpu_library()
+    if cpu_frames > 0 and olib:
+        ms_cpu, _, _ = run(GLLibrary(olib, lk), cpu_frames, "list")
+        out["cpu_baseline"] = {"ms_per_frame": round(ms_cpu, 2), "cores": 1, "kind": kind, "cpu": cpu_model(),
+                               "sample": f"{cpu_frames} frames of the list workload, one thread"}
     return out
 
 
@@ -724,12 +753,25 @@ def main():
     bytes_per_launch = st.frag_bytes / args.steps   # one fragment-stage launch per frame
     achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
 
+    prof = profile_figures()
+    lds_fig = valu_fig = None
+    if "lds_bytes_per_launch" in prof and prof.get("lds_kernel_us"):
+        lds_tbs = prof["lds_bytes_per_launch"] / (prof["lds_kernel_us"] * 1e-6) / 1e12
+        lds_fig = {"achieved": round(lds_tbs, 3), "peak": round(LDS_PEAK_TBS, 1), "unit": "TB/s",
+                   "frac": round(lds_tbs / LDS_PEAK_TBS, 4),
+                   "source": "rocprofv3 SQ_INSTS_LDS_LOAD/STORE_BANDWIDTH x 64 B over the traced kernel time "
+                             "(profiles/fragment_stage_textured_sphere_1080p.md)"}
+    if "valu_busy" in prof:
+        valu_fig = {"busy": prof["valu_busy"], "lane_utilization": prof.get("valu_lane_utilization"),
+                    "occupancy": prof.get("occupancy"),
+                    "source": "rocprofv3 SQ_ACTIVE_INST_VALU, SQ_THREAD_CYCLES_VALU, SQ_WAVE_CYCLES, GRBM_GUI_ACTIVE"}
+
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     cpu_obj = None
     if cpu:
         cpu_fps = cpu["frames"] / cpu["seconds"]
         cpu_obj = {"value": round(frags_per_frame * cpu_fps / 1e6, 4), "unit": "Mfragments/s", "cores": 1,
-                   "kind": cpu["kind"],
+                   "kind": cpu["kind"], "cpu": cpu_model(),
                    "sample": f"{cpu['frames']} frames of the same 1080p textured-sphere workload in "
                              f"{cpu['seconds']:.1f} s, one thread ({os.cpu_count()} host CPUs visible)",
                    "frames_per_s": round(cpu_fps, 4)}
@@ -762,9 +804,10 @@ def main():
                    "texture": "256x256 RGBA8, GL_LINEAR_MIPMAP_LINEAR/GL_LINEAR, GL_REPEAT",
                    "fog": "GL_LINEAR 1..3"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic_from_profile(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": prof.get("traffic_bytes_per_launch"),
+                     "lds": lds_fig, "valu": valu_fig,
                      "kernel": "fragment stage (k_raster_depth: coverage, in-order depth pass, shading of the last passing fragment)", "kernel_ms": round(k_ms, 5),
-                     "launches": int(st.raster_launches),
+                     "launches_timed": int(st.raster_launches), "timed_one_in": TIMING_EVERY,
                      "bytes_per_launch": int(bytes_per_launch)},
         "cpu_baseline": cpu_obj,
         "batched": batched,

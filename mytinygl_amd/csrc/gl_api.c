@@ -1244,9 +1244,32 @@ MTGL_EXPORT void glFogfv(GLenum pname, const GLfloat *params)
 /* lighting (reference src/gl_api.c:1943-2262)                                */
 /* ------------------------------------------------------------------------- */
 
-MTGL_EXPORT void glLightfv(GLenum light, GLenum pname, const GLfloat *p)
+/* values a light, material or light-model parameter takes: the caller's array
+ * holds that many, no more (an unknown name reads one, then reports the error) */
+static int light_param_count(GLenum pname)
+{
+    switch (pname) {
+    case GL_AMBIENT: case GL_DIFFUSE: case GL_SPECULAR: case GL_POSITION: return 4;
+    case GL_SPOT_DIRECTION: return 3;
+    default: return 1;
+    }
+}
+
+static int material_param_count(GLenum pname)
+{
+    switch (pname) {
+    case GL_AMBIENT: case GL_DIFFUSE: case GL_SPECULAR: case GL_EMISSION: case GL_AMBIENT_AND_DIFFUSE: return 4;
+    default: return 1;
+    }
+}
+
+static int light_model_param_count(GLenum pname) { return pname == GL_LIGHT_MODEL_AMBIENT ? 4 : 1; }
+
+MTGL_EXPORT void glLightfv(GLenum light, GLenum pname, const GLfloat *params)
 {
     CTX_OR_RETURN();
+    GLfloat p[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    memcpy(p, params, (size_t)light_param_count(pname) * sizeof(float));
     gls_list_cmd cmd = { LOP_LIGHTFV, { light, pname } };
     memcpy(cmd.f, p, 4 * sizeof(float));
     if (gls_list_record(c, &cmd)) return;
@@ -1294,13 +1317,16 @@ MTGL_EXPORT void glLighti(GLenum light, GLenum pname, GLint param) { glLightf(li
 
 MTGL_EXPORT void glLightiv(GLenum light, GLenum pname, const GLint *params)
 {
-    GLfloat p[4] = { (GLfloat)params[0], (GLfloat)params[1], (GLfloat)params[2], (GLfloat)params[3] };
+    GLfloat p[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    for (int i = 0; i < light_param_count(pname); i++) p[i] = (GLfloat)params[i];
     glLightfv(light, pname, p);
 }
 
-MTGL_EXPORT void glMaterialfv(GLenum face, GLenum pname, const GLfloat *p)
+MTGL_EXPORT void glMaterialfv(GLenum face, GLenum pname, const GLfloat *params)
 {
     CTX_OR_RETURN();
+    GLfloat p[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    memcpy(p, params, (size_t)material_param_count(pname) * sizeof(float));
     gls_list_cmd cmd = { LOP_MATERIALFV, { face, pname } };
     memcpy(cmd.f, p, 4 * sizeof(float));
     if (gls_list_record(c, &cmd)) return;
@@ -1338,7 +1364,8 @@ MTGL_EXPORT void glMateriali(GLenum face, GLenum pname, GLint param) { glMateria
 
 MTGL_EXPORT void glMaterialiv(GLenum face, GLenum pname, const GLint *params)
 {
-    GLfloat p[4] = { (GLfloat)params[0], (GLfloat)params[1], (GLfloat)params[2], (GLfloat)params[3] };
+    GLfloat p[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    for (int i = 0; i < material_param_count(pname); i++) p[i] = (GLfloat)params[i];
     glMaterialfv(face, pname, p);
 }
 
@@ -1364,7 +1391,8 @@ MTGL_EXPORT void glLightModeli(GLenum pname, GLint param) { glLightModelf(pname,
 
 MTGL_EXPORT void glLightModeliv(GLenum pname, const GLint *params)
 {
-    GLfloat p[4] = { (GLfloat)params[0], (GLfloat)params[1], (GLfloat)params[2], (GLfloat)params[3] };
+    GLfloat p[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    for (int i = 0; i < light_model_param_count(pname); i++) p[i] = (GLfloat)params[i];
     glLightModelfv(pname, p);
 }
 

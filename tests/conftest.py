@@ -19,7 +19,8 @@ sys.path.insert(0, ROOT)
 from mytinygl_amd.gl import GLLibrary  # noqa: E402
 
 MTGL_LIB = os.path.join(ROOT, "mytinygl_amd", "libmtgl.so")
-ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
+# MTGL_ORACLE_LIB selects another build of the oracle (the sanitizer build)
+ORACLE_LIB = os.environ.get("MTGL_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "liboracle_gl.so")
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libmtgl_ref.so")
 
 
