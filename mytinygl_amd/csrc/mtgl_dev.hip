@@ -564,6 +564,17 @@ __device__ __forceinline__ T wave_inclusive_scan(T v)
     return v;
 }
 
+// 32-bit values: DPP row shifts and broadcasts, no LDS permutes
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) { return dpp_incl_scan(v); }
+
+// lane 63's value of v, to every lane
+template <typename T>
+__device__ __forceinline__ T wave_last(T v) { return __shfl(v, 63); }
+__device__ __forceinline__ uint32_t wave_last(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // clip-space vertex carried through clipping (the object-space normal the
 // reference also interpolates is never read after clipping and is dropped)
 struct CV {
@@ -987,17 +998,18 @@ __device__ __forceinline__ void stage_words(void *dst, const void *src, uint32_t
 // chain of dependent loads, not arithmetic); MULTI draws switch vertex states
 // per vertex and read them from HBM.
 template <bool MULTI>
-__global__ void __launch_bounds__(SETUP_BLOCK) k_setup(const SetupParams *Ss, const uint32_t *block_pre,
-                                                       const uint32_t *block_scene, uint32_t nscenes,
-                                                       uint32_t *blk_kept, unsigned long long *blk_pairs)
+__global__ void __launch_bounds__(SETUP_BLOCK) k_setup(const SetupParams *Ss, const VertexParams *Vs,
+                                                       const uint32_t *block_pre, const uint32_t *block_scene,
+                                                       uint32_t nscenes, uint32_t *blk_kept,
+                                                       unsigned long long *blk_pairs)
 {
     __shared__ SetupStage st;
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
+    // two rounds of loads: the scene's tables, then the states they point to
     stage_words(&st.sp, Ss + sc, sizeof(SetupParams));
+    stage_words(&st.vp, Vs + sc, sizeof(VertexParams));
     __syncthreads();
-    stage_words(&st.vp, st.sp.vp, sizeof(VertexParams));
     stage_words(&st.rs, st.sp.rs, offsetof(mtgl_raster_state, lighting));
-    __syncthreads();
     if (!MULTI) stage_words(&st.vs, st.vp.vstates, sizeof(mtgl_vertex_state));
     __syncthreads();
     const SetupParams &P = st.sp;
@@ -1201,7 +1213,7 @@ __global__ void __launch_bounds__(SCAN1_THREADS) k_scan_single(const T *in, uint
                 out[i] = carry + inc - v[k];
                 if (clear) clear[i] = (T)0;
             }
-            carry += __shfl(inc, 63);
+            carry += wave_last(inc);
         }
     }
     if (total_out && threadIdx.x == 0) *total_out = total;
@@ -2140,20 +2152,23 @@ __device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, in
                                                 int32_t fb_h, int32_t r0, int32_t r1)
 {
     const float fx0 = cv.fx0, fy0 = cv.fy0, fx1 = cv.fx1, fy1 = cv.fy1, fx2 = cv.fx2, fy2 = cv.fy2;
-    const float d0y = cv.d0y, d0x = cv.d0x, d1y = cv.d1y, d1x = cv.d1x, d2y = cv.d2y, d2x = cv.d2x;
     const bool live = valid && (area > 0.0f || area < 0.0f);
     const float sg = area > 0.0f ? 1.0f : -1.0f;
+    // sg * fl(p * q) == fl(p * (sg * q)) for sg = +-1 (rounding to nearest is
+    // symmetric), so the sign folds into the edge deltas once per lane
+    const float d0y = sg * cv.d0y, d0x = sg * cv.d0x, d1y = sg * cv.d1y, d1x = sg * cv.d1x;
+    const float d2y = sg * cv.d2y, d2x = sg * cv.d2x;
     float a0[TILE], a1[TILE], a2[TILE], b0[TILE], b1[TILE], b2[TILE];
 #pragma unroll
     for (int i = 0; i < TILE; i++) {
         const int32_t xi = X0 + i, yi = Y0 + i;
         const float pxi = (float)xi, pyi = (float)yi;
-        a0[i] = sg * ((pxi - fx1) * d0y);
-        a1[i] = sg * ((pxi - fx2) * d1y);
-        a2[i] = sg * ((pxi - fx0) * d2y);
-        b0[i] = sg * ((pyi - fy1) * d0x);
-        b1[i] = sg * ((pyi - fy2) * d1x);
-        b2[i] = sg * ((pyi - fy0) * d2x);
+        a0[i] = (pxi - fx1) * d0y;
+        a1[i] = (pxi - fx2) * d1y;
+        a2[i] = (pxi - fx0) * d2y;
+        b0[i] = (pyi - fy1) * d0x;
+        b1[i] = (pyi - fy2) * d1x;
+        b2[i] = (pyi - fy0) * d2x;
         if (!(live && xi >= minX && xi <= maxX && xi < fb_w)) a0[i] = -__builtin_inff();
         if (!(yi >= minY && yi <= maxY && yi < fb_h)) b0[i] = __builtin_inff();
     }
@@ -3868,7 +3883,11 @@ struct Launch {
     const uint32_t *tile_pre = nullptr;
     Arena maps;                    // block/tile -> scene maps of a launch with several scenes
     const uint32_t *vmap = nullptr, *smap = nullptr, *tmap = nullptr;
+    // the bytes the device copy of `params` holds (blocks up to PARAM_SHADOW_MAX):
+    // a launch whose prepared bytes equal them does not upload them again
+    std::vector<uint8_t> shadow;
 };
+constexpr size_t PARAM_SHADOW_MAX = 64u << 10;
 
 }  // namespace
 
@@ -4346,10 +4365,13 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     size_t bytes = 0;
     const uint64_t pairs_cap = L.pairs_cap;
     const Arena params = L.params, maps = L.maps;
+    std::vector<uint8_t> shadow;
+    shadow.swap(L.shadow);
     L = Launch();
     L.params = params;
     L.maps = maps;
     L.pairs_cap = pairs_cap;
+    L.shadow.swap(shadow);
     for (uint32_t i = 0; i < n; i++) {
         const mtgl_draw *dr = in[i].dr;
         if (!draw_mode_ok(dr->mode)) {
@@ -4401,7 +4423,9 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
             if ((rc = arena_reserve(x, d->writer, (size_t)d->w * (size_t)d->h * 4))) return rc;
         }
     }
+    const size_t params_cap = L.params.cap;
     if ((rc = arena_reserve(x, L.params, bytes))) return rc;
+    if (L.params.cap != params_cap) L.shadow.clear();   // a new device block holds nothing of ours
     uint8_t *host = ring_take(x, bytes);
     if (!host) return MTGL_ERR_OUT_OF_MEMORY;
     L.host = host;
@@ -4515,7 +4539,13 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     sbp[ns] = sbo;
     tp[ns] = toff;
     tlp[ns] = tileoff;
-    HIPC(hipMemcpyAsync(L.params.ptr, host, bytes, hipMemcpyHostToDevice, x.stream));
+    // redundant uploads are skipped: the device block already holds these
+    // bytes when the previous launch of L uploaded the same ones
+    if (!(L.shadow.size() == bytes && memcmp(L.shadow.data(), host, bytes) == 0)) {
+        HIPC(hipMemcpyAsync(L.params.ptr, host, bytes, hipMemcpyHostToDevice, x.stream));
+        if (bytes <= PARAM_SHADOW_MAX) L.shadow.assign(host, host + bytes);
+        else L.shadow.clear();
+    }
     L.vp = (const VertexParams *)(dev + off_vp);
     L.sp = (const SetupParams *)(dev + off_sp);
     L.sr = (const SceneRaster *)(dev + off_sr);
@@ -4588,11 +4618,11 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     unsigned long long *blk_pairs = as<unsigned long long>(x.blk_pairs);
     hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
     if (L.multi_vs)
-        hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap,
-                           L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+        hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.vp, L.sblock_pre,
+                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
     else
-        hipLaunchKernelGGL(k_setup<false>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap,
-                           L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+        hipLaunchKernelGGL(k_setup<false>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.vp, L.sblock_pre,
+                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
     HIPC(hipGetLastError());
 
     // ---- K3: pair offsets and total (read on the device: the host never
