@@ -2097,30 +2097,50 @@ __device__ __forceinline__ uint32_t texel_taps(const RasterConsts &K, float lod)
     return filter == GL_LINEAR ? 4u : 1u;
 }
 
-template <int R, int I>
-__device__ __forceinline__ void cover_pixel(const float *a0, const float *a1, const float *a2, float b0, float b1,
-                                            float b2, int32_t &mlo, int32_t &mhi)
+// Orders this wave's LDS accesses before and after it.  The raster kernels
+// run one wave per block, whose LDS operations complete in issue order, so
+// no s_barrier (and no wait on outstanding global loads) is needed.
+__device__ __forceinline__ void wave_sync()
 {
-    const bool p = (a0[I] >= b0) & (a1[I] >= b1) & (a2[I] >= b2);
-    const uint64_t bal = __builtin_amdgcn_ballot_w64(p);
-    if ((int)threadIdx.x == R * TILE + I) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The mask of the chunk's entries covering pixel (R, I): the AND of three
+// ballots (each compare writes its lane mask straight to scalar registers).
+// VIA_LDS: lane 0 stores it to the pixel's LDS slot and the pixel's lane
+// reads it back once every pixel is done (one wave: LDS operations complete
+// in order), 4 vector instructions per pixel; else the pixel's lane selects
+// it from the scalar result (no LDS traffic, about twice the instructions:
+// the general kernel, whose LDS is busier, keeps this form).
+template <int R, int I, bool VIA_LDS>
+__device__ __forceinline__ void cover_pixel(const float *a0, const float *a1, const float *a2, float b0, float b1,
+                                            float b2, uint64_t *s_mask, int32_t &mlo, int32_t &mhi)
+{
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(a0[I] >= b0) & __builtin_amdgcn_ballot_w64(a1[I] >= b1) &
+                         __builtin_amdgcn_ballot_w64(a2[I] >= b2);
+    if (VIA_LDS) {
+        if (threadIdx.x == 0) s_mask[R * TILE + I] = bal;
+    } else if ((int)threadIdx.x == R * TILE + I) {
         mlo = (int32_t)(uint32_t)bal;
         mhi = (int32_t)(uint32_t)(bal >> 32);
     }
 }
 
-template <int R>
+template <int R, bool VIA_LDS>
 __device__ __forceinline__ void cover_row(const float *a0, const float *a1, const float *a2, const float *b0,
-                                          const float *b1, const float *b2, int32_t &mlo, int32_t &mhi)
+                                          const float *b1, const float *b2, uint64_t *s_mask, int32_t &mlo,
+                                          int32_t &mhi)
 {
-    cover_pixel<R, 0>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 1>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 2>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 3>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 4>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 5>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 6>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
-    cover_pixel<R, 7>(a0, a1, a2, b0[R], b1[R], b2[R], mlo, mhi);
+    cover_pixel<R, 0, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 1, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 2, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 3, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 4, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 5, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 6, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
+    cover_pixel<R, 7, VIA_LDS>(a0, a1, a2, b0[R], b1[R], b2[R], s_mask, mlo, mhi);
 }
 
 
@@ -2147,9 +2167,10 @@ __device__ __forceinline__ RCov make_cov(const float4 *q)
 // covering it, kept by that pixel's lane.  Negative-area triangles are
 // tested with A and B negated; the bounding box and the framebuffer edge
 // fold in as -inf / +inf.
+template <bool VIA_LDS>
 __device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, int32_t minY, int32_t maxX, int32_t maxY,
                                                 float area, bool valid, int32_t X0, int32_t Y0, int32_t fb_w,
-                                                int32_t fb_h, int32_t r0, int32_t r1)
+                                                int32_t fb_h, int32_t r0, int32_t r1, uint64_t *s_mask)
 {
     const float fx0 = cv.fx0, fy0 = cv.fy0, fx1 = cv.fx1, fy1 = cv.fy1, fx2 = cv.fx2, fy2 = cv.fy2;
     const bool live = valid && (area > 0.0f || area < 0.0f);
@@ -2174,15 +2195,20 @@ __device__ __forceinline__ uint64_t cover_chunk(const RCov &cv, int32_t minX, in
     }
     // only the rows [r0, r1) of the item (a part of a heavy tile, or all)
     int32_t mlo = 0, mhi = 0;
-    if (r0 <= 0 && 0 < r1) cover_row<0>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 1 && 1 < r1) cover_row<1>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 2 && 2 < r1) cover_row<2>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 3 && 3 < r1) cover_row<3>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 4 && 4 < r1) cover_row<4>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 5 && 5 < r1) cover_row<5>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 6 && 6 < r1) cover_row<6>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    if (r0 <= 7 && 7 < r1) cover_row<7>(a0, a1, a2, b0, b1, b2, mlo, mhi);
-    return ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
+    if (r0 <= 0 && 0 < r1) cover_row<0, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 1 && 1 < r1) cover_row<1, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 2 && 2 < r1) cover_row<2, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 3 && 3 < r1) cover_row<3, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 4 && 4 < r1) cover_row<4, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 5 && 5 < r1) cover_row<5, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 6 && 6 < r1) cover_row<6, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (r0 <= 7 && 7 < r1) cover_row<7, VIA_LDS>(a0, a1, a2, b0, b1, b2, s_mask, mlo, mhi);
+    if (!VIA_LDS) return ((uint64_t)(uint32_t)mhi << 32) | (uint32_t)mlo;
+    wave_sync();
+    const int32_t row = (int32_t)(threadIdx.x >> 3);
+    const uint64_t m = row >= r0 && row < r1 ? s_mask[threadIdx.x] : 0ull;
+    wave_sync();   // read before the next chunk's masks are stored
+    return m;
 }
 
 // Depth and stencil of this lane's pixel for the entries in m, in
@@ -2364,16 +2390,6 @@ __device__ __forceinline__ void add_stats(const RasterParams &P, unsigned long l
     }
 }
 
-// Orders this wave's LDS accesses before and after it.  The raster kernels
-// run one wave per block, whose LDS operations complete in issue order, so
-// no s_barrier (and no wait on outstanding global loads) is needed.
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 constexpr int DCAP = 512;          // fragment depths held in LDS per batch (depth kernel)
 
 // K4a, the depth kernel of draws without blending, alpha test on a texture or
@@ -2393,6 +2409,7 @@ template <bool STENCIL>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster_depth(RasterParams P)
 {
     __shared__ RCov s_cov[CHUNK];
+    __shared__ uint64_t s_mask[TILE * TILE];   // coverage masks of a chunk, per pixel
     __shared__ uint32_t s_rid[CHUNK];
     __shared__ uint32_t s_tap[CHUNK];
     __shared__ float s_dep[DCAP];
@@ -2554,8 +2571,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
             s_rid[lane] = rid;
             s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
-        const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
-                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1);
+        const uint64_t m = cover_chunk<true>(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
+                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1, s_mask);
         const uint32_t cnt = (uint32_t)__popcll(m);
         frag_count += cnt;
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int32_t)dpp_incl_scan(cnt), 63);
@@ -2675,6 +2692,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
 {
     __shared__ RFrag s_rec[CHUNK];
     __shared__ RCov s_cov[CHUNK];
+    __shared__ uint64_t s_mask[TILE * TILE];   // coverage masks of a chunk, per pixel
     __shared__ uint32_t s_rid[CHUNK];
     __shared__ uint32_t s_tap[CHUNK];
     __shared__ float4 s_col[FCAP];
@@ -2747,8 +2765,8 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
             s_rid[lane] = rid;
             s_tap[lane] = texel_taps(K, REC_F(q, lod));
         }
-        const uint64_t m = cover_chunk(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
-                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1);
+        const uint64_t m = cover_chunk<false>(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
+                                       REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1, s_mask);
         frag_count += (unsigned long long)__popcll(m);
         wave_sync();   // staged records visible to every lane
 
