@@ -1455,23 +1455,20 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
 
 // The work items of the launch: HEAVY_SPLIT parts (TILE / HEAVY_SPLIT pixel
 // rows each) of every tile whose bin holds more than heavy_bin entries and
-// every tile with more
-// than MED_BIN entries whole, from the front of `list`, and every other
-// non-empty tile whole, from its back.  The raster kernels walk the list from
-// the front and their blocks start in list order, so the long items start
-// first.  One thread per tile; one atomic per block and list end.
-__global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
-                                                   const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
-                                                   uint32_t heavy_bin, const uint32_t *tile_scene,
-                                                   const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
-                                                   uint32_t list_cap, Counters *ctr)
+// every tile with more than MED_BIN entries whole, from the front of `list`,
+// and every other non-empty tile whole, from its back.  The raster kernels
+// walk the list from the front and their blocks start in list order, so the
+// long items start first.  One tile per thread of a block of NW waves (ne:
+// its bin [start, end) is non-empty), every thread of the block calls it; one
+// atomic per block and list end.  s_n and s_base are the block's LDS.
+template <int NW>
+__device__ __forceinline__ void append_tile_items(bool ne, uint32_t t, uint32_t start, uint32_t end,
+                                                  uint32_t heavy_bin, const uint32_t *tile_scene,
+                                                  const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
+                                                  uint32_t list_cap, Counters *ctr, uint32_t (*s_n)[NW],
+                                                  uint32_t *s_base)
 {
-    __shared__ uint32_t s_n[2][4];
-    __shared__ uint32_t s_base[2];
-    if (exec_poisoned(ctr)) return;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool ne = t < ntiles && tile_gen[t] == gen;
-    const uint32_t len = ne ? tile_end[t] - tile_start[t] : 0u;
+    const uint32_t len = ne ? end - start : 0u;
     const bool hv = ne && len > heavy_bin;
     const bool md = ne && !hv && len > MED_BIN;
     const bool lt = ne && !hv && !md;
@@ -1484,7 +1481,7 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t th = 0, tl = 0;
-        for (int w = 0; w < 4; w++) {
+        for (int w = 0; w < NW; w++) {
             const uint32_t a = s_n[0][w], c = s_n[1][w];
             s_n[0][w] = th;
             s_n[1][w] = tl;
@@ -1500,7 +1497,7 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     uint4 item = make_uint4(0u, 0u, 0u, 0u);
     if (ne) {
         const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
-        item = make_uint4(t - tile_pre[sc], tile_start[t], tile_end[t], sc);
+        item = make_uint4(t - tile_pre[sc], start, end, sc);
     }
     const uint32_t of = s_base[0] + s_n[0][wave] + HEAVY_SPLIT * (uint32_t)__popcll(bh & below) +
                         (uint32_t)__popcll(bm & below);
@@ -1512,6 +1509,197 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
     if (lt) {
         const uint32_t o = s_base[1] + s_n[1][wave] + (uint32_t)__popcll(bl & below);
         if (o < list_cap) list[list_cap - 1u - o] = item;
+    }
+    __syncthreads();   // s_n and s_base are read before a later call rewrites them
+}
+
+// The work list from the tile ranges k_segments stamped (one tile per thread).
+__global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
+                                                   const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
+                                                   uint32_t heavy_bin, const uint32_t *tile_scene,
+                                                   const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
+                                                   uint32_t list_cap, Counters *ctr)
+{
+    __shared__ uint32_t s_n[2][4];
+    __shared__ uint32_t s_base[2];
+    if (exec_poisoned(ctr)) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ne = t < ntiles && tile_gen[t] == gen;
+    append_tile_items<4>(ne, t, ne ? tile_start[t] : 0u, ne ? tile_end[t] : 0u, heavy_bin, tile_scene, tile_pre,
+                      nscenes, list, list_cap, ctr, s_n, s_base);
+}
+
+// The second walk of k_tile_bins over one wave's share [ws, we) of a
+// bucket: each pair goes to its tile's cursor plus its rank among the equal
+// tiles of its 64-pair chunk (ballots over the `shift` bits of the tile);
+// the first lane of each tile then advances the cursor, after every lane has
+// read it (one wave's LDS operations complete in order).  lk / val hold the
+// share when it fits one round (pre), else each round is loaded here.
+template <uint32_t R>
+__device__ __forceinline__ void bins_scatter(uint32_t (&lk)[R], uint32_t (&val)[R], uint32_t ws, uint32_t we,
+                                             bool pre, const uint32_t *keys, const uint32_t *vals, uint32_t t0,
+                                             uint32_t nb, int shift, uint32_t *cur, uint32_t *vals_out)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t c = ws; c < we; c += 64u * R) {   // (uniform in the wave)
+        if (!pre) {
+#pragma unroll
+            for (uint32_t r = 0; r < R; r++) {
+                const uint32_t i = c + r * 64u + (uint32_t)lane;
+                lk[r] = i < we ? keys[i] - t0 : 0xFFFFFFFFu;
+                val[r] = i < we ? vals[i] : 0u;
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++) {
+            if (c + r * 64u >= we) break;   // (uniform)
+            const bool ok = lk[r] < nb;
+            const uint32_t k = ok ? lk[r] : 0u;
+            uint64_t peers = __ballot(ok);
+            for (int bit = 0; bit < shift; bit++) {
+                const uint64_t m = __ballot((k >> bit) & 1u);
+                peers &= ((k >> bit) & 1u) ? m : ~m;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            const uint32_t base = ok ? cur[k] : 0u;
+            if (ok) vals_out[base + rank] = val[r];
+            if (ok && rank == 0u) cur[k] = base + (uint32_t)__popcll(peers);
+        }
+    }
+}
+
+// Second level of the tile sort when every tile key is below
+// RADIX << BINS_MAX_SHIFT.  The first level (k_sort_hist / k_sort_scatter on
+// the key's bits from `shift` up) left the pairs of each bucket of 1 << shift
+// consecutive tiles contiguous and in submission order; one workgroup per
+// bucket counts its tiles, sorts its pairs by tile, stably, stamps its
+// non-empty tiles with the launch generation and appends their work items.
+// Each wave owns a contiguous 1/BINS_WAVES of the bucket: a first walk counts
+// its pairs per tile, so every wave knows where its pairs of each tile start;
+// a second walk ranks equal tiles within each 64-pair chunk by ballots and
+// advances the wave's own cursors, so no barrier is needed between chunks.
+// A wave whose share fits BINS_ROUND chunks loads it once, for both walks;
+// the list slots are reserved before the second walk and filled after it,
+// so the atomics' round trip overlaps the walk.
+constexpr int BINS_WAVES = 16;
+constexpr int BINS_THREADS = 64 * BINS_WAVES;
+constexpr int BINS_MAX_SHIFT = 10;
+constexpr uint32_t BINS_ROUND = 8;
+static_assert((1 << BINS_MAX_SHIFT) <= BINS_THREADS, "one tile of a bucket per thread");
+__global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys, const uint32_t *vals,
+                                                            uint32_t n_max, const unsigned long long *n_dev,
+                                                            int shift, uint32_t nblocks,
+                                                            const uint32_t *offsets, uint32_t ntiles,
+                                                            uint32_t *vals_out, uint32_t *tile_gen, uint32_t gen,
+                                                            uint32_t heavy_bin, const uint32_t *tile_scene,
+                                                            const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
+                                                            uint32_t list_cap, Counters *ctr)
+{
+    __shared__ uint32_t cur[BINS_WAVES][1 << BINS_MAX_SHIFT];   // per wave and tile: count, then cursor
+    __shared__ uint32_t s_waves[BINS_WAVES];
+    __shared__ uint32_t s_n[2][BINS_WAVES];
+    __shared__ uint32_t s_base[2];
+    const uint32_t bk = blockIdx.x;
+    // the bucket's pairs: the scanned first-level counts are digit-major
+    // (the three loads are independent; nothing is written when poisoned)
+    const uint32_t poison = __builtin_amdgcn_readfirstlane(*(volatile const uint32_t *)&ctr->poison);
+    const uint32_t n = (uint32_t)min((unsigned long long)n_max, *n_dev);
+    const uint32_t oa = offsets[bk * nblocks];
+    const uint32_t oe = bk + 1u < (uint32_t)RADIX ? offsets[(bk + 1u) * nblocks] : 0xFFFFFFFFu;
+    if (poison != 0u) return;
+    const uint32_t a = min(oa, n), e = min(oe, n);
+    const uint32_t t0 = bk << shift;
+    const uint32_t nb = min(1u << shift, ntiles - t0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t len = e > a ? e - a : 0u;
+    const uint32_t share = ((len + BINS_WAVES - 1u) / BINS_WAVES + 63u) & ~63u;
+    const uint32_t ws = min(a + (uint32_t)w * share, e), we = min(ws + share, e);
+    const bool one_round = share <= 64u * BINS_ROUND;   // (uniform)
+    uint32_t lk[BINS_ROUND], val[BINS_ROUND];
+    auto load = [&](uint32_t c, bool with_vals) {
+#pragma unroll
+        for (uint32_t r = 0; r < BINS_ROUND; r++) {
+            const uint32_t i = c + r * 64u + (uint32_t)lane;
+            lk[r] = i < we ? keys[i] - t0 : 0xFFFFFFFFu;
+            if (with_vals) val[r] = i < we ? vals[i] : 0u;
+        }
+    };
+    if (one_round) load(ws, true);
+    for (uint32_t k = threadIdx.x; k < nb; k += BINS_THREADS)
+        for (int v = 0; v < BINS_WAVES; v++) cur[v][k] = 0u;
+    __syncthreads();
+    for (uint32_t c = ws; c < we; c += 64u * BINS_ROUND) {
+        if (!one_round) load(c, false);
+#pragma unroll
+        for (uint32_t r = 0; r < BINS_ROUND; r++)
+            if (lk[r] < nb) atomicAdd(&cur[w][lk[r]], 1u);
+    }
+    __syncthreads();
+    // tile k0 of the bucket per thread: its range, and its list slots
+    // reserved now and filled last
+    const uint32_t k0 = threadIdx.x;
+    uint32_t sum = 0;
+    if (k0 < nb)
+        for (int v = 0; v < BINS_WAVES; v++) sum += cur[v][k0];
+    uint32_t total;
+    uint32_t pos = a + block_exclusive_scan<uint32_t>(sum, s_waves, total);
+    const uint32_t start = pos;
+    if (k0 < nb)
+        for (int v = 0; v < BINS_WAVES; v++) {
+            const uint32_t cnt = cur[v][k0];
+            cur[v][k0] = pos;
+            pos += cnt;
+        }
+    const bool ne = k0 < nb && pos > start;
+    const uint32_t blen = pos - start;
+    const bool hv = ne && blen > heavy_bin;
+    const bool md = ne && !hv && blen > MED_BIN;
+    const bool ltl = ne && !hv && !md;
+    const uint64_t bh = __ballot(hv), bm = __ballot(md), bl = __ballot(ltl);
+    if (lane == 0) {
+        s_n[0][w] = HEAVY_SPLIT * (uint32_t)__popcll(bh) + (uint32_t)__popcll(bm);   // front items
+        s_n[1][w] = (uint32_t)__popcll(bl);
+    }
+    __syncthreads();
+    uint32_t hb = 0, lb = 0;
+    if (threadIdx.x == 0) {
+        uint32_t th = 0, tl = 0;
+        for (int v = 0; v < BINS_WAVES; v++) {
+            const uint32_t x0 = s_n[0][v], x1 = s_n[1][v];
+            s_n[0][v] = th;
+            s_n[1][v] = tl;
+            th += x0;
+            tl += x1;
+        }
+        hb = th ? atomicAdd(&ctr->nheavy, th) : 0u;
+        lb = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
+    }
+    __syncthreads();   // cursors and wave offsets in place
+    bins_scatter(lk, val, ws, we, one_round, keys, vals, t0, nb, shift, cur[w], vals_out);
+    if (threadIdx.x == 0) {
+        s_base[0] = hb;
+        s_base[1] = lb;
+    }
+    __syncthreads();
+    if (ne) {
+        tile_gen[t0 + k0] = gen;
+        const uint64_t below = (1ull << lane) - 1ull;
+        const uint32_t t = t0 + k0;
+        const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
+        const uint4 item = make_uint4(t - tile_pre[sc], start, pos, sc);
+        const uint32_t of = s_base[0] + s_n[0][w] + HEAVY_SPLIT * (uint32_t)__popcll(bh & below) +
+                            (uint32_t)__popcll(bm & below);
+        if (hv) {
+            for (uint32_t q = 0; q < HEAVY_SPLIT; q++)
+                if (of + q < list_cap)
+                    list[of + q] = make_uint4(item.x | ((q + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
+        }
+        if (md && of < list_cap) list[of] = item;
+        if (ltl) {
+            const uint32_t o = s_base[1] + s_n[1][w] + (uint32_t)__popcll(bl & below);
+            if (o < list_cap) list[list_cap - 1u - o] = item;
+        }
     }
 }
 
@@ -4661,25 +4849,43 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
                        x.stream, (const unsigned long long *)pair_base, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
                        keys, vals, (const uint32_t *)as<uint32_t>(x.big_list), (const Counters *)x.ctr);
-    // stable LSD radix sort by tile: pairs keep submission order within a tile
-    int key_bits = 1;
-    while ((1u << key_bits) <= ntiles) key_bits++;
     uint32_t *counts = as<uint32_t>(x.sort_tmp);
-    for (int shift = 0; shift < key_bits; shift += RADIX_BITS) {
+    // tile keys below RADIX << BINS_MAX_SHIFT: one sort pass on the key's
+    // bits from `shift` up puts each bucket of 1 << shift tiles in place,
+    // then one workgroup per bucket sorts it by tile and emits the work list
+    int shift = 0;
+    while (shift <= BINS_MAX_SHIFT && ((ntiles - 1u) >> shift) >= (uint32_t)RADIX) shift++;
+    if (ntiles > 0u && shift <= BINS_MAX_SHIFT) {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, cap, np,
                            shift, nblocks, counts);
         if ((rc = device_scan<uint32_t>(x, counts, counts, (uint64_t)RADIX * nblocks, frozen))) return rc;
         hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
                            (const uint32_t *)vals, cap, np, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
-        std::swap(keys, keys2);
-        std::swap(vals, vals2);
+        hipLaunchKernelGGL(k_tile_bins, dim3(((ntiles - 1u) >> shift) + 1u), dim3(BINS_THREADS), 0, x.stream,
+                           (const uint32_t *)keys2, (const uint32_t *)vals2, cap, np, shift, nblocks,
+                           (const uint32_t *)counts, ntiles, vals, as<uint32_t>(x.tile_gen), x.gen, x.heavy_bin,
+                           L.tmap, L.tile_pre, L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
+    } else {
+        // stable LSD radix sort by tile: pairs keep submission order within a tile
+        int key_bits = 1;
+        while ((1u << key_bits) <= ntiles) key_bits++;
+        for (int sh = 0; sh < key_bits; sh += RADIX_BITS) {
+            hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, cap,
+                               np, sh, nblocks, counts);
+            if ((rc = device_scan<uint32_t>(x, counts, counts, (uint64_t)RADIX * nblocks, frozen))) return rc;
+            hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
+                               (const uint32_t *)vals, cap, np, sh, nblocks, (const uint32_t *)counts, keys2, vals2);
+            std::swap(keys, keys2);
+            std::swap(vals, vals2);
+        }
+        hipLaunchKernelGGL(k_segments, dim3(blocks_for(cap, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, cap,
+                           np, ntiles, as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen),
+                           x.gen);
+        hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
+                           (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
+                           (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
+                           L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
     }
-    hipLaunchKernelGGL(k_segments, dim3(blocks_for(cap, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, cap, np,
-                       ntiles, as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen), x.gen);
-    hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
-                       (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
-                       (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
-                       L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
     HIPC(hipGetLastError());
     // raster grids: no more blocks than items, which the estimated pairs bound
     const uint64_t est_items = std::min<uint64_t>(cap, (uint64_t)((double)x.pair_ratio * (double)ttot) + 64u);
