@@ -994,16 +994,18 @@ __device__ __forceinline__ void stage_words(void *dst, const void *src, uint32_t
 // frustum plane pass through the clipper unchanged (each plane re-emits the
 // vertices in order), so they are set up directly into slot t; the others go
 // through setup_clipped.  Every kept record's tiles are counted into the
-// bins.  One wave per block (blocks spread over every CU: the kernel is a
-// chain of dependent loads, not arithmetic); MULTI draws switch vertex states
-// per vertex and read them from HBM.
+// bins.  A block takes SETUP_BLOCK triangles with three waves: wave c runs
+// the vertex pipeline of every triangle's corner c into LDS (three SIMDs at
+// once), then wave 0 sets the triangles up.  MULTI draws switch vertex
+// states per vertex and read them from HBM.
 template <bool MULTI>
-__global__ void __launch_bounds__(SETUP_BLOCK) k_setup(const SetupParams *Ss, const VertexParams *Vs,
+__global__ void __launch_bounds__(3 * SETUP_BLOCK) k_setup(const SetupParams *Ss, const VertexParams *Vs,
                                                        const uint32_t *block_pre, const uint32_t *block_scene,
                                                        uint32_t nscenes, uint32_t *blk_kept,
                                                        unsigned long long *blk_pairs)
 {
     __shared__ SetupStage st;
+    __shared__ CV s_corner[3][SETUP_BLOCK];
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     // two rounds of loads: the scene's tables, then the states they point to
     stage_words(&st.sp, Ss + sc, sizeof(SetupParams));
@@ -1015,20 +1017,25 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_setup(const SetupParams *Ss, co
     const SetupParams &P = st.sp;
     const VertexParams &V = st.vp;
     const mtgl_vertex_state *vs0 = MULTI ? V.vstates : &st.vs;
-    const uint32_t t = (blockIdx.x - block_pre[sc]) * blockDim.x + threadIdx.x;
+    const uint32_t l = threadIdx.x % SETUP_BLOCK, corner = threadIdx.x / SETUP_BLOCK;
+    const uint32_t t = (blockIdx.x - block_pre[sc]) * SETUP_BLOCK + l;
     const bool act = t < P.ntri;
-    const mtgl_raster_state *rs = &st.rs;
     const bool cold = P.cold != 0u;
+    if (act) {
+        uint32_t iv[3];
+        tri_vertices(P.mode, t, iv[0], iv[1], iv[2]);
+        const uint32_t i = corner == 0u ? iv[0] : corner == 1u ? iv[1] : iv[2];
+        CV o;
+        vertex_at<MULTI>(V, i, fetch_index(V, i), cold, o, vs0);
+        s_corner[corner][l] = o;
+    }
+    __syncthreads();
+    if (corner != 0u) return;   // (whole waves)
+    const mtgl_raster_state *rs = &st.rs;
     uint32_t nout = 0;
     unsigned long long tiles = 0;
     if (act) {
-        uint32_t ia, ib, ic;
-        tri_vertices(P.mode, t, ia, ib, ic);
-        const int32_t xa = fetch_index(V, ia), xb = fetch_index(V, ib), xc = fetch_index(V, ic);
-        CV v[3];
-        vertex_at<MULTI>(V, ia, xa, cold, v[0], vs0);
-        vertex_at<MULTI>(V, ib, xb, cold, v[1], vs0);
-        vertex_at<MULTI>(V, ic, xc, cold, v[2], vs0);
+        CV v[3] = {s_corner[0][l], s_corner[1][l], s_corner[2][l]};
         bool all_inside = true;
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -4824,11 +4831,11 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     unsigned long long *blk_pairs = as<unsigned long long>(x.blk_pairs);
     hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
     if (L.multi_vs)
-        hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.vp, L.sblock_pre,
-                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+        hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(3 * SETUP_BLOCK), 0, x.stream, L.sp, L.vp,
+                           L.sblock_pre, L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
     else
-        hipLaunchKernelGGL(k_setup<false>, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.vp, L.sblock_pre,
-                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+        hipLaunchKernelGGL(k_setup<false>, dim3(L.sblocks), dim3(3 * SETUP_BLOCK), 0, x.stream, L.sp, L.vp,
+                           L.sblock_pre, L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
     HIPC(hipGetLastError());
 
     // ---- K3: pair offsets and total (read on the device: the host never
