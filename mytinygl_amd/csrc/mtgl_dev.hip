@@ -133,7 +133,7 @@ struct __align__(16) RCold {
 // Counters of an Exec.  The first group belongs to one launch and is reset
 // by the launch's first kernel; the second persists across launches.
 struct Counters {
-    unsigned long long npairs;    // (tile, record) pairs of the launch, written by the bin-count scan
+    unsigned long long npairs;    // (tile, record) pairs of the launch (the pair-count scan or k_expand)
     unsigned int nheavy;          // work items at the front of the tile list: heavy-tile parts, medium tiles
     unsigned int nlight;          // the other non-empty tiles
     unsigned int unsupported;
@@ -1017,6 +1017,12 @@ __global__ void __launch_bounds__(3 * SETUP_BLOCK) k_setup(const SetupParams *Ss
     const SetupParams &P = st.sp;
     const VertexParams &V = st.vp;
     const mtgl_vertex_state *vs0 = MULTI ? V.vstates : &st.vs;
+    // the counters later kernels of the launch accumulate start at zero
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.ctr->nheavy = 0u;
+        P.ctr->nlight = 0u;
+        P.ctr->nbig = 0u;
+    }
     const uint32_t l = threadIdx.x % SETUP_BLOCK, corner = threadIdx.x / SETUP_BLOCK;
     const uint32_t t = (blockIdx.x - block_pre[sc]) * SETUP_BLOCK + l;
     const bool act = t < P.ntri;
@@ -1258,9 +1264,16 @@ __device__ __forceinline__ void expand_record(const RBox &q, uint32_t rec, uint6
 // blocks' pair sums) and follow in triangle order.  A thread writes its
 // triangle's pairs when there are at most EXPAND_SMALL, longer triangles are
 // queued for k_expand_big (one wave each) with their first pair in pair_base.
+// With blk_off null (launches of at most EXPAND_FOLD set-up blocks) each
+// block sums the pair counts of the blocks before it itself, and block 0
+// sums them all into npairs: no scan launch.  Block 0 also ends the set-up's
+// use of the per-launch counters: the clip pool, its overflow bits (kept in
+// sticky) and the unsupported flags start the next launch at zero.
+constexpr uint32_t EXPAND_FOLD = 2048;
 __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, const uint32_t *block_pre,
                                                         const uint32_t *block_scene, uint32_t nscenes,
                                                         const unsigned long long *blk_off,
+                                                        const unsigned long long *blk_pairs,
                                                         const unsigned long long *tri_tiles,
                                                         unsigned long long *pair_base, const uint32_t *n_rt,
                                                         const uint32_t *rec_base, const RBox *box, uint64_t cap,
@@ -1269,22 +1282,34 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
                                                         uint32_t nsblk)
 {
     const int lane = threadIdx.x;
+    unsigned long long first = 0, sum_all = 0;
+    if (blk_off) {
+        first = blk_off[blockIdx.x];
+    } else {
+        const uint32_t upto = blockIdx.x == 0 ? nsblk : blockIdx.x;
+        unsigned long long v = 0;
+        for (uint32_t b = lane; b < upto; b += 64) v += blk_pairs[b];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (blockIdx.x == 0) sum_all = v;
+        else first = v;
+    }
     if (blockIdx.x == 0) {
         // fan triangles kept by k_setup, one count per set-up block
         uint32_t kept = 0;
         for (uint32_t b = lane; b < nsblk; b += 64) kept += blk_kept[b];
         kept = (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(kept), 63);
         if (lane == 0) {
+            if (!blk_off) ctr->npairs = sum_all;
             // the launch's pair total against the capacity of the pair arrays:
             // a launch that overflowed (pairs, or the clip pool) poisons the
             // Exec with its sequence number, so that its raster stage and
             // every later framebuffer write are skipped until the host has
             // grown the arrays and run it again; a launch that fits adds to
             // the running counts
-            const unsigned long long total = ctr->npairs;
+            const unsigned long long total = blk_off ? ctr->npairs : sum_all;
             const uint32_t over = ctr->overflow_full | (total > cap ? 2u : 0u);
             if (over) {
-                ctr->overflow_full = over;
                 if (atomicCAS(&ctr->poison, 0u, seq) == 0u) {
                     ctr->poison_pairs = total;
                     ctr->poison_flags = over;
@@ -1293,6 +1318,10 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
                 ctr->acc_tri_raster += kept;
                 ctr->acc_bin_pairs += total;
             }
+            ctr->sticky |= over;
+            ctr->overflow_full = 0u;
+            ctr->noverflow = 0u;
+            ctr->unsupported = 0u;
         }
     }
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
@@ -1300,7 +1329,7 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
     const uint32_t tl = (blockIdx.x - block_pre[sc]) * SETUP_BLOCK + (uint32_t)lane;
     const uint32_t t = tri_off + tl;
     const uint64_t np = tl < ntri ? tri_tiles[t] : 0ull;
-    uint64_t p0 = blk_off[blockIdx.x] + wave_inclusive_scan(np) - np;
+    uint64_t p0 = first + wave_inclusive_scan(np) - np;
     const bool is_big = np > EXPAND_SMALL;
     // one queue slot reservation per wave
     const uint64_t bal = __ballot(is_big);
@@ -1716,7 +1745,9 @@ __global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys
 // ---------------------------------------------------------------------------
 
 
-// per-launch counters (k_launch_begin, before any other kernel of the launch)
+// per-launch counters at zero: k_launch_begin before the point / line path,
+// and before a tile launch when the previous one stopped before k_expand
+// (which, with k_setup, otherwise leaves them at zero)
 __device__ __forceinline__ void reset_counters(Counters *ctr)
 {
     ctr->npairs = 0;
@@ -4067,6 +4098,7 @@ struct Exec {
     std::vector<unsigned long long> stat_host;
     mtgl_dev_stats stats = {};
     bool timing = false;
+    bool ctr_clean = false;                        // per-launch counters at zero (k_expand's block 0 leaves them so)
     uint32_t timing_every = 1;     // time one fragment-stage launch in this many
     uint32_t timing_count = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_events;
@@ -4829,7 +4861,10 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     // ---- K2: set-up, with the vertex transform
     unsigned long long *tri_tiles = as<unsigned long long>(x.tri_tiles);
     unsigned long long *blk_pairs = as<unsigned long long>(x.blk_pairs);
-    hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
+    // k_setup and k_expand reset the per-launch counters as they go; a
+    // launch that stopped between them leaves them to k_launch_begin
+    if (!x.ctr_clean) hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
+    x.ctr_clean = false;
     if (L.multi_vs)
         hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(3 * SETUP_BLOCK), 0, x.stream, L.sp, L.vp,
                            L.sblock_pre, L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
@@ -4843,15 +4878,19 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     // ranges and the work list
     unsigned long long *pair_base = as<unsigned long long>(x.pair_base);
     unsigned long long *blk_off = as<unsigned long long>(x.blk_off);
-    if ((rc = device_scan<unsigned long long>(x, blk_pairs, blk_off, L.sblocks, frozen, &x.ctr->npairs))) return rc;
+    const bool fold = L.sblocks <= EXPAND_FOLD;
+    if (!fold && (rc = device_scan<unsigned long long>(x, blk_pairs, blk_off, L.sblocks, frozen, &x.ctr->npairs)))
+        return rc;
     const unsigned long long *np = &x.ctr->npairs;
     uint32_t *keys = as<uint32_t>(x.keys[0]), *vals = as<uint32_t>(x.vals[0]);
     uint32_t *keys2 = as<uint32_t>(x.keys[1]), *vals2 = as<uint32_t>(x.vals[1]);
     hipLaunchKernelGGL(k_expand, dim3(L.sblocks), dim3(SETUP_BLOCK), 0, x.stream, L.sp, L.sblock_pre, L.smap, L.ns,
-                       (const unsigned long long *)blk_off, (const unsigned long long *)tri_tiles, pair_base,
+                       fold ? nullptr : (const unsigned long long *)blk_off, (const unsigned long long *)blk_pairs,
+                       (const unsigned long long *)tri_tiles, pair_base,
                        (const uint32_t *)as<uint32_t>(x.n_rt), (const uint32_t *)as<uint32_t>(x.rec_base),
                        (const RBox *)as<RBox>(x.box), (uint64_t)cap, keys, vals, as<uint32_t>(x.big_list), x.ctr, seq,
                        (const uint32_t *)as<uint32_t>(x.blk_kept), L.sblocks);
+    x.ctr_clean = true;
     hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
                        x.stream, (const unsigned long long *)pair_base, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
