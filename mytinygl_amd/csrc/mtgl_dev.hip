@@ -1566,27 +1566,21 @@ __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, c
 }
 
 // The second walk of k_tile_bins over one wave's share [ws, we) of a
-// bucket: each pair goes to its tile's cursor plus its rank among the equal
-// tiles of its 64-pair chunk (ballots over the `shift` bits of the tile);
-// the first lane of each tile then advances the cursor, after every lane has
-// read it (one wave's LDS operations complete in order).  lk / val hold the
-// share when it fits one round (pre), else each round is loaded here.
-template <uint32_t R>
+// bucket (positions in the bucket's order of pairs): each pair goes to its
+// tile's cursor plus its rank among the equal tiles of its 64-pair chunk
+// (ballots over the `shift` bits of the tile); the first lane of each tile
+// then advances the cursor, after every lane has read it (one wave's LDS
+// operations complete in order).  lk / val hold the share when it fits one
+// round (pre), else load(c) fills them with the R chunks from position c.
+template <uint32_t R, typename Load>
 __device__ __forceinline__ void bins_scatter(uint32_t (&lk)[R], uint32_t (&val)[R], uint32_t ws, uint32_t we,
-                                             bool pre, const uint32_t *keys, const uint32_t *vals, uint32_t t0,
-                                             uint32_t nb, int shift, uint32_t *cur, uint32_t *vals_out)
+                                             bool pre, Load &&load, uint32_t nb, int shift, uint32_t *cur,
+                                             uint32_t *vals_out)
 {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (uint32_t c = ws; c < we; c += 64u * R) {   // (uniform in the wave)
-        if (!pre) {
-#pragma unroll
-            for (uint32_t r = 0; r < R; r++) {
-                const uint32_t i = c + r * 64u + (uint32_t)lane;
-                lk[r] = i < we ? keys[i] - t0 : 0xFFFFFFFFu;
-                val[r] = i < we ? vals[i] : 0u;
-            }
-        }
+        if (!pre) load(c, true);
 #pragma unroll
         for (uint32_t r = 0; r < R; r++) {
             if (c + r * 64u >= we) break;   // (uniform)
@@ -1605,12 +1599,99 @@ __device__ __forceinline__ void bins_scatter(uint32_t (&lk)[R], uint32_t (&val)[
     }
 }
 
+// First level of the tile sort, one workgroup per SORT_TILE pairs: the
+// block's pairs sorted by bucket (key >> shift), stably, within the block's
+// own range of keys_out / vals_out, and per bucket the block's count and
+// the start of its run within the block (rows of RADIX in counts and loc;
+// zero rows for blocks past the pairs).  Each wave owns a contiguous quarter
+// of the block: a first walk counts its pairs per bucket, a second ranks
+// equal buckets within each 64-pair chunk by ballots and advances the wave's
+// own cursors.
+constexpr uint32_t LOCAL_PER_WAVE = SORT_TILE / (SORT_THREADS / 64);
+constexpr uint32_t LOCAL_CHUNKS = LOCAL_PER_WAVE / 64;
+static_assert(SORT_THREADS == RADIX && LOCAL_CHUNKS * 64 * (SORT_THREADS / 64) == SORT_TILE, "local sort shape");
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_local(const uint32_t *keys_in, const uint32_t *vals_in,
+                                                             uint32_t n_max, const unsigned long long *n_dev,
+                                                             int shift, uint32_t *counts, uint32_t *loc,
+                                                             uint32_t *keys_out, uint32_t *vals_out)
+{
+    constexpr int W = SORT_THREADS / 64;
+    __shared__ uint32_t cur[W][RADIX];
+    __shared__ uint32_t s_waves[W];
+    const uint32_t n = (uint32_t)min((unsigned long long)n_max, *n_dev);
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    const uint32_t row = blockIdx.x * RADIX + threadIdx.x;
+    if (base >= n) {
+        counts[row] = 0u;
+        loc[row] = 0u;
+        return;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t kk[LOCAL_CHUNKS], vv[LOCAL_CHUNKS], d[LOCAL_CHUNKS];
+#pragma unroll
+    for (uint32_t r = 0; r < LOCAL_CHUNKS; r++) {
+        const uint32_t i = base + (uint32_t)w * LOCAL_PER_WAVE + r * 64u + (uint32_t)lane;
+        const bool valid = i < n;
+        kk[r] = valid ? keys_in[i] : 0u;
+        vv[r] = valid ? vals_in[i] : 0u;
+        d[r] = valid ? (kk[r] >> shift) & (RADIX - 1) : (uint32_t)RADIX;
+    }
+#pragma unroll
+    for (int v = 0; v < W; v++) cur[v][threadIdx.x] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < LOCAL_CHUNKS; r++)
+        if (d[r] < (uint32_t)RADIX) atomicAdd(&cur[w][d[r]], 1u);
+    __syncthreads();
+    // bucket threadIdx.x: the block's count, its start, each wave's cursor
+    uint32_t c[W], tot = 0;
+#pragma unroll
+    for (int v = 0; v < W; v++) {
+        c[v] = cur[v][threadIdx.x];
+        tot += c[v];
+    }
+    uint32_t all;
+    const uint32_t ex = block_exclusive_scan<uint32_t>(tot, s_waves, all);
+    counts[row] = tot;
+    loc[row] = ex;
+    uint32_t run = base + ex;
+#pragma unroll
+    for (int v = 0; v < W; v++) {
+        cur[v][threadIdx.x] = run;
+        run += c[v];
+    }
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (uint32_t r = 0; r < LOCAL_CHUNKS; r++) {
+        const bool ok = d[r] < (uint32_t)RADIX;
+        const uint32_t k = ok ? d[r] : 0u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int bit = 0; bit < RADIX_BITS; bit++) {
+            const uint64_t m = __ballot((k >> bit) & 1u);
+            peers &= ((k >> bit) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        const uint32_t at = ok ? cur[w][k] : 0u;
+        if (ok) {
+            keys_out[at + rank] = kk[r];
+            vals_out[at + rank] = vv[r];
+        }
+        if (ok && rank == 0u) cur[w][k] = at + (uint32_t)__popcll(peers);
+    }
+}
+
 // Second level of the tile sort when every tile key is below
-// RADIX << BINS_MAX_SHIFT.  The first level (k_sort_hist / k_sort_scatter on
-// the key's bits from `shift` up) left the pairs of each bucket of 1 << shift
-// consecutive tiles contiguous and in submission order; one workgroup per
-// bucket counts its tiles, sorts its pairs by tile, stably, stamps its
-// non-empty tiles with the launch generation and appends their work items.
+// RADIX << BINS_MAX_SHIFT: one workgroup per bucket of 1 << shift
+// consecutive tiles counts its tiles, sorts its pairs by tile, stably, into
+// vals_out, stamps its non-empty tiles with the launch generation and
+// appends their work items.  The bucket's pairs, in submission order, are
+// either one contiguous run (offsets: the scanned, digit-major counts of a
+// global first pass), or the runs k_sort_local left in each block, taken in
+// block order (counts / loc: a row of RADIX per block, at most
+// BINS_MAX_SEGS blocks); the bucket's place in vals_out is then the sum of
+// every block's start of its run.
 // Each wave owns a contiguous 1/BINS_WAVES of the bucket: a first walk counts
 // its pairs per tile, so every wave knows where its pairs of each tile start;
 // a second walk ranks equal tiles within each 64-pair chunk by ballots and
@@ -1622,49 +1703,94 @@ constexpr int BINS_WAVES = 16;
 constexpr int BINS_THREADS = 64 * BINS_WAVES;
 constexpr int BINS_MAX_SHIFT = 10;
 constexpr uint32_t BINS_ROUND = 8;
+constexpr uint32_t BINS_MAX_SEGS = BINS_THREADS;
 static_assert((1 << BINS_MAX_SHIFT) <= BINS_THREADS, "one tile of a bucket per thread");
 __global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys, const uint32_t *vals,
                                                             uint32_t n_max, const unsigned long long *n_dev,
                                                             int shift, uint32_t nblocks,
-                                                            const uint32_t *offsets, uint32_t ntiles,
+                                                            const uint32_t *offsets, const uint32_t *counts,
+                                                            const uint32_t *loc, uint32_t ntiles,
                                                             uint32_t *vals_out, uint32_t *tile_gen, uint32_t gen,
                                                             uint32_t heavy_bin, const uint32_t *tile_scene,
                                                             const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
                                                             uint32_t list_cap, Counters *ctr)
 {
     __shared__ uint32_t cur[BINS_WAVES][1 << BINS_MAX_SHIFT];   // per wave and tile: count, then cursor
+    __shared__ uint32_t seg_v[BINS_MAX_SEGS + 1];   // runs of the bucket: first position in bucket order
+    __shared__ uint32_t seg_src[BINS_MAX_SEGS];     // ... and in keys / vals
     __shared__ uint32_t s_waves[BINS_WAVES];
     __shared__ uint32_t s_n[2][BINS_WAVES];
     __shared__ uint32_t s_base[2];
     const uint32_t bk = blockIdx.x;
-    // the bucket's pairs: the scanned first-level counts are digit-major
-    // (the three loads are independent; nothing is written when poisoned)
+    // (the loads are independent; nothing is written when poisoned)
     const uint32_t poison = __builtin_amdgcn_readfirstlane(*(volatile const uint32_t *)&ctr->poison);
     const uint32_t n = (uint32_t)min((unsigned long long)n_max, *n_dev);
-    const uint32_t oa = offsets[bk * nblocks];
-    const uint32_t oe = bk + 1u < (uint32_t)RADIX ? offsets[(bk + 1u) * nblocks] : 0xFFFFFFFFu;
-    if (poison != 0u) return;
-    const uint32_t a = min(oa, n), e = min(oe, n);
+    uint32_t a, len, nseg = 1u;
+    if (!counts) {
+        const uint32_t oa = offsets[bk * nblocks];
+        const uint32_t oe = bk + 1u < (uint32_t)RADIX ? offsets[(bk + 1u) * nblocks] : 0xFFFFFFFFu;
+        if (poison != 0u) return;
+        a = min(oa, n);
+        const uint32_t e = min(oe, n);
+        len = e > a ? e - a : 0u;
+        if (threadIdx.x == 0) {
+            seg_v[0] = 0u;
+            seg_src[0] = a;
+            seg_v[1] = len;
+        }
+    } else {
+        // thread i: block i's run of the bucket
+        const uint32_t i = threadIdx.x;
+        const uint32_t c = i < nblocks ? counts[i * RADIX + bk] : 0u;
+        const uint32_t lo = i < nblocks ? loc[i * RADIX + bk] : 0u;
+        if (poison != 0u) return;
+        // one scan of (pairs, runs): a block holds at most SORT_TILE pairs,
+        // so the bucket at most BINS_MAX_SEGS * SORT_TILE < 1 << 21
+        uint32_t tot;
+        const uint32_t packed = block_exclusive_scan<uint32_t>(c | (c ? 1u << 21 : 0u), s_waves, tot);
+        len = tot & ((1u << 21) - 1u);
+        nseg = tot >> 21;
+        block_exclusive_scan<uint32_t>(lo, s_waves, a);   // a: the pairs of every earlier bucket
+        if (c) {
+            seg_v[packed >> 21] = packed & ((1u << 21) - 1u);
+            seg_src[packed >> 21] = i * (uint32_t)SORT_TILE + lo;
+        }
+        if (i == 0) seg_v[nseg] = len;
+    }
     const uint32_t t0 = bk << shift;
     const uint32_t nb = min(1u << shift, ntiles - t0);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t len = e > a ? e - a : 0u;
     const uint32_t share = ((len + BINS_WAVES - 1u) / BINS_WAVES + 63u) & ~63u;
-    const uint32_t ws = min(a + (uint32_t)w * share, e), we = min(ws + share, e);
+    const uint32_t ws = min((uint32_t)w * share, len), we = min(ws + share, len);
     const bool one_round = share <= 64u * BINS_ROUND;   // (uniform)
     uint32_t lk[BINS_ROUND], val[BINS_ROUND];
+    for (uint32_t k = threadIdx.x; k < nb; k += BINS_THREADS)
+        for (int v = 0; v < BINS_WAVES; v++) cur[v][k] = 0u;
+    __syncthreads();   // the runs are in place
+    // positions [c, c + 64 * BINS_ROUND) of the bucket, through its runs: the
+    // run of position c by a search every lane makes alike, then each lane
+    // steps over the (few) run ends before its own positions
     auto load = [&](uint32_t c, bool with_vals) {
+        uint32_t j = 0, hi = nseg;
+        while (hi - j > 1u) {
+            const uint32_t mid = (j + hi) >> 1;
+            if (seg_v[mid] <= c) j = mid;
+            else hi = mid;
+        }
 #pragma unroll
         for (uint32_t r = 0; r < BINS_ROUND; r++) {
-            const uint32_t i = c + r * 64u + (uint32_t)lane;
-            lk[r] = i < we ? keys[i] - t0 : 0xFFFFFFFFu;
-            if (with_vals) val[r] = i < we ? vals[i] : 0u;
+            const uint32_t v = c + r * 64u + (uint32_t)lane;
+            const bool in = v < we;
+            uint32_t i = 0;
+            if (in) {
+                while (seg_v[j + 1u] <= v) j++;   // (seg_v[nseg] = len ends every walk)
+                i = seg_src[j] + (v - seg_v[j]);
+            }
+            lk[r] = in ? keys[i] - t0 : 0xFFFFFFFFu;
+            if (with_vals) val[r] = in ? vals[i] : 0u;
         }
     };
     if (one_round) load(ws, true);
-    for (uint32_t k = threadIdx.x; k < nb; k += BINS_THREADS)
-        for (int v = 0; v < BINS_WAVES; v++) cur[v][k] = 0u;
-    __syncthreads();
     for (uint32_t c = ws; c < we; c += 64u * BINS_ROUND) {
         if (!one_round) load(c, false);
 #pragma unroll
@@ -1712,7 +1838,7 @@ __global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys
         lb = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
     }
     __syncthreads();   // cursors and wave offsets in place
-    bins_scatter(lk, val, ws, we, one_round, keys, vals, t0, nb, shift, cur[w], vals_out);
+    bins_scatter(lk, val, ws, we, one_round, load, nb, shift, cur[w], vals_out);
     if (threadIdx.x == 0) {
         s_base[0] = hb;
         s_base[1] = lb;
@@ -4843,7 +4969,8 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     const uint32_t list_cap = HEAVY_SPLIT * ntiles;   // up to HEAVY_SPLIT items per tile
     if ((rc = arena_reserve(x, x.tile_list, (size_t)list_cap * sizeof(uint4), frozen))) return rc;
     const uint32_t nblocks = (cap + SORT_TILE - 1) / SORT_TILE;
-    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)RADIX * nblocks * 4, frozen))) return rc;
+    // digit counts (and, for the local sort, the runs' starts)
+    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)2 * RADIX * nblocks * 4, frozen))) return rc;
     if (!frozen) {
         if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
             if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
@@ -4896,12 +5023,22 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
                        keys, vals, (const uint32_t *)as<uint32_t>(x.big_list), (const Counters *)x.ctr);
     uint32_t *counts = as<uint32_t>(x.sort_tmp);
-    // tile keys below RADIX << BINS_MAX_SHIFT: one sort pass on the key's
-    // bits from `shift` up puts each bucket of 1 << shift tiles in place,
+    // tile keys below RADIX << BINS_MAX_SHIFT: buckets of 1 << shift tiles
+    // by the key's bits from `shift` up, either sorted within each block of
+    // SORT_TILE pairs (up to BINS_MAX_SEGS blocks) or by one global pass,
     // then one workgroup per bucket sorts it by tile and emits the work list
     int shift = 0;
     while (shift <= BINS_MAX_SHIFT && ((ntiles - 1u) >> shift) >= (uint32_t)RADIX) shift++;
-    if (ntiles > 0u && shift <= BINS_MAX_SHIFT) {
+    if (ntiles > 0u && shift <= BINS_MAX_SHIFT && nblocks <= BINS_MAX_SEGS) {
+        // each block sorted by bucket in place; the buckets gather their runs
+        uint32_t *loc = counts + (size_t)RADIX * nblocks;
+        hipLaunchKernelGGL(k_sort_local, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
+                           (const uint32_t *)vals, cap, np, shift, counts, loc, keys2, vals2);
+        hipLaunchKernelGGL(k_tile_bins, dim3(((ntiles - 1u) >> shift) + 1u), dim3(BINS_THREADS), 0, x.stream,
+                           (const uint32_t *)keys2, (const uint32_t *)vals2, cap, np, shift, nblocks, nullptr,
+                           (const uint32_t *)counts, (const uint32_t *)loc, ntiles, vals, as<uint32_t>(x.tile_gen),
+                           x.gen, x.heavy_bin, L.tmap, L.tile_pre, L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
+    } else if (ntiles > 0u && shift <= BINS_MAX_SHIFT) {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys, cap, np,
                            shift, nblocks, counts);
         if ((rc = device_scan<uint32_t>(x, counts, counts, (uint64_t)RADIX * nblocks, frozen))) return rc;
@@ -4909,8 +5046,8 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
                            (const uint32_t *)vals, cap, np, shift, nblocks, (const uint32_t *)counts, keys2, vals2);
         hipLaunchKernelGGL(k_tile_bins, dim3(((ntiles - 1u) >> shift) + 1u), dim3(BINS_THREADS), 0, x.stream,
                            (const uint32_t *)keys2, (const uint32_t *)vals2, cap, np, shift, nblocks,
-                           (const uint32_t *)counts, ntiles, vals, as<uint32_t>(x.tile_gen), x.gen, x.heavy_bin,
-                           L.tmap, L.tile_pre, L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
+                           (const uint32_t *)counts, nullptr, nullptr, ntiles, vals, as<uint32_t>(x.tile_gen), x.gen,
+                           x.heavy_bin, L.tmap, L.tile_pre, L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
     } else {
         // stable LSD radix sort by tile: pairs keep submission order within a tile
         int key_bits = 1;
