@@ -1279,9 +1279,22 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
                                                         const uint32_t *rec_base, const RBox *box, uint64_t cap,
                                                         uint32_t *keys, uint32_t *vals, uint32_t *big,
                                                         Counters *ctr, uint32_t seq, const uint32_t *blk_kept,
-                                                        uint32_t nsblk)
+                                                        uint32_t nsblk, uint32_t ttot)
 {
     const int lane = threadIdx.x;
+    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
+    // one scene: its triangles are the launch's (no parameter loads)
+    const uint32_t ntri = nscenes == 1u ? ttot : Ss[sc].ntri;
+    const uint32_t tri_off = nscenes == 1u ? 0u : Ss[sc].tri_off;
+    const uint32_t bpre = nscenes == 1u ? 0u : block_pre[sc];
+    const uint32_t tl = (blockIdx.x - bpre) * SETUP_BLOCK + (uint32_t)lane;
+    const uint32_t t = tri_off + tl;
+    const bool act = tl < ntri;
+    // the triangle's set-up outputs, loaded with the block sums below (the
+    // one record of an unclipped triangle is its own slot t)
+    const uint64_t np = act ? tri_tiles[t] : 0ull;
+    const uint32_t nr = act ? n_rt[t] : 0u, rb = act ? rec_base[t] : 0u;
+    const RBox q0 = act ? box[t] : RBox{0u, 1u, 0u, 0u};
     unsigned long long first = 0, sum_all = 0;
     if (blk_off) {
         first = blk_off[blockIdx.x];
@@ -1307,28 +1320,27 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
             // every later framebuffer write are skipped until the host has
             // grown the arrays and run it again; a launch that fits adds to
             // the running counts
+            // every counter this reads, in one round of loads (only this
+            // lane of the launch writes the running counts and sticky)
             const unsigned long long total = blk_off ? ctr->npairs : sum_all;
-            const uint32_t over = ctr->overflow_full | (total > cap ? 2u : 0u);
+            const uint32_t ovf = ctr->overflow_full, poison = ctr->poison, sticky = ctr->sticky;
+            const unsigned long long acc_tri = ctr->acc_tri_raster, acc_pairs = ctr->acc_bin_pairs;
+            const uint32_t over = ovf | (total > cap ? 2u : 0u);
             if (over) {
                 if (atomicCAS(&ctr->poison, 0u, seq) == 0u) {
                     ctr->poison_pairs = total;
                     ctr->poison_flags = over;
                 }
-            } else if (ctr->poison == 0u) {   // (a launch after a poisoned one runs again: counted then)
-                ctr->acc_tri_raster += kept;
-                ctr->acc_bin_pairs += total;
+            } else if (poison == 0u) {   // (a launch after a poisoned one runs again: counted then)
+                ctr->acc_tri_raster = acc_tri + kept;
+                ctr->acc_bin_pairs = acc_pairs + total;
             }
-            ctr->sticky |= over;
+            ctr->sticky = sticky | over;
             ctr->overflow_full = 0u;
             ctr->noverflow = 0u;
             ctr->unsupported = 0u;
         }
     }
-    const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
-    const uint32_t ntri = Ss[sc].ntri, tri_off = Ss[sc].tri_off;
-    const uint32_t tl = (blockIdx.x - block_pre[sc]) * SETUP_BLOCK + (uint32_t)lane;
-    const uint32_t t = tri_off + tl;
-    const uint64_t np = tl < ntri ? tri_tiles[t] : 0ull;
     uint64_t p0 = first + wave_inclusive_scan(np) - np;
     const bool is_big = np > EXPAND_SMALL;
     // one queue slot reservation per wave
@@ -1344,9 +1356,8 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
         }
     }
     if (np == 0) return;
-    const uint32_t nr = n_rt[t], rb = rec_base[t];
     for (uint32_t j = 0; j < nr; j++) {
-        const RBox q = box[rb + j];
+        const RBox q = j == 0u && rb == t ? q0 : box[rb + j];
         expand_record(q, rb + j, p0, 0u, q.ntiles, 1u, cap, keys, vals);
         p0 += q.ntiles;
     }
@@ -5016,7 +5027,7 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
                        (const unsigned long long *)tri_tiles, pair_base,
                        (const uint32_t *)as<uint32_t>(x.n_rt), (const uint32_t *)as<uint32_t>(x.rec_base),
                        (const RBox *)as<RBox>(x.box), (uint64_t)cap, keys, vals, as<uint32_t>(x.big_list), x.ctr, seq,
-                       (const uint32_t *)as<uint32_t>(x.blk_kept), L.sblocks);
+                       (const uint32_t *)as<uint32_t>(x.blk_kept), L.sblocks, ttot);
     x.ctr_clean = true;
     hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
                        x.stream, (const unsigned long long *)pair_base, (const uint32_t *)as<uint32_t>(x.n_rt),
