@@ -479,8 +479,9 @@ def pixels_bench(reps: int, cpu_reps: int):
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 5), "bytes_per_launch": 20 * px,
                         "kernel": "k_draw_pixels"},
-           "note": "ms_per_call includes the host-to-device copy of the image (pageable host memory -> pinned "
-                   "staging -> HBM); the kernel reads an image already in HBM"}
+           "note": "ms_per_call includes the host-to-device copy of the image (pageable host memory -> HBM, "
+                   "after the context's queued work: images of 1 MB and more are not staged); the kernel "
+                   "reads an image already in HBM"}
     lib, lk, kind = cpu_library()
     if cpu_reps > 0 and lib:
         o = GLLibrary(lib, lk)

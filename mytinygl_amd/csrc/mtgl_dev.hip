@@ -72,7 +72,8 @@ constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | part
 constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
 static_assert(TILE % HEAVY_SPLIT == 0 && HEAVY_SPLIT < (1u << (32 - ITEM_TILE_BITS)), "heavy-tile parts");
 constexpr uint32_t MED_BIN = 40;        // whole tiles with longer bins join the front of the tile list
-constexpr int SETUP_BLOCK = 64;         // threads per block of the set-up kernel (one wave)
+constexpr int SETUP_BLOCK = 64;         // triangles per block of the set-up kernel (one per lane)
+constexpr uint32_t SETUP_SPLIT_CU = 4;  // set-up launches of at most this many blocks per CU: a wave per corner
 constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
@@ -135,7 +136,7 @@ struct __align__(16) RCold {
 struct Counters {
     unsigned long long npairs;    // (tile, record) pairs of the launch (the pair-count scan or k_expand)
     unsigned int nheavy;          // work items at the front of the tile list: heavy-tile parts, medium tiles
-    unsigned int nlight;          // the other non-empty tiles
+    unsigned int nlight;          // the other non-empty tiles (with nheavy: one 64-bit word for the list atomics)
     unsigned int unsupported;
     unsigned int noverflow;       // overflow slots requested by clipped triangles
     unsigned int overflow_full;   // bit 0: the clip pool was too small; bit 1: the pair arrays were
@@ -151,6 +152,9 @@ struct Counters {
     unsigned long long acc_tri_raster; // fan triangles and pairs of every launch that did not overflow
     unsigned long long acc_bin_pairs;
 };
+
+static_assert(offsetof(Counters, nheavy) % 8 == 0 && offsetof(Counters, nlight) == offsetof(Counters, nheavy) + 4,
+              "nheavy and nlight form one 64-bit word");
 
 // attribute stream resolved to a device pointer
 struct DArr {
@@ -994,18 +998,21 @@ __device__ __forceinline__ void stage_words(void *dst, const void *src, uint32_t
 // frustum plane pass through the clipper unchanged (each plane re-emits the
 // vertices in order), so they are set up directly into slot t; the others go
 // through setup_clipped.  Every kept record's tiles are counted into the
-// bins.  A block takes SETUP_BLOCK triangles with three waves: wave c runs
-// the vertex pipeline of every triangle's corner c into LDS (three SIMDs at
-// once), then wave 0 sets the triangles up.  MULTI draws switch vertex
-// states per vertex and read them from HBM.
-template <bool MULTI>
-__global__ void __launch_bounds__(3 * SETUP_BLOCK) k_setup(const SetupParams *Ss, const VertexParams *Vs,
-                                                       const uint32_t *block_pre, const uint32_t *block_scene,
-                                                       uint32_t nscenes, uint32_t *blk_kept,
-                                                       unsigned long long *blk_pairs)
+// bins.  A block takes SETUP_BLOCK triangles.  With NW = 3 waves (launches
+// of few blocks, which leave SIMDs idle) wave c runs the vertex pipeline of
+// every triangle's corner c into LDS, three SIMDs at once, then wave 0 sets
+// the triangles up; with NW = 1 (launches that fill the GPU) the one wave
+// runs all three corners itself.  MULTI draws switch vertex states per
+// vertex and read them from HBM.
+template <bool MULTI, int NW>
+__global__ void __launch_bounds__(NW * SETUP_BLOCK) k_setup(const SetupParams *Ss, const VertexParams *Vs,
+                                                            const uint32_t *block_pre, const uint32_t *block_scene,
+                                                            uint32_t nscenes, uint32_t *blk_kept,
+                                                            unsigned long long *blk_pairs)
 {
+    static_assert(NW == 1 || NW == 3, "one wave, or one per corner");
     __shared__ SetupStage st;
-    __shared__ CV s_corner[3][SETUP_BLOCK];
+    __shared__ CV s_corner[NW == 3 ? 3 : 1][NW == 3 ? SETUP_BLOCK : 1];
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     // two rounds of loads: the scene's tables, then the states they point to
     stage_words(&st.sp, Ss + sc, sizeof(SetupParams));
@@ -1027,21 +1034,34 @@ __global__ void __launch_bounds__(3 * SETUP_BLOCK) k_setup(const SetupParams *Ss
     const uint32_t t = (blockIdx.x - block_pre[sc]) * SETUP_BLOCK + l;
     const bool act = t < P.ntri;
     const bool cold = P.cold != 0u;
-    if (act) {
-        uint32_t iv[3];
-        tri_vertices(P.mode, t, iv[0], iv[1], iv[2]);
-        const uint32_t i = corner == 0u ? iv[0] : corner == 1u ? iv[1] : iv[2];
-        CV o;
-        vertex_at<MULTI>(V, i, fetch_index(V, i), cold, o, vs0);
-        s_corner[corner][l] = o;
+    if (NW == 3) {
+        if (act) {
+            uint32_t iv[3];
+            tri_vertices(P.mode, t, iv[0], iv[1], iv[2]);
+            const uint32_t i = corner == 0u ? iv[0] : corner == 1u ? iv[1] : iv[2];
+            CV o;
+            vertex_at<MULTI>(V, i, fetch_index(V, i), cold, o, vs0);
+            s_corner[NW == 3 ? corner : 0u][NW == 3 ? l : 0u] = o;
+        }
+        __syncthreads();
+        if (corner != 0u) return;   // (whole waves)
     }
-    __syncthreads();
-    if (corner != 0u) return;   // (whole waves)
     const mtgl_raster_state *rs = &st.rs;
     uint32_t nout = 0;
     unsigned long long tiles = 0;
     if (act) {
-        CV v[3] = {s_corner[0][l], s_corner[1][l], s_corner[2][l]};
+        CV v[3];
+        if (NW == 3) {
+            for (int k = 0; k < 3; k++) v[k] = s_corner[NW == 3 ? k : 0][NW == 3 ? l : 0u];
+        } else {
+            // the three indices first, so their loads overlap
+            uint32_t ia, ib, ic;
+            tri_vertices(P.mode, t, ia, ib, ic);
+            const int32_t xa = fetch_index(V, ia), xb = fetch_index(V, ib), xc = fetch_index(V, ic);
+            vertex_at<MULTI>(V, ia, xa, cold, v[0], vs0);
+            vertex_at<MULTI>(V, ib, xb, cold, v[1], vs0);
+            vertex_at<MULTI>(V, ic, xc, cold, v[2], vs0);
+        }
         bool all_inside = true;
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -1308,9 +1328,16 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
         else first = v;
     }
     if (blockIdx.x == 0) {
-        // fan triangles kept by k_setup, one count per set-up block
+        // fan triangles kept by k_setup, one count per set-up block (16 B
+        // loads, several in flight: a batch launch has tens of thousands)
         uint32_t kept = 0;
-        for (uint32_t b = lane; b < nsblk; b += 64) kept += blk_kept[b];
+        const uint4 *kv = reinterpret_cast<const uint4 *>(blk_kept);
+#pragma unroll 8
+        for (uint32_t b = lane; b < nsblk / 4u; b += 64) {
+            const uint4 q = kv[b];
+            kept += q.x + q.y + q.z + q.w;
+        }
+        for (uint32_t b = (nsblk & ~3u) + lane; b < nsblk; b += 64) kept += blk_kept[b];
         kept = (uint32_t)__builtin_amdgcn_readlane((int)dpp_incl_scan(kept), 63);
         if (lane == 0) {
             if (!blk_off) ctr->npairs = sum_all;
@@ -1505,75 +1532,71 @@ __global__ void __launch_bounds__(256) k_segments(const uint32_t *keys, uint32_t
 // every tile with more than MED_BIN entries whole, from the front of `list`,
 // and every other non-empty tile whole, from its back.  The raster kernels
 // walk the list from the front and their blocks start in list order, so the
-// long items start first.  One tile per thread of a block of NW waves (ne:
-// its bin [start, end) is non-empty), every thread of the block calls it; one
-// atomic per block and list end.  s_n and s_base are the block's LDS.
-template <int NW>
-__device__ __forceinline__ void append_tile_items(bool ne, uint32_t t, uint32_t start, uint32_t end,
-                                                  uint32_t heavy_bin, const uint32_t *tile_scene,
-                                                  const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
-                                                  uint32_t list_cap, Counters *ctr, uint32_t (*s_n)[NW],
-                                                  uint32_t *s_base)
-{
-    const uint32_t len = ne ? end - start : 0u;
-    const bool hv = ne && len > heavy_bin;
-    const bool md = ne && !hv && len > MED_BIN;
-    const bool lt = ne && !hv && !md;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t bh = __ballot(hv), bm = __ballot(md), bl = __ballot(lt);
-    if (lane == 0) {
-        s_n[0][wave] = HEAVY_SPLIT * (uint32_t)__popcll(bh) + (uint32_t)__popcll(bm);   // front items
-        s_n[1][wave] = (uint32_t)__popcll(bl);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t th = 0, tl = 0;
-        for (int w = 0; w < NW; w++) {
-            const uint32_t a = s_n[0][w], c = s_n[1][w];
-            s_n[0][w] = th;
-            s_n[1][w] = tl;
-            th += a;
-            tl += c;
-        }
-        s_base[0] = th ? atomicAdd(&ctr->nheavy, th) : 0u;
-        s_base[1] = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
-    }
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
-    // an item: scene-local tile | part << ITEM_TILE_BITS, bin start, bin end, scene
-    uint4 item = make_uint4(0u, 0u, 0u, 0u);
-    if (ne) {
-        const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
-        item = make_uint4(t - tile_pre[sc], start, end, sc);
-    }
-    const uint32_t of = s_base[0] + s_n[0][wave] + HEAVY_SPLIT * (uint32_t)__popcll(bh & below) +
-                        (uint32_t)__popcll(bm & below);
-    if (hv) {
-        for (uint32_t q = 0; q < HEAVY_SPLIT; q++)
-            if (of + q < list_cap) list[of + q] = make_uint4(item.x | ((q + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
-    }
-    if (md && of < list_cap) list[of] = item;
-    if (lt) {
-        const uint32_t o = s_base[1] + s_n[1][wave] + (uint32_t)__popcll(bl & below);
-        if (o < list_cap) list[list_cap - 1u - o] = item;
-    }
-    __syncthreads();   // s_n and s_base are read before a later call rewrites them
-}
-
-// The work list from the tile ranges k_segments stamped (one tile per thread).
+// long items start first.  The tile ranges are those k_segments stamped;
+// each thread takes TILE_LIST_PER consecutive tiles, so a block's items keep
+// tile order and one atomic (both list ends: nheavy and nlight are the low
+// and high words of one aligned 64-bit counter) serves TILE_LIST_PER * 256
+// tiles.
+constexpr uint32_t TILE_LIST_PER = 4;
 __global__ void __launch_bounds__(256) k_tile_list(const uint32_t *tile_start, const uint32_t *tile_end,
                                                    const uint32_t *tile_gen, uint32_t gen, uint32_t ntiles,
                                                    uint32_t heavy_bin, const uint32_t *tile_scene,
                                                    const uint32_t *tile_pre, uint32_t nscenes, uint4 *list,
                                                    uint32_t list_cap, Counters *ctr)
 {
-    __shared__ uint32_t s_n[2][4];
+    __shared__ uint32_t s_waves[4];
     __shared__ uint32_t s_base[2];
     if (exec_poisoned(ctr)) return;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool ne = t < ntiles && tile_gen[t] == gen;
-    append_tile_items<4>(ne, t, ne ? tile_start[t] : 0u, ne ? tile_end[t] : 0u, heavy_bin, tile_scene, tile_pre,
-                      nscenes, list, list_cap, ctr, s_n, s_base);
+    const uint32_t t0 = (blockIdx.x * 256u + threadIdx.x) * TILE_LIST_PER;
+    uint32_t st[TILE_LIST_PER], en[TILE_LIST_PER];
+    uint32_t front = 0, back = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < TILE_LIST_PER; q++) {
+        const uint32_t t = t0 + q;
+        const bool ne = t < ntiles && tile_gen[t] == gen;
+        st[q] = ne ? tile_start[t] : 0u;
+        en[q] = ne ? tile_end[t] : 0u;
+        const uint32_t len = en[q] - st[q];
+        front += len > heavy_bin ? HEAVY_SPLIT : len > MED_BIN ? 1u : 0u;
+        back += len > 0u && len <= heavy_bin && len <= MED_BIN ? 1u : 0u;
+    }
+    // one scan of (front, back): a block's front items are at most
+    // HEAVY_SPLIT * TILE_LIST_PER * 256 < 1 << 16
+    static_assert(HEAVY_SPLIT * TILE_LIST_PER * 256 < (1u << 16), "packed list counts");
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<uint32_t>(front | (back << 16), s_waves, tot);
+    if (threadIdx.x == 0) {
+        const uint32_t th = tot & 0xFFFFu, tl = tot >> 16;
+        const unsigned long long hl =
+            th | tl ? atomicAdd(reinterpret_cast<unsigned long long *>(&ctr->nheavy),
+                                ((unsigned long long)tl << 32) | th)
+                    : 0ull;
+        s_base[0] = (uint32_t)hl;
+        s_base[1] = (uint32_t)(hl >> 32);
+    }
+    __syncthreads();
+    uint32_t of = s_base[0] + (ex & 0xFFFFu), ob = s_base[1] + (ex >> 16);
+#pragma unroll
+    for (uint32_t q = 0; q < TILE_LIST_PER; q++) {
+        const uint32_t len = en[q] - st[q];
+        if (len == 0u) continue;
+        const uint32_t t = t0 + q;
+        const uint32_t sc = nscenes == 1u ? 0u : tile_scene[t];
+        // an item: scene-local tile | part << ITEM_TILE_BITS, bin start, bin end, scene
+        const uint4 item = make_uint4(t - tile_pre[sc], st[q], en[q], sc);
+        if (len > heavy_bin) {
+            for (uint32_t h = 0; h < HEAVY_SPLIT; h++)
+                if (of + h < list_cap)
+                    list[of + h] = make_uint4(item.x | ((h + 1u) << ITEM_TILE_BITS), item.y, item.z, item.w);
+            of += HEAVY_SPLIT;
+        } else if (len > MED_BIN) {
+            if (of < list_cap) list[of] = item;
+            of++;
+        } else {
+            if (ob < list_cap) list[list_cap - 1u - ob] = item;
+            ob++;
+        }
+    }
 }
 
 // The second walk of k_tile_bins over one wave's share [ws, we) of a
@@ -1845,8 +1868,12 @@ __global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys
             th += x0;
             tl += x1;
         }
-        hb = th ? atomicAdd(&ctr->nheavy, th) : 0u;
-        lb = tl ? atomicAdd(&ctr->nlight, tl) : 0u;
+        const unsigned long long hl =
+            th | tl ? atomicAdd(reinterpret_cast<unsigned long long *>(&ctr->nheavy),
+                                ((unsigned long long)tl << 32) | th)
+                    : 0ull;
+        hb = (uint32_t)hl;
+        lb = (uint32_t)(hl >> 32);
     }
     __syncthreads();   // cursors and wave offsets in place
     bins_scatter(lk, val, ws, we, one_round, load, nb, shift, cur[w], vals_out);
@@ -4399,6 +4426,7 @@ unsigned blocks_for(uint64_t n, unsigned threads) { return (unsigned)((n + threa
 
 constexpr size_t RING_BLOCK = 8u << 20;
 constexpr size_t RING_LIMIT = 256u << 20;   // staging a context holds before it waits for its queued work
+constexpr size_t PIXELS_DIRECT = 1u << 20;   // glDrawPixels images at least this large skip the staging ring
 
 // bytes of pinned staging from the ring (nullptr when the host is out of pinned memory)
 uint8_t *ring_take(Exec &x, size_t bytes)
@@ -5003,12 +5031,19 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     // launch that stopped between them leaves them to k_launch_begin
     if (!x.ctr_clean) hipLaunchKernelGGL(k_launch_begin, dim3(1), dim3(64), 0, x.stream, x.ctr);
     x.ctr_clean = false;
-    if (L.multi_vs)
-        hipLaunchKernelGGL(k_setup<true>, dim3(L.sblocks), dim3(3 * SETUP_BLOCK), 0, x.stream, L.sp, L.vp,
-                           L.sblock_pre, L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
-    else
-        hipLaunchKernelGGL(k_setup<false>, dim3(L.sblocks), dim3(3 * SETUP_BLOCK), 0, x.stream, L.sp, L.vp,
-                           L.sblock_pre, L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+    // a wave per corner while the blocks leave SIMDs idle
+    const bool split = L.sblocks <= SETUP_SPLIT_CU * x.cu_count;
+    auto setup_kernel = [&](auto kern, int nw) {
+        hipLaunchKernelGGL(kern, dim3(L.sblocks), dim3(nw * SETUP_BLOCK), 0, x.stream, L.sp, L.vp, L.sblock_pre,
+                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+    };
+    if (L.multi_vs) {
+        if (split) setup_kernel(k_setup<true, 3>, 3);
+        else setup_kernel(k_setup<true, 1>, 1);
+    } else {
+        if (split) setup_kernel(k_setup<false, 3>, 3);
+        else setup_kernel(k_setup<false, 1>, 1);
+    }
     HIPC(hipGetLastError());
 
     // ---- K3: pair offsets and total (read on the device: the host never
@@ -5075,7 +5110,7 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
         hipLaunchKernelGGL(k_segments, dim3(blocks_for(cap, 256)), dim3(256), 0, x.stream, (const uint32_t *)keys, cap,
                            np, ntiles, as<uint32_t>(x.tile_start), as<uint32_t>(x.tile_end), as<uint32_t>(x.tile_gen),
                            x.gen);
-        hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256)), dim3(256), 0, x.stream,
+        hipLaunchKernelGGL(k_tile_list, dim3(blocks_for(ntiles, 256 * TILE_LIST_PER)), dim3(256), 0, x.stream,
                            (const uint32_t *)as<uint32_t>(x.tile_start), (const uint32_t *)as<uint32_t>(x.tile_end),
                            (const uint32_t *)as<uint32_t>(x.tile_gen), x.gen, ntiles, x.heavy_bin, L.tmap, L.tile_pre,
                            L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
@@ -6302,6 +6337,16 @@ MTGL_API int mtgl_dev_draw_pixels(mtgl_dev *d, const mtgl_pixels *p)
     if (rc) return rc;
     Exec &x = d->ex;
     const size_t bytes = (size_t)n * bpp;
+    if (bytes >= PIXELS_DIRECT) {
+        // a large image is copied straight from the caller's memory once the
+        // queued work has taken effect (the log is then empty, so the image
+        // never needs to be drawn again): one wait instead of a host copy
+        // into the staging ring
+        if ((rc = confirm(d))) return rc;
+        if ((rc = arena_reserve(x, d->pixels, bytes))) return rc;
+        HIPC(hipMemcpy(d->pixels.ptr, p->pixels, bytes, hipMemcpyHostToDevice));
+        return draw_pixels_kernel(d, p, (const uint8_t *)d->pixels.ptr, n, true);
+    }
     if (x.ring_used + bytes > RING_LIMIT && (rc = confirm(d))) return rc;
     uint8_t *host = ring_take(x, bytes);
     if (!host) return MTGL_ERR_OUT_OF_MEMORY;
