@@ -1008,19 +1008,28 @@ template <bool MULTI, int NW>
 __global__ void __launch_bounds__(NW * SETUP_BLOCK) k_setup(const SetupParams *Ss, const VertexParams *Vs,
                                                             const uint32_t *block_pre, const uint32_t *block_scene,
                                                             uint32_t nscenes, uint32_t *blk_kept,
-                                                            unsigned long long *blk_pairs)
+                                                            unsigned long long *blk_pairs,
+                                                            const mtgl_raster_state *rs1,
+                                                            const mtgl_vertex_state *vs1)
 {
     static_assert(NW == 1 || NW == 3, "one wave, or one per corner");
     __shared__ SetupStage st;
     __shared__ CV s_corner[NW == 3 ? 3 : 1][NW == 3 ? SETUP_BLOCK : 1];
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
-    // two rounds of loads: the scene's tables, then the states they point to
+    // the scene's tables and the states they point to: one round of loads
+    // when the launch passes the states of its one scene (rs1, vs1), else two
     stage_words(&st.sp, Ss + sc, sizeof(SetupParams));
     stage_words(&st.vp, Vs + sc, sizeof(VertexParams));
+    if (rs1) {
+        stage_words(&st.rs, rs1, offsetof(mtgl_raster_state, lighting));
+        if (!MULTI) stage_words(&st.vs, vs1, sizeof(mtgl_vertex_state));
+    }
     __syncthreads();
-    stage_words(&st.rs, st.sp.rs, offsetof(mtgl_raster_state, lighting));
-    if (!MULTI) stage_words(&st.vs, st.vp.vstates, sizeof(mtgl_vertex_state));
-    __syncthreads();
+    if (!rs1) {
+        stage_words(&st.rs, st.sp.rs, offsetof(mtgl_raster_state, lighting));
+        if (!MULTI) stage_words(&st.vs, st.vp.vstates, sizeof(mtgl_vertex_state));
+        __syncthreads();
+    }
     const SetupParams &P = st.sp;
     const VertexParams &V = st.vp;
     const mtgl_vertex_state *vs0 = MULTI ? V.vstates : &st.vs;
@@ -1633,7 +1642,7 @@ __device__ __forceinline__ void bins_scatter(uint32_t (&lk)[R], uint32_t (&val)[
     }
 }
 
-// First level of the tile sort, one workgroup per SORT_TILE pairs: the
+// First level of the tile sort, one workgroup per LOCAL_TILE pairs: the
 // block's pairs sorted by bucket (key >> shift), stably, within the block's
 // own range of keys_out / vals_out, and per bucket the block's count and
 // the start of its run within the block (rows of RADIX in counts and loc;
@@ -1641,9 +1650,10 @@ __device__ __forceinline__ void bins_scatter(uint32_t (&lk)[R], uint32_t (&val)[
 // of the block: a first walk counts its pairs per bucket, a second ranks
 // equal buckets within each 64-pair chunk by ballots and advances the wave's
 // own cursors.
-constexpr uint32_t LOCAL_PER_WAVE = SORT_TILE / (SORT_THREADS / 64);
+constexpr uint32_t LOCAL_TILE = 1024;
+constexpr uint32_t LOCAL_PER_WAVE = LOCAL_TILE / (SORT_THREADS / 64);
 constexpr uint32_t LOCAL_CHUNKS = LOCAL_PER_WAVE / 64;
-static_assert(SORT_THREADS == RADIX && LOCAL_CHUNKS * 64 * (SORT_THREADS / 64) == SORT_TILE, "local sort shape");
+static_assert(SORT_THREADS == RADIX && LOCAL_CHUNKS * 64 * (SORT_THREADS / 64) == LOCAL_TILE, "local sort shape");
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_local(const uint32_t *keys_in, const uint32_t *vals_in,
                                                              uint32_t n_max, const unsigned long long *n_dev,
                                                              int shift, uint32_t *counts, uint32_t *loc,
@@ -1653,7 +1663,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_local(const uint32_t *key
     __shared__ uint32_t cur[W][RADIX];
     __shared__ uint32_t s_waves[W];
     const uint32_t n = (uint32_t)min((unsigned long long)n_max, *n_dev);
-    const uint32_t base = blockIdx.x * SORT_TILE;
+    const uint32_t base = blockIdx.x * LOCAL_TILE;
     const uint32_t row = blockIdx.x * RADIX + threadIdx.x;
     if (base >= n) {
         counts[row] = 0u;
@@ -1778,8 +1788,8 @@ __global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys
         const uint32_t c = i < nblocks ? counts[i * RADIX + bk] : 0u;
         const uint32_t lo = i < nblocks ? loc[i * RADIX + bk] : 0u;
         if (poison != 0u) return;
-        // one scan of (pairs, runs): a block holds at most SORT_TILE pairs,
-        // so the bucket at most BINS_MAX_SEGS * SORT_TILE < 1 << 21
+        // one scan of (pairs, runs): a block holds at most LOCAL_TILE pairs,
+        // so the bucket at most BINS_MAX_SEGS * LOCAL_TILE < 1 << 21
         uint32_t tot;
         const uint32_t packed = block_exclusive_scan<uint32_t>(c | (c ? 1u << 21 : 0u), s_waves, tot);
         len = tot & ((1u << 21) - 1u);
@@ -1787,7 +1797,7 @@ __global__ void __launch_bounds__(BINS_THREADS) k_tile_bins(const uint32_t *keys
         block_exclusive_scan<uint32_t>(lo, s_waves, a);   // a: the pairs of every earlier bucket
         if (c) {
             seg_v[packed >> 21] = packed & ((1u << 21) - 1u);
-            seg_src[packed >> 21] = i * (uint32_t)SORT_TILE + lo;
+            seg_src[packed >> 21] = i * LOCAL_TILE + lo;
         }
         if (i == 0) seg_v[nseg] = len;
     }
@@ -4290,6 +4300,10 @@ struct Launch {
     const uint32_t *tri_pre = nullptr;
     const SceneRaster *sr = nullptr;
     const uint32_t *tile_pre = nullptr;
+    // one scene: its raster state and first vertex state (set-up stages them
+    // with its tables instead of after them)
+    const mtgl_raster_state *rs1 = nullptr;
+    const mtgl_vertex_state *vs1 = nullptr;
     Arena maps;                    // block/tile -> scene maps of a launch with several scenes
     const uint32_t *vmap = nullptr, *smap = nullptr, *tmap = nullptr;
     // the bytes the device copy of `params` holds (blocks up to PARAM_SHADOW_MAX):
@@ -4900,6 +4914,10 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         q.tex_w = p.tx ? p.tx->w : 0;
         q.tex_h = p.tx ? p.tx->h : 0;
         q.rs = rs_dev;
+        if (s == 0) {
+            L.rs1 = rs_dev;
+            L.vs1 = v.vstates;
+        }
         q.rt = as<RTri>(x.rt) + toff;
         q.rcold = L.any_cold ? as<RCold>(x.rcold) + toff : nullptr;
         q.box = as<RBox>(x.box) + toff;
@@ -5008,8 +5026,9 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     const uint32_t list_cap = HEAVY_SPLIT * ntiles;   // up to HEAVY_SPLIT items per tile
     if ((rc = arena_reserve(x, x.tile_list, (size_t)list_cap * sizeof(uint4), frozen))) return rc;
     const uint32_t nblocks = (cap + SORT_TILE - 1) / SORT_TILE;
-    // digit counts (and, for the local sort, the runs' starts)
-    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)2 * RADIX * nblocks * 4, frozen))) return rc;
+    const uint32_t nlocal = (cap + LOCAL_TILE - 1) / LOCAL_TILE;
+    // digit counts, or the local sort's bucket counts and runs' starts
+    if ((rc = arena_reserve(x, x.sort_tmp, (size_t)2 * RADIX * std::max(nblocks, nlocal) * 4, frozen))) return rc;
     if (!frozen) {
         if (x.tile_gen.cap < (size_t)ntiles * 4 || x.gen == 0xFFFFFFFFu) {
             if ((rc = arena_reserve(x, x.tile_gen, (size_t)ntiles * 4))) return rc;
@@ -5035,7 +5054,8 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     const bool split = L.sblocks <= SETUP_SPLIT_CU * x.cu_count;
     auto setup_kernel = [&](auto kern, int nw) {
         hipLaunchKernelGGL(kern, dim3(L.sblocks), dim3(nw * SETUP_BLOCK), 0, x.stream, L.sp, L.vp, L.sblock_pre,
-                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs);
+                           L.smap, L.ns, as<uint32_t>(x.blk_kept), blk_pairs, L.ns == 1u ? L.rs1 : nullptr,
+                           L.ns == 1u ? L.vs1 : nullptr);
     };
     if (L.multi_vs) {
         if (split) setup_kernel(k_setup<true, 3>, 3);
@@ -5071,17 +5091,17 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     uint32_t *counts = as<uint32_t>(x.sort_tmp);
     // tile keys below RADIX << BINS_MAX_SHIFT: buckets of 1 << shift tiles
     // by the key's bits from `shift` up, either sorted within each block of
-    // SORT_TILE pairs (up to BINS_MAX_SEGS blocks) or by one global pass,
+    // LOCAL_TILE pairs (up to BINS_MAX_SEGS blocks) or by one global pass,
     // then one workgroup per bucket sorts it by tile and emits the work list
     int shift = 0;
     while (shift <= BINS_MAX_SHIFT && ((ntiles - 1u) >> shift) >= (uint32_t)RADIX) shift++;
-    if (ntiles > 0u && shift <= BINS_MAX_SHIFT && nblocks <= BINS_MAX_SEGS) {
+    if (ntiles > 0u && shift <= BINS_MAX_SHIFT && nlocal <= BINS_MAX_SEGS) {
         // each block sorted by bucket in place; the buckets gather their runs
-        uint32_t *loc = counts + (size_t)RADIX * nblocks;
-        hipLaunchKernelGGL(k_sort_local, dim3(nblocks), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
+        uint32_t *loc = counts + (size_t)RADIX * nlocal;
+        hipLaunchKernelGGL(k_sort_local, dim3(nlocal), dim3(SORT_THREADS), 0, x.stream, (const uint32_t *)keys,
                            (const uint32_t *)vals, cap, np, shift, counts, loc, keys2, vals2);
         hipLaunchKernelGGL(k_tile_bins, dim3(((ntiles - 1u) >> shift) + 1u), dim3(BINS_THREADS), 0, x.stream,
-                           (const uint32_t *)keys2, (const uint32_t *)vals2, cap, np, shift, nblocks, nullptr,
+                           (const uint32_t *)keys2, (const uint32_t *)vals2, cap, np, shift, nlocal, nullptr,
                            (const uint32_t *)counts, (const uint32_t *)loc, ntiles, vals, as<uint32_t>(x.tile_gen),
                            x.gen, x.heavy_bin, L.tmap, L.tile_pre, L.ns, as<uint4>(x.tile_list), list_cap, x.ctr);
     } else if (ntiles > 0u && shift <= BINS_MAX_SHIFT) {
