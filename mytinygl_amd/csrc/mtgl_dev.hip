@@ -114,13 +114,19 @@ static_assert(sizeof(RTri) == 176, "RTri layout");
 
 // what the binner reads of a record (16 B, same slot as the record): the
 // framebuffer-clamped tile box as the launch-wide key of its first tile, its
-// width, its tile count and the row pitch of the record's scene
+// width, its tile count and the row pitch of the record's scene; a box of at
+// most 64 tiles carries the mask of the tiles the record can cover (bit k:
+// tile k of the box, row-major), and ntiles counts those
 struct __align__(16) RBox {
     uint32_t key0;                       // scene tile offset + ty0 * tiles_x + tx0
     uint32_t tw;                         // tiles per box row
-    uint32_t ntiles;
+    uint32_t ntiles;                     // pairs of the record
     uint32_t tiles_x;
+    unsigned long long mask;             // (masked) tiles of the box with pairs
+    uint32_t masked;
+    uint32_t pad;
 };
+constexpr uint32_t BOX_MASK_TILES = 64;
 constexpr int RT_FRAG_VEC4 = 11;         // 176 bytes = 11 x 16 B staged in LDS per bin entry
 constexpr int RT_COV_VEC4 = 4;           // 64 bytes: vertices, box, area, 1/area, z, lod
 static_assert(offsetof(RTri, lod) == 60, "RTri layout");
@@ -892,6 +898,76 @@ __device__ __forceinline__ void cold_record(const CV &A, const CV &B, const CV &
     q.pad[0] = q.pad[1] = 0.0f;
 }
 
+// The tiles of a record's box (tw x th tiles from (tx0, ty0), at most 64)
+// that can hold a covered pixel: bit k for tile k, row-major.  The raster
+// covers a pixel when, for each edge, A(x) = fl((x - fx) * dy) >= B(y) =
+// fl((y - fy) * dx), formed exactly as in cover_chunk (the sign of the area
+// folded into the deltas), within the record's pixel box and the
+// framebuffer.  A is monotone in x and B in y (a correctly rounded product of
+// an exact difference), so a tile holds a pixel passing edge k exactly when
+// A at the tile's column end that maximises it is >= B at the row end that
+// minimises it; a tile failing one edge this way holds no covered pixel.
+__device__ __forceinline__ unsigned long long box_tile_mask(const int32_t *X, const int32_t *Y, float area,
+                                                            int32_t minX, int32_t minY, int32_t maxX, int32_t maxY,
+                                                            int32_t tx0, int32_t ty0, int32_t tw, int32_t th,
+                                                            int32_t fb_w, int32_t fb_h)
+{
+    const float fx0 = (float)X[0], fy0 = (float)Y[0], fx1 = (float)X[1], fy1 = (float)Y[1];
+    const float fx2 = (float)X[2], fy2 = (float)Y[2];
+    const float sg = area > 0.0f ? 1.0f : -1.0f;
+    const float ex[3] = {fx1, fx2, fx0}, ey[3] = {fy1, fy2, fy0};
+    const float dy[3] = {sg * (fy2 - fy1), sg * (fy0 - fy2), sg * (fy1 - fy0)};
+    const float dx[3] = {sg * (fx2 - fx1), sg * (fx0 - fx2), sg * (fx1 - fx0)};
+    const int32_t xl = minX, xh = min(maxX, fb_w - 1), yl = minY, yh = min(maxY, fb_h - 1);
+    // A of tile column i (its end maximising it), B of tile row j (its end
+    // minimising it), per edge
+    auto col_a = [&](int32_t i, float *a) {
+        const int32_t tx = tx0 + i;
+        const float xa = (float)max(tx * TILE, xl), xb = (float)min(tx * TILE + TILE - 1, xh);
+#pragma unroll
+        for (int e = 0; e < 3; e++) a[e] = ((dy[e] >= 0.0f ? xb : xa) - ex[e]) * dy[e];
+    };
+    auto row_b = [&](int32_t j, float *b) {
+        const int32_t ty = ty0 + j;
+        const float ya = (float)max(ty * TILE, yl), yb = (float)min(ty * TILE + TILE - 1, yh);
+#pragma unroll
+        for (int e = 0; e < 3; e++) b[e] = ((dx[e] >= 0.0f ? ya : yb) - ey[e]) * dx[e];
+    };
+    // the box holds at most 64 tiles, so its shorter side at most 8: those
+    // terms are formed once, the longer side's once per step
+    unsigned long long m = 0ull;
+    if (th <= tw) {
+        float b[8][3];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (j < th) row_b(j, b[j]);
+        for (int32_t i = 0; i < tw; i++) {
+            float a[3];
+            col_a(i, a);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const bool in = j < th && (a[0] >= b[j][0]) & (a[1] >= b[j][1]) & (a[2] >= b[j][2]);
+                m |= in ? 1ull << (uint32_t)(j * tw + i) : 0ull;   // (j * tw + i < 64 when j < th)
+            }
+        }
+    } else {
+        float a[8][3];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (i < tw) col_a(i, a[i]);
+        for (int32_t j = 0; j < th; j++) {
+            float b[3];
+            row_b(j, b);
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const bool in = i < tw && (a[i][0] >= b[0]) & (a[i][1] >= b[1]) & (a[i][2] >= b[2]);
+                m |= in ? 1ull << (uint32_t)(j * tw + i) : 0ull;
+            }
+        }
+    }
+    return m;
+}
+
 __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, const CV &A, const CV &B, const CV &C,
                          bool cold, RTri *outp, RCold *coldp, RBox *boxp, int *tiles_out)
 {
@@ -922,6 +998,17 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     bx.tw = (uint32_t)(tx1 - tx0 + 1);
     bx.ntiles = bx.tw * (uint32_t)(ty1 - ty0 + 1);
     bx.tiles_x = (uint32_t)P.tiles_x;
+    bx.mask = 0ull;
+    bx.masked = 0u;
+    bx.pad = 0u;
+    if (bx.ntiles <= BOX_MASK_TILES) {
+        // the tiles of a small box the record can cover: only those get pairs
+        const unsigned long long m = box_tile_mask(X, Y, r.area, r.minX, r.minY, r.maxX, r.maxY, tx0, ty0,
+                                                   (int32_t)bx.tw, ty1 - ty0 + 1, P.fb_w, P.fb_h);
+        bx.mask = m;
+        bx.masked = 1u;
+        bx.ntiles = (uint32_t)__popcll(m);
+    }
     *boxp = bx;
     if (cold) {
         RCold q;
@@ -1271,6 +1358,27 @@ __global__ void __launch_bounds__(SCAN1_THREADS) k_scan_single(const T *in, uint
 __device__ __forceinline__ void expand_record(const RBox &q, uint32_t rec, uint64_t p0, uint32_t k0, uint32_t k1,
                                               uint32_t step, uint64_t cap, uint32_t *keys, uint32_t *vals)
 {
+    if (q.masked) {
+        // the box's tiles with pairs, in box order: one thread takes them
+        // all (step 1), or lane k0 of a wave takes tile k0 (step 64)
+        const uint32_t tw = q.tw, base = q.key0, pitch = q.tiles_x;
+        if (step == 1u) {
+            uint64_t p = p0;
+            for (unsigned long long m = q.mask; m; m &= m - 1ull, p++) {
+                if (p >= cap) break;
+                const uint32_t k = (uint32_t)__builtin_ctzll(m);
+                keys[p] = base + (k / tw) * pitch + k % tw;
+                vals[p] = rec;
+            }
+        } else if (k0 < 64u && ((q.mask >> k0) & 1ull)) {
+            const uint64_t p = p0 + (uint64_t)__popcll(q.mask & ((1ull << k0) - 1ull));
+            if (p < cap) {
+                keys[p] = base + (k0 / tw) * pitch + k0 % tw;
+                vals[p] = rec;
+            }
+        }
+        return;
+    }
     const uint32_t tw = q.tw, base = q.key0, pitch = q.tiles_x;
     uint32_t ky = k0 / tw, kx = k0 % tw;
     const uint32_t sy = step / tw, sx = step % tw;
