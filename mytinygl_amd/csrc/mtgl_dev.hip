@@ -2907,14 +2907,15 @@ constexpr int DCAP = 512;          // fragment depths held in LDS per batch (dep
 // pixel 1 + the record slot of that last fragment (0: none), then shades that
 // one fragment per pixel (barycentrics, colour, lighting, texture, fog) from
 // its record.  One 8x8 tile per one-wave block; the depth pass reads only the
-// first 64 bytes of each record.  Three waves per SIMD: the shading needs
-// more registers than four would leave without spilling.
+// first 64 bytes of each record.  WPE waves per SIMD: with three the shading
+// keeps its registers (one scene's launch is faster so); with four a few
+// spill, which a launch of many small scenes gains back in latency hiding.
 // Per chunk: coverage (cover_chunk); then batches of at most DCAP fragments
 // in pixel-major order: the fragments' window depths one per lane, and per
 // pixel lane the depth/stencil steps in submission order over its own
 // fragments, which only load, compare and select.
-template <bool STENCIL>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster_depth(RasterParams P)
+template <bool STENCIL, int WPE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_raster_depth(RasterParams P)
 {
     __shared__ RCov s_cov[CHUNK];
     __shared__ uint64_t s_mask[TILE * TILE];   // coverage masks of a chunk, per pixel
@@ -4355,7 +4356,9 @@ struct Arena {
 struct Exec {
     hipStream_t stream = nullptr;
     int cu_count = 256;
-    uint32_t resident[3] = {256, 256, 256};   // blocks at once: depth<stencil>, depth<no stencil>, general
+    // blocks at once: depth<stencil>, depth<no stencil>, general, then the
+    // depth kernels at four waves per SIMD (launches of several scenes)
+    uint32_t resident[5] = {256, 256, 256, 256, 256};
     Counters *ctr = nullptr;                       // device
     Counters *ctr_host = nullptr;                  // pinned mirror for readback
     uint8_t *pinned = nullptr;                     // staging for uploads that wait for their copy
@@ -4673,9 +4676,10 @@ int exec_init(Exec &x, int device)
         x.cu_count <= 0)
         x.cu_count = 256;
     // resident one-wave blocks of each raster kernel (registers and LDS bound)
-    const void *kern[3] = {(const void *)k_raster_depth<true>, (const void *)k_raster_depth<false>,
-                           (const void *)k_raster};
-    for (int k = 0; k < 3; k++) {
+    const void *kern[5] = {(const void *)k_raster_depth<true, 3>, (const void *)k_raster_depth<false, 3>,
+                           (const void *)k_raster, (const void *)k_raster_depth<true, 4>,
+                           (const void *)k_raster_depth<false, 4>};
+    for (int k = 0; k < 5; k++) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
         x.resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)x.cu_count, STAT_SLOTS);
@@ -5283,8 +5287,15 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     const uint32_t lw_stencil = (1u << (LW | ST)) | (1u << (LW | ST | CO));
     const uint32_t lw_plain = (1u << LW) | (1u << (LW | CO));
     const uint32_t general = (1u << 0) | (1u << ST) | (1u << CO) | (1u << (ST | CO));
-    if (L.kinds & lw_stencil) hipLaunchKernelGGL(k_raster_depth<true>, grid(0), dim3(64), 0, x.stream, rp);
-    if (L.kinds & lw_plain) hipLaunchKernelGGL(k_raster_depth<false>, grid(1), dim3(64), 0, x.stream, rp);
+    const bool many = L.ns > 1u;   // many small scenes: more waves per SIMD
+    if (L.kinds & lw_stencil) {
+        if (many) hipLaunchKernelGGL((k_raster_depth<true, 4>), grid(3), dim3(64), 0, x.stream, rp);
+        else hipLaunchKernelGGL((k_raster_depth<true, 3>), grid(0), dim3(64), 0, x.stream, rp);
+    }
+    if (L.kinds & lw_plain) {
+        if (many) hipLaunchKernelGGL((k_raster_depth<false, 4>), grid(4), dim3(64), 0, x.stream, rp);
+        else hipLaunchKernelGGL((k_raster_depth<false, 3>), grid(1), dim3(64), 0, x.stream, rp);
+    }
     if (L.kinds & general) hipLaunchKernelGGL(k_raster, grid(2), dim3(64), 0, x.stream, rp);
     HIPC(hipGetLastError());
     if (ev0 && ev1) {
