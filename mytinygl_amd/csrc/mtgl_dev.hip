@@ -3878,8 +3878,12 @@ __global__ void __launch_bounds__(256) k_clear_span(uint32_t *color, float *dept
     const uint64_t end = p0 + n;
     if (q >= end) return;
     if (q >= p0 && q + 4 <= end) {
-        if (color) *reinterpret_cast<uint4 *>(color + q) = make_uint4(cv, cv, cv, cv);
-        if (depth) *reinterpret_cast<float4 *>(depth + q) = make_float4(dv, dv, dv, dv);
+        // streaming stores: the cleared lines are not kept in L2 for reading
+        typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+        const v4u cvv = {cv, cv, cv, cv};
+        const v4u dvv = {__float_as_uint(dv), __float_as_uint(dv), __float_as_uint(dv), __float_as_uint(dv)};
+        if (color) __builtin_nontemporal_store(cvv, reinterpret_cast<v4u *>(color + q));
+        if (depth) __builtin_nontemporal_store(dvv, reinterpret_cast<v4u *>(depth + q));
         if (stencil) *reinterpret_cast<uint32_t *>(stencil + q) = 0x01010101u * sv;
         return;
     }
