@@ -60,6 +60,18 @@ def test_feature_scene_odd_size(mtgl, oracle, name, size):
     _check_frame(f"{name}/{W}x{H}", a, b)
 
 
+# glDrawPixels images of 1 MB and more take their own path (a wait for the
+# queued work, then a copy straight from the caller's memory): at 1920x1080
+# the scene's rectangles reach 960x540 pixels, 2 MB in GL_RGBA
+@pytest.mark.parametrize("seed", [1, 2])
+def test_draw_pixels_large_images(mtgl, oracle, seed):
+    W, H = 1920, 1080
+    fn = S.FEATURE_SCENES["draw_pixels"]
+    (a, ea), (b, eb) = S.render(oracle, fn, W, H, seed), S.render(mtgl, fn, W, H, seed)
+    assert ea == eb
+    _check_frame(f"draw_pixels/{W}x{H}/{seed}", a, b)
+
+
 def _render_config(g, setup, draw, W, H):
     ctx = g.create_context(W, H)
     try:
