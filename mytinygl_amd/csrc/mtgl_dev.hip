@@ -66,7 +66,7 @@ constexpr int SCAN1_THREADS = 1024;     // the one-workgroup scan
 template <typename T> constexpr uint32_t SCAN1_ROWS = sizeof(T) == 4 ? 32u : 16u;
 template <typename T> constexpr uint64_t SCAN1_MAX = (uint64_t)SCAN1_ROWS<T> * SCAN1_THREADS;
 constexpr uint64_t SCAN2_MAX_BLOCKS = 256;   // scans of at most this many tiles take two launches
-constexpr uint32_t HEAVY_BIN = 256;     // default: tiles with longer bins are rasterised first, in parts
+constexpr uint32_t HEAVY_BIN = 192;     // default: tiles with longer bins are rasterised first, in parts
 constexpr uint32_t HEAVY_SPLIT = 8;     // parts of a heavy tile, TILE / HEAVY_SPLIT pixel rows each
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | part << 28 (0: whole tile, 1..HEAVY_SPLIT)
 constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
