@@ -71,7 +71,7 @@ constexpr uint32_t HEAVY_SPLIT = 8;     // parts of a heavy tile, TILE / HEAVY_S
 constexpr uint32_t ITEM_TILE_BITS = 28; // raster work item = global tile | part << 28 (0: whole tile, 1..HEAVY_SPLIT)
 constexpr int32_t PART_ROWS = TILE / (int32_t)HEAVY_SPLIT;
 static_assert(TILE % HEAVY_SPLIT == 0 && HEAVY_SPLIT < (1u << (32 - ITEM_TILE_BITS)), "heavy-tile parts");
-constexpr uint32_t MED_BIN = 40;        // whole tiles with longer bins join the front of the tile list
+constexpr uint32_t MED_BIN = 32;        // whole tiles with longer bins join the front of the tile list
 constexpr int SETUP_BLOCK = 64;         // triangles per block of the set-up kernel (one per lane)
 constexpr uint32_t SETUP_SPLIT_CU = 4;  // set-up launches of at most this many blocks per CU: a wave per corner
 constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
