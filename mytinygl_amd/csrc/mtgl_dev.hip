@@ -75,6 +75,7 @@ constexpr uint32_t MED_BIN = 32;        // whole tiles with longer bins join the
 constexpr int SETUP_BLOCK = 64;         // triangles per block of the set-up kernel (one per lane)
 constexpr uint32_t SETUP_SPLIT_CU = 4;  // set-up launches of at most this many blocks per CU: a wave per corner
 constexpr uint32_t EXPAND_SMALL = 64;   // pairs one thread writes; triangles with more go to a wave each
+constexpr uint32_t EXPAND_WAVE = 512;   // ... its own set-up block's wave up to this many, else one in k_expand_big
 constexpr uint32_t STAT_SLOTS = 16384;  // per-block fragment statistics (raster grids are capped at this)
 
 // ---------------------------------------------------------------------------
@@ -1399,8 +1400,10 @@ __device__ __forceinline__ void expand_record(const RBox &q, uint32_t rec, uint6
 // Pairs of every triangle, in (triangle, fan record, tile) order: one wave
 // per set-up block, whose pairs start at blk_off (the scan of the set-up
 // blocks' pair sums) and follow in triangle order.  A thread writes its
-// triangle's pairs when there are at most EXPAND_SMALL, longer triangles are
-// queued for k_expand_big (one wave each) with their first pair in pair_base.
+// triangle's pairs when there are at most EXPAND_SMALL, the block's wave
+// writes those of a triangle with at most EXPAND_WAVE together, and longer
+// triangles are queued for k_expand_big (one wave each) with their first
+// pair in pair_base.
 // With blk_off null (launches of at most EXPAND_FOLD set-up blocks) each
 // block sums the pair counts of the blocks before it itself, and block 0
 // sums them all into npairs: no scan launch.  Block 0 also ends the set-up's
@@ -1486,24 +1489,40 @@ __global__ void __launch_bounds__(SETUP_BLOCK) k_expand(const SetupParams *Ss, c
         }
     }
     uint64_t p0 = first + wave_inclusive_scan(np) - np;
-    const bool is_big = np > EXPAND_SMALL;
-    // one queue slot reservation per wave
-    const uint64_t bal = __ballot(is_big);
+    // up to EXPAND_SMALL pairs a thread writes alone; up to EXPAND_WAVE the
+    // whole wave writes them after that (a lane per tile); longer triangles
+    // are queued for k_expand_big, one queue slot reservation per wave
+    const bool coop = np > EXPAND_SMALL && np <= EXPAND_WAVE;
+    const bool queued = np > EXPAND_WAVE;
+    const uint64_t bal = __ballot(queued);
     if (bal) {
         uint32_t base = 0;
         if (lane == __builtin_ctzll(bal)) base = atomicAdd(&ctr->nbig, (uint32_t)__popcll(bal));
         base = __shfl(base, __builtin_ctzll(bal));
-        if (is_big) {
+        if (queued) {
             big[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = t;
             pair_base[t] = p0;
-            return;
         }
     }
-    if (np == 0) return;
-    for (uint32_t j = 0; j < nr; j++) {
-        const RBox q = j == 0u && rb == t ? q0 : box[rb + j];
-        expand_record(q, rb + j, p0, 0u, q.ntiles, 1u, cap, keys, vals);
-        p0 += q.ntiles;
+    if (np != 0u && !coop && !queued) {
+        uint64_t p = p0;
+        for (uint32_t j = 0; j < nr; j++) {
+            const RBox q = j == 0u && rb == t ? q0 : box[rb + j];
+            expand_record(q, rb + j, p, 0u, q.ntiles, 1u, cap, keys, vals);
+            p += q.ntiles;
+        }
+    }
+    for (uint64_t cb = __ballot(coop); cb; cb &= cb - 1ull) {   // (uniform)
+        const int src = __builtin_ctzll(cb);
+        const uint32_t rbs = (uint32_t)__builtin_amdgcn_readlane((int)rb, src);
+        const uint32_t nrs = (uint32_t)__builtin_amdgcn_readlane((int)nr, src);
+        uint64_t p = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(p0 >> 32), src) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)p0, src);
+        for (uint32_t j = 0; j < nrs; j++) {
+            const RBox q = box[rbs + j];
+            expand_record(q, rbs + j, p, (uint32_t)lane, q.ntiles, 64u, cap, keys, vals);
+            p += q.ntiles;
+        }
     }
 }
 
@@ -5200,7 +5219,8 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
                        (const RBox *)as<RBox>(x.box), (uint64_t)cap, keys, vals, as<uint32_t>(x.big_list), x.ctr, seq,
                        (const uint32_t *)as<uint32_t>(x.blk_kept), L.sblocks, ttot);
     x.ctr_clean = true;
-    hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 1024u)), dim3(64), 0,
+    // (the queue holds only triangles of over EXPAND_WAVE pairs: few waves)
+    hipLaunchKernelGGL(k_expand_big, dim3(std::min<uint32_t>(std::max<uint32_t>(ttot, 1u), 256u)), dim3(64), 0,
                        x.stream, (const unsigned long long *)pair_base, (const uint32_t *)as<uint32_t>(x.n_rt),
                        (const uint32_t *)as<uint32_t>(x.rec_base), (const RBox *)as<RBox>(x.box), (uint64_t)cap,
                        keys, vals, (const uint32_t *)as<uint32_t>(x.big_list), (const Counters *)x.ctr);
