@@ -2917,7 +2917,7 @@ __device__ __forceinline__ void add_stats(const RasterParams &P, unsigned long l
     }
 }
 
-constexpr int DCAP = 512;          // fragment depths held in LDS per batch (depth kernel)
+constexpr int DCAP = 512;          // fragment depths held in LDS per batch (depth kernel at 4 waves per SIMD)
 
 // K4a, the depth kernel of draws without blending, alpha test on a texture or
 // a partial colour mask: when nothing blends, only the last fragment that
@@ -2929,19 +2929,21 @@ constexpr int DCAP = 512;          // fragment depths held in LDS per batch (dep
 // first 64 bytes of each record.  WPE waves per SIMD: with three the shading
 // keeps its registers (one scene's launch is faster so); with four a few
 // spill, which a launch of many small scenes gains back in latency hiding.
-// Per chunk: coverage (cover_chunk); then batches of at most DCAP fragments
+// Per chunk: coverage (cover_chunk); then batches of at most DCAP_W fragments
 // in pixel-major order: the fragments' window depths one per lane, and per
 // pixel lane the depth/stencil steps in submission order over its own
 // fragments, which only load, compare and select.
 template <bool STENCIL, int WPE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_raster_depth(RasterParams P)
 {
+    // fragments per batch: at three waves per SIMD LDS has room for more
+    constexpr int DCAP_W = WPE == 3 ? 2 * DCAP : DCAP;
     __shared__ RCov s_cov[CHUNK];
     __shared__ uint64_t s_mask[TILE * TILE];   // coverage masks of a chunk, per pixel
     __shared__ uint32_t s_rid[CHUNK];
     __shared__ uint32_t s_tap[CHUNK];
-    __shared__ float s_dep[DCAP];
-    __shared__ uint16_t s_slot[DCAP];   // taps << 12 | entry << 6 | pixel
+    __shared__ float s_dep[DCAP_W];
+    __shared__ uint16_t s_slot[DCAP_W];   // taps << 12 | entry << 6 | pixel
 
     if (exec_poisoned(P.ctr)) return;   // the launch runs again once its arrays have grown
     RasterConsts K;
@@ -3109,26 +3111,26 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
         bool first = true;
         while (total && (first || __builtin_amdgcn_ballot_w64(rem != 0ull))) {
             // each pixel takes its next L fragments, L the largest quota that
-            // keeps the batch within DCAP (64 always fits)
+            // keeps the batch within DCAP_W (64 always fits)
             uint32_t L = 64;
-            if (!first || total > (uint32_t)DCAP) {
+            if (!first || total > (uint32_t)DCAP_W) {
                 const uint32_t rc = (uint32_t)__popcll(rem);
                 L = 1;
 #pragma unroll
                 for (uint32_t step = 32; step >= 1; step >>= 1) {
                     const uint32_t sum = (uint32_t)__builtin_amdgcn_readlane((int32_t)dpp_incl_scan(min(rc, L + step)), 63);
-                    if (sum <= (uint32_t)DCAP) L += step;
+                    if (sum <= (uint32_t)DCAP_W) L += step;
                 }
             }
             first = false;
             const uint32_t take = min((uint32_t)__popcll(rem), L);
             const uint32_t ij = dpp_incl_scan(take);
             const uint32_t off = ij - take;
-            const uint32_t fr = min((uint32_t)__builtin_amdgcn_readlane((int32_t)ij, 63), (uint32_t)DCAP);
+            const uint32_t fr = min((uint32_t)__builtin_amdgcn_readlane((int32_t)ij, 63), (uint32_t)DCAP_W);
             for (uint32_t o = off, qn_ = 0; qn_ < take; qn_++, o++) {
                 const uint32_t k = (uint32_t)__builtin_ctzll(rem);
                 rem &= rem - 1;
-                if (o < (uint32_t)DCAP) s_slot[o] = (uint16_t)((k << 6) | (uint32_t)lane);
+                if (o < (uint32_t)DCAP_W) s_slot[o] = (uint16_t)((k << 6) | (uint32_t)lane);
             }
             wave_sync();
             // window depth of every fragment of the batch, one per lane
@@ -3154,7 +3156,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
             wave_sync();
             // per pixel, in submission order; the LDS reads of four steps are
             // issued together, ahead of the dependent compares
-            const uint32_t nst = off < (uint32_t)DCAP ? min(take, (uint32_t)DCAP - off) : 0u;
+            const uint32_t nst = off < (uint32_t)DCAP_W ? min(take, (uint32_t)DCAP_W - off) : 0u;
             uint32_t bytes32 = 0;
             uint32_t qn_ = 0;
             for (; qn_ + 4u <= nst; qn_ += 4u) {
