@@ -911,7 +911,8 @@ __device__ __forceinline__ void cold_record(const CV &A, const CV &B, const CV &
 __device__ __forceinline__ unsigned long long box_tile_mask(const int32_t *X, const int32_t *Y, float area,
                                                             int32_t minX, int32_t minY, int32_t maxX, int32_t maxY,
                                                             int32_t tx0, int32_t ty0, int32_t tw, int32_t th,
-                                                            int32_t fb_w, int32_t fb_h)
+                                                            int32_t fb_w, int32_t fb_h, int32_t lo = 0,
+                                                            int32_t hi = 64)
 {
     const float fx0 = (float)X[0], fy0 = (float)Y[0], fx1 = (float)X[1], fy1 = (float)Y[1];
     const float fx2 = (float)X[2], fy2 = (float)Y[2];
@@ -935,14 +936,15 @@ __device__ __forceinline__ unsigned long long box_tile_mask(const int32_t *X, co
         for (int e = 0; e < 3; e++) b[e] = ((dx[e] >= 0.0f ? ya : yb) - ey[e]) * dx[e];
     };
     // the box holds at most 64 tiles, so its shorter side at most 8: those
-    // terms are formed once, the longer side's once per step
+    // terms are formed once, the longer side's once per step (the steps
+    // [lo, hi) of it)
     unsigned long long m = 0ull;
     if (th <= tw) {
         float b[8][3];
 #pragma unroll
         for (int j = 0; j < 8; j++)
             if (j < th) row_b(j, b[j]);
-        for (int32_t i = 0; i < tw; i++) {
+        for (int32_t i = lo; i < min(tw, hi); i++) {
             float a[3];
             col_a(i, a);
 #pragma unroll
@@ -956,7 +958,7 @@ __device__ __forceinline__ unsigned long long box_tile_mask(const int32_t *X, co
 #pragma unroll
         for (int i = 0; i < 8; i++)
             if (i < tw) col_a(i, a[i]);
-        for (int32_t j = 0; j < th; j++) {
+        for (int32_t j = lo; j < min(th, hi); j++) {
             float b[3];
             row_b(j, b);
 #pragma unroll
@@ -969,8 +971,19 @@ __device__ __forceinline__ unsigned long long box_tile_mask(const int32_t *X, co
     return m;
 }
 
+// What box_tile_mask needs of one record, handed from the set-up wave to
+// the waves that form the mask when set-up defers it (want: a box of at
+// most BOX_MASK_TILES tiles)
+struct MaskIn {
+    int32_t X[3], Y[3];
+    float area;
+    int32_t minX, minY, maxX, maxY;
+    int32_t tx0, ty0, tw, th;
+    int32_t want;
+};
+
 __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, const CV &A, const CV &B, const CV &C,
-                         bool cold, RTri *outp, RCold *coldp, RBox *boxp, int *tiles_out)
+                         bool cold, RTri *outp, RCold *coldp, RBox *boxp, int *tiles_out, MaskIn *defer = nullptr)
 {
     const int32_t vx = rs->viewport[0], vy = rs->viewport[1], vw = rs->viewport[2], vh = rs->viewport[3];
     const CV *v[3] = {&A, &B, &C};
@@ -1002,7 +1015,24 @@ __device__ int setup_fan(const SetupParams &P, const mtgl_raster_state *rs, cons
     bx.mask = 0ull;
     bx.masked = 0u;
     bx.pad = 0u;
-    if (bx.ntiles <= BOX_MASK_TILES) {
+    if (defer) {
+        // the caller has the mask formed elsewhere and completes the box
+        MaskIn &d = *defer;
+        for (int k = 0; k < 3; k++) {
+            d.X[k] = X[k];
+            d.Y[k] = Y[k];
+        }
+        d.area = r.area;
+        d.minX = r.minX;
+        d.minY = r.minY;
+        d.maxX = r.maxX;
+        d.maxY = r.maxY;
+        d.tx0 = tx0;
+        d.ty0 = ty0;
+        d.tw = (int32_t)bx.tw;
+        d.th = ty1 - ty0 + 1;
+        d.want = bx.ntiles <= BOX_MASK_TILES ? 1 : 0;
+    } else if (bx.ntiles <= BOX_MASK_TILES) {
         // the tiles of a small box the record can cover: only those get pairs
         const unsigned long long m = box_tile_mask(X, Y, r.area, r.minX, r.minY, r.maxX, r.maxY, tx0, ty0,
                                                    (int32_t)bx.tw, ty1 - ty0 + 1, P.fb_w, P.fb_h);
@@ -1103,6 +1133,8 @@ __global__ void __launch_bounds__(NW * SETUP_BLOCK) k_setup(const SetupParams *S
     static_assert(NW == 1 || NW == 3, "one wave, or one per corner");
     __shared__ SetupStage st;
     __shared__ CV s_corner[NW == 3 ? 3 : 1][NW == 3 ? SETUP_BLOCK : 1];
+    __shared__ MaskIn s_mi[NW == 3 ? SETUP_BLOCK : 1];                       // records whose mask waves 1, 2 form
+    __shared__ unsigned long long s_mpart[NW == 3 ? 2 : 1][NW == 3 ? SETUP_BLOCK : 1];   // ... their halves
     const uint32_t sc = scene_of(block_scene, nscenes, blockIdx.x);
     // the scene's tables and the states they point to: one round of loads
     // when the launch passes the states of its one scene (rs1, vs1), else two
@@ -1141,12 +1173,17 @@ __global__ void __launch_bounds__(NW * SETUP_BLOCK) k_setup(const SetupParams *S
             s_corner[NW == 3 ? corner : 0u][NW == 3 ? l : 0u] = o;
         }
         __syncthreads();
-        if (corner != 0u) return;   // (whole waves)
     }
     const mtgl_raster_state *rs = &st.rs;
-    uint32_t nout = 0;
+    uint32_t nout = 0, rb = 0;
     unsigned long long tiles = 0;
-    if (act) {
+    // NW = 3: the set-up wave leaves the tile masks of its unclipped records
+    // to waves 1 and 2 (half of each box's longer side each) and completes
+    // those boxes afterwards
+    MaskIn mi;
+    mi.want = 0;
+    RBox bxd;
+    if (corner == 0u && act) {
         CV v[3];
         if (NW == 3) {
             for (int k = 0; k < 3; k++) v[k] = s_corner[NW == 3 ? k : 0][NW == 3 ? l : 0u];
@@ -1165,21 +1202,55 @@ __global__ void __launch_bounds__(NW * SETUP_BLOCK) k_setup(const SetupParams *S
 #pragma unroll
             for (int pl = 0; pl < 6; pl++)
                 all_inside = all_inside && (plane_dist(v[k], pl) >= 0);
-        uint32_t rb = P.tri_off + t;
+        rb = P.tri_off + t;
         if (all_inside) {
             for (int k = 0; k < 3; k++) perspective_divide(v[k]);
             int nt = 0;
             RBox bx;
-            if (setup_fan(P, rs, v[0], v[1], v[2], cold, P.rt + t, cold ? P.rcold + t : nullptr, &bx, &nt)) {
+            if (setup_fan(P, rs, v[0], v[1], v[2], cold, P.rt + t, cold ? P.rcold + t : nullptr, &bx, &nt,
+                          NW == 3 ? &mi : nullptr)) {
                 nout = 1;
-                P.box[t] = bx;
                 tiles = (unsigned long long)nt;
+                if (mi.want) bxd = bx;
+                else P.box[t] = bx;
+            } else {
+                mi.want = 0;
             }
         } else {
             uint32_t base;
             nout = setup_clipped(P, rs, v[0], v[1], v[2], cold, base, tiles);
             rb = P.over_base + base;
         }
+    }
+    if (NW == 3) {
+        if (corner == 0u) {
+            if (mi.want) s_mi[l] = mi;
+            else s_mi[l].want = 0;
+        }
+        __syncthreads();
+        if (corner != 0u) {
+            const MaskIn &d = s_mi[l];
+            unsigned long long m = 0ull;
+            if (d.want) {
+                const int32_t half = (max(d.tw, d.th) + 1) / 2;
+                const int32_t lo = corner == 1u ? 0 : half, hi = corner == 1u ? half : 64;
+                m = box_tile_mask(d.X, d.Y, d.area, d.minX, d.minY, d.maxX, d.maxY, d.tx0, d.ty0, d.tw, d.th,
+                                  P.fb_w, P.fb_h, lo, hi);
+            }
+            s_mpart[corner - 1u][l] = m;
+        }
+        __syncthreads();
+        if (corner != 0u) return;   // (whole waves)
+        if (mi.want) {
+            const unsigned long long m = s_mpart[0][l] | s_mpart[1][l];
+            bxd.mask = m;
+            bxd.masked = 1u;
+            bxd.ntiles = (uint32_t)__popcll(m);
+            P.box[t] = bxd;
+            tiles = bxd.ntiles;
+        }
+    }
+    if (act) {
         P.n_rt[t] = nout;
         P.rec_base[t] = rb;
         P.tri_tiles[t] = tiles;
