@@ -30,6 +30,7 @@
 // and rounded once; spot cutoff cosines arrive precomputed by the host C
 // library.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -4751,6 +4752,19 @@ hipEvent_t take_event(Exec &x)
     return e;
 }
 
+// Both events of a timed launch, or neither (an event taken alone goes back
+// to the pool).
+bool take_event_pair(Exec &x, hipEvent_t &a, hipEvent_t &b)
+{
+    a = take_event(x);
+    b = take_event(x);
+    if (a && b) return true;
+    if (a) x.free_events.push_back(a);
+    if (b) x.free_events.push_back(b);
+    a = b = nullptr;
+    return false;
+}
+
 int resolve_timing(Exec &x)
 {
     for (auto &pr : x.pending_events) {
@@ -5364,11 +5378,7 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     rp.ctr = x.ctr;
     rp.stats = x.stat_slots;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (x.timing && !frozen && x.timing_count++ % x.timing_every == 0u) {
-        ev0 = take_event(x);
-        ev1 = take_event(x);
-        if (ev0) HIPC(hipEventRecord(ev0, x.stream));
-    }
+    if (x.timing && !frozen && x.timing_count++ % x.timing_every == 0u) (void)take_event_pair(x, ev0, ev1);
     // persistent grids over the work list (longest items first): at most one
     // block per item, at most as many as are resident at once
     const uint32_t nlist = (uint32_t)std::min<uint64_t>(est_items, (uint64_t)HEAVY_SPLIT * ntiles);
@@ -5385,19 +5395,35 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     const uint32_t lw_plain = (1u << LW) | (1u << (LW | CO));
     const uint32_t general = (1u << 0) | (1u << ST) | (1u << CO) | (1u << (ST | CO));
     const bool many = L.ns > 1u;   // many small scenes: more waves per SIMD
+    const void *fns[3];
+    dim3 grids[3];
+    int nl = 0;
     if (L.kinds & lw_stencil) {
-        if (many) hipLaunchKernelGGL((k_raster_depth<true, 4>), grid(3), dim3(64), 0, x.stream, rp);
-        else hipLaunchKernelGGL((k_raster_depth<true, 3>), grid(0), dim3(64), 0, x.stream, rp);
+        fns[nl] = many ? (const void *)k_raster_depth<true, 4> : (const void *)k_raster_depth<true, 3>;
+        grids[nl++] = grid(many ? 3 : 0);
     }
     if (L.kinds & lw_plain) {
-        if (many) hipLaunchKernelGGL((k_raster_depth<false, 4>), grid(4), dim3(64), 0, x.stream, rp);
-        else hipLaunchKernelGGL((k_raster_depth<false, 3>), grid(1), dim3(64), 0, x.stream, rp);
+        fns[nl] = many ? (const void *)k_raster_depth<false, 4> : (const void *)k_raster_depth<false, 3>;
+        grids[nl++] = grid(many ? 4 : 1);
     }
-    if (L.kinds & general) hipLaunchKernelGGL(k_raster, grid(2), dim3(64), 0, x.stream, rp);
-    HIPC(hipGetLastError());
-    if (ev0 && ev1) {
-        HIPC(hipEventRecord(ev1, x.stream));
+    if (L.kinds & general) {
+        fns[nl] = (const void *)k_raster;
+        grids[nl++] = grid(2);
+    }
+    void *args[] = {&rp};
+    if (ev0 && nl > 0) {
+        // timed: the events are taken at the first kernel's start and the
+        // last one's end, not around the launches' dispatch
+        for (int i = 0; i < nl; i++)
+            HIPC(hipExtLaunchKernel(fns[i], grids[i], dim3(64), args, 0, x.stream, i == 0 ? ev0 : nullptr,
+                                    i == nl - 1 ? ev1 : nullptr, 0));
         x.pending_events.emplace_back(ev0, ev1);
+    } else {
+        for (int i = 0; i < nl; i++) HIPC(hipLaunchKernel(fns[i], grids[i], dim3(64), args, 0, x.stream));
+        if (ev0) {
+            x.free_events.push_back(ev0);
+            x.free_events.push_back(ev1);
+        }
     }
     return L.unsupported ? MTGL_ERR_UNSUPPORTED : MTGL_OK;
 }
@@ -6493,18 +6519,13 @@ static int draw_pixels_kernel(mtgl_dev *d, const mtgl_pixels *p, const uint8_t *
     P.blend_dst = p->blend_dst;
     P.alpha_ref = p->alpha_ref;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (timed && x.timing && x.timing_count++ % x.timing_every == 0u) {
-        ev0 = take_event(x);
-        ev1 = take_event(x);
-        if (ev0) HIPC(hipEventRecord(ev0, x.stream));
-    }
-    hipLaunchKernelGGL(k_draw_pixels, dim3(blocks_for(n, 256)), dim3(256), 0, x.stream, P, src, d->color, d->depth,
-                       (const Counters *)x.ctr);
-    HIPC(hipGetLastError());
-    if (ev0 && ev1) {
-        HIPC(hipEventRecord(ev1, x.stream));
-        x.pending_events.emplace_back(ev0, ev1);
-    }
+    if (timed && x.timing && x.timing_count++ % x.timing_every == 0u) (void)take_event_pair(x, ev0, ev1);
+    const Counters *ctr = (const Counters *)x.ctr;
+    void *args[] = {&P, &src, &d->color, &d->depth, &ctr};
+    // a timed launch takes its events at the kernel's own start and end
+    HIPC(hipExtLaunchKernel((const void *)k_draw_pixels, dim3(blocks_for(n, 256)), dim3(256), args, 0, x.stream, ev0,
+                            ev1, 0));
+    if (ev0) x.pending_events.emplace_back(ev0, ev1);
     d->dirty = true;
     return MTGL_OK;
 }
