@@ -3040,7 +3040,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (!(kind & KIND_LAST_WRITER) || ((kind & KIND_STENCIL) != 0u) != STENCIL) continue;
     const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
-    if (sc != cur) {
+    // at three waves per SIMD the scene's constants are read again for every
+    // item: held across items they spill scalar registers on every pass
+    if (WPE == 3 || sc != cur) {
         raster_consts(S, K);
         K.stencil_on = STENCIL;   // the instantiation matches the scene's state
         cur = sc;
@@ -3303,10 +3305,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     __shared__ uint8_t s_meta[FCAP];
 
     if (exec_poisoned(P.ctr)) return;   // the launch runs again once its arrays have grown
-    RasterConsts K;
-    uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
-    bool need_color = false, alpha_dep = false;
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
     unsigned long long frag_count = 0, frag_bytes = 0;
@@ -3320,13 +3319,13 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (kind & KIND_LAST_WRITER) continue;
     const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
-    if (sc != cur) {
-        raster_consts(S, K);
-        need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
-        // the alpha test reads the texture, and its outcome gates the depth write
-        alpha_dep = K.alpha_on && K.textured;
-        cur = sc;
-    }
+    // the scene's constants are read again for every item: held across
+    // items they spill scalar registers on every pass
+    RasterConsts K;
+    raster_consts(S, K);
+    const bool need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
+    // the alpha test reads the texture, and its outcome gates the depth write
+    const bool alpha_dep = K.alpha_on && K.textured;
     const int32_t X0 = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE;
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
     const float xf = (float)x, yf = (float)y;
