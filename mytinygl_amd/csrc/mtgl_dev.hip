@@ -3020,7 +3020,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 
     if (exec_poisoned(P.ctr)) return;   // the launch runs again once its arrays have grown
     RasterConsts K;
-    uint32_t cur = 0xFFFFFFFFu;
     const int lane = threadIdx.x;
     uint32_t nh, it0, nt;
     list_counts(P, nh, it0, nt);
@@ -3041,13 +3040,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
     const uint32_t kind = __builtin_amdgcn_readfirstlane(S.kind);
     if (!(kind & KIND_LAST_WRITER) || ((kind & KIND_STENCIL) != 0u) != STENCIL) continue;
     const int32_t r0 = quarter ? ((int32_t)quarter - 1) * PART_ROWS : 0, r1 = quarter ? r0 + PART_ROWS : TILE;
-    // at three waves per SIMD the scene's constants are read again for every
-    // item: held across items they spill scalar registers on every pass
-    if (WPE == 3 || sc != cur) {
-        raster_consts(S, K);
-        K.stencil_on = STENCIL;   // the instantiation matches the scene's state
-        cur = sc;
-    }
+    // the scene's constants are read for every item (held across items they
+    // spill scalar registers on every pass), and again for the shading below
+    raster_consts(S, K);
+    K.stencil_on = STENCIL;   // the instantiation matches the scene's state
     const int32_t X0 = (int32_t)(tile % (uint32_t)S.tiles_x) * TILE, Y0 = (int32_t)(tile / (uint32_t)S.tiles_x) * TILE;
     const int32_t x = X0 + (lane & 7), y = Y0 + (lane >> 3);
     const int32_t fb_w = S.fb_w, fb_h = S.fb_h;
@@ -3261,6 +3257,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
         // the colour of the pixel's last passing fragment, from its record
         // (its loads are issued before this item's stores, which they would
         // otherwise wait behind)
+        // the constants again for the shading (a compiler barrier keeps the
+        // loads here): those only the shading reads are not held through the
+        // depth pass, which takes 18 VGPRs off the kernel
+        __asm__ volatile("" ::: "memory");
+        raster_consts(S, K);
+        K.stencil_on = STENCIL;
         const bool shade = in_fb && (kind & KIND_COLOR) && writer;
         uint32_t packed = 0;
         if (shade) {
