@@ -129,7 +129,6 @@ struct __align__(16) RBox {
     uint32_t pad;
 };
 constexpr uint32_t BOX_MASK_TILES = 64;
-constexpr int RT_FRAG_VEC4 = 11;         // 176 bytes = 11 x 16 B staged in LDS per bin entry
 constexpr int RT_COV_VEC4 = 4;           // 64 bytes: vertices, box, area, 1/area, z, lod
 static_assert(offsetof(RTri, lod) == 60, "RTri layout");
 
@@ -2655,11 +2654,6 @@ __device__ __forceinline__ void edge_values(const RTri *r, int32_t x, int32_t y,
 constexpr int CHUNK = 64;          // bin entries per coverage pass: one bit each in a pixel's mask
 constexpr int FCAP = 128;          // fragment colours held in LDS per batch (general kernel)
 
-// the colour-stage part of a record, as staged in LDS
-struct __align__(16) RFrag {
-    float4 v[RT_FRAG_VEC4];
-};
-
 // what the in-order depth pass reads per bin entry: the edge-function terms
 // in the reference's operation order, 1/area and the three window z
 struct __align__(16) RCov {
@@ -3294,10 +3288,13 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 // in-order depth/stencil pass; then colours for the fragments whose colour
 // step runs (all covered ones under an alpha test on a texture), one
 // fragment per lane in batches of at most FCAP, applied per pixel in
-// submission order.
-__global__ void __launch_bounds__(64) k_raster(RasterParams P)
+// submission order.  The colour step reads each fragment's record from
+// global memory (L1/L2: a chunk's fragments share few records) rather than
+// staging the chunk's 176-byte records in LDS: with 7.5 KB of LDS and at
+// most 168 VGPRs (a few spill) three waves share a SIMD instead of two, and
+// the 4K overdraw stage takes 1.41 ms instead of 1.57.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster(RasterParams P)
 {
-    __shared__ RFrag s_rec[CHUNK];
     __shared__ RCov s_cov[CHUNK];
     __shared__ uint64_t s_mask[TILE * TILE];   // coverage masks of a chunk, per pixel
     __shared__ uint32_t s_rid[CHUNK];
@@ -3343,15 +3340,16 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
     }
 
     // the next chunk's record indices are loaded while the current chunk is
-    // processed (its records then take one memory wait, not two)
+    // processed (its records' first 64 bytes then take one memory wait, not
+    // two)
     uint32_t ridn = (uint32_t)lane < e1 - e0 ? P.vals[e0 + lane] : 0u;
     for (uint32_t c0 = e0; c0 < e1; c0 += CHUNK) {
         const uint32_t cn = min((uint32_t)CHUNK, e1 - c0);
         const bool valid = (uint32_t)lane < cn;
         wave_sync();   // the previous chunk's records are no longer read
-        float4 q[RT_FRAG_VEC4];
+        float4 q[RT_COV_VEC4];
 #pragma unroll
-        for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < RT_COV_VEC4; i++) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         uint32_t rid = 0;
         const uint32_t rid_here = ridn;
         if (c0 + CHUNK < e1) ridn = (uint32_t)lane < e1 - (c0 + CHUNK) ? P.vals[c0 + CHUNK + lane] : 0u;
@@ -3359,9 +3357,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
             rid = rid_here;
             const float4 *src = reinterpret_cast<const float4 *>(&P.rt[rid]);
 #pragma unroll
-            for (int i = 0; i < RT_FRAG_VEC4; i++) q[i] = src[i];
-#pragma unroll
-            for (int i = 0; i < RT_FRAG_VEC4; i++) s_rec[lane].v[i] = q[i];
+            for (int i = 0; i < RT_COV_VEC4; i++) q[i] = src[i];
         }
         const RCov cv = make_cov(q);
         if (valid) {
@@ -3372,7 +3368,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
         const uint64_t m = cover_chunk<false>(cv, REC_I(q, minX), REC_I(q, minY), REC_I(q, maxX), REC_I(q, maxY),
                                        REC_F(q, area), valid, X0, Y0, fb_w, fb_h, r0, r1, s_mask);
         frag_count += (unsigned long long)__popcll(m);
-        wave_sync();   // staged records visible to every lane
+        wave_sync();   // staged entries visible to every lane
 
         const uint64_t todo = alpha_dep ? m : depth_pass(K, s_cov, s_tap, m, xf, yf, px, frag_bytes);
         // batches: each pixel takes its next L fragments, L the largest quota
@@ -3403,7 +3399,7 @@ __global__ void __launch_bounds__(64) k_raster(RasterParams P)
                 if (f < fr) {
                     const uint32_t sl = s_slot[f];
                     const uint32_t k = sl >> 6, j = sl & 63u;
-                    const RTri *r = reinterpret_cast<const RTri *>(&s_rec[k]);
+                    const RTri *r = P.rt + s_rid[k];
                     float w0e, w1e, w2e;
                     edge_values(r, X0 + (int32_t)(j & 7u), Y0 + (int32_t)(j >> 3), w0e, w1e, w2e);
                     const float inv = r->inv_area;
