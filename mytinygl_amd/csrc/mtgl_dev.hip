@@ -3292,8 +3292,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 // global memory (L1/L2: a chunk's fragments share few records) rather than
 // staging the chunk's 176-byte records in LDS: with 7.5 KB of LDS and at
 // most 168 VGPRs (a few spill) three waves share a SIMD instead of two, and
-// the 4K overdraw stage takes 1.41 ms instead of 1.57.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_raster(RasterParams P)
+// the 4K overdraw stage takes 1.41 ms instead of 1.57.  PLAIN: every scene
+// of the launch that this kernel draws has no texture and no per-fragment
+// lighting (Phong or two-sided), so those paths compile away: 124 VGPRs and
+// four waves per SIMD.
+template <bool PLAIN>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PLAIN ? 4 : 3))) k_raster(RasterParams P)
 {
     __shared__ RCov s_cov[CHUNK];
     __shared__ uint64_t s_mask[TILE * TILE];   // coverage masks of a chunk, per pixel
@@ -3323,6 +3327,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     // items they spill scalar registers on every pass
     RasterConsts K;
     raster_consts(S, K);
+    if (PLAIN) {
+        // what these scenes hold already (the launch checked them), as
+        // constants; phong and two_side only matter with lighting on
+        K.textured = false;
+        K.chain_on = false;
+        K.phong = false;
+        K.two_side = false;
+    }
     const bool need_color = K.blend_on || (K.cmask != 0xFu && K.cmask != 0u);
     // the alpha test reads the texture, and its outcome gates the depth write
     const bool alpha_dep = K.alpha_on && K.textured;
@@ -4453,7 +4465,7 @@ struct Exec {
     int cu_count = 256;
     // blocks at once: depth<stencil>, depth<no stencil>, general, then the
     // depth kernels at four waves per SIMD (launches of several scenes)
-    uint32_t resident[5] = {256, 256, 256, 256, 256};
+    uint32_t resident[6] = {256, 256, 256, 256, 256, 256};
     Counters *ctr = nullptr;                       // device
     Counters *ctr_host = nullptr;                  // pinned mirror for readback
     uint8_t *pinned = nullptr;                     // staging for uploads that wait for their copy
@@ -4498,6 +4510,7 @@ struct Launch {
     bool multi_vs = false;         // some scene switches vertex states per vertex
     bool unsupported = false;
     uint32_t kinds = 0;            // bit (1 << kind) for each raster kind present
+    bool plain = true;             // no scene for the general kernel has a texture or per-fragment lighting
     uint64_t pairs_cap = 0;        // pairs of the sizing run (frozen replays use it)
     const VertexParams *vp = nullptr;
     const uint32_t *vblock_pre = nullptr;
@@ -4784,10 +4797,10 @@ int exec_init(Exec &x, int device)
         x.cu_count <= 0)
         x.cu_count = 256;
     // resident one-wave blocks of each raster kernel (registers and LDS bound)
-    const void *kern[5] = {(const void *)k_raster_depth<true, 3>, (const void *)k_raster_depth<false, 3>,
-                           (const void *)k_raster, (const void *)k_raster_depth<true, 4>,
-                           (const void *)k_raster_depth<false, 4>};
-    for (int k = 0; k < 5; k++) {
+    const void *kern[6] = {(const void *)k_raster_depth<true, 3>, (const void *)k_raster_depth<false, 3>,
+                           (const void *)k_raster<false>, (const void *)k_raster_depth<true, 4>,
+                           (const void *)k_raster_depth<false, 4>, (const void *)k_raster<true>};
+    for (int k = 0; k < 6; k++) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
         x.resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)x.cu_count, STAT_SLOTS);
@@ -5058,6 +5071,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     L.vblocks = 0;
     L.sblocks = (uint32_t)sb;
     L.any_cold = z.cold;
+    L.plain = true;
 
     int rc;
     if ((rc = reserve_scene_buffers(x, z))) return rc;
@@ -5174,6 +5188,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         r.tile_off = tileoff;
         sr[s] = r;
         L.kinds |= 1u << r.kind;
+        if (!(r.kind & KIND_LAST_WRITER) && (r.tex != nullptr || needs_cold(dr->raster))) L.plain = false;
 
         vbp[s] = 0;
         sbp[s] = sbo;
@@ -5387,7 +5402,7 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
         // twice is 20% slower on the 1080p depth stage, four and six times
         // equal to three there; on the 4K overdraw stage six times is 1.5%
         // faster than three)
-        const uint32_t over = k == 2 ? 6u : 3u;
+        const uint32_t over = k == 2 || k == 5 ? 6u : 3u;
         const uint32_t g = std::min<uint32_t>(over * x.resident[k] / x.grid_div, STAT_SLOTS);
         uint32_t n = std::max<uint32_t>(1u, std::min<uint32_t>(nlist, g));
         if (n >= 64u) n &= ~63u;
@@ -5410,8 +5425,8 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
         grids[nl++] = grid(many ? 4 : 1);
     }
     if (L.kinds & general) {
-        fns[nl] = (const void *)k_raster;
-        grids[nl++] = grid(2);
+        fns[nl] = L.plain ? (const void *)k_raster<true> : (const void *)k_raster<false>;
+        grids[nl++] = grid(L.plain ? 5 : 2);
     }
     void *args[] = {&rp};
     if (ev0 && nl > 0) {
