@@ -3000,7 +3000,7 @@ constexpr int DCAP = 512;          // fragment depths held in LDS per batch (dep
 // in pixel-major order: the fragments' window depths one per lane, and per
 // pixel lane the depth/stencil steps in submission order over its own
 // fragments, which only load, compare and select.
-template <bool STENCIL, int WPE>
+template <bool STENCIL, int WPE, bool SIMPLE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) k_raster_depth(RasterParams P)
 {
     // fragments per batch: at three waves per SIMD LDS has room for more
@@ -3257,6 +3257,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
         __asm__ volatile("" ::: "memory");
         raster_consts(S, K);
         K.stencil_on = STENCIL;
+        if (SIMPLE) {
+            // no scene of the launch lights per fragment or samples the full
+            // chain (the launch checked them): those paths compile away, 16
+            // fewer VGPRs, more blocks resident
+            K.phong = false;
+            K.two_side = false;
+            K.chain_on = false;
+        }
         const bool shade = in_fb && (kind & KIND_COLOR) && writer;
         uint32_t packed = 0;
         if (shade) {
@@ -4465,7 +4473,7 @@ struct Exec {
     int cu_count = 256;
     // blocks at once: depth<stencil>, depth<no stencil>, general, then the
     // depth kernels at four waves per SIMD (launches of several scenes)
-    uint32_t resident[6] = {256, 256, 256, 256, 256, 256};
+    uint32_t resident[8] = {256, 256, 256, 256, 256, 256, 256, 256};
     Counters *ctr = nullptr;                       // device
     Counters *ctr_host = nullptr;                  // pinned mirror for readback
     uint8_t *pinned = nullptr;                     // staging for uploads that wait for their copy
@@ -4511,6 +4519,7 @@ struct Launch {
     bool unsupported = false;
     uint32_t kinds = 0;            // bit (1 << kind) for each raster kind present
     bool plain = true;             // no scene for the general kernel has a texture or per-fragment lighting
+    bool simple = true;            // no scene for the depth kernel lights per fragment or samples the full chain
     uint64_t pairs_cap = 0;        // pairs of the sizing run (frozen replays use it)
     const VertexParams *vp = nullptr;
     const uint32_t *vblock_pre = nullptr;
@@ -4797,10 +4806,11 @@ int exec_init(Exec &x, int device)
         x.cu_count <= 0)
         x.cu_count = 256;
     // resident one-wave blocks of each raster kernel (registers and LDS bound)
-    const void *kern[6] = {(const void *)k_raster_depth<true, 3>, (const void *)k_raster_depth<false, 3>,
-                           (const void *)k_raster<false>, (const void *)k_raster_depth<true, 4>,
-                           (const void *)k_raster_depth<false, 4>, (const void *)k_raster<true>};
-    for (int k = 0; k < 6; k++) {
+    const void *kern[8] = {(const void *)k_raster_depth<true, 3, false>, (const void *)k_raster_depth<false, 3, false>,
+                           (const void *)k_raster<false>, (const void *)k_raster_depth<true, 4, false>,
+                           (const void *)k_raster_depth<false, 4, false>, (const void *)k_raster<true>,
+                           (const void *)k_raster_depth<true, 3, true>, (const void *)k_raster_depth<false, 3, true>};
+    for (int k = 0; k < 8; k++) {
         int nb = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern[k], 64, 0) != hipSuccess || nb <= 0) nb = 2;
         x.resident[k] = std::min<uint32_t>((uint32_t)nb * (uint32_t)x.cu_count, STAT_SLOTS);
@@ -5072,6 +5082,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
     L.sblocks = (uint32_t)sb;
     L.any_cold = z.cold;
     L.plain = true;
+    L.simple = true;
 
     int rc;
     if ((rc = reserve_scene_buffers(x, z))) return rc;
@@ -5189,6 +5200,7 @@ int prepare_launch(Exec &x, Launch &L, const SceneIn *in, uint32_t n)
         sr[s] = r;
         L.kinds |= 1u << r.kind;
         if (!(r.kind & KIND_LAST_WRITER) && (r.tex != nullptr || needs_cold(dr->raster))) L.plain = false;
+        if ((r.kind & KIND_LAST_WRITER) && (needs_cold(dr->raster) || r.lod_mode == MTGL_LOD_FULL_CHAIN)) L.simple = false;
 
         vbp[s] = 0;
         sbp[s] = sbo;
@@ -5416,13 +5428,16 @@ int run_launch(Exec &x, Launch &L, bool frozen, uint32_t seq)
     const void *fns[3];
     dim3 grids[3];
     int nl = 0;
+    // one scene: three waves per SIMD, SIMPLE when it allows
     if (L.kinds & lw_stencil) {
-        fns[nl] = many ? (const void *)k_raster_depth<true, 4> : (const void *)k_raster_depth<true, 3>;
-        grids[nl++] = grid(many ? 3 : 0);
+        fns[nl] = many ? (const void *)k_raster_depth<true, 4, false>
+                       : L.simple ? (const void *)k_raster_depth<true, 3, true> : (const void *)k_raster_depth<true, 3, false>;
+        grids[nl++] = grid(many ? 3 : L.simple ? 6 : 0);
     }
     if (L.kinds & lw_plain) {
-        fns[nl] = many ? (const void *)k_raster_depth<false, 4> : (const void *)k_raster_depth<false, 3>;
-        grids[nl++] = grid(many ? 4 : 1);
+        fns[nl] = many ? (const void *)k_raster_depth<false, 4, false>
+                       : L.simple ? (const void *)k_raster_depth<false, 3, true> : (const void *)k_raster_depth<false, 3, false>;
+        grids[nl++] = grid(many ? 4 : L.simple ? 7 : 1);
     }
     if (L.kinds & general) {
         fns[nl] = L.plain ? (const void *)k_raster<true> : (const void *)k_raster<false>;
