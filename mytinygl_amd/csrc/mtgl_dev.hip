@@ -3299,11 +3299,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) 
 // submission order.  The colour step reads each fragment's record from
 // global memory (L1/L2: a chunk's fragments share few records) rather than
 // staging the chunk's 176-byte records in LDS: with 7.5 KB of LDS and at
-// most 168 VGPRs (a few spill) three waves share a SIMD instead of two, and
-// the 4K overdraw stage takes 1.41 ms instead of 1.57.  PLAIN: every scene
-// of the launch that this kernel draws has no texture and no per-fragment
-// lighting (Phong or two-sided), so those paths compile away: 124 VGPRs and
-// four waves per SIMD.
+// most 168 VGPRs (a few spill) three waves share a SIMD instead of two (the
+// 4K overdraw stage took 1.41 ms so, 1.57 with the records in LDS).  PLAIN:
+// every scene of the launch that this kernel draws has no texture and no
+// per-fragment lighting (Phong or two-sided), so those paths compile away:
+// 124 VGPRs, no spills and four waves per SIMD (that stage: 0.94 ms).
 template <bool PLAIN>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PLAIN ? 4 : 3))) k_raster(RasterParams P)
 {
